@@ -1,0 +1,776 @@
+// C ABI of the MI355X nonbonded cutoff force (include/pmnb.h).
+//
+// One context = one force object on one GPU.  The host side mirrors what
+// NonbondedCutoffSystemForce::evaluate does per call
+// (force/nonbonded/NonbondedCutoffSystemForce.h:45-55): rebuild the cell
+// list from the current positions, enumerate cell pairs, evaluate pairs,
+// accumulate forces, energies and virials.  All of it runs on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pmnb.h"
+#include "pmnb_common.h"
+#include "pmnb_cells.h"
+#include "pmnb_pair.h"
+
+using namespace pmnb;
+
+namespace {
+
+constexpr double kEpsilon = 1.0e-14;                   // base/PMConstants.cpp:47
+constexpr int kMaxInt2 = 2147483647 / 2;               // base/PMConstants.cpp:44
+constexpr double kPi = 3.14159265358979323846;
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t want) {
+    if (want <= bytes && p) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      p = nullptr;
+      bytes = 0;
+      if (e != hipSuccess) return e;
+    }
+    size_t b = want < 256 ? 256 : want;
+    hipError_t e = hipMalloc(&p, b);
+    if (e == hipSuccess) bytes = b;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+}  // namespace
+
+struct pmnb_ctx {
+  pmnb_params prm{};
+  hipStream_t stream = nullptr;
+  std::string err;
+  // topology
+  bool have_topo = false, have_box = false;
+  int n = 0, ntypes = 0, nmol = 0, max_excl_delta = 0;
+  double csf = 1.0;
+  // box / grid
+  Box box{};
+  Grid grid{};
+  std::vector<int4> shell;  // (0,0,0) then the reference's delta list
+  int per_pair_wrap = 0;
+  long long last_pairs = 0;
+  // device: topology (original order)
+  DevBuf d_type, d_q, d_mol, d_xfull, d_xmod, d_lj, d_ljf;
+  // device: per evaluation
+  DevBuf d_pos, d_force, d_out, d_com, d_cell_of, d_count, d_start, d_fill,
+      d_sorted, d_sx, d_sy, d_sz, d_loc, d_stype, d_sq, d_smol, d_scell,
+      d_shell, d_partial, d_npairs, d_pairs;
+  int shell_uploaded = 0;
+  // pair-kernel timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+};
+
+namespace {
+
+int fail(pmnb_ctx *c, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(call)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      return fail(ctx, PMNB_ERR_HIP, "%s failed: %s (%s:%d)", #call,        \
+                  hipGetErrorString(e_), __FILE__, __LINE__);               \
+  } while (0)
+
+int blocks_for(long long n, int b = kBlock) {
+  return (int)((n + b - 1) / b);
+}
+
+// Host restatement of the switching constants
+// (switch/C2SwitchingFunction.cpp:12-18, switch/C1SwitchingFunction.cpp:12-16).
+Switch make_c2(double on, double cut) {
+  Switch s{};
+  double d = cut * cut - on * on;
+  s.on2 = on * on;
+  s.cut2 = cut * cut;
+  s.s1 = 1.0 / ((d * d) * d);
+  s.s2 = cut * cut - 3.0 * on * on;
+  s.s3 = 4.0 / ((d * d) * d);
+  return s;
+}
+
+Switch make_c1(double cut) {
+  Switch s{};
+  s.on2 = 0.0;
+  s.cut2 = cut * cut;
+  s.c1_15c1 = 1.5 / cut;
+  s.c1_05c3 = 0.5 / (cut * cut * cut);
+  s.c1_15c3 = 1.5 / (cut * cut * cut);
+  return s;
+}
+
+// Box + grid + shell offsets exactly as PeriodicBoundaryConditions::set,
+// CubicCellManager::initialize, ArrayCellListStructure::initialize and the
+// periodic enumerator's delta list compute them.
+int build_box(pmnb_ctx *ctx, const double *e1, const double *e2,
+              const double *e3, const double *origin) {
+  if (e1[1] != 0.0 || e1[2] != 0.0 || e2[0] != 0.0 || e2[2] != 0.0 ||
+      e3[0] != 0.0 || e3[1] != 0.0)
+    return fail(ctx, PMNB_ERR_UNSUPPORTED, "only orthogonal boxes are built");
+  if (!(e1[0] > 0.0) || !(e2[1] > 0.0) || !(e3[2] > 0.0))
+    return fail(ctx, PMNB_ERR_ARG, "box vectors must be positive");
+  Box b{};
+  const double E[3] = {e1[0], e2[1], e3[2]};
+  // e_kr = cross / dot; for a diagonal box this is 1/e_k computed as the
+  // reference does: (e2 x e3).x / (e1 . (e2 x e3)).
+  const double a1 = e2[1] * e3[2] - e2[2] * e3[1];
+  const double a2 = e3[2] * e1[0] - e3[0] * e1[2];
+  const double a3 = e1[0] * e2[1] - e1[1] * e2[0];
+  b.er[0] = a1 / (e1[0] * a1 + e1[1] * 0.0 + e1[2] * 0.0);
+  b.er[1] = a2 / (e2[0] * 0.0 + e2[1] * a2 + e2[2] * 0.0);
+  b.er[2] = a3 / (e3[0] * 0.0 + e3[1] * 0.0 + e3[2] * a3);
+  for (int k = 0; k < 3; k++) {
+    b.e[k] = E[k];
+    b.origin[k] = origin[k];
+    const double s = (k == 0 ? e1[0] + e2[0] + e3[0]
+                             : (k == 1 ? e1[1] + e2[1] + e3[1]
+                                       : e1[2] + e2[2] + e3[2]));
+    const double lo = origin[k] - s * 0.5, hi = origin[k] + s * 0.5;
+    const double mn = std::min(lo, hi), mx = std::max(lo, hi);
+    b.mn[k] = mn;
+    b.H[k] = mx - mn;
+    b.H2[k] = b.H[k] * 0.5;
+    b.shift[k] = origin[k] - mn;
+  }
+  const double dx = e1[0] * 0.5, dy = e2[1] * 0.5, dz = e3[2] * 0.5;
+  const double DX = (dx * dx) * 1.0, DY = (dy * dy) * 1.0, DZ = (dz * dz) * 1.0;
+  b.D = std::min(DX, std::min(DY, DZ));
+
+  const double cs0 = ctx->prm.cell_size;
+  Grid g{};
+  int maxN[3], n1[3], maxN1[3];
+  for (int k = 0; k < 3; k++) {
+    const double vd = b.H[k];
+    const double f = std::floor(vd / cs0);
+    const double cs = vd / std::max(1.0, f);
+    g.cs[k] = cs;
+    g.inv[k] = 1.0 / cs;
+    const int n = (int)std::floor(vd / cs + kEpsilon);
+    g.n[k] = std::max(1, n);
+    maxN[k] = kMaxInt2 - kMaxInt2 % g.n[k];
+    n1[k] = -g.n[k] / 2;
+    maxN1[k] = maxN[k] - n1[k];
+  }
+  const long long ncell = (long long)g.n[0] * g.n[1] * g.n[2];
+  if (ncell > (1ll << 26))
+    return fail(ctx, PMNB_ERR_UNSUPPORTED,
+                "cell grid %dx%dx%d too large; raise the cell size", g.n[0],
+                g.n[1], g.n[2]);
+  g.ncell = (int)ncell;
+
+  // Delta list (CellListEnumeratorPeriodicBoundaries.h:46-83).
+  const double cut = ctx->prm.algorithm_cutoff;
+  const int nx = (int)(cut / g.cs[0] + 1.0 + kEpsilon);
+  const int ny = (int)(cut / g.cs[1] + 1.0 + kEpsilon);
+  const int nz = (int)(cut / g.cs[2] + 1.0 + kEpsilon);
+  const int r0 = std::min(nx, g.n[0]), r1 = std::min(ny, g.n[1]),
+            r2 = std::min(nz, g.n[2]);
+  const double xx = g.cs[0] * g.cs[0], yy = g.cs[1] * g.cs[1],
+               zz = g.cs[2] * g.cs[2], c2 = cut * cut;
+  std::vector<int4> dl;
+  auto seen = [&](int x, int y, int z) {
+    if (x == 0 && y == 0 && z == 0) return true;
+    for (const int4 &v : dl)
+      if (v.x == x && v.y == y && v.z == z) return true;
+    return false;
+  };
+  for (int k = -r0; k <= r0; k++) {
+    int x = std::abs(k) - 1;
+    if (x < 0) x = 0;
+    const double d0 = x * x * xx;
+    for (int l = -r1; l <= r1; l++) {
+      int y = std::abs(l) - 1;
+      if (y < 0) y = 0;
+      const double d1 = d0 + y * y * yy;
+      for (int m = -r2; m <= r2; m++) {
+        int z = std::abs(m) - 1;
+        if (z < 0) z = 0;
+        if (d1 + z * z * zz < c2) {
+          const int bx = (k + maxN1[0]) % g.n[0] + n1[0];
+          const int by = (l + maxN1[1]) % g.n[1] + n1[1];
+          const int bz = (m + maxN1[2]) % g.n[2] + n1[2];
+          if (!seen(bx, by, bz)) dl.push_back(make_int4(bx, by, bz, 0));
+        }
+      }
+    }
+  }
+  std::sort(dl.begin(), dl.end(), [](const int4 &p, const int4 &q) {
+    if (p.x != q.x) return p.x < q.x;
+    if (p.y != q.y) return p.y < q.y;
+    return p.z < q.z;
+  });
+  ctx->shell.clear();
+  ctx->shell.push_back(make_int4(0, 0, 0, 0));
+  ctx->shell.insert(ctx->shell.end(), dl.begin(), dl.end());
+  // The mixed kernel takes the image of a neighbour atom from its cell's
+  // offset; that is the minimum image only when every offset in
+  // [-reach, reach] is its own periodic class.
+  int reach[3] = {0, 0, 0};
+  for (const int4 &v : ctx->shell) {
+    reach[0] = std::max(reach[0], std::abs(v.x));
+    reach[1] = std::max(reach[1], std::abs(v.y));
+    reach[2] = std::max(reach[2], std::abs(v.z));
+  }
+  ctx->per_pair_wrap = 0;
+  for (int k = 0; k < 3; k++)
+    if (g.n[k] < 2 * reach[k] + 1) ctx->per_pair_wrap = 1;
+  ctx->box = b;
+  ctx->grid = g;
+  ctx->shell_uploaded = 0;
+  ctx->have_box = true;
+  return PMNB_OK;
+}
+
+ForceParams make_params(const pmnb_ctx *ctx, uint32_t flags) {
+  ForceParams P{};
+  P.box = ctx->box;
+  P.grid = ctx->grid;
+  P.sw_lj = make_c2(ctx->prm.lj_switchon, ctx->prm.lj_cutoff);
+  if (ctx->prm.elec_switch == PMNB_SWITCH_C1)
+    P.sw_el = make_c1(ctx->prm.elec_cutoff);
+  else
+    P.sw_el = make_c2(ctx->prm.elec_switchon, ctx->prm.elec_cutoff);
+  // OneAtomPairTwo::mySquaredCutoff: max of the two switching cutoffs
+  // (force/OneAtomPairTwo.h:38-40).  With no electrostatic functor the
+  // second switch still exists in the reference's two-force template, so
+  // its cutoff still counts.
+  P.cut2 = std::max(P.sw_lj.cut2, P.sw_el.cut2);
+  P.alpha = ctx->prm.ewald_alpha;
+  P.two_alpha_pi = 2.0 * ctx->prm.ewald_alpha / std::sqrt(kPi);
+  P.csf = ctx->csf;
+  P.n_atoms = ctx->n;
+  P.n_types = ctx->ntypes;
+  P.ndelta = (int)ctx->shell.size();
+  P.elec = ctx->prm.elec;
+  P.el_switch = ctx->prm.elec_switch;
+  P.max_excl_delta = ctx->max_excl_delta;
+  P.per_pair_wrap = ctx->per_pair_wrap;
+  P.flags = flags;
+  P.band = 1.0e-3f * (float)std::max(1.0, P.cut2 / 100.0);
+  return P;
+}
+
+void launch_fill_i32(int *p, int n, int v, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n)), dim3(kBlock), 0, s, p,
+                       n, v);
+}
+
+int ensure_eval_buffers(pmnb_ctx *ctx) {
+  const int n = ctx->n, nc = ctx->grid.ncell;
+  const size_t N = (size_t)std::max(n, 1);
+  HIPCHK(ctx->d_cell_of.ensure(N * sizeof(int)));
+  HIPCHK(ctx->d_count.ensure((size_t)nc * sizeof(int)));
+  HIPCHK(ctx->d_start.ensure((size_t)nc * sizeof(int)));
+  HIPCHK(ctx->d_fill.ensure((size_t)nc * sizeof(int)));
+  HIPCHK(ctx->d_sorted.ensure(N * sizeof(int)));
+  HIPCHK(ctx->d_sx.ensure(N * sizeof(double)));
+  HIPCHK(ctx->d_sy.ensure(N * sizeof(double)));
+  HIPCHK(ctx->d_sz.ensure(N * sizeof(double)));
+  HIPCHK(ctx->d_loc.ensure(N * sizeof(float4)));
+  HIPCHK(ctx->d_stype.ensure(N * sizeof(int)));
+  HIPCHK(ctx->d_sq.ensure(N * sizeof(double)));
+  HIPCHK(ctx->d_smol.ensure(N * sizeof(int)));
+  HIPCHK(ctx->d_scell.ensure(N * sizeof(int)));
+  HIPCHK(ctx->d_partial.ensure((size_t)blocks_for(n) * ACC_N * sizeof(double)));
+  HIPCHK(ctx->d_npairs.ensure(sizeof(unsigned long long)));
+  if (!ctx->shell_uploaded) {
+    HIPCHK(ctx->d_shell.ensure(ctx->shell.size() * sizeof(int4)));
+    HIPCHK(hipMemcpy(ctx->d_shell.p, ctx->shell.data(),
+                     ctx->shell.size() * sizeof(int4), hipMemcpyHostToDevice));
+    ctx->shell_uploaded = 1;
+  }
+  return PMNB_OK;
+}
+
+// Cell-list build on the device from d_pos (original order).
+int build_cells(pmnb_ctx *ctx, const double *d_pos, hipStream_t s) {
+  const int n = ctx->n, nc = ctx->grid.ncell;
+  launch_fill_i32(ctx->d_count.as<int>(), nc, 0, s);
+  launch_fill_i32(ctx->d_fill.as<int>(), nc, 0, s);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_assign, dim3(blocks_for(n)), dim3(kBlock), 0, s,
+                       ctx->box, ctx->grid, d_pos, n, ctx->d_cell_of.as<int>(),
+                       ctx->d_count.as<int>());
+  }
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, ctx->d_count.as<int>(),
+                     ctx->d_start.as<int>(), nc);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, s,
+                       ctx->d_cell_of.as<int>(), n, ctx->d_start.as<int>(),
+                       ctx->d_fill.as<int>(), ctx->d_sorted.as<int>());
+    hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(nc)), dim3(kBlock), 0, s,
+                       ctx->d_start.as<int>(), ctx->d_count.as<int>(), nc,
+                       ctx->d_sorted.as<int>());
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for(n)), dim3(kBlock), 0, s,
+                       ctx->box, ctx->grid, d_pos, ctx->d_sorted.as<int>(),
+                       ctx->d_cell_of.as<int>(), ctx->d_type.as<int>(),
+                       ctx->d_q.as<double>(), ctx->d_mol.as<int>(), n,
+                       ctx->d_sx.as<double>(), ctx->d_sy.as<double>(),
+                       ctx->d_sz.as<double>(), ctx->d_loc.as<float4>(),
+                       ctx->d_stype.as<int>(), ctx->d_sq.as<double>(),
+                       ctx->d_smol.as<int>(), ctx->d_scell.as<int>());
+  }
+  HIPCHK(hipGetLastError());
+  return PMNB_OK;
+}
+
+Sorted make_sorted(const pmnb_ctx *ctx, const double *d_com) {
+  Sorted S{};
+  S.x = ctx->d_sx.as<double>();
+  S.y = ctx->d_sy.as<double>();
+  S.z = ctx->d_sz.as<double>();
+  S.loc = ctx->d_loc.as<float4>();
+  S.atom = ctx->d_sorted.as<int>();
+  S.type = ctx->d_stype.as<int>();
+  S.q = ctx->d_sq.as<double>();
+  S.mol = ctx->d_smol.as<int>();
+  S.cell = ctx->d_scell.as<int>();
+  S.cstart = ctx->d_start.as<int>();
+  S.ccount = ctx->d_count.as<int>();
+  S.delta = ctx->d_shell.as<int4>();
+  S.xfull = ctx->d_xfull.as<unsigned long long>();
+  S.xmod = ctx->d_xmod.as<unsigned long long>();
+  S.lj = ctx->d_lj.as<double>();
+  S.ljf = ctx->d_ljf.as<float4>();
+  S.com = d_com;
+  return S;
+}
+
+int check_ready(pmnb_ctx *ctx) {
+  if (!ctx) return PMNB_ERR_ARG;
+  if (!ctx->have_topo)
+    return fail(ctx, PMNB_ERR_STATE, "pmnb_set_topology has not been called");
+  if (!ctx->have_box)
+    return fail(ctx, PMNB_ERR_STATE, "pmnb_set_box has not been called");
+  return PMNB_OK;
+}
+
+int run_device(pmnb_ctx *ctx, const double *d_pos, double *d_forces,
+               double *d_out, const double *d_com, uint32_t flags,
+               hipStream_t s) {
+  int rc = ensure_eval_buffers(ctx);
+  if (rc) return rc;
+  if ((flags & PMNB_FLAG_MOLVIRIAL) && !d_com)
+    return fail(ctx, PMNB_ERR_ARG, "molecular virial needs molecule centres");
+  rc = build_cells(ctx, d_pos, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s,
+                     ctx->d_npairs.as<unsigned long long>(), 1);
+  const ForceParams P = make_params(ctx, flags);
+  const Sorted S = make_sorted(ctx, d_com);
+  const int nb = std::max(1, blocks_for(ctx->n));
+  double *part = ctx->d_partial.as<double>();
+  unsigned long long *np = ctx->d_npairs.as<unsigned long long>();
+  const bool mixed = ctx->prm.precision == PMNB_PREC_MIXED;
+  const int el = ctx->prm.elec, sw = ctx->prm.elec_switch;
+  hipEvent_t ev_stop = nullptr;
+  if (ctx->timing) {
+    while (ctx->ev_pool.size() < ctx->ev_used + 2) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      ctx->ev_pool.push_back(e);
+    }
+    HIPCHK(hipEventRecord(ctx->ev_pool[ctx->ev_used], s));
+    ev_stop = ctx->ev_pool[ctx->ev_used + 1];
+    ctx->ev_used += 2;
+  }
+  if (mixed) {
+    if (el == PMNB_ELEC_NONE)
+      hipLaunchKernelGGL(k_pair_mixed<0>, dim3(nb), dim3(kBlock), 0, s, P, S,
+                         d_forces, part, np);
+    else if (el == PMNB_ELEC_COULOMB)
+      hipLaunchKernelGGL(k_pair_mixed<1>, dim3(nb), dim3(kBlock), 0, s, P, S,
+                         d_forces, part, np);
+    else
+      hipLaunchKernelGGL(k_pair_mixed<2>, dim3(nb), dim3(kBlock), 0, s, P, S,
+                         d_forces, part, np);
+  } else {
+    if (el == PMNB_ELEC_NONE)
+      hipLaunchKernelGGL((k_pair_fp64<0, 0>), dim3(nb), dim3(kBlock), 0, s, P,
+                         S, d_forces, part, np);
+    else if (el == PMNB_ELEC_COULOMB && sw == PMNB_SWITCH_C2)
+      hipLaunchKernelGGL((k_pair_fp64<1, 0>), dim3(nb), dim3(kBlock), 0, s, P,
+                         S, d_forces, part, np);
+    else if (el == PMNB_ELEC_COULOMB)
+      hipLaunchKernelGGL((k_pair_fp64<1, 1>), dim3(nb), dim3(kBlock), 0, s, P,
+                         S, d_forces, part, np);
+    else if (sw == PMNB_SWITCH_C2)
+      hipLaunchKernelGGL((k_pair_fp64<2, 0>), dim3(nb), dim3(kBlock), 0, s, P,
+                         S, d_forces, part, np);
+    else
+      hipLaunchKernelGGL((k_pair_fp64<2, 1>), dim3(nb), dim3(kBlock), 0, s, P,
+                         S, d_forces, part, np);
+  }
+  if (ev_stop) HIPCHK(hipEventRecord(ev_stop, s));
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, s, part, nb, d_out);
+  HIPCHK(hipGetLastError());
+  return PMNB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pmnb_version(void) { return 100; }
+
+int pmnb_create(const pmnb_params *params, pmnb_ctx **out) {
+  pmnb_ctx *ctx = nullptr;
+  if (!params || !out) return PMNB_ERR_ARG;
+  *out = nullptr;
+  if (params->elec < PMNB_ELEC_NONE || params->elec > PMNB_ELEC_EWALD_REAL)
+    return PMNB_ERR_ARG;
+  if (params->elec_switch != PMNB_SWITCH_C2 &&
+      params->elec_switch != PMNB_SWITCH_C1)
+    return PMNB_ERR_ARG;
+  if (params->precision != PMNB_PREC_FP64 &&
+      params->precision != PMNB_PREC_MIXED)
+    return PMNB_ERR_ARG;
+  if (params->precision == PMNB_PREC_MIXED &&
+      params->elec_switch == PMNB_SWITCH_C1)
+    return PMNB_ERR_UNSUPPORTED;
+  // C2SwitchingFunction::make: 0 <= switchon < cutoff
+  // (switch/C2SwitchingFunction.cpp:30-41).
+  if (!(params->lj_switchon >= 0.0) ||
+      !(params->lj_switchon < params->lj_cutoff))
+    return PMNB_ERR_ARG;
+  if (params->elec_switch == PMNB_SWITCH_C2 &&
+      (!(params->elec_switchon >= 0.0) ||
+       !(params->elec_switchon < params->elec_cutoff)))
+    return PMNB_ERR_ARG;
+  if (!(params->elec_cutoff > 0.0) || !(params->algorithm_cutoff > 0.0) ||
+      !(params->cell_size > 0.0))
+    return PMNB_ERR_ARG;
+  if (params->elec == PMNB_ELEC_EWALD_REAL && !(params->ewald_alpha > 0.0))
+    return PMNB_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PMNB_ERR_HIP;
+  if (params->device < 0 || params->device >= ndev) return PMNB_ERR_ARG;
+  if (hipSetDevice(params->device) != hipSuccess) return PMNB_ERR_HIP;
+  ctx = new pmnb_ctx();
+  ctx->prm = *params;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) !=
+      hipSuccess) {
+    delete ctx;
+    return PMNB_ERR_HIP;
+  }
+  *out = ctx;
+  return PMNB_OK;
+}
+
+int pmnb_set_topology(pmnb_ctx *ctx, int32_t n_atoms, const int32_t *type,
+                      const double *scaled_charge, const int32_t *molecule,
+                      int32_t n_types, const double *lj,
+                      double coulomb_scaling_factor, int32_t n_excl,
+                      const int32_t *excl) {
+  if (!ctx) return PMNB_ERR_ARG;
+  if (n_atoms < 0 || n_types <= 0 || n_excl < 0)
+    return fail(ctx, PMNB_ERR_ARG, "negative sizes");
+  if (n_atoms > 0 && (!type || !scaled_charge || !molecule))
+    return fail(ctx, PMNB_ERR_ARG, "null per-atom array");
+  if (!lj) return fail(ctx, PMNB_ERR_ARG, "null LJ table");
+  if (n_excl > 0 && !excl) return fail(ctx, PMNB_ERR_ARG, "null exclusions");
+  int nmol = 0;
+  for (int i = 0; i < n_atoms; i++) {
+    if (type[i] < 0 || type[i] >= n_types)
+      return fail(ctx, PMNB_ERR_ARG, "atom %d has type %d outside [0,%d)", i,
+                  type[i], n_types);
+    if (molecule[i] < 0)
+      return fail(ctx, PMNB_ERR_ARG, "atom %d has molecule id %d", i,
+                  molecule[i]);
+    nmol = std::max(nmol, molecule[i] + 1);
+  }
+  HIPCHK(hipSetDevice(ctx->prm.device));
+  // Exclusion masks: last writer wins per pair (ExclusionTable.cpp:105-120).
+  std::vector<unsigned long long> full((size_t)std::max(n_atoms, 1), 0ull),
+      mod((size_t)std::max(n_atoms, 1), 0ull);
+  int maxd = 0;
+  for (int k = 0; k < n_excl; k++) {
+    int a = excl[3 * k], b = excl[3 * k + 1], cls = excl[3 * k + 2];
+    if (a < 0 || b < 0 || a >= n_atoms || b >= n_atoms)
+      return fail(ctx, PMNB_ERR_ARG, "exclusion %d (%d,%d) out of range", k, a,
+                  b);
+    if (cls < PMNB_EXCL_NONE || cls > PMNB_EXCL_FULL)
+      return fail(ctx, PMNB_ERR_ARG, "exclusion %d has class %d", k, cls);
+    if (a == b) continue;
+    if (a > b) std::swap(a, b);
+    const int d = b - a;
+    if (molecule[a] != molecule[b]) continue;  // never consulted
+    if (d > kMaxExclDelta)
+      return fail(ctx, PMNB_ERR_UNSUPPORTED,
+                  "exclusion (%d,%d) spans %d atoms; at most %d are held", a,
+                  b, d, kMaxExclDelta);
+    const unsigned long long bit = 1ull << (d - 1);
+    full[a] &= ~bit;
+    mod[a] &= ~bit;
+    if (cls == PMNB_EXCL_FULL) full[a] |= bit;
+    else if (cls == PMNB_EXCL_MODIFIED) mod[a] |= bit;
+    maxd = std::max(maxd, d);
+  }
+  // LJ table: entries (i <= j) mirrored, as LennardJonesParameterTable::set
+  // holds one record per unordered type pair.
+  const int T = n_types;
+  std::vector<double> ljd((size_t)T * T * 4);
+  std::vector<float4> ljf((size_t)T * T);
+  for (int i = 0; i < T; i++)
+    for (int j = 0; j < T; j++) {
+      const int lo = std::min(i, j), hi = std::max(i, j);
+      const double *src = lj + 4 * ((size_t)lo * T + hi);
+      for (int k = 0; k < 4; k++) ljd[4 * ((size_t)i * T + j) + k] = src[k];
+      ljf[(size_t)i * T + j] = make_float4((float)src[0], (float)src[1],
+                                           (float)src[2], (float)src[3]);
+    }
+  const size_t N = (size_t)std::max(n_atoms, 1);
+  HIPCHK(ctx->d_type.ensure(N * sizeof(int)));
+  HIPCHK(ctx->d_q.ensure(N * sizeof(double)));
+  HIPCHK(ctx->d_mol.ensure(N * sizeof(int)));
+  HIPCHK(ctx->d_xfull.ensure(N * sizeof(unsigned long long)));
+  HIPCHK(ctx->d_xmod.ensure(N * sizeof(unsigned long long)));
+  HIPCHK(ctx->d_lj.ensure(ljd.size() * sizeof(double)));
+  HIPCHK(ctx->d_ljf.ensure(ljf.size() * sizeof(float4)));
+  if (n_atoms > 0) {
+    HIPCHK(hipMemcpy(ctx->d_type.p, type, n_atoms * sizeof(int),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_q.p, scaled_charge, n_atoms * sizeof(double),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_mol.p, molecule, n_atoms * sizeof(int),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_xfull.p, full.data(), n_atoms * 8,
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_xmod.p, mod.data(), n_atoms * 8,
+                     hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemcpy(ctx->d_lj.p, ljd.data(), ljd.size() * sizeof(double),
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->d_ljf.p, ljf.data(), ljf.size() * sizeof(float4),
+                   hipMemcpyHostToDevice));
+  ctx->n = n_atoms;
+  ctx->ntypes = T;
+  ctx->nmol = nmol;
+  ctx->csf = coulomb_scaling_factor;
+  ctx->max_excl_delta = maxd;
+  ctx->have_topo = true;
+  return PMNB_OK;
+}
+
+int pmnb_set_box(pmnb_ctx *ctx, const double e1[3], const double e2[3],
+                 const double e3[3], const double origin[3]) {
+  if (!ctx || !e1 || !e2 || !e3 || !origin) return PMNB_ERR_ARG;
+  return build_box(ctx, e1, e2, e3, origin);
+}
+
+int pmnb_evaluate_device(pmnb_ctx *ctx, const double *d_positions,
+                         double *d_forces, double *d_out,
+                         const double *d_mol_com, uint32_t flags,
+                         void *stream) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (!d_positions || !d_forces || !d_out)
+    return fail(ctx, PMNB_ERR_ARG, "null device pointer");
+  HIPCHK(hipSetDevice(ctx->prm.device));
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  return run_device(ctx, d_positions, d_forces, d_out, d_mol_com, flags, s);
+}
+
+int pmnb_evaluate(pmnb_ctx *ctx, const double *positions, double *forces,
+                  double *energies, double *virial, double *molvirial,
+                  const double *mol_com, uint32_t flags) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (ctx->n > 0 && (!positions || !forces))
+    return fail(ctx, PMNB_ERR_ARG, "null positions or forces");
+  if (!energies) return fail(ctx, PMNB_ERR_ARG, "null energies");
+  if ((flags & (PMNB_FLAG_VIRIAL | PMNB_FLAG_MOLVIRIAL)) && !virial)
+    return fail(ctx, PMNB_ERR_ARG, "null virial");
+  if ((flags & PMNB_FLAG_MOLVIRIAL) && (!molvirial || !mol_com))
+    return fail(ctx, PMNB_ERR_ARG, "null molecular virial or centres");
+  HIPCHK(hipSetDevice(ctx->prm.device));
+  const size_t N = (size_t)std::max(ctx->n, 1);
+  hipStream_t s = ctx->stream;
+  HIPCHK(ctx->d_pos.ensure(N * 3 * sizeof(double)));
+  HIPCHK(ctx->d_force.ensure(N * 3 * sizeof(double)));
+  HIPCHK(ctx->d_out.ensure(20 * sizeof(double)));
+  if (ctx->n > 0)
+    HIPCHK(hipMemcpyAsync(ctx->d_pos.p, positions, ctx->n * 3 * sizeof(double),
+                          hipMemcpyHostToDevice, s));
+  launch_fill_i32(ctx->d_force.as<int>(), (int)(N * 6), 0, s);
+  launch_fill_i32(ctx->d_out.as<int>(), 40, 0, s);
+  const double *d_com = nullptr;
+  if (flags & PMNB_FLAG_MOLVIRIAL) {
+    // centres are indexed by molecule id
+    const int nmol = std::max(ctx->nmol, 1);
+    HIPCHK(ctx->d_com.ensure((size_t)nmol * 3 * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(ctx->d_com.p, mol_com, ctx->nmol * 3 * sizeof(double),
+                          hipMemcpyHostToDevice, s));
+    d_com = ctx->d_com.as<double>();
+  }
+  rc = run_device(ctx, ctx->d_pos.as<double>(), ctx->d_force.as<double>(),
+                  ctx->d_out.as<double>(), d_com, flags, s);
+  if (rc) return rc;
+  std::vector<double> f(N * 3);
+  double out[20];
+  unsigned long long np = 0;
+  if (ctx->n > 0)
+    HIPCHK(hipMemcpyAsync(f.data(), ctx->d_force.p, ctx->n * 3 * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(out, ctx->d_out.p, sizeof out, hipMemcpyDeviceToHost,
+                        s));
+  HIPCHK(hipMemcpyAsync(&np, ctx->d_npairs.p, sizeof np, hipMemcpyDeviceToHost,
+                        s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (size_t i = 0; i < (size_t)ctx->n * 3; i++) forces[i] += f[i];
+  energies[0] += out[0];
+  energies[1] += out[1];
+  if (flags & (PMNB_FLAG_VIRIAL | PMNB_FLAG_MOLVIRIAL))
+    for (int k = 0; k < 9; k++) virial[k] += out[2 + k];
+  if (flags & PMNB_FLAG_MOLVIRIAL)
+    for (int k = 0; k < 9; k++) molvirial[k] += out[11 + k];
+  ctx->last_pairs = (long long)np;
+  return PMNB_OK;
+}
+
+int pmnb_pairset(pmnb_ctx *ctx, const double *positions, int32_t *pairs,
+                 int64_t capacity, int64_t *n_pairs) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (!n_pairs || (ctx->n > 0 && !positions) || capacity < 0)
+    return fail(ctx, PMNB_ERR_ARG, "bad pairset arguments");
+  HIPCHK(hipSetDevice(ctx->prm.device));
+  hipStream_t s = ctx->stream;
+  const size_t N = (size_t)std::max(ctx->n, 1);
+  HIPCHK(ctx->d_pos.ensure(N * 3 * sizeof(double)));
+  if (ctx->n > 0)
+    HIPCHK(hipMemcpyAsync(ctx->d_pos.p, positions, ctx->n * 3 * sizeof(double),
+                          hipMemcpyHostToDevice, s));
+  rc = ensure_eval_buffers(ctx);
+  if (rc) return rc;
+  rc = build_cells(ctx, ctx->d_pos.as<double>(), s);
+  if (rc) return rc;
+  const long long cap = pairs ? capacity : 0;
+  HIPCHK(ctx->d_pairs.ensure((size_t)std::max<long long>(cap, 1) * 3 *
+                             sizeof(int)));
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s,
+                     ctx->d_npairs.as<unsigned long long>(), 1);
+  const ForceParams P = make_params(ctx, 0);
+  const Sorted S = make_sorted(ctx, nullptr);
+  if (ctx->n > 0)
+    hipLaunchKernelGGL(k_pairset, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, s,
+                       P, S, ctx->d_pairs.as<int>(), cap,
+                       ctx->d_npairs.as<unsigned long long>());
+  HIPCHK(hipGetLastError());
+  unsigned long long np = 0;
+  HIPCHK(hipMemcpyAsync(&np, ctx->d_npairs.p, sizeof np, hipMemcpyDeviceToHost,
+                        s));
+  HIPCHK(hipStreamSynchronize(s));
+  *n_pairs = (int64_t)np;
+  if (!pairs) return PMNB_OK;
+  if ((long long)np > cap)
+    return fail(ctx, PMNB_ERR_CAPACITY, "pair set has %llu pairs, capacity %lld",
+                np, cap);
+  if (np > 0)
+    HIPCHK(hipMemcpy(pairs, ctx->d_pairs.p, (size_t)np * 3 * sizeof(int),
+                     hipMemcpyDeviceToHost));
+  // sort by (i, j)
+  std::vector<long long> key((size_t)np);
+  std::vector<int32_t> cls((size_t)np);
+  for (size_t k = 0; k < (size_t)np; k++) {
+    key[k] = (long long)pairs[3 * k] * ctx->n + pairs[3 * k + 1];
+    cls[k] = pairs[3 * k + 2];
+  }
+  std::vector<size_t> ord((size_t)np);
+  for (size_t k = 0; k < ord.size(); k++) ord[k] = k;
+  std::sort(ord.begin(), ord.end(),
+            [&](size_t x, size_t y) { return key[x] < key[y]; });
+  for (size_t k = 0; k < ord.size(); k++) {
+    pairs[3 * k] = (int32_t)(key[ord[k]] / ctx->n);
+    pairs[3 * k + 1] = (int32_t)(key[ord[k]] % ctx->n);
+    pairs[3 * k + 2] = cls[ord[k]];
+  }
+  return PMNB_OK;
+}
+
+int pmnb_kernel_timing(pmnb_ctx *ctx, int32_t enable, double *pair_ms,
+                       int64_t *launches) {
+  if (!ctx) return PMNB_ERR_ARG;
+  HIPCHK(hipSetDevice(ctx->prm.device));
+  double total = 0.0;
+  const size_t used = ctx->ev_used;
+  if (used) HIPCHK(hipEventSynchronize(ctx->ev_pool[used - 1]));
+  for (size_t k = 0; k + 1 < used; k += 2) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev_pool[k], ctx->ev_pool[k + 1]));
+    total += ms;
+  }
+  if (pair_ms) *pair_ms = total;
+  if (launches) *launches = (int64_t)(used / 2);
+  ctx->ev_used = 0;
+  ctx->timing = enable != 0;
+  return PMNB_OK;
+}
+
+int pmnb_last_pair_count(pmnb_ctx *ctx, int64_t *n_pairs) {
+  if (!ctx || !n_pairs) return PMNB_ERR_ARG;
+  if (ctx->d_npairs.p) {
+    // the counter of the last launch, on whichever stream it ran
+    HIPCHK(hipSetDevice(ctx->prm.device));
+    HIPCHK(hipDeviceSynchronize());
+    unsigned long long np = 0;
+    HIPCHK(hipMemcpy(&np, ctx->d_npairs.p, sizeof np, hipMemcpyDeviceToHost));
+    ctx->last_pairs = (long long)np;
+  }
+  *n_pairs = ctx->last_pairs;
+  return PMNB_OK;
+}
+
+const char *pmnb_last_error(const pmnb_ctx *ctx) {
+  return ctx ? ctx->err.c_str() : "null context";
+}
+
+void pmnb_destroy(pmnb_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->prm.device);
+  (void)hipStreamSynchronize(ctx->stream);
+  DevBuf *bufs[] = {&ctx->d_type, &ctx->d_q, &ctx->d_mol, &ctx->d_xfull,
+                    &ctx->d_xmod, &ctx->d_lj, &ctx->d_ljf, &ctx->d_pos,
+                    &ctx->d_force, &ctx->d_out, &ctx->d_com, &ctx->d_cell_of,
+                    &ctx->d_count, &ctx->d_start, &ctx->d_fill, &ctx->d_sorted,
+                    &ctx->d_sx, &ctx->d_sy, &ctx->d_sz, &ctx->d_loc,
+                    &ctx->d_stype, &ctx->d_sq, &ctx->d_smol, &ctx->d_scell,
+                    &ctx->d_shell, &ctx->d_partial, &ctx->d_npairs,
+                    &ctx->d_pairs};
+  for (DevBuf *b : bufs) b->release();
+  for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+}  // extern "C"

@@ -1,0 +1,161 @@
+// GPU cell-list build: integer cell hashing, counting sort, per-cell
+// ordering and the gather into sorted structure-of-arrays.
+//
+// Replaces Topology::updateCellLists (topology/Topology.h:44-86) with
+// CubicCellManager::findCell (topology/CubicCellManager.cpp:40-44).  The
+// reference builds singly linked lists in ascending atom order; here each
+// cell is a contiguous segment of the sorted arrays, also in ascending atom
+// order, so traversal order inside a cell is the reference's.
+#pragma once
+
+#include "pmnb_common.h"
+
+namespace pmnb {
+
+// Fills n ints with v (used instead of hipMemsetAsync so every clear is an
+// ordinary kernel node if a caller captures the stream into a graph).
+__global__ void k_fill_i32(int *p, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long *p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+// Cell of atom i: findCell(origin - min + minimalPosition(x_i)).
+__device__ __forceinline__ void cell_coords(const Box &b, const Grid &g,
+                                            double x, double y, double z,
+                                            int &cx, int &cy, int &cz,
+                                            double &vx, double &vy,
+                                            double &vz) {
+  double px, py, pz;
+  min_pos_exact(b, x, y, z, px, py, pz);
+  vx = b.shift[0] + px;
+  vy = b.shift[1] + py;
+  vz = b.shift[2] + pz;
+  double fx = floor(vx * g.inv[0]);
+  double fy = floor(vy * g.inv[1]);
+  double fz = floor(vz * g.inv[2]);
+  // Clamp before the integer conversion so a NaN or huge value cannot
+  // produce an undefined conversion.
+  fx = (fx == fx) ? fmin(fmax(fx, -2.0), (double)g.n[0] + 1.0) : 0.0;
+  fy = (fy == fy) ? fmin(fmax(fy, -2.0), (double)g.n[1] + 1.0) : 0.0;
+  fz = (fz == fz) ? fmin(fmax(fz, -2.0), (double)g.n[2] + 1.0) : 0.0;
+  int rx = (int)fx, ry = (int)fy, rz = (int)fz;
+  cx = wrap_cell(rx, g.n[0]);
+  cy = wrap_cell(ry, g.n[1]);
+  cz = wrap_cell(rz, g.n[2]);
+  // Keep the wrapped coordinate consistent with the wrapped cell.
+  if (rx >= g.n[0]) vx -= b.H[0];
+  else if (rx < 0) vx += b.H[0];
+  if (ry >= g.n[1]) vy -= b.H[1];
+  else if (ry < 0) vy += b.H[1];
+  if (rz >= g.n[2]) vz -= b.H[2];
+  else if (rz < 0) vz += b.H[2];
+}
+
+__global__ void k_assign(Box b, Grid g, const double *__restrict__ pos,
+                         int n, int *__restrict__ cell_of,
+                         int *__restrict__ count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int cx, cy, cz;
+  double vx, vy, vz;
+  cell_coords(b, g, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], cx, cy, cz,
+              vx, vy, vz);
+  int c = (cx * g.n[1] + cy) * g.n[2] + cz;
+  cell_of[i] = c;
+  atomicAdd(&count[c], 1);
+}
+
+// Exclusive scan of count[0..n) into start[0..n) by one block of 1024.
+__global__ void __launch_bounds__(1024)
+k_scan(const int *__restrict__ count, int *__restrict__ start, int n) {
+  __shared__ int part[1024];
+  int t = threadIdx.x;
+  int chunk = (n + 1023) / 1024;
+  int b0 = t * chunk, b1 = min(n, b0 + chunk);
+  int s = 0;
+  for (int i = b0; i < b1; i++) s += count[i];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = (t >= off) ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - s;  // exclusive prefix of this chunk
+  for (int i = b0; i < b1; i++) {
+    start[i] = run;
+    run += count[i];
+  }
+}
+
+__global__ void k_scatter(const int *__restrict__ cell_of, int n,
+                          const int *__restrict__ start,
+                          int *__restrict__ fill, int *__restrict__ sorted) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c = cell_of[i];
+  int slot = atomicAdd(&fill[c], 1);
+  sorted[start[c] + slot] = i;
+}
+
+// Ascending atom order inside each cell (the reference's list order).
+__global__ void k_sort_cells(const int *__restrict__ start,
+                             const int *__restrict__ count, int ncell,
+                             int *__restrict__ sorted) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  int s0 = start[c], m = count[c];
+  int *seg = sorted + s0;
+  for (int i = 1; i < m; i++) {
+    int v = seg[i];
+    int j = i - 1;
+    while (j >= 0 && seg[j] > v) {
+      seg[j + 1] = seg[j];
+      j--;
+    }
+    seg[j + 1] = v;
+  }
+}
+
+// Gathers per-atom data into sorted order.  loc holds, for the mixed path,
+// the FP32 offset of the atom from its cell's lower corner (computed in
+// FP64) and its scaled charge.
+__global__ void k_gather(Box b, Grid g, const double *__restrict__ pos,
+                         const int *__restrict__ sorted,
+                         const int *__restrict__ cell_of,
+                         const int *__restrict__ type,
+                         const double *__restrict__ q,
+                         const int *__restrict__ mol, int n,
+                         double *__restrict__ sx, double *__restrict__ sy,
+                         double *__restrict__ sz, float4 *__restrict__ sloc,
+                         int *__restrict__ stype, double *__restrict__ sq,
+                         int *__restrict__ smol, int *__restrict__ scell) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int a = sorted[s];
+  double x = pos[3 * a], y = pos[3 * a + 1], z = pos[3 * a + 2];
+  sx[s] = x;
+  sy[s] = y;
+  sz[s] = z;
+  int cx, cy, cz;
+  double vx, vy, vz;
+  cell_coords(b, g, x, y, z, cx, cy, cz, vx, vy, vz);
+  float4 l;
+  l.x = (float)(vx - (double)cx * g.cs[0]);
+  l.y = (float)(vy - (double)cy * g.cs[1]);
+  l.z = (float)(vz - (double)cz * g.cs[2]);
+  l.w = (float)q[a];
+  sloc[s] = l;
+  stype[s] = type[a];
+  sq[s] = q[a];
+  smol[s] = mol[a];
+  scell[s] = cell_of[a];
+}
+
+}  // namespace pmnb

@@ -1,0 +1,186 @@
+"""Writes the golden fixtures under tests/golden/ (run in a checkout that has
+the reference tree at /root/reference and its path built: `make -C oracle
+ref`).
+
+1. water423.npz  -- inputs and expected outputs of the reference's
+   regression case test/tests/water_CHARMM_PERIODIC_L.conf: 141 TIP3 waters
+   (data/water_423.{psf,pos.pdb}), CHARMM parameters for OT/HT
+   (data/par_all27_prot_lipid.inp), the step-0 row of
+   expected/water_CHARMM_PERIODIC_L.energy and the first frame of
+   expected/water_CHARMM_PERIODIC_L.forces.
+2. pentane320.npz -- inputs and the step-0 VdW energy of the feature test
+   protomol-test/ConfigTests/data/EnergiesTest.conf (VelacTest.{psf,coor,par},
+   EnergiesTest.energies.orig).
+3. ref_<name>.npz -- outputs of the reference's own NonbondedCutoffSystemForce
+   (oracle/_ref/ref_driver, compiled from the reference sources) on seeded
+   systems: forces, energies, virials and the SHA-256 of the sorted pair set,
+   with the input positions.
+
+Only data (inputs and expected outputs) is written; no reference source.
+"""
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+REF = "/root/reference"
+WATER_DIR = os.path.join(REF, "test/tests")
+PENT_DIR = os.path.join(REF, "protomol-test/ConfigTests/data")
+
+
+def read_psf(path):
+    lines = open(path).read().splitlines()
+    out = {}
+
+    def section(tag):
+        for k, l in enumerate(lines):
+            if tag in l:
+                return int(l.split()[0]), k + 1
+        return 0, None
+
+    n, k = section("!NATOM")
+    names, types, q, m = [], [], [], []
+    for l in lines[k:k + n]:
+        f = l.split()
+        names.append(f[4])
+        types.append(f[5])
+        q.append(float(f[6]))
+        m.append(float(f[7]))
+    out["atom_type"] = np.array(types)
+    out["charge"] = np.array(q)
+    out["mass"] = np.array(m)
+    for tag, width, key in (("!NBOND", 2, "bonds"), ("!NTHETA", 3, "angles"),
+                            ("!NPHI", 4, "dihedrals")):
+        cnt, k = section(tag)
+        vals = []
+        while len(vals) < cnt * width:
+            vals += [int(v) for v in lines[k].split()]
+            k += 1
+        out[key] = np.array(vals, dtype=np.int64).reshape(-1, width) - 1
+    return out
+
+
+def read_pdb(path):
+    xyz = []
+    for l in open(path):
+        if l.startswith(("ATOM", "HETATM")):
+            xyz.append([float(l[30:38]), float(l[38:46]), float(l[46:54])])
+    return np.array(xyz)
+
+
+def read_par_nonbonded(path, wanted):
+    """CHARMM NONBONDED rows: name, ignored, epsilon, Rmin/2[, ignored,
+    eps14, Rmin14/2] (io/PARReader.cpp:416-450)."""
+    out = {}
+    in_nb = False
+    for l in open(path):
+        s = l.split("!")[0].strip()
+        if not s:
+            continue
+        w = s.split()
+        if w[0].upper().startswith("NONBONDED"):
+            in_nb = True
+            continue
+        if w[0].upper() in ("HBOND", "NBFIX", "END", "BONDS", "ANGLES",
+                            "DIHEDRALS", "IMPROPER", "CMAP"):
+            in_nb = False
+        if in_nb and w[0] in wanted and w[0] not in out:
+            try:
+                vals = [float(v) for v in w[1:]]
+            except ValueError:
+                continue
+            eps, rmin = vals[1], vals[2]
+            eps14, rmin14 = (vals[4], vals[5]) if len(vals) >= 6 else (eps, rmin)
+            out[w[0]] = (eps, rmin, eps14, rmin14)
+    return out
+
+
+def water423():
+    psf = read_psf(os.path.join(WATER_DIR, "data/water_423.psf"))
+    pos = read_pdb(os.path.join(WATER_DIR, "data/water_423.pos.pdb"))
+    par = read_par_nonbonded(os.path.join(WATER_DIR, "data/par_all27_prot_lipid.inp"),
+                             {"OT", "HT"})
+    names = ["OT", "HT"]
+    types = np.array([names.index(t) for t in psf["atom_type"]], dtype=np.int32)
+    lj = np.array([par[n] for n in names])  # eps, rmin/2, eps14, rmin14/2
+    energy = np.array([float(v) for v in open(os.path.join(
+        WATER_DIR, "expected/water_CHARMM_PERIODIC_L.energy")).readline().split()])
+    fl = open(os.path.join(WATER_DIR, "expected/water_CHARMM_PERIODIC_L.forces")
+              ).read().splitlines()
+    n = int(fl[1].split()[0])
+    forces = np.array([[float(v) for v in l.split()[1:4]] for l in fl[2:2 + n]])
+    np.savez_compressed(
+        os.path.join(HERE, "water423.npz"), positions=pos, types=types,
+        charges=psf["charge"], masses=psf["mass"], bonds=psf["bonds"],
+        angles=psf["angles"], lj_eps_rmin=lj,
+        # OT-HT bond 450.0 / 0.9572 and HT-OT-HT angle 55.0 / 104.52 deg
+        # (par_all27_prot_lipid.inp:359,817)
+        bond_k_r0=np.array([450.0, 0.9572]),
+        angle_k_theta0=np.array([55.0, 104.52]),
+        box=np.array([28.0, 28.0, 28.0]), origin=np.zeros(3), cell_size=6.5,
+        lj_switch=np.array([0.1, 6.5]), coulomb_c1_cutoff=6.5,
+        algorithm_cutoff=6.5, energy_row0=energy, forces_frame0=forces)
+    print("water423.npz", pos.shape, energy[9:11])
+
+
+def pentane320():
+    psf = read_psf(os.path.join(PENT_DIR, "VelacTest.psf"))
+    raw = open(os.path.join(PENT_DIR, "VelacTest.coor"), "rb").read()
+    n = int(np.frombuffer(raw[:4], np.int32)[0])
+    pos = np.frombuffer(raw[4:4 + 24 * n], np.float64).reshape(n, 3).copy()
+    par = read_par_nonbonded(os.path.join(PENT_DIR, "VelacTest.par"),
+                             {"C2", "C3"})
+    names = ["C2", "C3"]
+    types = np.array([names.index(t) for t in psf["atom_type"]], dtype=np.int32)
+    lj = np.array([par[k] for k in names])
+    row = open(os.path.join(PENT_DIR, "EnergiesTest.energies.orig")).readline()
+    energy = np.array([float(v) for v in row.split()])
+    np.savez_compressed(
+        os.path.join(HERE, "pentane320.npz"), positions=pos, types=types,
+        charges=psf["charge"], bonds=psf["bonds"], angles=psf["angles"],
+        dihedrals=psf["dihedrals"], lj_eps_rmin=lj,
+        box=np.array([23.0, 23.0, 23.0]), origin=np.array([6.8, 7.3, 8.2]),
+        cell_size=5.0, lj_switch=np.array([8.0, 9.0]), algorithm_cutoff=9.0,
+        energy_row0=energy)
+    print("pentane320.npz", pos.shape, energy[9])
+
+
+def pair_hash(pairs):
+    p = np.asarray(pairs, dtype=np.int64)
+    key = np.sort(p[:, 0] * (1 << 32) + p[:, 1])
+    cls = p[np.argsort(p[:, 0] * (1 << 32) + p[:, 1], kind="stable"), 2]
+    h = hashlib.sha256()
+    h.update(key.astype("<i8").tobytes())
+    h.update(cls.astype("<i4").tobytes())
+    return h.hexdigest()
+
+
+def ref_vectors():
+    from oracle import oracle as O
+    from protomoladdon_amd import systems
+    cases = {
+        "config0": systems.config0(),
+        "chains2k": systems.small_chains(n_chains=100),
+        "replica167": systems.config3_replica(0, 167),
+    }
+    for name, w in cases.items():
+        s = O.System.from_workload(w, flags=3)
+        r = O.run_ref(s)
+        np.savez_compressed(
+            os.path.join(HERE, "ref_%s.npz" % name), positions=w.positions,
+            forces=r.forces, energies=r.energies, virial=r.virial,
+            molvirial=r.molvirial, n_pairs=r.n_pairs,
+            pair_sha256=pair_hash(r.pairs),
+            pair_class_counts=np.bincount(r.pairs[:, 2], minlength=3))
+        print("ref_%s.npz" % name, w.n_atoms, r.n_pairs)
+
+
+if __name__ == "__main__":
+    water423()
+    pentane320()
+    ref_vectors()

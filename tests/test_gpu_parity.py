@@ -1,0 +1,219 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 oracle and the
+reference's golden outputs.
+
+Bars: pair sets bit-exact; FP64 forces within 1e-12 of the largest oracle
+force component (different summation order only) and energies/virials
+within 1e-11 relative; mixed precision RMS relative force error <= 1e-5
+(the stated accuracy target), energies within 1e-6 relative."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from protomoladdon_amd import systems
+from tests import fixtures_lib as FX
+
+pytestmark = pytest.mark.gpu
+
+
+def rms_rel(f, ref):
+    return float(np.sqrt(np.mean(np.sum((f - ref) ** 2, axis=1)) /
+                         np.mean(np.sum(ref ** 2, axis=1))))
+
+
+def run_gpu(w, precision, flags=0, positions=None, with_pairs=True):
+    from protomoladdon_amd.topology import ScalarStructure
+    force = w.make_force(precision)
+    pos = w.positions if positions is None else positions
+    f = np.zeros((w.n_atoms, 3))
+    e = ScalarStructure(virial=bool(flags & 1), molecular_virial=bool(flags & 2))
+    force.evaluate(w.topology, pos, f, e)
+    npairs = force.last_pair_count()
+    pairs = force.pair_set(w.topology, pos) if with_pairs else None
+    force.close()
+    return f, e, npairs, pairs
+
+
+def check_fp64(w, flags=3, positions=None):
+    s = O.System.from_workload(w, positions=positions, flags=flags)
+    ref = O.evaluate(s, record_pairs=True)
+    f, e, npairs, pairs = run_gpu(w, "fp64", flags, positions)
+    print(w.name, "n", w.n_atoms, "pairs", ref.n_pairs, flush=True)
+    np.testing.assert_array_equal(pairs, O.sorted_pairs(ref.pairs))
+    assert npairs == ref.n_pairs
+    scale = max(1.0, np.abs(ref.forces).max())
+    assert np.abs(f - ref.forces).max() <= 1e-12 * scale * 10
+    np.testing.assert_allclose([e[1], e[0]], ref.energies, rtol=1e-11,
+                               atol=1e-9)
+    if flags & 3:
+        vir = e.table[7:16]
+        np.testing.assert_allclose(vir, ref.virial, rtol=1e-11,
+                                   atol=1e-9 * np.abs(ref.virial).max())
+    if flags & 2:
+        mv = e.table[16:25]
+        np.testing.assert_allclose(mv, ref.molvirial, rtol=1e-11,
+                                   atol=1e-9 * np.abs(ref.molvirial).max())
+    return ref
+
+
+def check_mixed(w, flags=1):
+    s = O.System.from_workload(w, flags=flags)
+    ref = O.evaluate(s)
+    f, e, npairs, pairs = run_gpu(w, "mixed", flags)
+    err = rms_rel(f, ref.forces)
+    print(w.name, "mixed rms rel force err", err, flush=True)
+    assert err <= 1e-5
+    assert npairs == ref.n_pairs  # pair set decided in FP64 near rc
+    np.testing.assert_allclose([e[1], e[0]], ref.energies, rtol=1e-6,
+                               atol=1e-6)
+    if flags & 1:
+        np.testing.assert_allclose(e.table[7:16], ref.virial, rtol=2e-6,
+                                   atol=1e-6 * np.abs(ref.virial).max())
+    return err
+
+
+# ------------------------------------------------------------------ FP64
+@pytest.mark.parametrize("make", [
+    systems.config0,
+    lambda: systems.small_water(n_mol=400),
+    lambda: systems.small_chains(n_chains=120),
+    lambda: systems.config3_replica(0, 167),
+    lambda: systems.config3_replica(7, 2667),
+], ids=["config0", "water_ewald", "chains", "replica_small", "replica_large"])
+def test_fp64_matches_oracle(make):
+    check_fp64(make())
+
+
+def test_fp64_golden_reference_vectors():
+    for name, make in (("config0", systems.config0),
+                       ("chains2k", lambda: systems.small_chains(n_chains=100)),
+                       ("replica167", lambda: systems.config3_replica(0, 167))):
+        g = FX.load("ref_%s.npz" % name)
+        w = make()
+        f, e, npairs, pairs = run_gpu(w, "fp64", 3)
+        assert npairs == int(g["n_pairs"])
+        from tests.golden.make_fixtures import pair_hash
+        assert pair_hash(pairs) == str(g["pair_sha256"])
+        scale = max(1.0, np.abs(g["forces"]).max())
+        assert np.abs(f - g["forces"]).max() <= 1e-11 * scale
+        np.testing.assert_allclose([e[1], e[0]], g["energies"], rtol=1e-11)
+
+
+def test_fp64_water423_reference_output():
+    # the reference's own regression case, Coulomb with C1 switching
+    w, d = FX.water423()
+    f, e, _, _ = run_gpu(w, "fp64", 0)
+    assert abs(e[0] - d["energy_row0"][9]) < 6e-5
+    assert abs(e[1] - d["energy_row0"][10]) < 6e-5
+    fb, _, _ = FX.harmonic_bonded_forces(
+        w.positions, d["bonds"], d["angles"], d["bond_k_r0"][0],
+        d["bond_k_r0"][1], d["angle_k_theta0"][0], d["angle_k_theta0"][1],
+        d["box"])
+    assert np.abs(f + fb - d["forces_frame0"]).max() < 1e-6
+
+
+def test_fp64_pentane_reference_output():
+    w, d = FX.pentane320()
+    f, e, _, _ = run_gpu(w, "fp64", 0)
+    assert abs(e[0] - d["energy_row0"][9]) < 6e-6
+
+
+def test_fp64_edge_positions():
+    # atoms on box faces, far images and a non-zero origin
+    w = systems.small_water(n_mol=300)
+    L = w.topology.boundary_conditions.lengths()[0]
+    p = w.positions.copy()
+    p[0] = (0.0, L, L / 2)
+    p[3] = (L, 0.0, np.nextafter(L, 0))
+    p[6] += (-L, 2 * L, 3 * L)
+    p[9] += (5 * L, 0.0, -7 * L)
+    check_fp64(w, positions=p)
+    bc = w.topology.boundary_conditions
+    bc.set(bc.e1, bc.e2, bc.e3, (6.8, 7.3, 8.2))
+    w.topology.touch()
+    check_fp64(w)
+
+
+def test_fp64_empty_and_tiny_systems():
+    from protomoladdon_amd.topology import ScalarStructure
+    w = systems.small_water(n_mol=1)
+    check_fp64(w)
+    # zero atoms
+    t = w.topology
+    from protomoladdon_amd.topology import ExclusionTable, Topology
+    t0 = Topology(np.zeros(0, np.int32), np.zeros(0), np.zeros(0, np.int32),
+                  t.lennard_jones_parameters, ExclusionTable(0),
+                  t.boundary_conditions, t.cell_manager)
+    force = w.make_force("fp64")
+    f = np.zeros((0, 3))
+    e = ScalarStructure()
+    force.evaluate(t0, np.zeros((0, 3)), f, e)
+    assert e[0] == 0.0 and e[1] == 0.0
+    assert len(force.pair_set(t0, np.zeros((0, 3)))) == 0
+
+
+def test_fp64_is_deterministic_and_accumulates():
+    from protomoladdon_amd.topology import ScalarStructure
+    w = systems.small_water(n_mol=500)
+    force = w.make_force("fp64")
+    f1 = np.zeros((w.n_atoms, 3))
+    e1 = ScalarStructure(virial=True)
+    force.evaluate(w.topology, w.positions, f1, e1)
+    f2 = np.ones((w.n_atoms, 3))
+    e2 = ScalarStructure(virial=True)
+    e2[0] = 5.0
+    force.evaluate(w.topology, w.positions, f2, e2)
+    np.testing.assert_array_equal(f2 - 1.0, f1)
+    assert e2[0] - 5.0 == e1[0]
+    f3 = np.zeros((w.n_atoms, 3))
+    e3 = ScalarStructure(virial=True)
+    force.evaluate(w.topology, w.positions, f3, e3)
+    np.testing.assert_array_equal(f3, f1)
+    np.testing.assert_array_equal(e3.table, e1.table)
+
+
+# ----------------------------------------------------------------- mixed
+@pytest.mark.parametrize("make", [
+    systems.config0,
+    lambda: systems.small_water(n_mol=400),
+    lambda: systems.small_chains(n_chains=120),
+    lambda: systems.config3_replica(0, 167),
+], ids=["config0", "water_ewald", "chains", "replica_small"])
+def test_mixed_matches_oracle(make):
+    check_mixed(make())
+
+
+def test_mixed_pair_set_and_determinism():
+    from protomoladdon_amd.topology import ScalarStructure
+    w = systems.small_water(n_mol=600)
+    force = w.make_force("mixed")
+    out = []
+    for _ in range(2):
+        f = np.zeros((w.n_atoms, 3))
+        e = ScalarStructure(virial=True)
+        force.evaluate(w.topology, w.positions, f, e)
+        out.append((f, e.table.copy(), force.last_pair_count()))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    ref = O.evaluate(O.System.from_workload(w))
+    assert out[0][2] == ref.n_pairs
+
+
+@pytest.mark.parametrize("which", [1, 2])
+def test_large_config_properties(which):
+    """Full-size configs: FP64 GPU vs the oracle (bit-exact pair count,
+    forces) and mixed RMS error; sizes the oracle runs in seconds."""
+    w = systems.config1() if which == 1 else systems.config2()
+    s = O.System.from_workload(w)
+    ref = O.evaluate(s)
+    f, e, npairs, _ = run_gpu(w, "fp64", 0, with_pairs=False)
+    print(w.name, "pairs", ref.n_pairs, flush=True)
+    assert npairs == ref.n_pairs
+    scale = max(1.0, np.abs(ref.forces).max())
+    assert np.abs(f - ref.forces).max() <= 1e-11 * scale
+    fm, em, npm, _ = run_gpu(w, "mixed", 0, with_pairs=False)
+    assert npm == ref.n_pairs
+    err = rms_rel(fm, ref.forces)
+    print(w.name, "mixed rms rel err", err, flush=True)
+    assert err <= 1e-5
+    # Newton's third law holds to the fixed-point resolution
+    assert np.abs(fm.sum(axis=0)).max() < 1e-6 * np.abs(fm).sum() + 1e-3
