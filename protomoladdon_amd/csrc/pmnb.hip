@@ -232,14 +232,10 @@ int build_box(pmnb_ctx *ctx, const double *e1, const double *e2,
   ctx->shell.push_back(make_int4(0, 0, 0, 0));
   ctx->shell.insert(ctx->shell.end(), dl.begin(), dl.end());
   // The mixed kernel takes the image of a neighbour atom from its cell's
-  // offset; that is the minimum image only when every offset in
-  // [-reach, reach] is its own periodic class.
-  int reach[3] = {0, 0, 0};
-  for (const int4 &v : ctx->shell) {
-    reach[0] = std::max(reach[0], std::abs(v.x));
-    reach[1] = std::max(reach[1], std::abs(v.y));
-    reach[2] = std::max(reach[2], std::abs(v.z));
-  }
+  // offset; that is the minimum image only when every cell offset a pair
+  // within the cutoff can have, [-reach, reach] with reach taken before the
+  // delta list folds offsets into one periodic class, is its own class.
+  const int reach[3] = {nx, ny, nz};
   ctx->per_pair_wrap = 0;
   for (int k = 0; k < 3; k++)
     if (g.n[k] < 2 * reach[k] + 1) ctx->per_pair_wrap = 1;

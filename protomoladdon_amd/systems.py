@@ -13,9 +13,13 @@ fixed here:
   and orientations are redrawn until no H is within 1.5 A of an atom of
   another molecule (30 rounds at most);
 * chains start on a seeded subset of a cubic lattice and grow with a fixed
-  111 degree bend; each new atom takes, of 48 random dihedrals, the one
+  111 degree bend; each new atom takes, of 24 random dihedrals, the one
   farthest from the atoms already placed (chains grow in four interleaved
-  groups per step so same-step atoms see each other);
+  groups per step so same-step atoms see each other); then, for up to 20
+  rounds, each chain with an atom within 1.2 A of a non-neighbour has its
+  tail from that atom rotated rigidly about the preceding bond to the best
+  of its current and 64 random angles (bond lengths and bend angles stay
+  exact);
 * configs 2 and 3 switch with C2 from rc - 1.5 A;
 * H atoms of water have epsilon 0, so Lorentz-Berthelot gives A = B = 0 for
   every pair involving H and LJ acts on O-O only.
@@ -198,7 +202,7 @@ CHAIN_EPS = (0.05, 0.1, 0.15, 0.2)
 
 
 def chain_positions(n_chains, chain_len, L, seed, bond=1.53, bend=111.0,
-                    candidates=48, batches=4):
+                    candidates=24, batches=4):
     """Random-walk chains with fixed bond length and bend angle.  Each new
     atom takes, of `candidates` random dihedrals, the one farthest from every
     atom already placed (other chains, and its own chain two or more bonds
@@ -251,7 +255,68 @@ def chain_positions(n_chains, chain_len, L, seed, bond=1.53, bend=111.0,
             best = np.argmax(score, axis=1)
             X[idx, j] = cand[np.arange(len(idx)), best]
             done[idx] = True
+    _repair_chains(X, L, bond, theta, rng)
     return np.ascontiguousarray(np.mod(X.reshape(-1, 3), L))
+
+
+def _chain_clashes(X, L, min_dist):
+    """Atoms (flat index) closer than min_dist to an atom that is not its
+    1-2 or 1-3 neighbour in the same chain."""
+    from scipy.spatial import cKDTree
+    n_chains, chain_len = X.shape[:2]
+    P = np.mod(X.reshape(-1, 3), L)
+    pairs = cKDTree(P, boxsize=L).query_pairs(min_dist, output_type="ndarray")
+    if not len(pairs):
+        return pairs
+    a, b = pairs[:, 0], pairs[:, 1]
+    near = (a // chain_len == b // chain_len) & (np.abs(a - b) <= 2)
+    return pairs[~near]
+
+
+def _repair_chains(X, L, bond, theta, rng, min_dist=1.2, candidates=64,
+                   rounds=20, batches=8):
+    """Pivot moves: for each chain that still has an atom within min_dist of
+    a non-neighbour, the tail from the earliest such atom is rotated rigidly
+    about the preceding bond (bond lengths and bend angles are unchanged);
+    of the current angle and `candidates` random ones, the one keeping the
+    tail farthest from every other atom is taken.  Chains are moved in
+    `batches` groups per round, each scored against all current positions."""
+    from scipy.spatial import cKDTree
+    n_chains, chain_len = X.shape[:2]
+    for _ in range(rounds):
+        pairs = _chain_clashes(X, L, min_dist)
+        if not len(pairs):
+            return
+        pos = pairs % chain_len
+        first = np.full(n_chains, chain_len)
+        for k in (0, 1):
+            np.minimum.at(first, pairs[:, k] // chain_len,
+                          np.maximum(pos[:, k], 2))
+        fix = rng.permutation(np.nonzero(first < chain_len)[0])
+        for group in np.array_split(fix, batches):
+            tree = cKDTree(np.mod(X.reshape(-1, 3), L), boxsize=L)
+            for c in group:
+                p = first[c]
+                axis = X[c, p - 1] - X[c, p - 2]
+                axis /= np.linalg.norm(axis)
+                tail = X[c, p:] - X[c, p - 1]
+                psi = np.concatenate([[0.0], rng.uniform(0, 2 * np.pi,
+                                                         candidates)])
+                cs, sn = np.cos(psi)[:, None, None], np.sin(psi)[:, None, None]
+                kx = np.cross(axis[None, :], tail)[None]
+                kd = (tail @ axis)[None, :, None] * axis[None, None, :]
+                rot = tail[None] * cs + kx * sn + kd * (1 - cs)  # Rodrigues
+                cand = X[c, p - 1][None, None, :] + rot
+                dist, nb = tree.query(np.mod(cand.reshape(-1, 3), L), k=8)
+                nb = np.minimum(nb, n_chains * chain_len - 1)
+                tail_pos = np.tile(np.arange(p, chain_len),
+                                   len(psi))[:, None]
+                own = nb // chain_len == c
+                skip = own & ((nb % chain_len >= p)
+                              | (nb % chain_len >= tail_pos - 2))
+                dist = np.where(skip, np.inf, dist).min(axis=1)
+                score = dist.reshape(len(psi), -1).min(axis=1)
+                X[c, p:] = cand[int(np.argmax(score))]
 
 
 def chain_topology(n_chains, chain_len, L, seed, cell_size, sigma=CHAIN_SIGMA,
