@@ -52,7 +52,7 @@ def time_gpu(w, precision, steps, warmup, flags, torch):
         force.evaluate_device(w.topology, pos, frc, out, flags, stream=sp)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    kms, launches = force.kernel_timing(False)
+    bms, kms, launches = force.kernel_timing(False)
     # one more evaluation for the forces and the pair count of one step
     frc.zero_()
     out.zero_()
@@ -62,6 +62,7 @@ def time_gpu(w, precision, steps, warmup, flags, torch):
     f = frc.cpu().numpy()
     force.close()
     return {"ms_per_step": 1e3 * dt / steps, "kernel_ms": kms / max(launches, 1),
+            "build_ms": bms / max(launches, 1),
             "launches": launches, "pairs": npairs, "forces": f,
             "energies": out[:2].cpu().numpy()}
 
@@ -158,6 +159,7 @@ def main():
                      "ms_per_step": f64["ms_per_step"],
                      "pair_evals_per_s": f64["pairs"] * sps64,
                      "pair_kernel_ms": f64["kernel_ms"],
+                     "list_build_ms": f64["build_ms"],
                      "pair_kernel_tflops": achieved64,
                      "pairs_per_step": f64["pairs"]},
         "speedup_vs_fp64": sps / sps64,
@@ -165,8 +167,11 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "k_pair_mixed<2>",
+                     "kernel": "k_force_list_mixed<2>",
                      "kernel_ms": mx["kernel_ms"],
+                     "list_build_ms": mx["build_ms"],
+                     "achieved_incl_list_build": P * FLOP_PER_PAIR / (
+                         (mx["kernel_ms"] + mx["build_ms"]) * 1e-3) / 1e12,
                      "flop_per_pair": FLOP_PER_PAIR},
         "cpu_baseline": cpu,
     }

@@ -144,13 +144,14 @@ int pmnb_evaluate_device(pmnb_ctx *ctx, const double *d_positions,
 int pmnb_pairset(pmnb_ctx *ctx, const double *positions, int32_t *pairs,
                  int64_t capacity, int64_t *n_pairs);
 
-/* HIP-event timing of the pair kernel on the stream it is launched on.
+/* HIP-event timing of the pair stage on the stream it is launched on.
    enable = 1 starts (and clears) timing; enable = 0 stops it.  Either call
-   first waits for the recorded launches and returns, in *pair_ms and
-   *launches, the summed duration and count of pair-kernel launches since
-   timing was last enabled (either pointer may be NULL).  No counterpart in
-   the reference; it feeds the benchmark's per-kernel roofline figure. */
-int pmnb_kernel_timing(pmnb_ctx *ctx, int32_t enable, double *pair_ms,
+   first waits for the recorded launches and returns the summed durations
+   since timing was last enabled: ms[0] the neighbour-list build, ms[1] the
+   pair-force kernels, and *launches the number of evaluations (either
+   pointer may be NULL).  No counterpart in the reference; it feeds the
+   benchmark's per-kernel roofline figure. */
+int pmnb_kernel_timing(pmnb_ctx *ctx, int32_t enable, double *ms,
                        int64_t *launches);
 
 /* Number of in-cutoff pairs counted by the last evaluate or pairset call

@@ -90,8 +90,7 @@ def load():
     lib.pmnb_evaluate_device.restype = ctypes.c_int
     lib.pmnb_pairset.argtypes = [_p, _p, _p, _i64, ctypes.POINTER(_i64)]
     lib.pmnb_pairset.restype = ctypes.c_int
-    lib.pmnb_kernel_timing.argtypes = [_p, _i32, ctypes.POINTER(_dbl),
-                                       ctypes.POINTER(_i64)]
+    lib.pmnb_kernel_timing.argtypes = [_p, _i32, _p, ctypes.POINTER(_i64)]
     lib.pmnb_kernel_timing.restype = ctypes.c_int
     lib.pmnb_last_pair_count.argtypes = [_p, ctypes.POINTER(_i64)]
     lib.pmnb_last_pair_count.restype = ctypes.c_int

@@ -19,6 +19,7 @@
 #include "pmnb_common.h"
 #include "pmnb_cells.h"
 #include "pmnb_pair.h"
+#include "pmnb_list.h"
 
 using namespace pmnb;
 
@@ -73,11 +74,16 @@ struct pmnb_ctx {
   // device: per evaluation
   DevBuf d_pos, d_force, d_out, d_com, d_cell_of, d_count, d_start, d_fill,
       d_sorted, d_sx, d_sy, d_sz, d_loc, d_stype, d_sq, d_smol, d_scell,
-      d_shell, d_partial, d_npairs, d_pairs;
+      d_shell, d_partial, d_npairs, d_pairs, d_list, d_ncount, d_partial2,
+      d_shell_off;
   int shell_uploaded = 0;
+  std::vector<float4> shell_off;  // FP32 cell offsets of the shell entries
+  bool use_list = false;          // neighbour-list path available
+  double box_key[12] = {0};       // e1, e2, e3, origin of the current box
+  int cap = 0;                    // list slots per atom
   // pair-kernel timing
   bool timing = false;
-  std::vector<hipEvent_t> ev_pool;
+  std::vector<hipEvent_t> ev_pool;  // 3 per launch: start, built, done
   size_t ev_used = 0;
 };
 
@@ -239,11 +245,31 @@ int build_box(pmnb_ctx *ctx, const double *e1, const double *e2,
   ctx->per_pair_wrap = 0;
   for (int k = 0; k < 3; k++)
     if (g.n[k] < 2 * reach[k] + 1) ctx->per_pair_wrap = 1;
+  ctx->shell_off.clear();
+  for (const int4 &v : ctx->shell)
+    ctx->shell_off.push_back(make_float4((float)(v.x * g.cs[0]),
+                                         (float)(v.y * g.cs[1]),
+                                         (float)(v.z * g.cs[2]), 0.f));
+  ctx->use_list = (int)ctx->shell.size() <= kMaxShell;
   ctx->box = b;
   ctx->grid = g;
   ctx->shell_uploaded = 0;
   ctx->have_box = true;
   return PMNB_OK;
+}
+
+// List slots per atom: 1.4x the mean neighbour count at the system's mean
+// density plus 32, a multiple of 32, at most n - 1.  Atoms with more
+// neighbours are computed by the direct-scan kernel.
+int list_capacity(const pmnb_ctx *ctx, double cut2) {
+  const double vol = ctx->box.H[0] * ctx->box.H[1] * ctx->box.H[2];
+  const double rc = std::sqrt(cut2);
+  const double mean = ctx->n / vol * 4.0 / 3.0 * kPi * rc * rc * rc;
+  long long cap = (long long)(1.4 * mean) + 32;
+  cap = (cap + 31) / 32 * 32;
+  cap = std::min<long long>(cap, 4096);
+  cap = std::min<long long>(cap, std::max(ctx->n - 1, 1));
+  return (int)std::max<long long>(cap, 1);
 }
 
 ForceParams make_params(const pmnb_ctx *ctx, uint32_t flags) {
@@ -303,7 +329,22 @@ int ensure_eval_buffers(pmnb_ctx *ctx) {
     HIPCHK(ctx->d_shell.ensure(ctx->shell.size() * sizeof(int4)));
     HIPCHK(hipMemcpy(ctx->d_shell.p, ctx->shell.data(),
                      ctx->shell.size() * sizeof(int4), hipMemcpyHostToDevice));
+    HIPCHK(ctx->d_shell_off.ensure(ctx->shell_off.size() * sizeof(float4)));
+    HIPCHK(hipMemcpy(ctx->d_shell_off.p, ctx->shell_off.data(),
+                     ctx->shell_off.size() * sizeof(float4),
+                     hipMemcpyHostToDevice));
     ctx->shell_uploaded = 1;
+  }
+  const bool list_ok = ctx->use_list && n < (1 << kEntryAtomBits);
+  if (list_ok) {
+    const ForceParams P = make_params(ctx, 0);
+    ctx->cap = list_capacity(ctx, P.cut2);
+    HIPCHK(ctx->d_list.ensure((size_t)ctx->cap * N * sizeof(unsigned)));
+    HIPCHK(ctx->d_ncount.ensure(N * sizeof(int)));
+    HIPCHK(ctx->d_partial2.ensure((size_t)blocks_for(n) * ACC_N *
+                                  sizeof(double)));
+  } else {
+    ctx->cap = 0;
   }
   return PMNB_OK;
 }
@@ -311,8 +352,9 @@ int ensure_eval_buffers(pmnb_ctx *ctx) {
 // Cell-list build on the device from d_pos (original order).
 int build_cells(pmnb_ctx *ctx, const double *d_pos, hipStream_t s) {
   const int n = ctx->n, nc = ctx->grid.ncell;
-  launch_fill_i32(ctx->d_count.as<int>(), nc, 0, s);
-  launch_fill_i32(ctx->d_fill.as<int>(), nc, 0, s);
+  hipLaunchKernelGGL(k_clear, dim3(blocks_for(nc)), dim3(kBlock), 0, s,
+                     ctx->d_count.as<int>(), ctx->d_fill.as<int>(), nc,
+                     ctx->d_npairs.as<unsigned long long>());
   if (n > 0) {
     hipLaunchKernelGGL(k_assign, dim3(blocks_for(n)), dim3(kBlock), 0, s,
                        ctx->box, ctx->grid, d_pos, n, ctx->d_cell_of.as<int>(),
@@ -324,7 +366,8 @@ int build_cells(pmnb_ctx *ctx, const double *d_pos, hipStream_t s) {
     hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, s,
                        ctx->d_cell_of.as<int>(), n, ctx->d_start.as<int>(),
                        ctx->d_fill.as<int>(), ctx->d_sorted.as<int>());
-    hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(nc)), dim3(kBlock), 0, s,
+    hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(nc, kBlock / 64)),
+                       dim3(kBlock), 0, s,
                        ctx->d_start.as<int>(), ctx->d_count.as<int>(), nc,
                        ctx->d_sorted.as<int>());
     hipLaunchKernelGGL(k_gather, dim3(blocks_for(n)), dim3(kBlock), 0, s,
@@ -380,55 +423,102 @@ int run_device(pmnb_ctx *ctx, const double *d_pos, double *d_forces,
     return fail(ctx, PMNB_ERR_ARG, "molecular virial needs molecule centres");
   rc = build_cells(ctx, d_pos, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s,
-                     ctx->d_npairs.as<unsigned long long>(), 1);
   const ForceParams P = make_params(ctx, flags);
   const Sorted S = make_sorted(ctx, d_com);
   const int nb = std::max(1, blocks_for(ctx->n));
   double *part = ctx->d_partial.as<double>();
+  double *part2 = ctx->d_partial2.as<double>();
   unsigned long long *np = ctx->d_npairs.as<unsigned long long>();
   const bool mixed = ctx->prm.precision == PMNB_PREC_MIXED;
   const int el = ctx->prm.elec, sw = ctx->prm.elec_switch;
-  hipEvent_t ev_stop = nullptr;
+  const bool list = ctx->cap > 0;
+  const int cap = ctx->cap;
+  unsigned *lst = ctx->d_list.as<unsigned>();
+  int *cnt = ctx->d_ncount.as<int>();
+  hipEvent_t ev_built = nullptr, ev_stop = nullptr;
   if (ctx->timing) {
-    while (ctx->ev_pool.size() < ctx->ev_used + 2) {
+    while (ctx->ev_pool.size() < ctx->ev_used + 3) {
       hipEvent_t e;
       HIPCHK(hipEventCreate(&e));
       ctx->ev_pool.push_back(e);
     }
     HIPCHK(hipEventRecord(ctx->ev_pool[ctx->ev_used], s));
-    ev_stop = ctx->ev_pool[ctx->ev_used + 1];
-    ctx->ev_used += 2;
+    ev_built = ctx->ev_pool[ctx->ev_used + 1];
+    ev_stop = ctx->ev_pool[ctx->ev_used + 2];
+    ctx->ev_used += 3;
   }
+  // only / cap of the direct-scan kernels: everything (no list) or the
+  // atoms whose neighbours overflowed the list
+  const int *only = list ? cnt : nullptr;
+  if (list) {
+    if (ctx->per_pair_wrap)
+      hipLaunchKernelGGL(k_build_list<true>, dim3(nb), dim3(kBlock), 0, s, P,
+                         S, lst, cnt, cap, np);
+    else
+      hipLaunchKernelGGL(k_build_list<false>, dim3(nb), dim3(kBlock), 0, s, P,
+                         S, lst, cnt, cap, np);
+  }
+  if (ev_built) HIPCHK(hipEventRecord(ev_built, s));
+  if (list && mixed) {
+    const float4 *so = ctx->d_shell_off.as<float4>();
+    if (el == PMNB_ELEC_NONE)
+      hipLaunchKernelGGL(k_force_list_mixed<0>, dim3(nb), dim3(kBlock), 0, s,
+                         P, S, lst, cnt, cap, so, d_forces, part);
+    else if (el == PMNB_ELEC_COULOMB)
+      hipLaunchKernelGGL(k_force_list_mixed<1>, dim3(nb), dim3(kBlock), 0, s,
+                         P, S, lst, cnt, cap, so, d_forces, part);
+    else
+      hipLaunchKernelGGL(k_force_list_mixed<2>, dim3(nb), dim3(kBlock), 0, s,
+                         P, S, lst, cnt, cap, so, d_forces, part);
+  } else if (list) {
+    if (el == PMNB_ELEC_NONE)
+      hipLaunchKernelGGL((k_force_list_fp64<0, 0>), dim3(nb), dim3(kBlock), 0,
+                         s, P, S, lst, cnt, cap, d_forces, part);
+    else if (el == PMNB_ELEC_COULOMB && sw == PMNB_SWITCH_C2)
+      hipLaunchKernelGGL((k_force_list_fp64<1, 0>), dim3(nb), dim3(kBlock), 0,
+                         s, P, S, lst, cnt, cap, d_forces, part);
+    else if (el == PMNB_ELEC_COULOMB)
+      hipLaunchKernelGGL((k_force_list_fp64<1, 1>), dim3(nb), dim3(kBlock), 0,
+                         s, P, S, lst, cnt, cap, d_forces, part);
+    else if (sw == PMNB_SWITCH_C2)
+      hipLaunchKernelGGL((k_force_list_fp64<2, 0>), dim3(nb), dim3(kBlock), 0,
+                         s, P, S, lst, cnt, cap, d_forces, part);
+    else
+      hipLaunchKernelGGL((k_force_list_fp64<2, 1>), dim3(nb), dim3(kBlock), 0,
+                         s, P, S, lst, cnt, cap, d_forces, part);
+  }
+  // direct scan: the whole force without a list, else the overflow atoms
+  double *pd = list ? part2 : part;
   if (mixed) {
     if (el == PMNB_ELEC_NONE)
       hipLaunchKernelGGL(k_pair_mixed<0>, dim3(nb), dim3(kBlock), 0, s, P, S,
-                         d_forces, part, np);
+                         d_forces, pd, np, only, cap);
     else if (el == PMNB_ELEC_COULOMB)
       hipLaunchKernelGGL(k_pair_mixed<1>, dim3(nb), dim3(kBlock), 0, s, P, S,
-                         d_forces, part, np);
+                         d_forces, pd, np, only, cap);
     else
       hipLaunchKernelGGL(k_pair_mixed<2>, dim3(nb), dim3(kBlock), 0, s, P, S,
-                         d_forces, part, np);
+                         d_forces, pd, np, only, cap);
   } else {
     if (el == PMNB_ELEC_NONE)
       hipLaunchKernelGGL((k_pair_fp64<0, 0>), dim3(nb), dim3(kBlock), 0, s, P,
-                         S, d_forces, part, np);
+                         S, d_forces, pd, np, only, cap);
     else if (el == PMNB_ELEC_COULOMB && sw == PMNB_SWITCH_C2)
       hipLaunchKernelGGL((k_pair_fp64<1, 0>), dim3(nb), dim3(kBlock), 0, s, P,
-                         S, d_forces, part, np);
+                         S, d_forces, pd, np, only, cap);
     else if (el == PMNB_ELEC_COULOMB)
       hipLaunchKernelGGL((k_pair_fp64<1, 1>), dim3(nb), dim3(kBlock), 0, s, P,
-                         S, d_forces, part, np);
+                         S, d_forces, pd, np, only, cap);
     else if (sw == PMNB_SWITCH_C2)
       hipLaunchKernelGGL((k_pair_fp64<2, 0>), dim3(nb), dim3(kBlock), 0, s, P,
-                         S, d_forces, part, np);
+                         S, d_forces, pd, np, only, cap);
     else
       hipLaunchKernelGGL((k_pair_fp64<2, 1>), dim3(nb), dim3(kBlock), 0, s, P,
-                         S, d_forces, part, np);
+                         S, d_forces, pd, np, only, cap);
   }
   if (ev_stop) HIPCHK(hipEventRecord(ev_stop, s));
-  hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, s, part, nb, d_out);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, s, part, nb,
+                     list ? part2 : part, list ? nb : 0, d_out);
   HIPCHK(hipGetLastError());
   return PMNB_OK;
 }
@@ -581,7 +671,18 @@ int pmnb_set_topology(pmnb_ctx *ctx, int32_t n_atoms, const int32_t *type,
 int pmnb_set_box(pmnb_ctx *ctx, const double e1[3], const double e2[3],
                  const double e3[3], const double origin[3]) {
   if (!ctx || !e1 || !e2 || !e3 || !origin) return PMNB_ERR_ARG;
-  return build_box(ctx, e1, e2, e3, origin);
+  double v[12];
+  for (int k = 0; k < 3; k++) {
+    v[k] = e1[k];
+    v[3 + k] = e2[k];
+    v[6 + k] = e3[k];
+    v[9 + k] = origin[k];
+  }
+  if (ctx->have_box && std::memcmp(v, ctx->box_key, sizeof v) == 0)
+    return PMNB_OK;  // unchanged: keep the grid, shell and uploads
+  const int rc = build_box(ctx, e1, e2, e3, origin);
+  if (rc == PMNB_OK) std::memcpy(ctx->box_key, v, sizeof v);
+  return rc;
 }
 
 int pmnb_evaluate_device(pmnb_ctx *ctx, const double *d_positions,
@@ -674,8 +775,6 @@ int pmnb_pairset(pmnb_ctx *ctx, const double *positions, int32_t *pairs,
   const long long cap = pairs ? capacity : 0;
   HIPCHK(ctx->d_pairs.ensure((size_t)std::max<long long>(cap, 1) * 3 *
                              sizeof(int)));
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s,
-                     ctx->d_npairs.as<unsigned long long>(), 1);
   const ForceParams P = make_params(ctx, 0);
   const Sorted S = make_sorted(ctx, nullptr);
   if (ctx->n > 0)
@@ -714,20 +813,25 @@ int pmnb_pairset(pmnb_ctx *ctx, const double *positions, int32_t *pairs,
   return PMNB_OK;
 }
 
-int pmnb_kernel_timing(pmnb_ctx *ctx, int32_t enable, double *pair_ms,
+int pmnb_kernel_timing(pmnb_ctx *ctx, int32_t enable, double *ms,
                        int64_t *launches) {
   if (!ctx) return PMNB_ERR_ARG;
   HIPCHK(hipSetDevice(ctx->prm.device));
-  double total = 0.0;
+  double build = 0.0, pair = 0.0;
   const size_t used = ctx->ev_used;
   if (used) HIPCHK(hipEventSynchronize(ctx->ev_pool[used - 1]));
-  for (size_t k = 0; k + 1 < used; k += 2) {
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, ctx->ev_pool[k], ctx->ev_pool[k + 1]));
-    total += ms;
+  for (size_t k = 0; k + 2 < used; k += 3) {
+    float a = 0.f, b = 0.f;
+    HIPCHK(hipEventElapsedTime(&a, ctx->ev_pool[k], ctx->ev_pool[k + 1]));
+    HIPCHK(hipEventElapsedTime(&b, ctx->ev_pool[k + 1], ctx->ev_pool[k + 2]));
+    build += a;
+    pair += b;
   }
-  if (pair_ms) *pair_ms = total;
-  if (launches) *launches = (int64_t)(used / 2);
+  if (ms) {
+    ms[0] = build;
+    ms[1] = pair;
+  }
+  if (launches) *launches = (int64_t)(used / 3);
   ctx->ev_used = 0;
   ctx->timing = enable != 0;
   return PMNB_OK;

@@ -104,23 +104,51 @@ __global__ void k_scatter(const int *__restrict__ cell_of, int n,
   sorted[start[c] + slot] = i;
 }
 
-// Ascending atom order inside each cell (the reference's list order).
-__global__ void k_sort_cells(const int *__restrict__ start,
-                             const int *__restrict__ count, int ncell,
-                             int *__restrict__ sorted) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+// Ascending atom order inside each cell (the reference's list order): one
+// wave per cell sorts up to 64 entries with a bitonic network across lanes;
+// larger cells fall back to an insertion sort by one lane.
+__global__ void __launch_bounds__(kBlock)
+k_sort_cells(const int *__restrict__ start, const int *__restrict__ count,
+             int ncell, int *__restrict__ sorted) {
+  const int c = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (c >= ncell) return;
-  int s0 = start[c], m = count[c];
-  int *seg = sorted + s0;
-  for (int i = 1; i < m; i++) {
-    int v = seg[i];
-    int j = i - 1;
-    while (j >= 0 && seg[j] > v) {
-      seg[j + 1] = seg[j];
-      j--;
+  const int m = count[c];
+  if (m <= 1) return;
+  int *seg = sorted + start[c];
+  if (m <= 64) {
+    int v = lane < m ? seg[lane] : 0x7fffffff;
+    for (int k = 2; k <= 64; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const int other = __shfl_xor(v, j, 64);
+        const bool up = (lane & k) == 0;
+        const bool lower = (lane & j) == 0;
+        v = (lower == up) ? min(v, other) : max(v, other);
+      }
     }
-    seg[j + 1] = v;
+    if (lane < m) seg[lane] = v;
+  } else if (lane == 0) {
+    for (int i = 1; i < m; i++) {
+      int v = seg[i];
+      int j = i - 1;
+      while (j >= 0 && seg[j] > v) {
+        seg[j + 1] = seg[j];
+        j--;
+      }
+      seg[j + 1] = v;
+    }
   }
+}
+
+// Clears the per-cell counters and the pair counter in one launch.
+__global__ void k_clear(int *__restrict__ count, int *__restrict__ fill,
+                        int ncell, unsigned long long *__restrict__ npairs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ncell) {
+    count[i] = 0;
+    fill[i] = 0;
+  }
+  if (i == 0) *npairs = 0ull;
 }
 
 // Gathers per-atom data into sorted order.  loc holds, for the mixed path,

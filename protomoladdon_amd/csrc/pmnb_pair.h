@@ -94,13 +94,14 @@ template <int ELEC, int ELSW>
 __global__ void __launch_bounds__(kBlock)
 k_pair_fp64(ForceParams P, Sorted S, double *__restrict__ forces,
             double *__restrict__ partial,
-            unsigned long long *__restrict__ npairs) {
+            unsigned long long *__restrict__ npairs,
+            const int *__restrict__ only, int cap) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[ACC_N];
 #pragma unroll
   for (int k = 0; k < ACC_N; k++) acc[k] = 0.0;
   unsigned cnt = 0;
-  if (s < P.n_atoms) {
+  if (s < P.n_atoms && (!only || only[s] > cap)) {
     const double xi = S.x[s], yi = S.y[s], zi = S.z[s];
     const int a = S.atom[s], ti = S.type[s], mi = S.mol[s];
     const double qi = S.q[s];
@@ -189,8 +190,10 @@ k_pair_fp64(ForceParams P, Sorted S, double *__restrict__ forces,
     forces[3 * a + 2] += fz;
   }
   block_reduce_store(acc, partial);
-  unsigned long long tot = block_count(cnt);
-  if (threadIdx.x == 0 && tot) atomicAdd(npairs, tot);
+  if (!only) {
+    unsigned long long tot = block_count(cnt);
+    if (threadIdx.x == 0 && tot) atomicAdd(npairs, tot);
+  }
 }
 
 // ----------------------------------------------------------------- mixed
@@ -202,17 +205,123 @@ __device__ __forceinline__ long long to_fixed(double v) {
   return (long long)__builtin_rint(v * kFixedScale);
 }
 
+// FP32 constants of the two functors and their switches.
+struct F32Params {
+  float cut2, band, Lx, Ly, Lz, iLx, iLy, iLz;
+  float on2l, cut2l, s1l, s2l, s3l, on2e, cut2e, s1e, s2e, s3e;
+  float alpha, tapi, csf;
+};
+
+__device__ __forceinline__ F32Params f32_params(const ForceParams &P) {
+  F32Params F;
+  F.cut2 = (float)P.cut2;
+  F.band = P.band;
+  F.Lx = (float)P.box.e[0];
+  F.Ly = (float)P.box.e[1];
+  F.Lz = (float)P.box.e[2];
+  F.iLx = (float)P.box.er[0];
+  F.iLy = (float)P.box.er[1];
+  F.iLz = (float)P.box.er[2];
+  F.on2l = (float)P.sw_lj.on2;
+  F.cut2l = (float)P.sw_lj.cut2;
+  F.s1l = (float)P.sw_lj.s1;
+  F.s2l = (float)P.sw_lj.s2;
+  F.s3l = (float)P.sw_lj.s3;
+  F.on2e = (float)P.sw_el.on2;
+  F.cut2e = (float)P.sw_el.cut2;
+  F.s1e = (float)P.sw_el.s1;
+  F.s2e = (float)P.sw_el.s2;
+  F.s3e = (float)P.sw_el.s3;
+  F.alpha = (float)P.alpha;
+  F.tapi = (float)P.two_alpha_pi;
+  F.csf = (float)P.csf;
+  return F;
+}
+
+// FP32 pair of OneAtomPairTwo: returns the switched F/r (ff) and the two
+// switched energies.  A, B are already the MODIFIED (A14, B14) values when
+// excl == 1; qq is the raw charge product.
+template <int ELEC>
+__device__ __forceinline__ void pair_f32(const F32Params &F, float r2,
+                                         float A, float B, float qq,
+                                         int excl, float &ff, float &e1,
+                                         float &e2) {
+  const float rd2 = __builtin_amdgcn_rcpf(r2);
+  const float r6 = rd2 * rd2 * rd2;
+  const float r6B = B * r6;
+  const float r12A = A * r6 * r6;
+  e1 = r12A - r6B;
+  float f1 = fmaf(12.f, r12A, -6.f * r6B) * rd2;
+  float f2 = 0.f;
+  e2 = 0.f;
+  if (ELEC == 1) {
+    if (excl == 1) qq *= F.csf;
+    e2 = qq * __builtin_amdgcn_rsqf(r2);
+    f2 = e2 * rd2;
+  } else if (ELEC == 2) {
+    if (excl == 1) qq *= F.csf;
+    const float rr = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * rr;
+    const float ar = F.alpha * r;
+    // Abramowitz & Stegun 7.1.26 (base/MathUtilities.cpp:39-49)
+    const float tt = __builtin_amdgcn_rcpf(fmaf(0.3275911f, ar, 1.f));
+    const float poly =
+        tt * fmaf(tt, fmaf(tt, fmaf(tt, fmaf(tt, 1.061405429f, -1.453152027f),
+                                    1.421413741f),
+                           -0.284496736f),
+                  0.254829592f);
+    const float e = qq * __builtin_amdgcn_exp2f(-ar * ar * 1.4426950408889634f);
+    e2 = poly * e * rr;
+    f2 = fmaf(F.tapi, e, e2) * rd2;
+  }
+  float sv = 1.f, sd = 0.f;
+  if (r2 >= F.on2l) {
+    const float c2 = F.cut2l - r2;
+    const float c4 = c2 * fmaf(2.f, r2, F.s2l);
+    sv = F.s1l * (c2 * c4);
+    sd = F.s3l * fmaf(c2, c2, -c4);
+    if (r2 > F.cut2l) {
+      sv = 0.f;
+      sd = 0.f;
+    }
+  }
+  f1 = fmaf(f1, sv, -e1 * sd);
+  e1 = e1 * sv;
+  if (ELEC != 0) {
+    float se = 1.f, sde = 0.f;
+    if (r2 >= F.on2e) {
+      const float c2 = F.cut2e - r2;
+      const float c4 = c2 * fmaf(2.f, r2, F.s2e);
+      se = F.s1e * (c2 * c4);
+      sde = F.s3e * fmaf(c2, c2, -c4);
+      if (r2 > F.cut2e) {
+        se = 0.f;
+        sde = 0.f;
+      }
+    }
+    f2 = fmaf(f2, se, -e2 * sde);
+    e2 = e2 * se;
+  }
+  ff = f1 + f2;
+}
+
+// Direct scan of the shell cells for one atom per lane (mixed precision).
+// With `only` non-null, only lanes whose neighbour count exceeds `cap` run:
+// the overflow path of the neighbour-list kernels.
 template <int ELEC>
 __global__ void __launch_bounds__(kBlock)
 k_pair_mixed(ForceParams P, Sorted S, double *__restrict__ forces,
              double *__restrict__ partial,
-             unsigned long long *__restrict__ npairs) {
+             unsigned long long *__restrict__ npairs,
+             const int *__restrict__ only, int cap) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[ACC_N];
 #pragma unroll
   for (int k = 0; k < ACC_N; k++) acc[k] = 0.0;
   unsigned cnt = 0;
-  if (s < P.n_atoms) {
+  const bool run = s < P.n_atoms && (!only || only[s] > cap);
+  if (run) {
+    const F32Params F = f32_params(P);
     const float4 li = S.loc[s];
     const int a = S.atom[s], ti = S.type[s], mi = S.mol[s];
     const int c = S.cell[s];
@@ -220,20 +329,6 @@ k_pair_mixed(ForceParams P, Sorted S, double *__restrict__ forces,
     const int cy = (c / P.grid.n[2]) % P.grid.n[1];
     const int cz = c % P.grid.n[2];
     const float4 *ljrow = S.ljf + ti * P.n_types;
-    const float cut2 = (float)P.cut2;
-    const float band = P.band;
-    const float Lx = (float)P.box.e[0], Ly = (float)P.box.e[1],
-                Lz = (float)P.box.e[2];
-    const float iLx = (float)P.box.er[0], iLy = (float)P.box.er[1],
-                iLz = (float)P.box.er[2];
-    const float on2l = (float)P.sw_lj.on2, cut2l = (float)P.sw_lj.cut2,
-                s1l = (float)P.sw_lj.s1, s2l = (float)P.sw_lj.s2,
-                s3l = (float)P.sw_lj.s3;
-    const float on2e = (float)P.sw_el.on2, cut2e = (float)P.sw_el.cut2,
-                s1e = (float)P.sw_el.s1, s2e = (float)P.sw_el.s2,
-                s3e = (float)P.sw_el.s3;
-    const float alpha = (float)P.alpha, tapi = (float)P.two_alpha_pi,
-                csf = (float)P.csf;
     long long Fx = 0, Fy = 0, Fz = 0;
     for (int d = 0; d < P.ndelta; d++) {
       const int4 dd = S.delta[d];
@@ -251,13 +346,13 @@ k_pair_mixed(ForceParams P, Sorted S, double *__restrict__ forces,
         const float4 lj4 = S.loc[t];
         float dx = lj4.x + ox, dy = lj4.y + oy, dz = lj4.z + oz;
         if (P.per_pair_wrap) {
-          dx = fmaf(-Lx, rintf(dx * iLx), dx);
-          dy = fmaf(-Ly, rintf(dy * iLy), dy);
-          dz = fmaf(-Lz, rintf(dz * iLz), dz);
+          dx = fmaf(-F.Lx, rintf(dx * F.iLx), dx);
+          dy = fmaf(-F.Ly, rintf(dy * F.iLy), dy);
+          dz = fmaf(-F.Lz, rintf(dz * F.iLz), dz);
         }
         const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        if (r2 > cut2 + band) continue;
-        if (r2 >= cut2 - band) {
+        if (r2 > F.cut2 + F.band) continue;
+        if (r2 >= F.cut2 - F.band) {
           // Near the cutoff: decide with the reference's FP64 arithmetic.
           double ex, ey, ez, e2d;
           min_diff_exact(P.box, S.x[s], S.y[s], S.z[s], S.x[t], S.y[t],
@@ -269,65 +364,12 @@ k_pair_mixed(ForceParams P, Sorted S, double *__restrict__ forces,
         const int excl =
             same ? excl_class(S.xfull, S.xmod, P.max_excl_delta, a, b) : 0;
         if (excl == 2) continue;
-        const float rd2 = __builtin_amdgcn_rcpf(r2);
         const float4 lp = ljrow[S.type[t]];
-        const float A = excl != 1 ? lp.x : lp.z;
-        const float B = excl != 1 ? lp.y : lp.w;
-        const float r6 = rd2 * rd2 * rd2;
-        const float r6B = B * r6;
-        const float r12A = A * r6 * r6;
-        float e1 = r12A - r6B;
-        float f1 = fmaf(12.f, r12A, -6.f * r6B) * rd2;
-        float e2 = 0.f, f2 = 0.f;
-        if (ELEC == 1) {
-          float qq = li.w * lj4.w;
-          if (excl == 1) qq *= csf;
-          e2 = qq * __builtin_amdgcn_rsqf(r2);
-          f2 = e2 * rd2;
-        } else if (ELEC == 2) {
-          float qq = li.w * lj4.w;
-          if (excl == 1) qq *= csf;
-          const float rr = __builtin_amdgcn_rsqf(r2);
-          const float r = r2 * rr;
-          const float ar = alpha * r;
-          const float tt = __builtin_amdgcn_rcpf(fmaf(0.3275911f, ar, 1.f));
-          const float poly =
-              tt * fmaf(tt, fmaf(tt, fmaf(tt, fmaf(tt, 1.061405429f,
-                                                   -1.453152027f),
-                                          1.421413741f),
-                                 -0.284496736f),
-                        0.254829592f);
-          const float e = qq * __builtin_amdgcn_exp2f(-ar * ar *
-                                                      1.4426950408889634f);
-          e2 = poly * e * rr;
-          f2 = fmaf(tapi, e, e2) * rd2;
-        }
-        // C2 switching of each term.
-        float sv = 1.f, sd = 0.f;
-        if (r2 >= on2l) {
-          const float c2 = cut2l - r2;
-          const float c4 = c2 * fmaf(2.f, r2, s2l);
-          sv = s1l * (c2 * c4);
-          sd = s3l * fmaf(c2, c2, -c4);
-          if (r2 > cut2l) { sv = 0.f; sd = 0.f; }
-        }
-        f1 = fmaf(f1, sv, -e1 * sd);
-        e1 = e1 * sv;
-        if (ELEC != 0) {
-          float se = 1.f, sde = 0.f;
-          if (r2 >= on2e) {
-            const float c2 = cut2e - r2;
-            const float c4 = c2 * fmaf(2.f, r2, s2e);
-            se = s1e * (c2 * c4);
-            sde = s3e * fmaf(c2, c2, -c4);
-            if (r2 > cut2e) { se = 0.f; sde = 0.f; }
-          }
-          f2 = fmaf(f2, se, -e2 * sde);
-          e2 = e2 * se;
-        }
+        float ff, e1, e2;
+        pair_f32<ELEC>(F, r2, excl != 1 ? lp.x : lp.z, excl != 1 ? lp.y : lp.w,
+                       li.w * lj4.w, excl, ff, e1, e2);
         pelj += e1;
         peel += e2;
-        const float ff = f1 + f2;
         const float gx = dx * ff, gy = dy * ff, gz = dz * ff;
         px -= gx;
         py -= gy;
@@ -358,22 +400,22 @@ k_pair_mixed(ForceParams P, Sorted S, double *__restrict__ forces,
       Fz += to_fixed((double)pz);
       acc[ACC_ELJ] += 0.5 * (double)pelj;
       acc[ACC_EEL] += 0.5 * (double)peel;
-      if (P.flags & 3u) {
-        acc[ACC_VXX] += 0.5 * (double)v0;
-        acc[ACC_VXY] += 0.5 * (double)v1;
-        acc[ACC_VXZ] += 0.5 * (double)v2;
-        acc[ACC_VYY] += 0.5 * (double)v3;
-        acc[ACC_VYZ] += 0.5 * (double)v4;
-        acc[ACC_VZZ] += 0.5 * (double)v5;
-      }
+      acc[ACC_VXX] += 0.5 * (double)v0;
+      acc[ACC_VXY] += 0.5 * (double)v1;
+      acc[ACC_VXZ] += 0.5 * (double)v2;
+      acc[ACC_VYY] += 0.5 * (double)v3;
+      acc[ACC_VYZ] += 0.5 * (double)v4;
+      acc[ACC_VZZ] += 0.5 * (double)v5;
     }
     forces[3 * a] += (double)Fx * kFixedInv;
     forces[3 * a + 1] += (double)Fy * kFixedInv;
     forces[3 * a + 2] += (double)Fz * kFixedInv;
   }
   block_reduce_store(acc, partial);
-  unsigned long long tot = block_count(cnt);
-  if (threadIdx.x == 0 && tot) atomicAdd(npairs, tot);
+  if (!only) {
+    unsigned long long tot = block_count(cnt);
+    if (threadIdx.x == 0 && tot) atomicAdd(npairs, tot);
+  }
 }
 
 // ------------------------------------------------------- pair-set extract
@@ -415,28 +457,34 @@ k_pairset(ForceParams P, Sorted S, int *__restrict__ out, long long cap,
   }
 }
 
-// Sums the per-block partials in block order (one thread per slot) and
+// Sums the per-block partials of up to two kernels (one wave per slot,
+// lanes striding over blocks, then a fixed shuffle tree: deterministic) and
 // accumulates into out[] in the 20-slot layout of pmnb_evaluate_device.
-__global__ void k_finish(const double *__restrict__ partial, int nblocks,
-                         double *__restrict__ out) {
-  int k = threadIdx.x;
-  if (k >= ACC_N) return;
-  double s = 0.0;
-  for (int b = 0; b < nblocks; b++) s += partial[b * ACC_N + k];
-  // out: [0] LJ, [1] elec, [2..10] virial 3x3, [11..19] mol virial 3x3
-  if (k < 2) {
-    out[k] += s;
-  } else {
-    const int base = k < ACC_MXX ? 2 : 11;
-    const int u = k < ACC_MXX ? k - ACC_VXX : k - ACC_MXX;
-    // u: 0 xx, 1 xy, 2 xz, 3 yy, 4 yz, 5 zz
-    switch (u) {
-      case 0: out[base + 0] += s; break;
-      case 1: out[base + 1] += s; out[base + 3] += s; break;
-      case 2: out[base + 2] += s; out[base + 6] += s; break;
-      case 3: out[base + 4] += s; break;
-      case 4: out[base + 5] += s; out[base + 7] += s; break;
-      default: out[base + 8] += s; break;
+__global__ void __launch_bounds__(1024)
+k_finish(const double *__restrict__ p1, int nb1, const double *__restrict__ p2,
+         int nb2, double *__restrict__ out) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int k = wave; k < ACC_N; k += 16) {
+    double v = 0.0;
+    for (int b = lane; b < nb1; b += 64) v += p1[b * ACC_N + k];
+    for (int b = lane; b < nb2; b += 64) v += p2[b * ACC_N + k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane != 0) continue;
+    // out: [0] LJ, [1] elec, [2..10] virial 3x3, [11..19] mol virial 3x3
+    if (k < 2) {
+      out[k] += v;
+    } else {
+      const int base = k < ACC_MXX ? 2 : 11;
+      const int u = k < ACC_MXX ? k - ACC_VXX : k - ACC_MXX;
+      // u: 0 xx, 1 xy, 2 xz, 3 yy, 4 yz, 5 zz
+      switch (u) {
+        case 0: out[base + 0] += v; break;
+        case 1: out[base + 1] += v; out[base + 3] += v; break;
+        case 2: out[base + 2] += v; out[base + 6] += v; break;
+        case 3: out[base + 4] += v; break;
+        case 4: out[base + 5] += v; out[base + 7] += v; break;
+        default: out[base + 8] += v; break;
+      }
     }
   }
 }

@@ -253,13 +253,14 @@ class NonbondedCutoffSystemForce:
         return out[:n.value]
 
     def kernel_timing(self, enable):
-        """Starts (True) or stops (False) HIP-event timing of the pair kernel;
-        returns (milliseconds, launches) recorded since timing was enabled."""
-        ms = ctypes.c_double()
+        """Starts (True) or stops (False) HIP-event timing of the pair stage;
+        returns (build_ms, pair_ms, evaluations) recorded since timing was
+        enabled."""
+        ms = np.zeros(2)
         n = ctypes.c_int64()
         _lib.check(self._lib, self._ctx, self._lib.pmnb_kernel_timing(
-            self._ctx, 1 if enable else 0, ctypes.byref(ms), ctypes.byref(n)))
-        return ms.value, n.value
+            self._ctx, 1 if enable else 0, ms.ctypes.data, ctypes.byref(n)))
+        return float(ms[0]), float(ms[1]), n.value
 
     def last_pair_count(self):
         n = ctypes.c_int64()

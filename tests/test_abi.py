@@ -40,3 +40,25 @@ def test_null_arguments_are_rejected_without_a_device():
     assert lib.pmnb_create(None, None) == _lib.PMNB_ERR_ARG
     assert lib.pmnb_last_error(None) == b"null context"
     lib.pmnb_destroy(None)
+
+
+def test_protomol_adapter_compiles_against_the_reference_headers():
+    """The SystemForce adapter builds with ProtoMol's flags (skipped when the
+    reference tree is not present)."""
+    import shutil
+    import subprocess
+    import pytest
+    src = "/root/reference/protomol/src"
+    if not os.path.isdir(src) or not shutil.which("g++"):
+        pytest.skip("ProtoMol sources not present")
+    code = ('#include "protomoladdon_amd/adapter/PmnbCutoffSystemForce.h"\n'
+            "template class ProtoMol::PmnbCutoffSystemForce<"
+            "ProtoMol::C2SwitchingFunction>;\n"
+            "template class ProtoMol::PmnbCutoffSystemForce<"
+            "ProtoMol::C1SwitchingFunction>;\n")
+    r = subprocess.run(["g++", "-std=c++0x", "-fpermissive", "-w",
+                        "-fsyntax-only", "-I" + src,
+                        "-I" + os.path.join(ROOT, "include"), "-I" + ROOT,
+                        "-x", "c++", "-"], input=code, text=True,
+                       capture_output=True)
+    assert r.returncode == 0, r.stderr[-2000:]
