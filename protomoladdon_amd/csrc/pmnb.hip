@@ -20,6 +20,7 @@
 #include "pmnb_cells.h"
 #include "pmnb_pair.h"
 #include "pmnb_list.h"
+#include "pmnb_tile.h"
 
 using namespace pmnb;
 
@@ -79,6 +80,9 @@ struct pmnb_ctx {
   int shell_uploaded = 0;
   std::vector<float4> shell_off;  // FP32 cell offsets of the shell entries
   bool use_list = false;          // neighbour-list path available
+  bool use_tile = false;          // LDS-staged path (own column grid)
+  TileGrid tile{};
+  DevBuf d_wcount, d_woff, d_items, d_nitems;
   double box_key[12] = {0};       // e1, e2, e3, origin of the current box
   int cap = 0;                    // list slots per atom
   // pair-kernel timing
@@ -132,6 +136,28 @@ Switch make_c1(double cut) {
   s.c1_05c3 = 0.5 / (cut * cut * cut);
   s.c1_15c3 = 1.5 / (cut * cut * cut);
   return s;
+}
+
+// Switching constants and the pair cutoff, from the force parameters alone.
+ForceParams make_params_for(const pmnb_params &prm, double csf) {
+  ForceParams P{};
+  P.sw_lj = make_c2(prm.lj_switchon, prm.lj_cutoff);
+  if (prm.elec_switch == PMNB_SWITCH_C1)
+    P.sw_el = make_c1(prm.elec_cutoff);
+  else
+    P.sw_el = make_c2(prm.elec_switchon, prm.elec_cutoff);
+  // OneAtomPairTwo::mySquaredCutoff: max of the two switching cutoffs
+  // (force/OneAtomPairTwo.h:38-40).  With no electrostatic functor the
+  // second switch still exists in the reference's two-force template, so
+  // its cutoff still counts.
+  P.cut2 = std::max(P.sw_lj.cut2, P.sw_el.cut2);
+  P.alpha = prm.ewald_alpha;
+  P.two_alpha_pi = 2.0 * prm.ewald_alpha / std::sqrt(kPi);
+  P.csf = csf;
+  P.elec = prm.elec;
+  P.el_switch = prm.elec_switch;
+  P.band = 1.0e-3f * (float)std::max(1.0, P.cut2 / 100.0);
+  return P;
 }
 
 // Box + grid + shell offsets exactly as PeriodicBoundaryConditions::set,
@@ -245,12 +271,54 @@ int build_box(pmnb_ctx *ctx, const double *e1, const double *e2,
   ctx->per_pair_wrap = 0;
   for (int k = 0; k < 3; k++)
     if (g.n[k] < 2 * reach[k] + 1) ctx->per_pair_wrap = 1;
+  // LDS-staged path: its own grid of x/y columns at least one cutoff wide
+  // and z cells of about a quarter cutoff.  Used when the pair set does not
+  // depend on the cell grid (algorithm cutoff >= switching cutoff) and the
+  // box holds at least 3 columns per axis.
+  {
+    const ForceParams P0 = make_params_for(ctx->prm, ctx->csf);
+    const double rcp = std::sqrt(P0.cut2);
+    ctx->use_tile = false;
+    const int tx = (int)std::floor(b.H[0] / rcp), ty = (int)std::floor(b.H[1] / rcp);
+    const int tz = (int)std::floor(b.H[2] / (0.25 * rcp));
+    if (ctx->prm.algorithm_cutoff >= rcp && tx >= 3 && ty >= 3 && tz >= 3) {
+      const double csz = b.H[2] / tz;
+      const int rz = (int)std::floor(rcp / csz) + 1;
+      if (tz >= 2 * rz + 1 && (long long)tx * ty * tz < (1ll << 26)) {
+        Grid tg{};
+        tg.n[0] = tx;
+        tg.n[1] = ty;
+        tg.n[2] = tz;
+        tg.ncell = tx * ty * tz;
+        tg.cs[0] = b.H[0] / tx;
+        tg.cs[1] = b.H[1] / ty;
+        tg.cs[2] = csz;
+        for (int k = 0; k < 3; k++) tg.inv[k] = 1.0 / tg.cs[k];
+        ctx->tile.nx = tx;
+        ctx->tile.ny = ty;
+        ctx->tile.nz = tz;
+        ctx->tile.rz = rz;
+        ctx->tile.csx = (float)tg.cs[0];
+        ctx->tile.csy = (float)tg.cs[1];
+        ctx->tile.csz = (float)tg.cs[2];
+        g = tg;
+        // shell of this grid (pair-set extraction walks it)
+        ctx->shell.clear();
+        ctx->shell.push_back(make_int4(0, 0, 0, 0));
+        for (int dx = -1; dx <= 1; dx++)
+          for (int dy = -1; dy <= 1; dy++)
+            for (int dz = -rz; dz <= rz; dz++)
+              if (dx || dy || dz) ctx->shell.push_back(make_int4(dx, dy, dz, 0));
+        ctx->use_tile = true;
+      }
+    }
+  }
   ctx->shell_off.clear();
   for (const int4 &v : ctx->shell)
     ctx->shell_off.push_back(make_float4((float)(v.x * g.cs[0]),
                                          (float)(v.y * g.cs[1]),
                                          (float)(v.z * g.cs[2]), 0.f));
-  ctx->use_list = (int)ctx->shell.size() <= kMaxShell;
+  ctx->use_list = !ctx->use_tile && (int)ctx->shell.size() <= kMaxShell;
   ctx->box = b;
   ctx->grid = g;
   ctx->shell_uploaded = 0;
@@ -273,31 +341,15 @@ int list_capacity(const pmnb_ctx *ctx, double cut2) {
 }
 
 ForceParams make_params(const pmnb_ctx *ctx, uint32_t flags) {
-  ForceParams P{};
+  ForceParams P = make_params_for(ctx->prm, ctx->csf);
   P.box = ctx->box;
   P.grid = ctx->grid;
-  P.sw_lj = make_c2(ctx->prm.lj_switchon, ctx->prm.lj_cutoff);
-  if (ctx->prm.elec_switch == PMNB_SWITCH_C1)
-    P.sw_el = make_c1(ctx->prm.elec_cutoff);
-  else
-    P.sw_el = make_c2(ctx->prm.elec_switchon, ctx->prm.elec_cutoff);
-  // OneAtomPairTwo::mySquaredCutoff: max of the two switching cutoffs
-  // (force/OneAtomPairTwo.h:38-40).  With no electrostatic functor the
-  // second switch still exists in the reference's two-force template, so
-  // its cutoff still counts.
-  P.cut2 = std::max(P.sw_lj.cut2, P.sw_el.cut2);
-  P.alpha = ctx->prm.ewald_alpha;
-  P.two_alpha_pi = 2.0 * ctx->prm.ewald_alpha / std::sqrt(kPi);
-  P.csf = ctx->csf;
   P.n_atoms = ctx->n;
   P.n_types = ctx->ntypes;
   P.ndelta = (int)ctx->shell.size();
-  P.elec = ctx->prm.elec;
-  P.el_switch = ctx->prm.elec_switch;
   P.max_excl_delta = ctx->max_excl_delta;
   P.per_pair_wrap = ctx->per_pair_wrap;
   P.flags = flags;
-  P.band = 1.0e-3f * (float)std::max(1.0, P.cut2 / 100.0);
   return P;
 }
 
@@ -323,7 +375,10 @@ int ensure_eval_buffers(pmnb_ctx *ctx) {
   HIPCHK(ctx->d_sq.ensure(N * sizeof(double)));
   HIPCHK(ctx->d_smol.ensure(N * sizeof(int)));
   HIPCHK(ctx->d_scell.ensure(N * sizeof(int)));
-  HIPCHK(ctx->d_partial.ensure((size_t)blocks_for(n) * ACC_N * sizeof(double)));
+  const int ncol_t = ctx->use_tile ? ctx->tile.nx * ctx->tile.ny : 0;
+  const int tile_blocks = blocks_for(n / kTileAtoms + ncol_t + 1, kTileWaves);
+  HIPCHK(ctx->d_partial.ensure((size_t)std::max(blocks_for(n), tile_blocks) *
+                               ACC_N * sizeof(double)));
   HIPCHK(ctx->d_npairs.ensure(sizeof(unsigned long long)));
   if (!ctx->shell_uploaded) {
     HIPCHK(ctx->d_shell.ensure(ctx->shell.size() * sizeof(int4)));
@@ -334,6 +389,14 @@ int ensure_eval_buffers(pmnb_ctx *ctx) {
                      ctx->shell_off.size() * sizeof(float4),
                      hipMemcpyHostToDevice));
     ctx->shell_uploaded = 1;
+  }
+  if (ctx->use_tile) {
+    const int ncol = ctx->tile.nx * ctx->tile.ny;
+    HIPCHK(ctx->d_wcount.ensure((size_t)ncol * sizeof(int)));
+    HIPCHK(ctx->d_woff.ensure((size_t)ncol * sizeof(int)));
+    HIPCHK(ctx->d_items.ensure((size_t)(n / kTileAtoms + ncol + 1) *
+                               sizeof(int4)));
+    HIPCHK(ctx->d_nitems.ensure(sizeof(int)));
   }
   const bool list_ok = ctx->use_list && n < (1 << kEntryAtomBits);
   if (list_ok) {
@@ -446,6 +509,55 @@ int run_device(pmnb_ctx *ctx, const double *d_pos, double *d_forces,
     ev_built = ctx->ev_pool[ctx->ev_used + 1];
     ev_stop = ctx->ev_pool[ctx->ev_used + 2];
     ctx->ev_used += 3;
+  }
+  if (ctx->use_tile) {
+    const TileGrid G = ctx->tile;
+    const int ncol = G.nx * G.ny;
+    const int max_items = ctx->n / kTileAtoms + ncol + 1;
+    const int nbt = blocks_for(max_items, kTileWaves);
+    int *wc = ctx->d_wcount.as<int>(), *wo = ctx->d_woff.as<int>();
+    int4 *items = ctx->d_items.as<int4>();
+    int *nitems = ctx->d_nitems.as<int>();
+    hipLaunchKernelGGL(k_col_waves, dim3(blocks_for(ncol)), dim3(kBlock), 0, s,
+                       S.cstart, S.ccount, ncol, G.nz, wc);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, wc, wo, ncol);
+    hipLaunchKernelGGL(k_col_items, dim3(blocks_for(ncol)), dim3(kBlock), 0, s,
+                       S.cstart, S.ccount, wc, wo, ncol, G.nz, items, nitems);
+    if (ev_built) HIPCHK(hipEventRecord(ev_built, s));
+    if (ctx->n > 0) {
+      if (mixed) {
+        if (el == PMNB_ELEC_NONE)
+          hipLaunchKernelGGL((k_tile<false, 0, 0, 256>), dim3(nbt), dim3(kBlock),
+                             0, s, P, S, G, items, nitems, d_forces, part, np);
+        else if (el == PMNB_ELEC_COULOMB)
+          hipLaunchKernelGGL((k_tile<false, 1, 0, 256>), dim3(nbt), dim3(kBlock),
+                             0, s, P, S, G, items, nitems, d_forces, part, np);
+        else
+          hipLaunchKernelGGL((k_tile<false, 2, 0, 256>), dim3(nbt), dim3(kBlock),
+                             0, s, P, S, G, items, nitems, d_forces, part, np);
+      } else {
+        if (el == PMNB_ELEC_NONE)
+          hipLaunchKernelGGL((k_tile<true, 0, 0, 128>), dim3(nbt), dim3(kBlock),
+                             0, s, P, S, G, items, nitems, d_forces, part, np);
+        else if (el == PMNB_ELEC_COULOMB && sw == PMNB_SWITCH_C2)
+          hipLaunchKernelGGL((k_tile<true, 1, 0, 128>), dim3(nbt), dim3(kBlock),
+                             0, s, P, S, G, items, nitems, d_forces, part, np);
+        else if (el == PMNB_ELEC_COULOMB)
+          hipLaunchKernelGGL((k_tile<true, 1, 1, 128>), dim3(nbt), dim3(kBlock),
+                             0, s, P, S, G, items, nitems, d_forces, part, np);
+        else if (sw == PMNB_SWITCH_C2)
+          hipLaunchKernelGGL((k_tile<true, 2, 0, 128>), dim3(nbt), dim3(kBlock),
+                             0, s, P, S, G, items, nitems, d_forces, part, np);
+        else
+          hipLaunchKernelGGL((k_tile<true, 2, 1, 128>), dim3(nbt), dim3(kBlock),
+                             0, s, P, S, G, items, nitems, d_forces, part, np);
+      }
+    }
+    if (ev_stop) HIPCHK(hipEventRecord(ev_stop, s));
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, s, part,
+                       ctx->n > 0 ? nbt : 0, part, 0, d_out);
+    HIPCHK(hipGetLastError());
+    return PMNB_OK;
   }
   // only / cap of the direct-scan kernels: everything (no list) or the
   // atoms whose neighbours overflowed the list
