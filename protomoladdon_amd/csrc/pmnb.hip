@@ -76,7 +76,7 @@ struct pmnb_ctx {
   DevBuf d_pos, d_force, d_out, d_com, d_cell_of, d_count, d_start, d_fill,
       d_sorted, d_sx, d_sy, d_sz, d_loc, d_stype, d_sq, d_smol, d_scell,
       d_shell, d_partial, d_npairs, d_pairs, d_list, d_ncount, d_partial2,
-      d_shell_off;
+      d_shell_off, d_box;
   int shell_uploaded = 0;
   std::vector<float4> shell_off;  // FP32 cell offsets of the shell entries
   bool use_list = false;          // neighbour-list path available
@@ -350,6 +350,28 @@ ForceParams make_params(const pmnb_ctx *ctx, uint32_t flags) {
   P.max_excl_delta = ctx->max_excl_delta;
   P.per_pair_wrap = ctx->per_pair_wrap;
   P.flags = flags;
+  F32Params &F = P.f;
+  F.cut2 = (float)P.cut2;
+  F.band = P.band;
+  for (int k = 0; k < 3; k++) {
+    (&F.Lx)[k] = (float)P.box.e[k];
+    (&F.iLx)[k] = (float)P.box.er[k];
+  }
+  F.on2l = (float)P.sw_lj.on2;
+  F.cut2l = (float)P.sw_lj.cut2;
+  F.s1l = (float)P.sw_lj.s1;
+  F.s2l = (float)P.sw_lj.s2;
+  F.s3l = (float)P.sw_lj.s3;
+  F.on2e = (float)P.sw_el.on2;
+  F.cut2e = (float)P.sw_el.cut2;
+  F.s1e = (float)P.sw_el.s1;
+  F.s2e = (float)P.sw_el.s2;
+  F.s3e = (float)P.sw_el.s3;
+  F.alpha = (float)P.alpha;
+  F.tapi = (float)P.two_alpha_pi;
+  F.csf = (float)P.csf;
+  F.same_switch = P.sw_lj.on2 == P.sw_el.on2 && P.sw_lj.cut2 == P.sw_el.cut2 &&
+                  ctx->prm.elec_switch == PMNB_SWITCH_C2;
   return P;
 }
 
@@ -384,6 +406,9 @@ int ensure_eval_buffers(pmnb_ctx *ctx) {
     HIPCHK(ctx->d_shell.ensure(ctx->shell.size() * sizeof(int4)));
     HIPCHK(hipMemcpy(ctx->d_shell.p, ctx->shell.data(),
                      ctx->shell.size() * sizeof(int4), hipMemcpyHostToDevice));
+    HIPCHK(ctx->d_box.ensure(sizeof(Box)));
+    HIPCHK(hipMemcpy(ctx->d_box.p, &ctx->box, sizeof(Box),
+                     hipMemcpyHostToDevice));
     HIPCHK(ctx->d_shell_off.ensure(ctx->shell_off.size() * sizeof(float4)));
     HIPCHK(hipMemcpy(ctx->d_shell_off.p, ctx->shell_off.data(),
                      ctx->shell_off.size() * sizeof(float4),
@@ -465,6 +490,7 @@ Sorted make_sorted(const pmnb_ctx *ctx, const double *d_com) {
   S.lj = ctx->d_lj.as<double>();
   S.ljf = ctx->d_ljf.as<float4>();
   S.com = d_com;
+  S.box = ctx->d_box.as<Box>();
   return S;
 }
 
@@ -978,7 +1004,10 @@ void pmnb_destroy(pmnb_ctx *ctx) {
                     &ctx->d_sx, &ctx->d_sy, &ctx->d_sz, &ctx->d_loc,
                     &ctx->d_stype, &ctx->d_sq, &ctx->d_smol, &ctx->d_scell,
                     &ctx->d_shell, &ctx->d_partial, &ctx->d_npairs,
-                    &ctx->d_pairs};
+                    &ctx->d_pairs, &ctx->d_list, &ctx->d_ncount,
+                    &ctx->d_partial2, &ctx->d_shell_off, &ctx->d_box,
+                    &ctx->d_wcount, &ctx->d_woff, &ctx->d_items,
+                    &ctx->d_nitems};
   for (DevBuf *b : bufs) b->release();
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);

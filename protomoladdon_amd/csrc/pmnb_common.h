@@ -52,6 +52,15 @@ struct Switch {
   double c1_15c1, c1_05c3, c1_15c3;
 };
 
+// FP32 copies of the constants the mixed-precision kernels use (computed
+// on the host so they arrive in scalar registers).
+struct F32Params {
+  float cut2, band, Lx, Ly, Lz, iLx, iLy, iLz;
+  float on2l, cut2l, s1l, s2l, s3l, on2e, cut2e, s1e, s2e, s3e;
+  float alpha, tapi, csf;
+  int same_switch;  // LJ and electrostatic switches are the same C2
+};
+
 struct ForceParams {
   Box box;
   Grid grid;
@@ -65,6 +74,7 @@ struct ForceParams {
   int per_pair_wrap;     // mixed path: grid too small for cell-delta images
   uint32_t flags;
   float band;            // |d2_f32 - cut2| below which FP64 re-tests
+  F32Params f;
 };
 
 // ---------------------------------------------------------------- FP64 exact

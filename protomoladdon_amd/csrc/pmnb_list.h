@@ -118,7 +118,7 @@ k_force_list_mixed(ForceParams P, Sorted S, const unsigned *__restrict__ list,
   for (int k = 0; k < ACC_N; k++) acc[k] = 0.0;
   const int m = s < n ? count[s] : 0;
   if (s < n && m <= cap) {
-    const F32Params F = f32_params(P);
+    const F32Params &F = P.f;
     const float4 li = S.loc[s];
     const int a = S.atom[s], ti = S.type[s], mi = S.mol[s];
     const float4 *ljrow = S.ljf + ti * P.n_types;

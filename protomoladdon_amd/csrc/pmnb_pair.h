@@ -31,6 +31,7 @@ struct Sorted {
   const double *lj;              // T*T*4 {A, B, A14, B14}
   const float4 *ljf;             // same in FP32
   const double *com;             // molecule centres (molecular virial)
+  const Box *box;                // device copy of the box (read on demand)
 };
 
 // Wave-then-block reduction of ACC_N doubles in a fixed order; block b
@@ -205,39 +206,6 @@ __device__ __forceinline__ long long to_fixed(double v) {
   return (long long)__builtin_rint(v * kFixedScale);
 }
 
-// FP32 constants of the two functors and their switches.
-struct F32Params {
-  float cut2, band, Lx, Ly, Lz, iLx, iLy, iLz;
-  float on2l, cut2l, s1l, s2l, s3l, on2e, cut2e, s1e, s2e, s3e;
-  float alpha, tapi, csf;
-};
-
-__device__ __forceinline__ F32Params f32_params(const ForceParams &P) {
-  F32Params F;
-  F.cut2 = (float)P.cut2;
-  F.band = P.band;
-  F.Lx = (float)P.box.e[0];
-  F.Ly = (float)P.box.e[1];
-  F.Lz = (float)P.box.e[2];
-  F.iLx = (float)P.box.er[0];
-  F.iLy = (float)P.box.er[1];
-  F.iLz = (float)P.box.er[2];
-  F.on2l = (float)P.sw_lj.on2;
-  F.cut2l = (float)P.sw_lj.cut2;
-  F.s1l = (float)P.sw_lj.s1;
-  F.s2l = (float)P.sw_lj.s2;
-  F.s3l = (float)P.sw_lj.s3;
-  F.on2e = (float)P.sw_el.on2;
-  F.cut2e = (float)P.sw_el.cut2;
-  F.s1e = (float)P.sw_el.s1;
-  F.s2e = (float)P.sw_el.s2;
-  F.s3e = (float)P.sw_el.s3;
-  F.alpha = (float)P.alpha;
-  F.tapi = (float)P.two_alpha_pi;
-  F.csf = (float)P.csf;
-  return F;
-}
-
 // FP32 pair of OneAtomPairTwo: returns the switched F/r (ff) and the two
 // switched energies.  A, B are already the MODIFIED (A14, B14) values when
 // excl == 1; qq is the raw charge product.
@@ -287,7 +255,10 @@ __device__ __forceinline__ void pair_f32(const F32Params &F, float r2,
   }
   f1 = fmaf(f1, sv, -e1 * sd);
   e1 = e1 * sv;
-  if (ELEC != 0) {
+  if (ELEC != 0 && F.same_switch) {
+    f2 = fmaf(f2, sv, -e2 * sd);
+    e2 = e2 * sv;
+  } else if (ELEC != 0) {
     float se = 1.f, sde = 0.f;
     if (r2 >= F.on2e) {
       const float c2 = F.cut2e - r2;
@@ -321,7 +292,7 @@ k_pair_mixed(ForceParams P, Sorted S, double *__restrict__ forces,
   unsigned cnt = 0;
   const bool run = s < P.n_atoms && (!only || only[s] > cap);
   if (run) {
-    const F32Params F = f32_params(P);
+    const F32Params &F = P.f;
     const float4 li = S.loc[s];
     const int a = S.atom[s], ti = S.type[s], mi = S.mol[s];
     const int c = S.cell[s];

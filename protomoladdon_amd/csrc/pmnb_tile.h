@@ -84,6 +84,46 @@ __device__ __forceinline__ int lane_prefix(unsigned long long mask) {
                                    __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
 }
 
+// Tests buffered records split, split + 4, ... (< n) against one atom.  The
+// result bits are shifted in, so record split + 4 j sits at bit
+// (c0 - 1 - j) of lo for j < c0 = min(32, jmax) and at bit
+// (c1 - 1 - (j - 32)) of hi for the rest (c1 = jmax - c0).  Records n ..
+// 4 jmax - 1 hold a far sentinel; the atom itself passes and is skipped by
+// the pair code.  WRAPZ: minimum image along z.
+template <bool WRAPZ>
+__device__ __forceinline__ void tile_test(const float4 *__restrict__ rec,
+                                          int n, int split, float xi,
+                                          float yi, float zi, float hi,
+                                          float Lz, float iLz, unsigned &lo,
+                                          unsigned &hi_bits) {
+  const int jmax = (n + kTileSplit - 1) / kTileSplit;  // wave-uniform
+  const int c0 = min(jmax, 32);
+  unsigned m = 0u;
+#pragma unroll 4
+  for (int j = 0; j < c0; j++) {
+    const int k = split + kTileSplit * j;
+    const float4 r = rec[k];
+    const float dx = r.x - xi, dy = r.y - yi;
+    float dz = r.z - zi;
+    if (WRAPZ) dz = fmaf(-Lz, rintf(dz * iLz), dz);
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    m = (m << 1) + (unsigned)(r2 <= hi);
+  }
+  lo = m;
+  m = 0u;
+#pragma unroll 4
+  for (int j = 32; j < jmax; j++) {
+    const int k = split + kTileSplit * j;
+    const float4 r = rec[k];
+    const float dx = r.x - xi, dy = r.y - yi;
+    float dz = r.z - zi;
+    if (WRAPZ) dz = fmaf(-Lz, rintf(dz * iLz), dz);
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    m = (m << 1) + (unsigned)(r2 <= hi);
+  }
+  hi_bits = m;
+}
+
 template <bool F64, int ELEC, int ELSW, int BUF>
 __global__ void __launch_bounds__(kBlock)
 k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
@@ -107,7 +147,7 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
     const int i = a0 + slot;
     const bool active = i < a1;
     const int ii = active ? i : a0;
-    const F32Params F = f32_params(P);
+    const F32Params &F = P.f;
     const float hi = F.cut2 + F.band, lo = F.cut2 - F.band;
     const int zlo = S.cell[a0] % G.nz, zhi = S.cell[a1 - 1] % G.nz;
     const float4 li = S.loc[ii];
@@ -135,104 +175,133 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
     const bool vir = (P.flags & 3u) != 0;
     int nbuf = 0;
 
-    // Tests the buffered candidates against this lane's atom, then
-    // evaluates the pairs that pass.
-    auto process = [&](int n) {
-      wave_sync_lds();
-      unsigned long long mask = 0ull;
-      if (active) {
-        const int nj = (n - split + kTileSplit - 1) / kTileSplit;
-        for (int j = 0; j < nj; j++) {
-          const float4 r = recf[wave][split + kTileSplit * j];
-          const float dx = r.x - xi, dy = r.y - yi;
-          float dz = r.z - zi;
-          if (clampz) dz = fmaf(-Lz, rintf(dz * iLz), dz);
-          const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-          if (r2 <= hi) mask |= 1ull << j;
-        }
+    // FP32 partials of one buffer (flushed by process)
+    float px = 0.f, py = 0.f, pz = 0.f, pelj = 0.f, peel = 0.f;
+    float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
+
+    // One pair: candidate record k of this wave's buffer.
+    auto eval = [&](const float4 &r, const int4 &q4, int k) {
+      const int t = q4.x;
+      if (t == i) return;
+      float dx = r.x - xi, dy = r.y - yi, dz = r.z - zi;
+      if (clampz) dz = fmaf(-Lz, rintf(dz * iLz), dz);
+      const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+      double ex = 0.0, ey = 0.0, ez = 0.0, e2d = 0.0;
+      if (F64) {
+        const double4 rd = recd[wave][k];
+        min_diff_exact(*S.box, xd, yd, zd, rd.x, rd.y, rd.z, ex, ey, ez, e2d);
+        if (e2d > P.cut2) return;
+      } else if (r2 >= lo) {
+        // near the cutoff: decide with the reference's FP64 minimum image
+        min_diff_exact(*S.box, xd, yd, zd, S.x[t], S.y[t], S.z[t], ex, ey, ez,
+                       e2d);
+        if (e2d > P.cut2) return;
       }
-      float px = 0.f, py = 0.f, pz = 0.f, pelj = 0.f, peel = 0.f;
-      float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
-      while (mask) {
-        const int j = __builtin_ctzll(mask);
-        mask &= mask - 1ull;
-        const int k = split + kTileSplit * j;
-        const int4 q4 = reci[wave][k];
-        const int t = q4.x;
-        if (t == i) continue;
-        const float4 r = recf[wave][k];
-        float dx = r.x - xi, dy = r.y - yi, dz = r.z - zi;
-        if (clampz) dz = fmaf(-Lz, rintf(dz * iLz), dz);
-        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        double ex = 0.0, ey = 0.0, ez = 0.0, e2d = 0.0;
-        if (F64) {
-          const double4 rd = recd[wave][k];
-          min_diff_exact(P.box, xd, yd, zd, rd.x, rd.y, rd.z, ex, ey, ez,
-                         e2d);
-          if (e2d > P.cut2) continue;
-        } else if (r2 >= lo) {
-          min_diff_exact(P.box, xd, yd, zd, S.x[t], S.y[t], S.z[t], ex, ey,
-                         ez, e2d);
-          if (e2d > P.cut2) continue;
+      const bool same = q4.z == mi;
+      const int excl =
+          same ? excl_class(S.xfull, S.xmod, P.max_excl_delta, ai, q4.w) : 0;
+      if (excl == 2) return;
+      if (i < t) cnt++;
+      const bool molv = (P.flags & 2u) && !same;
+      if (F64) {
+        const double *lp = ljrow64 + 4 * q4.y;
+        double ff, e1, e2;
+        pair_f64<ELEC, ELSW>(P, e2d, excl != 1 ? lp[0] : lp[2],
+                             excl != 1 ? lp[1] : lp[3], qi * recd[wave][k].w,
+                             excl, ff, e1, e2);
+        acc[ACC_ELJ] += 0.5 * e1;
+        acc[ACC_EEL] += 0.5 * e2;
+        const double gx = ex * ff, gy = ey * ff, gz = ez * ff;
+        fx64 -= gx;
+        fy64 -= gy;
+        fz64 -= gz;
+        if (molv || (P.flags & 1u))
+          add_virial(acc, ACC_VXX, gx, gy, gz, ex, ey, ez, 0.5);
+        if (molv) {
+          double cdx, cdy, cdz, cd2;
+          min_diff_exact(*S.box, S.com[3 * mi], S.com[3 * mi + 1],
+                         S.com[3 * mi + 2], S.com[3 * q4.z],
+                         S.com[3 * q4.z + 1], S.com[3 * q4.z + 2], cdx, cdy,
+                         cdz, cd2);
+          add_virial(acc, ACC_MXX, gx, gy, gz, cdx, cdy, cdz, 0.5);
         }
-        const bool same = q4.z == mi;
-        const int excl =
-            same ? excl_class(S.xfull, S.xmod, P.max_excl_delta, ai, q4.w) : 0;
-        if (excl == 2) continue;
-        if (i < t) cnt++;
-        const bool molv = (P.flags & 2u) && !same;
-        if (F64) {
-          const double *lp = ljrow64 + 4 * q4.y;
-          double ff, e1, e2;
-          pair_f64<ELEC, ELSW>(P, e2d, excl != 1 ? lp[0] : lp[2],
-                               excl != 1 ? lp[1] : lp[3], qi * recd[wave][k].w,
-                               excl, ff, e1, e2);
-          acc[ACC_ELJ] += 0.5 * e1;
-          acc[ACC_EEL] += 0.5 * e2;
-          const double gx = ex * ff, gy = ey * ff, gz = ez * ff;
-          fx64 -= gx;
-          fy64 -= gy;
-          fz64 -= gz;
-          if (molv || (P.flags & 1u))
-            add_virial(acc, ACC_VXX, gx, gy, gz, ex, ey, ez, 0.5);
+      } else {
+        const float4 lp = ljrow[q4.y];
+        float ff, e1, e2;
+        pair_f32<ELEC>(F, r2, excl != 1 ? lp.x : lp.z, excl != 1 ? lp.y : lp.w,
+                       li.w * r.w, excl, ff, e1, e2);
+        pelj += e1;
+        peel += e2;
+        const float gx = dx * ff, gy = dy * ff, gz = dz * ff;
+        px -= gx;
+        py -= gy;
+        pz -= gz;
+        if (vir) {
+          if (molv || (P.flags & 1u)) {
+            v0 = fmaf(gx, dx, v0);
+            v1 = fmaf(gx, dy, v1);
+            v2 = fmaf(gx, dz, v2);
+            v3 = fmaf(gy, dy, v3);
+            v4 = fmaf(gy, dz, v4);
+            v5 = fmaf(gz, dz, v5);
+          }
           if (molv) {
             double cdx, cdy, cdz, cd2;
-            min_diff_exact(P.box, S.com[3 * mi], S.com[3 * mi + 1],
+            min_diff_exact(*S.box, S.com[3 * mi], S.com[3 * mi + 1],
                            S.com[3 * mi + 2], S.com[3 * q4.z],
                            S.com[3 * q4.z + 1], S.com[3 * q4.z + 2], cdx, cdy,
                            cdz, cd2);
             add_virial(acc, ACC_MXX, gx, gy, gz, cdx, cdy, cdz, 0.5);
           }
-        } else {
-          const float4 lp = ljrow[q4.y];
-          float ff, e1, e2;
-          pair_f32<ELEC>(F, r2, excl != 1 ? lp.x : lp.z,
-                         excl != 1 ? lp.y : lp.w, li.w * r.w, excl, ff, e1,
-                         e2);
-          pelj += e1;
-          peel += e2;
-          const float gx = dx * ff, gy = dy * ff, gz = dz * ff;
-          px -= gx;
-          py -= gy;
-          pz -= gz;
-          if (vir) {
-            if (molv || (P.flags & 1u)) {
-              v0 = fmaf(gx, dx, v0);
-              v1 = fmaf(gx, dy, v1);
-              v2 = fmaf(gx, dz, v2);
-              v3 = fmaf(gy, dy, v3);
-              v4 = fmaf(gy, dz, v4);
-              v5 = fmaf(gz, dz, v5);
-            }
-            if (molv) {
-              double cdx, cdy, cdz, cd2;
-              min_diff_exact(P.box, S.com[3 * mi], S.com[3 * mi + 1],
-                             S.com[3 * mi + 2], S.com[3 * q4.z],
-                             S.com[3 * q4.z + 1], S.com[3 * q4.z + 2], cdx,
-                             cdy, cdz, cd2);
-              add_virial(acc, ACC_MXX, gx, gy, gz, cdx, cdy, cdz, 0.5);
-            }
-          }
+        }
+      }
+    };
+
+    // Tests the buffered candidates against this lane's atom (the atom
+    // itself has r2 == 0 and is left out), then evaluates the pairs that
+    // pass; the next pair's records are read from LDS before the current
+    // pair's arithmetic.
+    auto process = [&](int n) {
+      // records n .. n + 3 (the last group of 4) are made far away
+      if (lane < kTileSplit && n + lane < BUF)
+        recf[wave][n + lane] = make_float4(1e18f, 1e18f, 1e18f, 0.f);
+      wave_sync_lds();
+      unsigned m0 = 0u, m1 = 0u;
+      if (clampz)
+        tile_test<true>(recf[wave], n, split, xi, yi, zi, hi, Lz, iLz, m0, m1);
+      else
+        tile_test<false>(recf[wave], n, split, xi, yi, zi, hi, Lz, iLz, m0, m1);
+      if (!active) m0 = m1 = 0u;
+      const int jmax = (n + kTileSplit - 1) / kTileSplit;
+      const int c0 = min(jmax, 32), c1 = jmax - c0;
+      // record index of the lowest set bit (and clear it)
+      auto pop = [&]() -> int {
+        if (m0) {
+          const int b = __builtin_ctz(m0);
+          m0 &= m0 - 1u;
+          return split + kTileSplit * (c0 - 1 - b);
+        }
+        const int b = __builtin_ctz(m1);
+        m1 &= m1 - 1u;
+        return split + kTileSplit * (32 + c1 - 1 - b);
+      };
+      px = py = pz = pelj = peel = 0.f;
+      v0 = v1 = v2 = v3 = v4 = v5 = 0.f;
+      if (m0 | m1) {
+        int k = pop();
+        float4 r = recf[wave][k];
+        int4 q4 = reci[wave][k];
+        while (true) {
+          const bool more = (m0 | m1) != 0u;
+          int kn = k;
+          if (more) kn = pop();
+          const float4 rn = recf[wave][kn];
+          const int4 q4n = reci[wave][kn];
+          eval(r, q4, k);
+          if (!more) break;
+          r = rn;
+          q4 = q4n;
+          k = kn;
         }
       }
       if (!F64) {
