@@ -167,7 +167,7 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "k_tile<mixed, Ewald-real, C2>",
+                     "kernel": "k_tq<mixed, Ewald-real, C2>",
                      "kernel_ms": mx["kernel_ms"],
                      "list_build_ms": mx["build_ms"],
                      "achieved_incl_list_build": P * FLOP_PER_PAIR / (

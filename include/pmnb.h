@@ -167,6 +167,108 @@ void pmnb_destroy(pmnb_ctx *ctx);
 /* ABI version (major * 100 + minor). */
 int pmnb_version(void);
 
+/* ------------------------------------------------------------ derivatives
+   Force matching differentiates the force itself.  No counterpart in the
+   reference's force interface; the per-pair second derivatives it follows
+   are ReducedHessLennardJones (force/hessian/ReducedHessLennardJones.cpp:
+   7-42), ReducedHessCoulomb (force/hessian/ReducedHessCoulomb.cpp:6-35) and
+   C2SwitchingFunction::hessian (switch/C2SwitchingFunction.cpp:43-52), as
+   Hessian::evaluatePairsMatrix (integrator/hessian/Hessian.cpp:581-717)
+   combines them. */
+
+/* Replaces the LJ table (d_lj: n_types*n_types*4 doubles {A, B, A14, B14},
+   symmetric) and/or the scaled charges (d_scaled_charge: n_atoms doubles)
+   from device arrays, enqueued on `stream`; either may be NULL (kept).  The
+   rest of the topology stays as uploaded. */
+int pmnb_set_parameters_device(pmnb_ctx *ctx, const double *d_lj,
+                               const double *d_scaled_charge, void *stream);
+
+/* Adjoint of the forces of pmnb_evaluate_device: with d_grad_forces = dL/dF
+   (3 doubles per atom), accumulates (+=)
+     d_grad_positions (3 per atom): dL/dx = -H dL/dF, a Hessian-vector
+       product with the Hessian of the pair energies (for the Ewald term,
+       the Jacobian of the force expression the force computes);
+     d_grad_charges (1 per atom): dL/d(scaled charge);
+     d_grad_lj (n_types*n_types*4): dL/d{A, B, A14, B14}, symmetrised.
+   Any output may be NULL.  The pairs are recomputed from the cell list, in
+   the context's precision.  Needs C2 switching, at most 4 LJ types and the
+   tile path (algorithm cutoff >= switching cutoff). */
+int pmnb_adjoint_device(pmnb_ctx *ctx, const double *d_positions,
+                        const double *d_grad_forces, double *d_grad_positions,
+                        double *d_grad_charges, double *d_grad_lj,
+                        void *stream);
+
+/* ------------------------------------------------------------------ batch
+   Batched replicas.  No counterpart in the reference: one ProtoMol process
+   evaluates one system through NonbondedCutoffSystemForce::evaluate
+   (force/nonbonded/NonbondedCutoffSystemForce.h:45-55).  A batch holds many
+   independent periodic systems that share one force (params, LJ table,
+   coulomb scaling factor) and evaluates them all with one launch of each
+   kernel; replica r's result is what pmnb_evaluate_device gives for that
+   replica alone.  Replicas use the tile path, which needs the algorithm
+   cutoff >= the switching cutoff and at most 8 LJ types. */
+typedef struct pmnb_batch pmnb_batch;
+
+/* Creates an empty batch; lj and coulomb_scaling_factor as in
+   pmnb_set_topology. */
+int pmnb_batch_create(const pmnb_params *params, int32_t n_types,
+                      const double *lj, double coulomb_scaling_factor,
+                      pmnb_batch **out);
+
+/* Appends a replica: per-atom arrays and exclusions as in
+   pmnb_set_topology (indices local to the replica), its box as in
+   pmnb_set_box.  In the batch's position and force arrays its atoms follow
+   those of the replicas added before it.  *replica receives its index. */
+int pmnb_batch_add(pmnb_batch *b, int32_t n_atoms, const int32_t *type,
+                   const double *scaled_charge, const int32_t *molecule,
+                   int32_t n_excl, const int32_t *excl, const double e1[3],
+                   const double e2[3], const double e3[3],
+                   const double origin[3], int32_t *replica);
+
+/* Replicas and atoms added so far. */
+int pmnb_batch_size(const pmnb_batch *b, int32_t *n_replicas,
+                    int64_t *n_atoms);
+
+/* One evaluation of every replica on device arrays, enqueued on `stream`:
+   d_positions / d_forces hold 3 doubles per atom of the batch, d_out 20
+   doubles per replica in the layout of pmnb_evaluate_device.  Accumulates.
+   PMNB_FLAG_MOLVIRIAL is not supported. */
+int pmnb_batch_evaluate_device(pmnb_batch *b, const double *d_positions,
+                               double *d_forces, double *d_out,
+                               uint32_t flags, void *stream);
+
+/* n_steps evaluations as replays of one captured HIP graph: each replay
+   zeroes d_forces and d_out, then evaluates every replica.  The graph is
+   captured on the first call and again whenever the pointers, flags or
+   kernel shape change; nothing is allocated during capture.  Enqueued on
+   `stream` without host synchronisation. */
+int pmnb_batch_run_graph(pmnb_batch *b, const double *d_positions,
+                         double *d_forces, double *d_out, uint32_t flags,
+                         int32_t n_steps, void *stream);
+
+/* Autotuner: times every pair-kernel shape (waves per block, buffered
+   candidates, force slots) and persistent grid size on these positions
+   and keeps the fastest for later evaluations; returns the shape index,
+   the grid cap in blocks (0 = one block per work group of items) and its
+   time per evaluation in ms (any pointer may be NULL).  The outputs are
+   overwritten with accumulated sums. */
+int pmnb_batch_autotune(pmnb_batch *b, const double *d_positions,
+                        double *d_forces, double *d_out, uint32_t flags,
+                        int32_t *shape, int32_t *grid_blocks, double *ms);
+
+/* Sets the pair-kernel shape (0 .. 4) and grid cap by hand. */
+int pmnb_batch_set_shape(pmnb_batch *b, int32_t shape, int32_t grid_blocks);
+
+/* In-cutoff pairs of all replicas in the last evaluation; waits for the
+   device. */
+int pmnb_batch_pair_count(pmnb_batch *b, int64_t *n_pairs);
+
+/* Text of the last error on this batch ("" if none). */
+const char *pmnb_batch_last_error(const pmnb_batch *b);
+
+/* Frees the batch and its device memory. */
+void pmnb_batch_destroy(pmnb_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

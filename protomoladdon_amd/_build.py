@@ -10,7 +10,8 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG_DIR, "csrc")
 LIB_PATH = os.path.join(PKG_DIR, "libpmnb.so")
 SOURCES = ["pmnb.hip"]
-HEADERS = ["pmnb_common.h", "pmnb_cells.h", "pmnb_pair.h"]
+HEADERS = ["pmnb_common.h", "pmnb_cells.h", "pmnb_pair.h", "pmnb_list.h",
+           "pmnb_tile.h", "pmnb_queue.h", "pmnb_adjoint.h"]
 
 # -ffp-contract=off: FP64 expressions round exactly as the reference's x86-64
 # build (no FMA); FP32 kernels ask for fused multiply-adds explicitly.

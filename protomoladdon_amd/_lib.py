@@ -32,6 +32,11 @@ EXPORTED = (
     "pmnb_evaluate_device", "pmnb_pairset", "pmnb_kernel_timing",
     "pmnb_last_pair_count",
     "pmnb_last_error", "pmnb_destroy", "pmnb_version",
+    "pmnb_batch_create", "pmnb_batch_add", "pmnb_batch_size",
+    "pmnb_batch_evaluate_device", "pmnb_batch_run_graph",
+    "pmnb_batch_autotune", "pmnb_batch_set_shape", "pmnb_batch_pair_count",
+    "pmnb_batch_last_error", "pmnb_batch_destroy",
+    "pmnb_set_parameters_device", "pmnb_adjoint_device",
 )
 
 
@@ -100,6 +105,36 @@ def load():
     lib.pmnb_destroy.restype = None
     lib.pmnb_version.argtypes = []
     lib.pmnb_version.restype = ctypes.c_int
+    lib.pmnb_batch_create.argtypes = [ctypes.POINTER(PmnbParams), _i32, _p,
+                                      _dbl, ctypes.POINTER(_p)]
+    lib.pmnb_batch_create.restype = ctypes.c_int
+    lib.pmnb_batch_add.argtypes = [_p, _i32, _p, _p, _p, _i32, _p, _p, _p, _p,
+                                   _p, ctypes.POINTER(_i32)]
+    lib.pmnb_batch_add.restype = ctypes.c_int
+    lib.pmnb_batch_size.argtypes = [_p, ctypes.POINTER(_i32),
+                                    ctypes.POINTER(_i64)]
+    lib.pmnb_batch_size.restype = ctypes.c_int
+    lib.pmnb_batch_evaluate_device.argtypes = [_p, _p, _p, _p, _u32, _p]
+    lib.pmnb_batch_evaluate_device.restype = ctypes.c_int
+    lib.pmnb_batch_run_graph.argtypes = [_p, _p, _p, _p, _u32, _i32, _p]
+    lib.pmnb_batch_run_graph.restype = ctypes.c_int
+    lib.pmnb_batch_autotune.argtypes = [_p, _p, _p, _p, _u32,
+                                        ctypes.POINTER(_i32),
+                                        ctypes.POINTER(_i32),
+                                        ctypes.POINTER(_dbl)]
+    lib.pmnb_batch_autotune.restype = ctypes.c_int
+    lib.pmnb_batch_set_shape.argtypes = [_p, _i32, _i32]
+    lib.pmnb_batch_set_shape.restype = ctypes.c_int
+    lib.pmnb_batch_pair_count.argtypes = [_p, ctypes.POINTER(_i64)]
+    lib.pmnb_batch_pair_count.restype = ctypes.c_int
+    lib.pmnb_batch_last_error.argtypes = [_p]
+    lib.pmnb_batch_last_error.restype = ctypes.c_char_p
+    lib.pmnb_batch_destroy.argtypes = [_p]
+    lib.pmnb_batch_destroy.restype = None
+    lib.pmnb_set_parameters_device.argtypes = [_p, _p, _p, _p]
+    lib.pmnb_set_parameters_device.restype = ctypes.c_int
+    lib.pmnb_adjoint_device.argtypes = [_p, _p, _p, _p, _p, _p, _p]
+    lib.pmnb_adjoint_device.restype = ctypes.c_int
     _LIB = lib
     return lib
 
@@ -107,4 +142,10 @@ def load():
 def check(lib, ctx, code):
     if code != PMNB_OK:
         msg = lib.pmnb_last_error(ctx) if ctx else b""
+        raise PmnbError(code, (msg or b"").decode(errors="replace"))
+
+
+def check_batch(lib, batch, code):
+    if code != PMNB_OK:
+        msg = lib.pmnb_batch_last_error(batch) if batch else b""
         raise PmnbError(code, (msg or b"").decode(errors="replace"))

@@ -21,6 +21,8 @@
 #include "pmnb_pair.h"
 #include "pmnb_list.h"
 #include "pmnb_tile.h"
+#include "pmnb_queue.h"
+#include "pmnb_adjoint.h"
 
 using namespace pmnb;
 
@@ -82,13 +84,44 @@ struct pmnb_ctx {
   bool use_list = false;          // neighbour-list path available
   bool use_tile = false;          // LDS-staged path (own column grid)
   TileGrid tile{};
-  DevBuf d_wcount, d_woff, d_items, d_nitems;
+  // tile path geometry: one replica for a single system, several for a
+  // batch (atoms of replica r then carry r in d_rep_of)
+  std::vector<TileRep> reps;
+  std::vector<int> col_rep;       // replica of each global column
+  int ncol_total = 0;
+  bool batch = false;
+  DevBuf d_rep_of;
+  int rep_uploaded = 0;           // reps and col_rep are on the device
+  DevBuf d_partial_lj;            // adjoint: LJ-parameter partials per item
+  int shape = 0;                  // tile-path kernel shape (mixed)
+  int grid_cap = 0;               // persistent pair grid: max blocks (0: none)
+  DevBuf d_wcount, d_woff, d_items, d_nitems, d_reps, d_col_rep, d_rep_first;
   double box_key[12] = {0};       // e1, e2, e3, origin of the current box
   int cap = 0;                    // list slots per atom
   // pair-kernel timing
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;  // 3 per launch: start, built, done
   size_t ev_used = 0;
+};
+
+struct pmnb_batch {
+  pmnb_ctx *ctx = nullptr;  // the concatenated system, in batch mode
+  int ntypes = 0;
+  std::vector<double> lj;
+  double csf = 1.0;
+  // replicas in order of addition
+  std::vector<int> type, mol, rep_of, excl;
+  std::vector<double> q;
+  std::vector<Box> boxes;
+  int next_mol = 0;
+  bool dirty = true;  // replicas added since the last upload
+  // the captured evaluation and what it was captured for
+  hipStream_t cap_stream = nullptr;  // non-blocking stream for capture
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  const void *gkey[3] = {nullptr, nullptr, nullptr};
+  uint32_t gflags = 0;
+  int gshape = 0, gcap = 0;
 };
 
 namespace {
@@ -160,17 +193,76 @@ ForceParams make_params_for(const pmnb_params &prm, double csf) {
   return P;
 }
 
+// Column grid of the tile path for box b and pair cutoff rcp: x/y columns
+// at least rcp wide, z cells of about rcp / 4.  An axis with fewer than 3
+// columns is marked for per-pair minimum image.  shell receives the distinct
+// cell offsets within reach ((0,0,0) first), which the pair-set extraction
+// walks.  Returns false when the grid is too large.
+bool make_tile_rep(const Box &b, double rcp, TileRep &rep,
+                   std::vector<int4> &shell) {
+  const int tx = std::max(1, (int)std::floor(b.H[0] / rcp));
+  const int ty = std::max(1, (int)std::floor(b.H[1] / rcp));
+  const int tz = std::max(1, (int)std::floor(b.H[2] / (0.25 * rcp)));
+  if ((long long)tx * ty * tz >= (1ll << 26)) return false;
+  rep = TileRep{};
+  const double csz = b.H[2] / tz;
+  const int rz = (int)std::floor(rcp / csz) + 1;
+  Grid &tg = rep.grid;
+  tg.n[0] = tx;
+  tg.n[1] = ty;
+  tg.n[2] = tz;
+  tg.ncell = tx * ty * tz;
+  tg.cs[0] = b.H[0] / tx;
+  tg.cs[1] = b.H[1] / ty;
+  tg.cs[2] = csz;
+  for (int k = 0; k < 3; k++) tg.inv[k] = 1.0 / tg.cs[k];
+  rep.box = b;
+  rep.tg.nx = tx;
+  rep.tg.ny = ty;
+  rep.tg.nz = tz;
+  rep.tg.rz = rz;
+  rep.tg.csx = (float)tg.cs[0];
+  rep.tg.csy = (float)tg.cs[1];
+  rep.tg.csz = (float)tg.cs[2];
+  for (int k = 0; k < 3; k++) {
+    rep.L[k] = (float)b.e[k];
+    rep.iL[k] = (float)b.er[k];
+  }
+  rep.wrap[0] = tx < 3;
+  rep.wrap[1] = ty < 3;
+  rep.ncol = tx * ty;
+  // distinct offsets per axis
+  auto offsets = [](int n, int reach) {
+    std::vector<int> o;
+    if (n >= 2 * reach + 1) {
+      for (int d = -reach; d <= reach; d++) o.push_back(d);
+    } else {
+      for (int d = 0; d < n; d++) o.push_back(d);
+    }
+    return o;
+  };
+  const std::vector<int> ox = offsets(tx, 1), oy = offsets(ty, 1),
+                         oz = offsets(tz, rz);
+  shell.clear();
+  shell.push_back(make_int4(0, 0, 0, 0));
+  for (int dx : ox)
+    for (int dy : oy)
+      for (int dz : oz)
+        if (dx || dy || dz) shell.push_back(make_int4(dx, dy, dz, 0));
+  return true;
+}
+
 // Box + grid + shell offsets exactly as PeriodicBoundaryConditions::set,
 // CubicCellManager::initialize, ArrayCellListStructure::initialize and the
 // periodic enumerator's delta list compute them.
-int build_box(pmnb_ctx *ctx, const double *e1, const double *e2,
-              const double *e3, const double *origin) {
+int make_box(pmnb_ctx *ctx, const double *e1, const double *e2,
+             const double *e3, const double *origin, Box &b) {
   if (e1[1] != 0.0 || e1[2] != 0.0 || e2[0] != 0.0 || e2[2] != 0.0 ||
       e3[0] != 0.0 || e3[1] != 0.0)
     return fail(ctx, PMNB_ERR_UNSUPPORTED, "only orthogonal boxes are built");
   if (!(e1[0] > 0.0) || !(e2[1] > 0.0) || !(e3[2] > 0.0))
     return fail(ctx, PMNB_ERR_ARG, "box vectors must be positive");
-  Box b{};
+  b = Box{};
   const double E[3] = {e1[0], e2[1], e3[2]};
   // e_kr = cross / dot; for a diagonal box this is 1/e_k computed as the
   // reference does: (e2 x e3).x / (e1 . (e2 x e3)).
@@ -196,6 +288,14 @@ int build_box(pmnb_ctx *ctx, const double *e1, const double *e2,
   const double dx = e1[0] * 0.5, dy = e2[1] * 0.5, dz = e3[2] * 0.5;
   const double DX = (dx * dx) * 1.0, DY = (dy * dy) * 1.0, DZ = (dz * dz) * 1.0;
   b.D = std::min(DX, std::min(DY, DZ));
+  return PMNB_OK;
+}
+
+int build_box(pmnb_ctx *ctx, const double *e1, const double *e2,
+              const double *e3, const double *origin) {
+  Box b;
+  const int brc = make_box(ctx, e1, e2, e3, origin, b);
+  if (brc != PMNB_OK) return brc;
 
   const double cs0 = ctx->prm.cell_size;
   Grid g{};
@@ -271,46 +371,27 @@ int build_box(pmnb_ctx *ctx, const double *e1, const double *e2,
   ctx->per_pair_wrap = 0;
   for (int k = 0; k < 3; k++)
     if (g.n[k] < 2 * reach[k] + 1) ctx->per_pair_wrap = 1;
-  // LDS-staged path: its own grid of x/y columns at least one cutoff wide
-  // and z cells of about a quarter cutoff.  Used when the pair set does not
-  // depend on the cell grid (algorithm cutoff >= switching cutoff) and the
-  // box holds at least 3 columns per axis.
+  // Tile path: its own grid of x/y columns at least one cutoff wide and z
+  // cells of about a quarter cutoff.  Used when the pair set does not depend
+  // on the cell grid (algorithm cutoff >= switching cutoff).
   {
     const ForceParams P0 = make_params_for(ctx->prm, ctx->csf);
     const double rcp = std::sqrt(P0.cut2);
-    ctx->use_tile = false;
-    const int tx = (int)std::floor(b.H[0] / rcp), ty = (int)std::floor(b.H[1] / rcp);
-    const int tz = (int)std::floor(b.H[2] / (0.25 * rcp));
-    if (ctx->prm.algorithm_cutoff >= rcp && tx >= 3 && ty >= 3 && tz >= 3) {
-      const double csz = b.H[2] / tz;
-      const int rz = (int)std::floor(rcp / csz) + 1;
-      if (tz >= 2 * rz + 1 && (long long)tx * ty * tz < (1ll << 26)) {
-        Grid tg{};
-        tg.n[0] = tx;
-        tg.n[1] = ty;
-        tg.n[2] = tz;
-        tg.ncell = tx * ty * tz;
-        tg.cs[0] = b.H[0] / tx;
-        tg.cs[1] = b.H[1] / ty;
-        tg.cs[2] = csz;
-        for (int k = 0; k < 3; k++) tg.inv[k] = 1.0 / tg.cs[k];
-        ctx->tile.nx = tx;
-        ctx->tile.ny = ty;
-        ctx->tile.nz = tz;
-        ctx->tile.rz = rz;
-        ctx->tile.csx = (float)tg.cs[0];
-        ctx->tile.csy = (float)tg.cs[1];
-        ctx->tile.csz = (float)tg.cs[2];
-        g = tg;
-        // shell of this grid (pair-set extraction walks it)
-        ctx->shell.clear();
-        ctx->shell.push_back(make_int4(0, 0, 0, 0));
-        for (int dx = -1; dx <= 1; dx++)
-          for (int dy = -1; dy <= 1; dy++)
-            for (int dz = -rz; dz <= rz; dz++)
-              if (dx || dy || dz) ctx->shell.push_back(make_int4(dx, dy, dz, 0));
-        ctx->use_tile = true;
-      }
+    // the pair kernel packs type and sorted index into 32 bits and holds
+    // the LJ table in LDS
+    const bool packs = ctx->ntypes * ctx->ntypes <= kQLj &&
+                       ctx->n < (1 << kQTypeShift);
+    std::vector<int4> tshell;
+    TileRep rep{};
+    ctx->use_tile = ctx->prm.algorithm_cutoff >= rcp && packs &&
+                    make_tile_rep(b, rcp, rep, tshell);
+    if (ctx->use_tile) {
+      ctx->tile = rep.tg;
+      g = rep.grid;
+      ctx->shell = tshell;  // pair-set extraction walks it
+      ctx->reps.assign(1, rep);
+      ctx->col_rep.assign(rep.ncol, 0);
+      ctx->ncol_total = rep.ncol;
     }
   }
   ctx->shell_off.clear();
@@ -322,6 +403,7 @@ int build_box(pmnb_ctx *ctx, const double *e1, const double *e2,
   ctx->box = b;
   ctx->grid = g;
   ctx->shell_uploaded = 0;
+  ctx->rep_uploaded = 0;
   ctx->have_box = true;
   return PMNB_OK;
 }
@@ -397,9 +479,9 @@ int ensure_eval_buffers(pmnb_ctx *ctx) {
   HIPCHK(ctx->d_sq.ensure(N * sizeof(double)));
   HIPCHK(ctx->d_smol.ensure(N * sizeof(int)));
   HIPCHK(ctx->d_scell.ensure(N * sizeof(int)));
-  const int ncol_t = ctx->use_tile ? ctx->tile.nx * ctx->tile.ny : 0;
-  const int tile_blocks = blocks_for(n / kTileAtoms + ncol_t + 1, kTileWaves);
-  HIPCHK(ctx->d_partial.ensure((size_t)std::max(blocks_for(n), tile_blocks) *
+  const int ncol_t = ctx->use_tile ? ctx->ncol_total : 0;
+  const int tile_items = n / kTileAtoms + ncol_t + 1;  // partial row per item
+  HIPCHK(ctx->d_partial.ensure((size_t)std::max(blocks_for(n), tile_items) *
                                ACC_N * sizeof(double)));
   HIPCHK(ctx->d_npairs.ensure(sizeof(unsigned long long)));
   if (!ctx->shell_uploaded) {
@@ -416,12 +498,22 @@ int ensure_eval_buffers(pmnb_ctx *ctx) {
     ctx->shell_uploaded = 1;
   }
   if (ctx->use_tile) {
-    const int ncol = ctx->tile.nx * ctx->tile.ny;
+    const int ncol = ctx->ncol_total, nrep = (int)ctx->reps.size();
     HIPCHK(ctx->d_wcount.ensure((size_t)ncol * sizeof(int)));
     HIPCHK(ctx->d_woff.ensure((size_t)ncol * sizeof(int)));
     HIPCHK(ctx->d_items.ensure((size_t)(n / kTileAtoms + ncol + 1) *
                                sizeof(int4)));
     HIPCHK(ctx->d_nitems.ensure(sizeof(int)));
+    HIPCHK(ctx->d_rep_first.ensure((size_t)(nrep + 1) * sizeof(int)));
+    if (!ctx->rep_uploaded) {
+      HIPCHK(ctx->d_reps.ensure((size_t)nrep * sizeof(TileRep)));
+      HIPCHK(hipMemcpy(ctx->d_reps.p, ctx->reps.data(),
+                       (size_t)nrep * sizeof(TileRep), hipMemcpyHostToDevice));
+      HIPCHK(ctx->d_col_rep.ensure((size_t)ncol * sizeof(int)));
+      HIPCHK(hipMemcpy(ctx->d_col_rep.p, ctx->col_rep.data(),
+                       (size_t)ncol * sizeof(int), hipMemcpyHostToDevice));
+      ctx->rep_uploaded = 1;
+    }
   }
   const bool list_ok = ctx->use_list && n < (1 << kEntryAtomBits);
   if (list_ok) {
@@ -437,13 +529,159 @@ int ensure_eval_buffers(pmnb_ctx *ctx) {
   return PMNB_OK;
 }
 
+// Tile-path kernel shapes the autotuner chooses from (mixed precision):
+// {waves per block, buffered records, force slots per atom, LDS atomics}.
+constexpr int kMixedShapes = 5;
+using MixedShape0 = TqShape<1, 128, 16, false>;
+using MixedShape1 = TqShape<2, 128, 16, false>;
+using MixedShape2 = TqShape<4, 128, 16, false>;
+using MixedShape3 = TqShape<2, 128, 32, false>;
+using MixedShape4 = TqShape<2, 256, 32, false>;
+using F64Shape = TqShape<1, 128, 32, true>;
+
+struct TqArgs {
+  int max_items;   // upper bound of the work items
+  int grid_cap;    // persistent grid: at most this many blocks (0: none)
+  hipStream_t s;
+  const ForceParams *P;
+  const Sorted *S;
+  const TileRep *reps;
+  const int4 *items;
+  const int *nitems;
+  double *forces, *partial;
+  unsigned long long *npairs;
+};
+
+template <bool F64, int EL, int SW, class SHP>
+int launch_tq_shape(const TqArgs &a) {
+  int nb = blocks_for(a.max_items, SHP::NW);
+  if (a.grid_cap > 0) nb = std::min(nb, a.grid_cap);
+  nb = std::max(nb, 1);
+  hipLaunchKernelGGL((k_tq<F64, EL, SW, SHP>), dim3(nb), dim3(64 * SHP::NW), 0,
+                     a.s, *a.P, *a.S, a.reps, a.items, a.nitems, a.forces,
+                     a.partial, a.npairs);
+  return nb;
+}
+
+template <bool F64, int EL, int SW>
+int launch_tq(int shape, const TqArgs &a) {
+  if constexpr (F64) {
+    return launch_tq_shape<true, EL, SW, F64Shape>(a);
+  } else {
+    switch (shape) {
+      case 1: return launch_tq_shape<false, EL, SW, MixedShape1>(a);
+      case 2: return launch_tq_shape<false, EL, SW, MixedShape2>(a);
+      case 3: return launch_tq_shape<false, EL, SW, MixedShape3>(a);
+      case 4: return launch_tq_shape<false, EL, SW, MixedShape4>(a);
+      default: return launch_tq_shape<false, EL, SW, MixedShape0>(a);
+    }
+  }
+}
+
+// Pair kernel of the tile path for the force's precision and functors.
+void launch_pair_tq(bool mixed, int el, int sw, int shape, const TqArgs &a) {
+  if (mixed) {
+    if (el == PMNB_ELEC_NONE) launch_tq<false, 0, 0>(shape, a);
+    else if (el == PMNB_ELEC_COULOMB) launch_tq<false, 1, 0>(shape, a);
+    else launch_tq<false, 2, 0>(shape, a);
+  } else {
+    if (el == PMNB_ELEC_NONE) launch_tq<true, 0, 0>(shape, a);
+    else if (el == PMNB_ELEC_COULOMB && sw == PMNB_SWITCH_C2)
+      launch_tq<true, 1, 0>(shape, a);
+    else if (el == PMNB_ELEC_COULOMB) launch_tq<true, 1, 1>(shape, a);
+    else if (sw == PMNB_SWITCH_C2) launch_tq<true, 2, 0>(shape, a);
+    else launch_tq<true, 2, 1>(shape, a);
+  }
+}
+
+// Work items of the tile path from the sorted cells (one or many replicas).
+void launch_items(const TileRep *reps, const int *col_rep, const int *cstart,
+                  const int *ccount, int ncol, int nrep, int *wc, int *wo,
+                  int4 *items, int *nitems, int *rep_first, hipStream_t s) {
+  hipLaunchKernelGGL(k_col_waves_r, dim3(blocks_for(ncol)), dim3(kBlock), 0, s,
+                     reps, col_rep, cstart, ccount, ncol, wc);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, wc, wo, ncol);
+  hipLaunchKernelGGL(k_col_items_r, dim3(blocks_for(ncol)), dim3(kBlock), 0, s,
+                     reps, col_rep, cstart, ccount, wc, wo, ncol, nrep, items,
+                     nitems, rep_first);
+}
+
 // Cell-list build on the device from d_pos (original order).
+// FP32 copy of the LJ table (T*T records {A, B, A14, B14}).
+__global__ void k_lj_to_f32(const double *__restrict__ lj, int n,
+                            float4 *__restrict__ ljf) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n)
+    ljf[k] = make_float4((float)lj[4 * k], (float)lj[4 * k + 1],
+                         (float)lj[4 * k + 2], (float)lj[4 * k + 3]);
+}
+
+template <typename R>
+AdjC<R> make_adj_consts(const ForceParams &P) {
+  AdjC<R> C;
+  C.on2l = (R)P.sw_lj.on2;
+  C.cut2l = (R)P.sw_lj.cut2;
+  C.s1l = (R)P.sw_lj.s1;
+  C.s2l = (R)P.sw_lj.s2;
+  C.s3l = (R)P.sw_lj.s3;
+  C.on2e = (R)P.sw_el.on2;
+  C.cut2e = (R)P.sw_el.cut2;
+  C.s1e = (R)P.sw_el.s1;
+  C.s2e = (R)P.sw_el.s2;
+  C.s3e = (R)P.sw_el.s3;
+  C.alpha = (R)P.alpha;
+  C.c2ap = (R)P.two_alpha_pi;
+  C.csf = (R)P.csf;
+  return C;
+}
+
+template <bool F64, int EL, int MK>
+void launch_adj(int max_items, hipStream_t s, const ForceParams &P,
+                const Sorted &S, const TileRep *reps, const int4 *items,
+                const int *nitems, const double *gf, double *gx, double *gq,
+                double *plj, unsigned long long *np) {
+  constexpr int NW = 1, BUF = 128;
+  const int nb = std::max(1, blocks_for(max_items, NW));
+  hipLaunchKernelGGL((k_tq_adj<F64, EL, NW, BUF, MK>), dim3(nb),
+                     dim3(64 * NW), 0, s, P, make_adj_consts<float>(P),
+                     make_adj_consts<double>(P), S, reps, items, nitems, gf,
+                     gx, gq, plj, np);
+}
+
+template <bool F64, int EL>
+void launch_adj_t(int T, int max_items, hipStream_t s, const ForceParams &P,
+                  const Sorted &S, const TileRep *reps, const int4 *items,
+                  const int *nitems, const double *gf, double *gx, double *gq,
+                  double *plj, double *glj, unsigned long long *np) {
+  if (T <= 2) {
+    launch_adj<F64, EL, 3>(max_items, s, P, S, reps, items, nitems, gf, gx,
+                           gq, plj, np);
+    if (glj)
+      hipLaunchKernelGGL(k_adj_lj_finish<3>, dim3(1), dim3(256), 0, s, plj,
+                         nitems, T, glj);
+  } else {
+    launch_adj<F64, EL, 10>(max_items, s, P, S, reps, items, nitems, gf, gx,
+                            gq, plj, np);
+    if (glj)
+      hipLaunchKernelGGL(k_adj_lj_finish<10>, dim3(1), dim3(256), 0, s, plj,
+                         nitems, T, glj);
+  }
+}
+
+// On the tile path every replica's cells are numbered from its offset
+// (ctx->grid.ncell is then the total).
 int build_cells(pmnb_ctx *ctx, const double *d_pos, hipStream_t s) {
   const int n = ctx->n, nc = ctx->grid.ncell;
+  const TileRep *reps = ctx->d_reps.as<TileRep>();
+  const int *rep_of = ctx->batch ? ctx->d_rep_of.as<int>() : nullptr;
   hipLaunchKernelGGL(k_clear, dim3(blocks_for(nc)), dim3(kBlock), 0, s,
                      ctx->d_count.as<int>(), ctx->d_fill.as<int>(), nc,
                      ctx->d_npairs.as<unsigned long long>());
-  if (n > 0) {
+  if (n > 0 && ctx->use_tile) {
+    hipLaunchKernelGGL(k_assign_r<TileRep>, dim3(blocks_for(n)), dim3(kBlock),
+                       0, s, reps, rep_of, d_pos, n, ctx->d_cell_of.as<int>(),
+                       ctx->d_count.as<int>());
+  } else if (n > 0) {
     hipLaunchKernelGGL(k_assign, dim3(blocks_for(n)), dim3(kBlock), 0, s,
                        ctx->box, ctx->grid, d_pos, n, ctx->d_cell_of.as<int>(),
                        ctx->d_count.as<int>());
@@ -458,6 +696,17 @@ int build_cells(pmnb_ctx *ctx, const double *d_pos, hipStream_t s) {
                        dim3(kBlock), 0, s,
                        ctx->d_start.as<int>(), ctx->d_count.as<int>(), nc,
                        ctx->d_sorted.as<int>());
+    if (ctx->use_tile)
+      hipLaunchKernelGGL(k_gather_r<TileRep>, dim3(blocks_for(n)),
+                         dim3(kBlock), 0, s, reps, rep_of, d_pos,
+                         ctx->d_sorted.as<int>(), ctx->d_cell_of.as<int>(),
+                         ctx->d_type.as<int>(), ctx->d_q.as<double>(),
+                         ctx->d_mol.as<int>(), n, ctx->d_sx.as<double>(),
+                         ctx->d_sy.as<double>(), ctx->d_sz.as<double>(),
+                         ctx->d_loc.as<float4>(), ctx->d_stype.as<int>(),
+                         ctx->d_sq.as<double>(), ctx->d_smol.as<int>(),
+                         ctx->d_scell.as<int>());
+    else
     hipLaunchKernelGGL(k_gather, dim3(blocks_for(n)), dim3(kBlock), 0, s,
                        ctx->box, ctx->grid, d_pos, ctx->d_sorted.as<int>(),
                        ctx->d_cell_of.as<int>(), ctx->d_type.as<int>(),
@@ -537,51 +786,23 @@ int run_device(pmnb_ctx *ctx, const double *d_pos, double *d_forces,
     ctx->ev_used += 3;
   }
   if (ctx->use_tile) {
-    const TileGrid G = ctx->tile;
-    const int ncol = G.nx * G.ny;
-    const int max_items = ctx->n / kTileAtoms + ncol + 1;
-    const int nbt = blocks_for(max_items, kTileWaves);
-    int *wc = ctx->d_wcount.as<int>(), *wo = ctx->d_woff.as<int>();
+    const int ncol = ctx->ncol_total, nrep = (int)ctx->reps.size();
+    const TileRep *reps = ctx->d_reps.as<TileRep>();
     int4 *items = ctx->d_items.as<int4>();
     int *nitems = ctx->d_nitems.as<int>();
-    hipLaunchKernelGGL(k_col_waves, dim3(blocks_for(ncol)), dim3(kBlock), 0, s,
-                       S.cstart, S.ccount, ncol, G.nz, wc);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, wc, wo, ncol);
-    hipLaunchKernelGGL(k_col_items, dim3(blocks_for(ncol)), dim3(kBlock), 0, s,
-                       S.cstart, S.ccount, wc, wo, ncol, G.nz, items, nitems);
+    int *rep_first = ctx->d_rep_first.as<int>();
+    launch_items(reps, ctx->d_col_rep.as<int>(), S.cstart, S.ccount, ncol,
+                 nrep, ctx->d_wcount.as<int>(), ctx->d_woff.as<int>(), items,
+                 nitems, rep_first, s);
     if (ev_built) HIPCHK(hipEventRecord(ev_built, s));
     if (ctx->n > 0) {
-      if (mixed) {
-        if (el == PMNB_ELEC_NONE)
-          hipLaunchKernelGGL((k_tile<false, 0, 0, 256>), dim3(nbt), dim3(kBlock),
-                             0, s, P, S, G, items, nitems, d_forces, part, np);
-        else if (el == PMNB_ELEC_COULOMB)
-          hipLaunchKernelGGL((k_tile<false, 1, 0, 256>), dim3(nbt), dim3(kBlock),
-                             0, s, P, S, G, items, nitems, d_forces, part, np);
-        else
-          hipLaunchKernelGGL((k_tile<false, 2, 0, 256>), dim3(nbt), dim3(kBlock),
-                             0, s, P, S, G, items, nitems, d_forces, part, np);
-      } else {
-        if (el == PMNB_ELEC_NONE)
-          hipLaunchKernelGGL((k_tile<true, 0, 0, 128>), dim3(nbt), dim3(kBlock),
-                             0, s, P, S, G, items, nitems, d_forces, part, np);
-        else if (el == PMNB_ELEC_COULOMB && sw == PMNB_SWITCH_C2)
-          hipLaunchKernelGGL((k_tile<true, 1, 0, 128>), dim3(nbt), dim3(kBlock),
-                             0, s, P, S, G, items, nitems, d_forces, part, np);
-        else if (el == PMNB_ELEC_COULOMB)
-          hipLaunchKernelGGL((k_tile<true, 1, 1, 128>), dim3(nbt), dim3(kBlock),
-                             0, s, P, S, G, items, nitems, d_forces, part, np);
-        else if (sw == PMNB_SWITCH_C2)
-          hipLaunchKernelGGL((k_tile<true, 2, 0, 128>), dim3(nbt), dim3(kBlock),
-                             0, s, P, S, G, items, nitems, d_forces, part, np);
-        else
-          hipLaunchKernelGGL((k_tile<true, 2, 1, 128>), dim3(nbt), dim3(kBlock),
-                             0, s, P, S, G, items, nitems, d_forces, part, np);
-      }
+      TqArgs a{ctx->n / kTileAtoms + ncol + 1, ctx->grid_cap, s, &P, &S, reps, items,
+               nitems, d_forces, part, np};
+      launch_pair_tq(mixed, el, sw, ctx->shape, a);
     }
     if (ev_stop) HIPCHK(hipEventRecord(ev_stop, s));
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, s, part,
-                       ctx->n > 0 ? nbt : 0, part, 0, d_out);
+    hipLaunchKernelGGL(k_finish_rep, dim3(nrep), dim3(1024), 0, s, part,
+                       rep_first, d_out);
     HIPCHK(hipGetLastError());
     return PMNB_OK;
   }
@@ -702,8 +923,9 @@ int pmnb_create(const pmnb_params *params, pmnb_ctx **out) {
   if (hipSetDevice(params->device) != hipSuccess) return PMNB_ERR_HIP;
   ctx = new pmnb_ctx();
   ctx->prm = *params;
-  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) !=
-      hipSuccess) {
+  // a blocking stream: work enqueued on the legacy default stream (a
+  // caller's stream handle 0) stays ordered with it
+  if (hipStreamCreate(&ctx->stream) != hipSuccess) {
     delete ctx;
     return PMNB_ERR_HIP;
   }
@@ -803,6 +1025,13 @@ int pmnb_set_topology(pmnb_ctx *ctx, int32_t n_atoms, const int32_t *type,
   ctx->csf = coulomb_scaling_factor;
   ctx->max_excl_delta = maxd;
   ctx->have_topo = true;
+  // the choice of pair kernel depends on the topology: rebuild the grid
+  if (ctx->have_box) {
+    const double *v = ctx->box_key;
+    const int rc = build_box(ctx, v, v + 3, v + 6, v + 9);
+    if (rc != PMNB_OK) ctx->have_box = false;
+    return rc;
+  }
   return PMNB_OK;
 }
 
@@ -1007,11 +1236,403 @@ void pmnb_destroy(pmnb_ctx *ctx) {
                     &ctx->d_pairs, &ctx->d_list, &ctx->d_ncount,
                     &ctx->d_partial2, &ctx->d_shell_off, &ctx->d_box,
                     &ctx->d_wcount, &ctx->d_woff, &ctx->d_items,
-                    &ctx->d_nitems};
+                    &ctx->d_nitems, &ctx->d_reps, &ctx->d_col_rep,
+                    &ctx->d_rep_first, &ctx->d_rep_of, &ctx->d_partial_lj};
   for (DevBuf *b : bufs) b->release();
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+}
+
+// ----------------------------------------------------------- derivatives
+
+int pmnb_set_parameters_device(pmnb_ctx *ctx, const double *d_lj,
+                               const double *d_scaled_charge, void *stream) {
+  if (!ctx) return PMNB_ERR_ARG;
+  if (!ctx->have_topo)
+    return fail(ctx, PMNB_ERR_STATE, "pmnb_set_topology has not been called");
+  HIPCHK(hipSetDevice(ctx->prm.device));
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  const int nlj = ctx->ntypes * ctx->ntypes;
+  if (d_lj) {
+    HIPCHK(hipMemcpyAsync(ctx->d_lj.p, d_lj, (size_t)nlj * 4 * sizeof(double),
+                          hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_lj_to_f32, dim3(blocks_for(nlj)), dim3(kBlock), 0, s,
+                       ctx->d_lj.as<double>(), nlj, ctx->d_ljf.as<float4>());
+  }
+  if (d_scaled_charge && ctx->n > 0)
+    HIPCHK(hipMemcpyAsync(ctx->d_q.p, d_scaled_charge,
+                          (size_t)ctx->n * sizeof(double),
+                          hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipGetLastError());
+  return PMNB_OK;
+}
+
+int pmnb_adjoint_device(pmnb_ctx *ctx, const double *d_positions,
+                        const double *d_grad_forces, double *d_grad_positions,
+                        double *d_grad_charges, double *d_grad_lj,
+                        void *stream) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (!d_positions || !d_grad_forces)
+    return fail(ctx, PMNB_ERR_ARG, "null device pointer");
+  if (!ctx->use_tile)
+    return fail(ctx, PMNB_ERR_UNSUPPORTED,
+                "the adjoint needs the tile path (algorithm cutoff >= "
+                "switching cutoff, at most 8 LJ types)");
+  if (ctx->prm.elec_switch != PMNB_SWITCH_C2)
+    return fail(ctx, PMNB_ERR_UNSUPPORTED, "the adjoint is built for C2 only");
+  if (ctx->ntypes > 4)
+    return fail(ctx, PMNB_ERR_UNSUPPORTED,
+                "the adjoint holds at most 4 LJ types");
+  HIPCHK(hipSetDevice(ctx->prm.device));
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  rc = ensure_eval_buffers(ctx);
+  if (rc) return rc;
+  const int ncol = ctx->ncol_total, nrep = (int)ctx->reps.size();
+  const int max_items = ctx->n / kTileAtoms + ncol + 1;
+  HIPCHK(ctx->d_partial_lj.ensure((size_t)max_items * 10 * 4 *
+                                  sizeof(double)));
+  rc = build_cells(ctx, d_positions, s);
+  if (rc) return rc;
+  const ForceParams P = make_params(ctx, 0);
+  const Sorted S = make_sorted(ctx, nullptr);
+  const TileRep *reps = ctx->d_reps.as<TileRep>();
+  int4 *items = ctx->d_items.as<int4>();
+  int *nitems = ctx->d_nitems.as<int>();
+  launch_items(reps, ctx->d_col_rep.as<int>(), S.cstart, S.ccount, ncol, nrep,
+               ctx->d_wcount.as<int>(), ctx->d_woff.as<int>(), items, nitems,
+               ctx->d_rep_first.as<int>(), s);
+  if (ctx->n > 0) {
+    const bool f64 = ctx->prm.precision == PMNB_PREC_FP64;
+    const int el = ctx->prm.elec, T = ctx->ntypes;
+    double *plj = ctx->d_partial_lj.as<double>();
+    unsigned long long *np = ctx->d_npairs.as<unsigned long long>();
+#define PMNB_ADJ(F, E)                                                        \
+  launch_adj_t<F, E>(T, max_items, s, P, S, reps, items, nitems,              \
+                     d_grad_forces, d_grad_positions, d_grad_charges, plj,    \
+                     d_grad_lj, np)
+    if (f64) {
+      if (el == PMNB_ELEC_NONE) PMNB_ADJ(true, 0);
+      else if (el == PMNB_ELEC_COULOMB) PMNB_ADJ(true, 1);
+      else PMNB_ADJ(true, 2);
+    } else {
+      if (el == PMNB_ELEC_NONE) PMNB_ADJ(false, 0);
+      else if (el == PMNB_ELEC_COULOMB) PMNB_ADJ(false, 1);
+      else PMNB_ADJ(false, 2);
+    }
+#undef PMNB_ADJ
+  }
+  HIPCHK(hipGetLastError());
+  return PMNB_OK;
+}
+
+// ------------------------------------------------------------------ batch
+// A batch is a context in batch mode: the replicas' atoms concatenated in
+// order of addition (molecule ids and exclusion indices offset), one tile
+// grid per replica, cells and columns numbered globally.
+
+int pmnb_batch_create(const pmnb_params *params, int32_t n_types,
+                      const double *lj, double coulomb_scaling_factor,
+                      pmnb_batch **out) {
+  if (!params || !out || !lj || n_types <= 0) return PMNB_ERR_ARG;
+  *out = nullptr;
+  pmnb_ctx *ctx = nullptr;
+  const int rc = pmnb_create(params, &ctx);
+  if (rc) return rc;
+  pmnb_batch *b = new pmnb_batch();
+  b->ctx = ctx;
+  b->ntypes = n_types;
+  b->lj.assign(lj, lj + (size_t)n_types * n_types * 4);
+  b->csf = coulomb_scaling_factor;
+  *out = b;
+  return PMNB_OK;
+}
+
+int pmnb_batch_add(pmnb_batch *b, int32_t n_atoms, const int32_t *type,
+                   const double *scaled_charge, const int32_t *molecule,
+                   int32_t n_excl, const int32_t *excl, const double e1[3],
+                   const double e2[3], const double e3[3],
+                   const double origin[3], int32_t *replica) {
+  if (!b) return PMNB_ERR_ARG;
+  pmnb_ctx *ctx = b->ctx;
+  if (n_atoms <= 0 || n_excl < 0 || !type || !scaled_charge || !molecule ||
+      (n_excl > 0 && !excl) || !e1 || !e2 || !e3 || !origin)
+    return fail(ctx, PMNB_ERR_ARG, "replica: invalid sizes or null arrays");
+  Box box;
+  const int rc = make_box(ctx, e1, e2, e3, origin, box);
+  if (rc) return rc;
+  const int off = (int)b->type.size();
+  if ((long long)off + n_atoms >= (1ll << kQTypeShift))
+    return fail(ctx, PMNB_ERR_UNSUPPORTED, "batch exceeds %d atoms",
+                (1 << kQTypeShift) - 1);
+  int nmol = 0;
+  for (int i = 0; i < n_atoms; i++) {
+    if (molecule[i] < 0)
+      return fail(ctx, PMNB_ERR_ARG, "atom %d has molecule id %d", i,
+                  molecule[i]);
+    nmol = std::max(nmol, molecule[i] + 1);
+  }
+  for (int k = 0; k < n_excl; k++) {
+    const int a = excl[3 * k], c = excl[3 * k + 1];
+    if (a < 0 || c < 0 || a >= n_atoms || c >= n_atoms)
+      return fail(ctx, PMNB_ERR_ARG, "exclusion %d (%d,%d) out of range", k, a,
+                  c);
+  }
+  const int r = (int)b->boxes.size();
+  for (int i = 0; i < n_atoms; i++) {
+    b->type.push_back(type[i]);
+    b->q.push_back(scaled_charge[i]);
+    b->mol.push_back(b->next_mol + molecule[i]);
+    b->rep_of.push_back(r);
+  }
+  for (int k = 0; k < n_excl; k++) {
+    b->excl.push_back(off + excl[3 * k]);
+    b->excl.push_back(off + excl[3 * k + 1]);
+    b->excl.push_back(excl[3 * k + 2]);
+  }
+  b->next_mol += nmol;
+  b->boxes.push_back(box);
+  b->dirty = true;
+  if (replica) *replica = r;
+  return PMNB_OK;
+}
+
+namespace {
+
+void drop_graph(pmnb_batch *b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+}
+
+// Uploads the concatenated topology and the replicas' tile grids.
+int batch_finalize(pmnb_batch *b) {
+  if (!b->dirty) return PMNB_OK;
+  pmnb_ctx *ctx = b->ctx;
+  const int R = (int)b->boxes.size();
+  if (R == 0) return fail(ctx, PMNB_ERR_STATE, "the batch has no replica");
+  drop_graph(b);
+  ctx->have_box = false;
+  ctx->batch = false;
+  int rc = pmnb_set_topology(ctx, (int)b->type.size(), b->type.data(),
+                             b->q.data(), b->mol.data(), b->ntypes,
+                             b->lj.data(), b->csf, (int)b->excl.size() / 3,
+                             b->excl.data());
+  if (rc) return rc;
+  const ForceParams P0 = make_params_for(ctx->prm, ctx->csf);
+  const double rcp = std::sqrt(P0.cut2);
+  if (ctx->prm.algorithm_cutoff < rcp)
+    return fail(ctx, PMNB_ERR_UNSUPPORTED,
+                "a batch needs the algorithm cutoff >= the switching cutoff");
+  if (ctx->ntypes * ctx->ntypes > kQLj)
+    return fail(ctx, PMNB_ERR_UNSUPPORTED, "a batch holds at most %d LJ types",
+                8);
+  ctx->reps.clear();
+  ctx->col_rep.clear();
+  long long cells = 0;
+  int cols = 0;
+  std::vector<int4> shell;
+  for (int r = 0; r < R; r++) {
+    TileRep rep{};
+    if (!make_tile_rep(b->boxes[r], rcp, rep, shell))
+      return fail(ctx, PMNB_ERR_UNSUPPORTED, "replica %d grid too large", r);
+    rep.cell_off = (int)cells;
+    rep.col_off = cols;
+    cells += rep.grid.ncell;
+    cols += rep.ncol;
+    if (cells >= (1ll << 28))
+      return fail(ctx, PMNB_ERR_UNSUPPORTED, "batch cell grid too large");
+    for (int c = 0; c < rep.ncol; c++) ctx->col_rep.push_back(r);
+    ctx->reps.push_back(rep);
+  }
+  ctx->use_tile = true;
+  ctx->use_list = false;
+  ctx->batch = true;
+  ctx->tile = ctx->reps[0].tg;
+  ctx->box = ctx->reps[0].box;
+  ctx->grid = Grid{};
+  ctx->grid.ncell = (int)cells;
+  ctx->ncol_total = cols;
+  ctx->shell.assign(1, make_int4(0, 0, 0, 0));
+  ctx->shell_off.assign(1, make_float4(0.f, 0.f, 0.f, 0.f));
+  ctx->shell_uploaded = 0;
+  ctx->rep_uploaded = 0;
+  ctx->have_box = true;
+  HIPCHK(hipSetDevice(ctx->prm.device));
+  HIPCHK(ctx->d_rep_of.ensure(b->rep_of.size() * sizeof(int)));
+  HIPCHK(hipMemcpy(ctx->d_rep_of.p, b->rep_of.data(),
+                   b->rep_of.size() * sizeof(int), hipMemcpyHostToDevice));
+  rc = ensure_eval_buffers(ctx);
+  if (rc) return rc;
+  b->dirty = false;
+  return PMNB_OK;
+}
+
+}  // namespace
+
+int pmnb_batch_size(const pmnb_batch *b, int32_t *n_replicas,
+                    int64_t *n_atoms) {
+  if (!b) return PMNB_ERR_ARG;
+  if (n_replicas) *n_replicas = (int32_t)b->boxes.size();
+  if (n_atoms) *n_atoms = (int64_t)b->type.size();
+  return PMNB_OK;
+}
+
+int pmnb_batch_evaluate_device(pmnb_batch *b, const double *d_positions,
+                               double *d_forces, double *d_out,
+                               uint32_t flags, void *stream) {
+  if (!b) return PMNB_ERR_ARG;
+  pmnb_ctx *ctx = b->ctx;
+  if (flags & PMNB_FLAG_MOLVIRIAL)
+    return fail(ctx, PMNB_ERR_UNSUPPORTED,
+                "the batch computes no molecular virial");
+  int rc = batch_finalize(b);
+  if (rc) return rc;
+  return pmnb_evaluate_device(ctx, d_positions, d_forces, d_out, nullptr,
+                              flags, stream);
+}
+
+int pmnb_batch_run_graph(pmnb_batch *b, const double *d_positions,
+                         double *d_forces, double *d_out, uint32_t flags,
+                         int32_t n_steps, void *stream) {
+  if (!b || n_steps < 0) return PMNB_ERR_ARG;
+  pmnb_ctx *ctx = b->ctx;
+  if (!d_positions || !d_forces || !d_out)
+    return fail(ctx, PMNB_ERR_ARG, "null device pointer");
+  if (flags & PMNB_FLAG_MOLVIRIAL)
+    return fail(ctx, PMNB_ERR_UNSUPPORTED,
+                "the batch computes no molecular virial");
+  int rc = batch_finalize(b);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(ctx->prm.device));
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  const bool same = b->gexec && b->gkey[0] == d_positions &&
+                    b->gkey[1] == d_forces && b->gkey[2] == d_out &&
+                    b->gflags == flags && b->gshape == ctx->shape &&
+                    b->gcap == ctx->grid_cap;
+  if (!same) {
+    drop_graph(b);
+    if (ctx->timing)
+      return fail(ctx, PMNB_ERR_STATE, "kernel timing is on: no capture");
+    // one step: clear forces and outputs, then the whole evaluation; the
+    // buffers are allocated by batch_finalize, so nothing allocates here
+    const long long n3 = 3ll * ctx->n, no = 20ll * (long long)ctx->reps.size();
+    if (!b->cap_stream)
+      HIPCHK(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+    hipStream_t cs = b->cap_stream;
+    HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((n3 + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, cs, d_forces, n3);
+    hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((no + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, cs, d_out, no);
+    rc = run_device(ctx, d_positions, d_forces, d_out, nullptr, flags, cs);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(cs, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    HIPCHK(e);
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+    b->gkey[0] = d_positions;
+    b->gkey[1] = d_forces;
+    b->gkey[2] = d_out;
+    b->gflags = flags;
+    b->gshape = ctx->shape;
+    b->gcap = ctx->grid_cap;
+  }
+  for (int k = 0; k < n_steps; k++) HIPCHK(hipGraphLaunch(b->gexec, s));
+  return PMNB_OK;
+}
+
+int pmnb_batch_set_shape(pmnb_batch *b, int32_t shape, int32_t grid_blocks) {
+  if (!b) return PMNB_ERR_ARG;
+  if (shape < 0 || shape >= kMixedShapes || grid_blocks < 0)
+    return fail(b->ctx, PMNB_ERR_ARG, "shape %d / grid %d out of range", shape,
+                grid_blocks);
+  b->ctx->shape = shape;
+  b->ctx->grid_cap = grid_blocks;
+  return PMNB_OK;
+}
+
+int pmnb_batch_autotune(pmnb_batch *b, const double *d_positions,
+                        double *d_forces, double *d_out, uint32_t flags,
+                        int32_t *shape, int32_t *grid_blocks, double *ms) {
+  if (!b) return PMNB_ERR_ARG;
+  pmnb_ctx *ctx = b->ctx;
+  int rc = batch_finalize(b);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(ctx->prm.device));
+  int ncu = 0;
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                               ctx->prm.device));
+  const bool mixed = ctx->prm.precision == PMNB_PREC_MIXED;
+  const int nshape = mixed ? kMixedShapes : 1;
+  const int caps[4] = {0, 2 * ncu, 4 * ncu, 8 * ncu};
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  double best = 1e300;
+  int bs = 0, bc = 0;
+  const bool timing = ctx->timing;
+  ctx->timing = false;
+  for (int sh = 0; sh < nshape && rc == PMNB_OK; sh++) {
+    for (int ci = 0; ci < 4 && rc == PMNB_OK; ci++) {
+      ctx->shape = sh;
+      ctx->grid_cap = caps[ci];
+      double t = 1e300;
+      for (int rep = 0; rep < 4 && rc == PMNB_OK; rep++) {
+        if (hipEventRecord(e0, ctx->stream) != hipSuccess) rc = PMNB_ERR_HIP;
+        if (rc == PMNB_OK)
+          rc = run_device(ctx, d_positions, d_forces, d_out, nullptr, flags,
+                          ctx->stream);
+        if (rc == PMNB_OK && (hipEventRecord(e1, ctx->stream) != hipSuccess ||
+                              hipEventSynchronize(e1) != hipSuccess))
+          rc = fail(ctx, PMNB_ERR_HIP, "autotune timing failed");
+        float f = 0.f;
+        if (rc == PMNB_OK && hipEventElapsedTime(&f, e0, e1) != hipSuccess)
+          rc = fail(ctx, PMNB_ERR_HIP, "autotune timing failed");
+        if (rep > 0) t = std::min(t, (double)f);  // first run warms up
+      }
+      if (t < best) {
+        best = t;
+        bs = sh;
+        bc = caps[ci];
+      }
+    }
+  }
+  ctx->timing = timing;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  ctx->shape = bs;
+  ctx->grid_cap = bc;
+  if (shape) *shape = bs;
+  if (grid_blocks) *grid_blocks = bc;
+  if (ms) *ms = best;
+  return PMNB_OK;
+}
+
+int pmnb_batch_pair_count(pmnb_batch *b, int64_t *n_pairs) {
+  if (!b) return PMNB_ERR_ARG;
+  return pmnb_last_pair_count(b->ctx, n_pairs);
+}
+
+const char *pmnb_batch_last_error(const pmnb_batch *b) {
+  return b ? b->ctx->err.c_str() : "null batch";
+}
+
+void pmnb_batch_destroy(pmnb_batch *b) {
+  if (!b) return;
+  (void)hipSetDevice(b->ctx->prm.device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  drop_graph(b);
+  if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+  pmnb_destroy(b->ctx);
+  delete b;
 }
 
 }  // extern "C"

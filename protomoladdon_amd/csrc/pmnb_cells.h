@@ -186,4 +186,67 @@ __global__ void k_gather(Box b, Grid g, const double *__restrict__ pos,
   scell[s] = cell_of[a];
 }
 
+// Replica-aware cell assignment and gather of the tile path: atom i belongs
+// to replica rep_of[i] (replica 0 when rep_of is null), whose box and
+// column grid place it; cells are numbered globally from the replica's
+// cell offset.  RepT provides box, grid and cell_off.
+template <class RepT>
+__global__ void k_assign_r(const RepT *__restrict__ reps,
+                           const int *__restrict__ rep_of,
+                           const double *__restrict__ pos, int n,
+                           int *__restrict__ cell_of, int *__restrict__ count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const RepT &R = reps[rep_of ? rep_of[i] : 0];
+  int cx, cy, cz;
+  double vx, vy, vz;
+  cell_coords(R.box, R.grid, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], cx,
+              cy, cz, vx, vy, vz);
+  int c = R.cell_off + (cx * R.grid.n[1] + cy) * R.grid.n[2] + cz;
+  cell_of[i] = c;
+  atomicAdd(&count[c], 1);
+}
+
+template <class RepT>
+__global__ void k_gather_r(const RepT *__restrict__ reps,
+                           const int *__restrict__ rep_of,
+                           const double *__restrict__ pos,
+                           const int *__restrict__ sorted,
+                           const int *__restrict__ cell_of,
+                           const int *__restrict__ type,
+                           const double *__restrict__ q,
+                           const int *__restrict__ mol, int n,
+                           double *__restrict__ sx, double *__restrict__ sy,
+                           double *__restrict__ sz, float4 *__restrict__ sloc,
+                           int *__restrict__ stype, double *__restrict__ sq,
+                           int *__restrict__ smol, int *__restrict__ scell) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int a = sorted[s];
+  const RepT &R = reps[rep_of ? rep_of[a] : 0];
+  double x = pos[3 * a], y = pos[3 * a + 1], z = pos[3 * a + 2];
+  sx[s] = x;
+  sy[s] = y;
+  sz[s] = z;
+  int cx, cy, cz;
+  double vx, vy, vz;
+  cell_coords(R.box, R.grid, x, y, z, cx, cy, cz, vx, vy, vz);
+  float4 l;
+  l.x = (float)(vx - (double)cx * R.grid.cs[0]);
+  l.y = (float)(vy - (double)cy * R.grid.cs[1]);
+  l.z = (float)(vz - (double)cz * R.grid.cs[2]);
+  l.w = (float)q[a];
+  sloc[s] = l;
+  stype[s] = type[a];
+  sq[s] = q[a];
+  smol[s] = mol[a];
+  scell[s] = cell_of[a];
+}
+
+// Sets n doubles to zero (a kernel node, not a memset, inside graphs).
+__global__ void k_zero_f64(double *p, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
 }  // namespace pmnb

@@ -35,10 +35,11 @@ struct Sorted {
 };
 
 // Wave-then-block reduction of ACC_N doubles in a fixed order; block b
-// writes partial[b * ACC_N + k].
+// of NW waves writes partial[b * ACC_N + k].
+template <int NW = kBlock / 64>
 __device__ __forceinline__ void block_reduce_store(double (&acc)[ACC_N],
                                                    double *partial) {
-  __shared__ double red[kBlock / 64][ACC_N];
+  __shared__ double red[NW][ACC_N];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < ACC_N; k++) {
@@ -49,13 +50,14 @@ __device__ __forceinline__ void block_reduce_store(double (&acc)[ACC_N],
   __syncthreads();
   if (threadIdx.x < ACC_N) {
     double s = 0.0;
-    for (int w = 0; w < kBlock / 64; w++) s += red[w][threadIdx.x];
+    for (int w = 0; w < NW; w++) s += red[w][threadIdx.x];
     partial[blockIdx.x * ACC_N + threadIdx.x] = s;
   }
 }
 
+template <int NW = kBlock / 64>
 __device__ __forceinline__ unsigned long long block_count(unsigned c) {
-  __shared__ unsigned long long cred[kBlock / 64];
+  __shared__ unsigned long long cred[NW];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   unsigned long long v = c;
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -63,7 +65,7 @@ __device__ __forceinline__ unsigned long long block_count(unsigned c) {
   __syncthreads();
   unsigned long long s = 0;
   if (threadIdx.x == 0)
-    for (int w = 0; w < kBlock / 64; w++) s += cred[w];
+    for (int w = 0; w < NW; w++) s += cred[w];
   return s;
 }
 

@@ -238,6 +238,35 @@ class NonbondedCutoffSystemForce:
             d_mol_com.data_ptr() if d_mol_com is not None else None,
             flags, sp))
 
+    def set_parameters_device(self, topology, d_lj=None, d_scaled_charges=None,
+                              stream=None):
+        """Replaces the device copy of the LJ table (T x T x 4 float64
+        {A, B, A14, B14}) and/or the scaled charges (N float64) from device
+        tensors; the host topology is left as it is (a later change of the
+        topology uploads it again)."""
+        self._ensure(topology)
+        lib = self._lib
+        _lib.check(lib, self._ctx, lib.pmnb_set_parameters_device(
+            self._ctx, d_lj.data_ptr() if d_lj is not None else None,
+            d_scaled_charges.data_ptr() if d_scaled_charges is not None
+            else None, ctypes.c_void_p(stream) if stream else None))
+
+    def adjoint_device(self, topology, d_positions, d_grad_forces,
+                       d_grad_positions=None, d_grad_charges=None,
+                       d_grad_lj=None, stream=None):
+        """Accumulates dL/dx (= -H dL/dF), dL/d(scaled charges) and
+        dL/d{A, B, A14, B14} for the upstream gradient d_grad_forces = dL/dF;
+        the pairs are recomputed.  Device tensors; any output may be None."""
+        self._ensure(topology)
+        lib = self._lib
+
+        def ptr(t):
+            return t.data_ptr() if t is not None else None
+        _lib.check(lib, self._ctx, lib.pmnb_adjoint_device(
+            self._ctx, d_positions.data_ptr(), d_grad_forces.data_ptr(),
+            ptr(d_grad_positions), ptr(d_grad_charges), ptr(d_grad_lj),
+            ctypes.c_void_p(stream) if stream else None))
+
     def pair_set(self, topology, positions):
         """Sorted (P, 3) int32 rows (i, j, exclusion class), i < j."""
         self._ensure(topology)
