@@ -100,6 +100,10 @@ def main():
                     help="steps of the FP64 path (default: steps/4)")
     ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of CPU baseline sampling (0 = skip)")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB "
+                         "(3 times): the reference kernel that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profiled run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; multi-GPU runs are not part of "
@@ -108,6 +112,16 @@ def main():
 
     import torch
     from protomoladdon_amd import systems
+
+    if args.copy_mib > 0:
+        n = args.copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print("copied %d MiB x3" % args.copy_mib, file=sys.stderr, flush=True)
+        del src, dst
 
     print("generating workload ...", file=sys.stderr, flush=True)
     w = systems.config1()

@@ -256,7 +256,7 @@ int pmnb_batch_autotune(pmnb_batch *b, const double *d_positions,
                         double *d_forces, double *d_out, uint32_t flags,
                         int32_t *shape, int32_t *grid_blocks, double *ms);
 
-/* Sets the pair-kernel shape (0 .. 4) and grid cap by hand. */
+/* Sets the pair-kernel shape (0 .. 5; FP64: 0 .. 1) and grid cap by hand. */
 int pmnb_batch_set_shape(pmnb_batch *b, int32_t shape, int32_t grid_blocks);
 
 /* In-cutoff pairs of all replicas in the last evaluation; waits for the

@@ -23,6 +23,7 @@
 #include "pmnb_tile.h"
 #include "pmnb_queue.h"
 #include "pmnb_adjoint.h"
+#include "pmnb_tilemask.h"
 
 using namespace pmnb;
 
@@ -95,7 +96,7 @@ struct pmnb_ctx {
   DevBuf d_partial_lj;            // adjoint: LJ-parameter partials per item
   int shape = 0;                  // tile-path kernel shape (mixed)
   int grid_cap = 0;               // persistent pair grid: max blocks (0: none)
-  DevBuf d_wcount, d_woff, d_items, d_nitems, d_reps, d_col_rep, d_rep_first;
+  DevBuf d_items, d_nitems, d_reps, d_col_rep, d_rep_first;
   double box_key[12] = {0};       // e1, e2, e3, origin of the current box
   int cap = 0;                    // list slots per atom
   // pair-kernel timing
@@ -194,8 +195,8 @@ ForceParams make_params_for(const pmnb_params &prm, double csf) {
 }
 
 // Column grid of the tile path for box b and pair cutoff rcp: x/y columns
-// at least rcp wide, z cells of about rcp / 4.  An axis with fewer than 3
-// columns is marked for per-pair minimum image.  shell receives the distinct
+// at least rcp wide, z cells of about rcp / 4.  An axis with a single column
+// is marked for per-pair minimum image.  shell receives the distinct
 // cell offsets within reach ((0,0,0) first), which the pair-set extraction
 // walks.  Returns false when the grid is too large.
 bool make_tile_rep(const Box &b, double rcp, TileRep &rep,
@@ -228,8 +229,11 @@ bool make_tile_rep(const Box &b, double rcp, TileRep &rep,
     rep.L[k] = (float)b.e[k];
     rep.iL[k] = (float)b.er[k];
   }
-  rep.wrap[0] = tx < 3;
-  rep.wrap[1] = ty < 3;
+  // one column per axis (box < 2 rcp): per-pair minimum image; with two,
+  // the column offsets -1 and +1 reach the same column at images one box
+  // length (>= 2 rcp) apart, so at most one of them is within the cutoff
+  rep.wrap[0] = tx == 1;
+  rep.wrap[1] = ty == 1;
   rep.ncol = tx * ty;
   // distinct offsets per axis
   auto offsets = [](int n, int reach) {
@@ -499,8 +503,6 @@ int ensure_eval_buffers(pmnb_ctx *ctx) {
   }
   if (ctx->use_tile) {
     const int ncol = ctx->ncol_total, nrep = (int)ctx->reps.size();
-    HIPCHK(ctx->d_wcount.ensure((size_t)ncol * sizeof(int)));
-    HIPCHK(ctx->d_woff.ensure((size_t)ncol * sizeof(int)));
     HIPCHK(ctx->d_items.ensure((size_t)(n / kTileAtoms + ncol + 1) *
                                sizeof(int4)));
     HIPCHK(ctx->d_nitems.ensure(sizeof(int)));
@@ -529,9 +531,11 @@ int ensure_eval_buffers(pmnb_ctx *ctx) {
   return PMNB_OK;
 }
 
-// Tile-path kernel shapes the autotuner chooses from (mixed precision):
-// {waves per block, buffered records, force slots per atom, LDS atomics}.
-constexpr int kMixedShapes = 5;
+// Tile-path pair kernels the autotuner chooses from.  Shape 0 is k_tm
+// (per-lane masks, register sums); the others are k_tq (pair queue) with
+// {waves per block, buffered records, force slots per atom, LDS atomics};
+// FP64 has shape 0 and k_tq as shape 1.
+constexpr int kMixedShapes = 6;
 using MixedShape0 = TqShape<1, 128, 16, false>;
 using MixedShape1 = TqShape<2, 128, 16, false>;
 using MixedShape2 = TqShape<4, 128, 16, false>;
@@ -550,6 +554,9 @@ struct TqArgs {
   const int *nitems;
   double *forces, *partial;
   unsigned long long *npairs;
+  int pstride;     // partial row stride (items)
+  bool single;     // one replica without wrapped x/y: k_tile for shape 0
+  TileGrid G;      // its column grid
 };
 
 template <bool F64, int EL, int SW, class SHP>
@@ -559,21 +566,37 @@ int launch_tq_shape(const TqArgs &a) {
   nb = std::max(nb, 1);
   hipLaunchKernelGGL((k_tq<F64, EL, SW, SHP>), dim3(nb), dim3(64 * SHP::NW), 0,
                      a.s, *a.P, *a.S, a.reps, a.items, a.nitems, a.forces,
-                     a.partial, a.npairs);
+                     a.partial, a.pstride, a.npairs);
   return nb;
 }
 
 template <bool F64, int EL, int SW>
 int launch_tq(int shape, const TqArgs &a) {
+  if (shape == 0 && a.single) {
+    const int nb = std::max(1, blocks_for(a.max_items, kTmWaves));
+    hipLaunchKernelGGL((k_tile<F64, EL, SW, F64 ? 128 : 256>), dim3(nb),
+                       dim3(kBlock), 0, a.s, *a.P, *a.S, a.G, a.items,
+                       a.nitems, a.forces, a.partial, a.pstride, a.npairs);
+    return nb;
+  }
+  if (shape == 0) {
+    int nb = blocks_for(a.max_items, kTmWaves);
+    if (a.grid_cap > 0) nb = std::min(nb, a.grid_cap);
+    nb = std::max(nb, 1);
+    hipLaunchKernelGGL((k_tm<F64, EL, SW, F64 ? 128 : 256>), dim3(nb),
+                       dim3(kBlock), 0, a.s, *a.P, *a.S, a.reps, a.items,
+                       a.nitems, a.forces, a.partial, a.pstride, a.npairs);
+    return nb;
+  }
   if constexpr (F64) {
     return launch_tq_shape<true, EL, SW, F64Shape>(a);
   } else {
     switch (shape) {
-      case 1: return launch_tq_shape<false, EL, SW, MixedShape1>(a);
-      case 2: return launch_tq_shape<false, EL, SW, MixedShape2>(a);
-      case 3: return launch_tq_shape<false, EL, SW, MixedShape3>(a);
-      case 4: return launch_tq_shape<false, EL, SW, MixedShape4>(a);
-      default: return launch_tq_shape<false, EL, SW, MixedShape0>(a);
+      case 1: return launch_tq_shape<false, EL, SW, MixedShape0>(a);
+      case 2: return launch_tq_shape<false, EL, SW, MixedShape1>(a);
+      case 3: return launch_tq_shape<false, EL, SW, MixedShape2>(a);
+      case 4: return launch_tq_shape<false, EL, SW, MixedShape3>(a);
+      default: return launch_tq_shape<false, EL, SW, MixedShape4>(a);
     }
   }
 }
@@ -596,14 +619,10 @@ void launch_pair_tq(bool mixed, int el, int sw, int shape, const TqArgs &a) {
 
 // Work items of the tile path from the sorted cells (one or many replicas).
 void launch_items(const TileRep *reps, const int *col_rep, const int *cstart,
-                  const int *ccount, int ncol, int nrep, int *wc, int *wo,
-                  int4 *items, int *nitems, int *rep_first, hipStream_t s) {
-  hipLaunchKernelGGL(k_col_waves_r, dim3(blocks_for(ncol)), dim3(kBlock), 0, s,
-                     reps, col_rep, cstart, ccount, ncol, wc);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, wc, wo, ncol);
-  hipLaunchKernelGGL(k_col_items_r, dim3(blocks_for(ncol)), dim3(kBlock), 0, s,
-                     reps, col_rep, cstart, ccount, wc, wo, ncol, nrep, items,
-                     nitems, rep_first);
+                  const int *ccount, int ncol, int nrep, int4 *items,
+                  int *nitems, int *rep_first, hipStream_t s) {
+  hipLaunchKernelGGL(k_items_r, dim3(1), dim3(1024), 0, s, reps, col_rep,
+                     cstart, ccount, ncol, nrep, items, nitems, rep_first);
 }
 
 // Cell-list build on the device from d_pos (original order).
@@ -787,22 +806,24 @@ int run_device(pmnb_ctx *ctx, const double *d_pos, double *d_forces,
   }
   if (ctx->use_tile) {
     const int ncol = ctx->ncol_total, nrep = (int)ctx->reps.size();
+    const int max_items = ctx->n / kTileAtoms + ncol + 1;
     const TileRep *reps = ctx->d_reps.as<TileRep>();
     int4 *items = ctx->d_items.as<int4>();
     int *nitems = ctx->d_nitems.as<int>();
     int *rep_first = ctx->d_rep_first.as<int>();
     launch_items(reps, ctx->d_col_rep.as<int>(), S.cstart, S.ccount, ncol,
-                 nrep, ctx->d_wcount.as<int>(), ctx->d_woff.as<int>(), items,
-                 nitems, rep_first, s);
+                 nrep, items, nitems, rep_first, s);
     if (ev_built) HIPCHK(hipEventRecord(ev_built, s));
     if (ctx->n > 0) {
-      TqArgs a{ctx->n / kTileAtoms + ncol + 1, ctx->grid_cap, s, &P, &S, reps, items,
-               nitems, d_forces, part, np};
+      const bool single = !ctx->batch && nrep == 1 &&
+                          !ctx->reps[0].wrap[0] && !ctx->reps[0].wrap[1];
+      TqArgs a{max_items, ctx->grid_cap, s, &P, &S, reps, items,
+               nitems, d_forces, part, np, max_items, single, ctx->tile};
       launch_pair_tq(mixed, el, sw, ctx->shape, a);
     }
     if (ev_stop) HIPCHK(hipEventRecord(ev_stop, s));
-    hipLaunchKernelGGL(k_finish_rep, dim3(nrep), dim3(1024), 0, s, part,
-                       rep_first, d_out);
+    hipLaunchKernelGGL(k_finish_rep, dim3(nrep, ACC_N), dim3(256), 0, s, part,
+                       max_items, rep_first, d_out);
     HIPCHK(hipGetLastError());
     return PMNB_OK;
   }
@@ -1235,7 +1256,7 @@ void pmnb_destroy(pmnb_ctx *ctx) {
                     &ctx->d_shell, &ctx->d_partial, &ctx->d_npairs,
                     &ctx->d_pairs, &ctx->d_list, &ctx->d_ncount,
                     &ctx->d_partial2, &ctx->d_shell_off, &ctx->d_box,
-                    &ctx->d_wcount, &ctx->d_woff, &ctx->d_items,
+                    &ctx->d_items,
                     &ctx->d_nitems, &ctx->d_reps, &ctx->d_col_rep,
                     &ctx->d_rep_first, &ctx->d_rep_of, &ctx->d_partial_lj};
   for (DevBuf *b : bufs) b->release();
@@ -1301,8 +1322,7 @@ int pmnb_adjoint_device(pmnb_ctx *ctx, const double *d_positions,
   int4 *items = ctx->d_items.as<int4>();
   int *nitems = ctx->d_nitems.as<int>();
   launch_items(reps, ctx->d_col_rep.as<int>(), S.cstart, S.ccount, ncol, nrep,
-               ctx->d_wcount.as<int>(), ctx->d_woff.as<int>(), items, nitems,
-               ctx->d_rep_first.as<int>(), s);
+               items, nitems, ctx->d_rep_first.as<int>(), s);
   if (ctx->n > 0) {
     const bool f64 = ctx->prm.precision == PMNB_PREC_FP64;
     const int el = ctx->prm.elec, T = ctx->ntypes;
@@ -1570,7 +1590,7 @@ int pmnb_batch_autotune(pmnb_batch *b, const double *d_positions,
   HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
                                ctx->prm.device));
   const bool mixed = ctx->prm.precision == PMNB_PREC_MIXED;
-  const int nshape = mixed ? kMixedShapes : 1;
+  const int nshape = mixed ? kMixedShapes : 2;
   const int caps[4] = {0, 2 * ncu, 4 * ncu, 8 * ncu};
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));

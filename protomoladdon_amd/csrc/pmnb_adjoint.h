@@ -356,8 +356,9 @@ k_tq_adj(ForceParams P, AdjC<float> C32, AdjC<double> C64, Sorted S,
 
     for (int oc = 0; oc < 9; oc++) {
       const int ox = oc / 3 - 1, oy = oc % 3 - 1;
-      if (wx && (G.nx == 1 ? ox != 0 : ox == -1)) continue;
-      if (wy && (G.ny == 1 ? oy != 0 : oy == -1)) continue;
+      // an axis with a single column takes it once
+      if (wx && ox != 0) continue;
+      if (wy && oy != 0) continue;
       const float gx = wx ? 0.f : fmaxf(0.f, fmaxf((float)ox * G.csx - bx1,
                                                   bx0 - (float)(ox + 1) * G.csx));
       const float gy = wy ? 0.f : fmaxf(0.f, fmaxf((float)oy * G.csy - by1,

@@ -23,10 +23,10 @@
 // fixed order.  Every pair is visited from both atoms; energies and virials
 // take half.
 //
-// Each replica has its own column grid (TileRep).  An axis with fewer than 3
-// columns takes its distinct neighbour columns once and the minimum image of
-// every pair along it (as z does when the cutoff reaches all z cells), so
-// boxes narrower than three cutoffs keep the reference's one-image pair set.
+// Each replica has its own column grid (TileRep).  An axis with a single
+// column (box shorter than two cutoffs) takes it once and the minimum image
+// of every pair along it (as z does when the cutoff reaches all z cells), so
+// small boxes keep the reference's one-image pair set.
 //
 // The kernel is persistent: each wave walks the work items with a grid
 // stride, and writes one partial row (energies, virials) per item.
@@ -54,7 +54,7 @@ struct TileRep {
   Grid grid;          // FP64 column grid (cell assignment)
   TileGrid tg;        // FP32 cell sizes and the z reach of the cutoff
   float L[3], iL[3];  // box lengths and inverses
-  int wrap[2];        // x, y: fewer than 3 columns (per-pair minimum image)
+  int wrap[2];        // x, y: a single column (per-pair minimum image)
   int cell_off;       // first global cell of the replica
   int col_off;        // first global column
   int ncol;           // columns (tg.nx * tg.ny)
@@ -73,6 +73,15 @@ __device__ __forceinline__ void lds_add(T *p, T v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// The box of a rare path (pairs near the cutoff, molecule centres): the
+// pointer passes through an empty asm statement, so the loads of the box
+// fields stay in that branch instead of being hoisted into scalar
+// registers held across the whole work item.
+__device__ __forceinline__ const Box &rare_box(const Box *b) {
+  __asm__ volatile("" : "+s"(b));
+  return *b;
+}
+
 // Marks a value equal on all lanes as wave-uniform (scalar register), so the
 // loops and branches it controls compile to scalar control flow.
 __device__ __forceinline__ int uni(int v) {
@@ -82,80 +91,90 @@ __device__ __forceinline__ float uni(float v) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
-// Waves per global column: ceil(atoms in the column / 16).
-__global__ void k_col_waves_r(const TileRep *__restrict__ reps,
-                              const int *__restrict__ col_rep,
-                              const int *__restrict__ cstart,
-                              const int *__restrict__ ccount, int ncol,
-                              int *__restrict__ wcount) {
-  const int gc = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gc >= ncol) return;
-  const TileRep &R = reps[col_rep[gc]];
-  const int nz = R.tg.nz;
-  const int first = R.cell_off + (gc - R.col_off) * nz, last = first + nz - 1;
-  const int n = cstart[last] + ccount[last] - cstart[first];
-  wcount[gc] = (n + kTileAtoms - 1) / kTileAtoms;
-}
-
-// Work items {first atom, end atom, local column, replica} from the scanned
-// wave counts; rep_first[r] = first item of replica r, rep_first[nrep] and
-// *nitems = the item count.
-__global__ void k_col_items_r(const TileRep *__restrict__ reps,
-                              const int *__restrict__ col_rep,
-                              const int *__restrict__ cstart,
-                              const int *__restrict__ ccount,
-                              const int *__restrict__ wcount,
-                              const int *__restrict__ woff, int ncol, int nrep,
-                              int4 *__restrict__ items,
-                              int *__restrict__ nitems,
-                              int *__restrict__ rep_first) {
-  const int gc = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gc >= ncol) return;
-  const int r = col_rep[gc];
-  const TileRep &R = reps[r];
-  const int nz = R.tg.nz, lc = gc - R.col_off;
-  const int first = R.cell_off + lc * nz, last = first + nz - 1;
-  const int a0 = cstart[first];
-  const int a1 = cstart[last] + ccount[last];
-  const int w = wcount[gc], o = woff[gc];
-  for (int k = 0; k < w; k++)
-    items[o + k] = make_int4(a0 + k * kTileAtoms,
-                             min(a0 + (k + 1) * kTileAtoms, a1), lc, r);
-  if (lc == 0) rep_first[r] = o;
-  if (gc == ncol - 1) {
-    *nitems = o + w;
-    rep_first[nrep] = o + w;
+// All work items in one block: each thread takes a contiguous run of global
+// columns, counts their waves (ceil(atoms / 16)), the block scans the counts,
+// then every thread writes the items {first atom, end atom, local column,
+// replica} of its columns.  rep_first[r] = first item of replica r;
+// rep_first[nrep] and *nitems = the item count.
+__global__ void __launch_bounds__(1024)
+k_items_r(const TileRep *__restrict__ reps, const int *__restrict__ col_rep,
+          const int *__restrict__ cstart, const int *__restrict__ ccount,
+          int ncol, int nrep, int4 *__restrict__ items,
+          int *__restrict__ nitems, int *__restrict__ rep_first) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int chunk = (ncol + 1023) / 1024;
+  const int b0 = min(ncol, t * chunk), b1 = min(ncol, b0 + chunk);
+  auto span = [&](int gc, int &r, int &lc, int &a0, int &a1) {
+    r = col_rep[gc];
+    const TileRep &R = reps[r];
+    const int nz = R.tg.nz;
+    lc = gc - R.col_off;
+    const int first = R.cell_off + lc * nz, last = first + nz - 1;
+    a0 = cstart[first];
+    a1 = cstart[last] + ccount[last];
+  };
+  int sum = 0;
+  for (int gc = b0; gc < b1; gc++) {
+    int r, lc, a0, a1;
+    span(gc, r, lc, a0, a1);
+    sum += (a1 - a0 + kTileAtoms - 1) / kTileAtoms;
+  }
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int o = part[t] - sum;  // first item of this thread's columns
+  for (int gc = b0; gc < b1; gc++) {
+    int r, lc, a0, a1;
+    span(gc, r, lc, a0, a1);
+    if (lc == 0) rep_first[r] = o;
+    for (int a = a0; a < a1; a += kTileAtoms)
+      items[o++] = make_int4(a, min(a + kTileAtoms, a1), lc, r);
+  }
+  if (t == 1023) {
+    *nitems = part[1023];
+    rep_first[nrep] = part[1023];
   }
 }
 
-// Per-replica sums of the item partials in item order; out[20 r + ..]:
+// Per-replica sums of the item partials (accumulator k of item b at
+// partial[k * pstride + b]), one block per (replica, accumulator) with a
+// fixed reduction tree; out[20 r + ..]:
 // [0] LJ, [1] electrostatic, [2..10] virial 3x3, [11..19] molecular virial.
-__global__ void __launch_bounds__(1024)
-k_finish_rep(const double *__restrict__ partial,
+__global__ void __launch_bounds__(256)
+k_finish_rep(const double *__restrict__ partial, int pstride,
              const int *__restrict__ rep_first, double *__restrict__ out) {
-  const int r = blockIdx.x;
+  __shared__ double red[4];
+  const int r = blockIdx.x, k = blockIdx.y;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i0 = rep_first[r], i1 = rep_first[r + 1];
+  double v = 0.0;
+  const double *pk = partial + (size_t)k * pstride;
+  for (int b = i0 + (int)threadIdx.x; b < i1; b += 256) v += pk[b];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  v = (red[0] + red[1]) + (red[2] + red[3]);
   double *o = out + 20 * r;
-  for (int k = wave; k < ACC_N; k += 16) {
-    double v = 0.0;
-    for (int b = i0 + lane; b < i1; b += 64) v += partial[b * ACC_N + k];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if (lane != 0) continue;
-    if (k < 2) {
-      o[k] += v;
-    } else {
-      const int base = k < ACC_MXX ? 2 : 11;
-      const int u = k < ACC_MXX ? k - ACC_VXX : k - ACC_MXX;
-      // u: 0 xx, 1 xy, 2 xz, 3 yy, 4 yz, 5 zz
-      switch (u) {
-        case 0: o[base + 0] += v; break;
-        case 1: o[base + 1] += v; o[base + 3] += v; break;
-        case 2: o[base + 2] += v; o[base + 6] += v; break;
-        case 3: o[base + 4] += v; break;
-        case 4: o[base + 5] += v; o[base + 7] += v; break;
-        default: o[base + 8] += v; break;
-      }
+  if (k < 2) {
+    o[k] += v;
+  } else {
+    const int base = k < ACC_MXX ? 2 : 11;
+    const int u = k < ACC_MXX ? k - ACC_VXX : k - ACC_MXX;
+    // u: 0 xx, 1 xy, 2 xz, 3 yy, 4 yz, 5 zz
+    switch (u) {
+      case 0: o[base + 0] += v; break;
+      case 1: o[base + 1] += v; o[base + 3] += v; break;
+      case 2: o[base + 2] += v; o[base + 6] += v; break;
+      case 3: o[base + 4] += v; break;
+      case 4: o[base + 5] += v; o[base + 7] += v; break;
+      default: o[base + 8] += v; break;
     }
   }
 }
@@ -173,7 +192,7 @@ template <bool F64, int ELEC, int ELSW, class SH_>
 __global__ void __launch_bounds__(64 * SH_::NW)
 k_tq(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
      const int4 *__restrict__ items, const int *__restrict__ nitems,
-     double *__restrict__ forces, double *__restrict__ partial,
+     double *__restrict__ forces, double *__restrict__ partial, int pstride,
      unsigned long long *__restrict__ npairs) {
   constexpr int NW = SH_::NW, BUF = SH_::BUF, SLOTS = SH_::SLOTS;
   constexpr bool ATOM = SH_::ATOM;
@@ -270,8 +289,8 @@ k_tq(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
         if (e2d > P.cut2) return;
       } else if (r2 >= lo) {
         // near the cutoff: decide with the reference's FP64 minimum image
-        min_diff_exact(R.box, S.x[i], S.y[i], S.z[i], S.x[t], S.y[t], S.z[t],
-                       ex, ey, ez, e2d);
+        min_diff_exact(rare_box(&R.box), S.x[i], S.y[i], S.z[i], S.x[t],
+                       S.y[t], S.z[t], ex, ey, ez, e2d);
         if (e2d > P.cut2) return;
       }
       const int mi = iq.y;
@@ -319,7 +338,7 @@ k_tq(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
       if (molv) {
         const int mj = rq.y;
         double cdx, cdy, cdz, cd2;
-        min_diff_exact(R.box, S.com[3 * mi], S.com[3 * mi + 1],
+        min_diff_exact(rare_box(&R.box), S.com[3 * mi], S.com[3 * mi + 1],
                        S.com[3 * mi + 2], S.com[3 * mj], S.com[3 * mj + 1],
                        S.com[3 * mj + 2], cdx, cdy, cdz, cd2);
         add_virial(acc, ACC_MXX, (double)gx, (double)gy, (double)gz, cdx, cdy,
@@ -401,10 +420,13 @@ k_tq(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
     // and appends the passing entries to the queue.  WX, WY, WZ: per-pair
     // minimum image along that axis.
     const int slot = lane & (kTileAtoms - 1), sub = lane >> 4;
-    auto test_group = [&](auto wxc, auto wyc, auto wzc, int c0, int nt,
-                          int &qn) {
-      constexpr bool WX = decltype(wxc)::value, WY = decltype(wyc)::value,
-                     WZ = decltype(wzc)::value;
+    // WMODE 0: no minimum image, 1: along z, 2: along x, y and z, 3: along
+    // the axes whose flag wx, wy, wz is set.
+    auto test_group = [&](auto mode, int c0, int nt, int &qn) {
+      constexpr int WMODE = decltype(mode)::value;
+      const bool WX = WMODE == 2 || (WMODE == 3 && wx);
+      const bool WY = WMODE == 2 || (WMODE == 3 && wy);
+      const bool WZ = WMODE == 1 || WMODE == 2 || (WMODE == 3 && wz);
 #pragma unroll 4
       for (int j = 0; j < 16; j++) {
         const int k = c0 + 4 * j + sub;
@@ -421,8 +443,6 @@ k_tq(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
         qn += __popcll(b);
       }
     };
-    using T_ = std::true_type;
-    using F_ = std::false_type;
 
     // Tests buffered records against the wave's atoms, queueing and
     // evaluating the pairs that pass (the atom itself passes and is skipped
@@ -435,16 +455,14 @@ k_tq(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
       wave_sync_lds();
       int qn = 0;
       for (int c0 = 0; c0 < nt; c0 += 64) {
-        switch (wmode) {
-          case 0: test_group(F_{}, F_{}, F_{}, c0, nt, qn); break;
-          case 4: test_group(F_{}, F_{}, T_{}, c0, nt, qn); break;
-          case 3: test_group(T_{}, T_{}, F_{}, c0, nt, qn); break;
-          case 7: test_group(T_{}, T_{}, T_{}, c0, nt, qn); break;
-          case 1: test_group(T_{}, F_{}, F_{}, c0, nt, qn); break;
-          case 2: test_group(F_{}, T_{}, F_{}, c0, nt, qn); break;
-          case 5: test_group(T_{}, F_{}, T_{}, c0, nt, qn); break;
-          default: test_group(F_{}, T_{}, T_{}, c0, nt, qn); break;
-        }
+        if (wmode == 0)
+          test_group(std::integral_constant<int, 0>{}, c0, nt, qn);
+        else if (wmode == 4)
+          test_group(std::integral_constant<int, 1>{}, c0, nt, qn);
+        else if (wmode == 7)
+          test_group(std::integral_constant<int, 2>{}, c0, nt, qn);
+        else
+          test_group(std::integral_constant<int, 3>{}, c0, nt, qn);
         if (qn > kQCap - kTileAtoms * 64) {
           wave_sync_lds();
           eval_queue(qn);
@@ -469,9 +487,9 @@ k_tq(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
 
     for (int oc = 0; oc < 9; oc++) {
       const int ox = oc / 3 - 1, oy = oc % 3 - 1;
-      // an axis with 1 or 2 columns takes each distinct column once
-      if (wx && (G.nx == 1 ? ox != 0 : ox == -1)) continue;
-      if (wy && (G.ny == 1 ? oy != 0 : oy == -1)) continue;
+      // an axis with a single column takes it once
+      if (wx && ox != 0) continue;
+      if (wy && oy != 0) continue;
       // column prefilter: gap from the wave box to the neighbour column
       const float gx = wx ? 0.f : fmaxf(0.f, fmaxf((float)ox * G.csx - bx1,
                                                   bx0 - (float)(ox + 1) * G.csx));
@@ -555,7 +573,7 @@ k_tq(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
     for (int k = 0; k < ACC_N; k++) {
       double v = acc[k];
       for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-      if (lane == 0) partial[(size_t)item_idx * ACC_N + k] = v;
+      if (lane == 0) partial[(size_t)k * pstride + item_idx] = v;
     }
     wave_sync_lds();  // facc and the buffers are reused by the next item
   }

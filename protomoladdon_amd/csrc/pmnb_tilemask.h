@@ -1,0 +1,731 @@
+// Tile kernel with per-lane pair masks (k_tm), for one system or a batch of
+// replicas (TileRep, pmnb_queue.h).
+//
+// One wave handles 16 consecutive sorted atoms of one column, 4 lanes per
+// atom (lane = atom slot + 16 * split).  Candidates of the neighbour columns
+// stream into an LDS buffer as in k_tq.  When the buffer is nearly full each
+// lane tests every 4th buffered candidate against its own atom into a 64-bit
+// mask, then evaluates the set bits one by one, reading the candidate from
+// LDS (the next candidate is read before the current pair's arithmetic).
+// Forces stay in registers: FP32 partials per buffer, added into 64-bit
+// fixed point (mixed) or FP64 sums, and the 4 lanes of an atom combine by
+// shuffles at the end.  Pairs within `band` of the cutoff are re-decided
+// with the reference's FP64 minimum image.  Every pair is visited from both
+// atoms; energies and virials take half.  Work items are walked with a grid
+// stride; each item writes one partial row of energies and virials.
+//
+// Against k_tq this trades lane idling (a lane with few pairs waits for the
+// lane of its wave with most) for no LDS traffic per pair and higher
+// occupancy; the autotuner picks between them.
+#pragma once
+
+#include "pmnb_queue.h"
+
+namespace pmnb {
+
+constexpr int kTmSplit = 4;                 // lanes per atom
+constexpr int kTmWaves = kBlock / 64;       // waves per block
+
+// Tests buffered records split, split + 4, ... (< n) against one atom.  The
+// result bits are shifted in, so record split + 4 j sits at bit
+// (c0 - 1 - j) of lo for j < c0 = min(32, jmax) and at bit (c1 - 1 - (j -
+// 32)) of hi for the rest (c1 = jmax - c0).  Records n .. 4 jmax - 1 hold a
+// far sentinel; the atom itself passes and is skipped by the pair code.
+// WMODE 0: no minimum image, 1: along z, 2: along x, y and z, 3: along the
+// axes whose runtime flag wx, wy, wz is set.
+template <int WMODE>
+__device__ __forceinline__ void tm_test(const float4 *__restrict__ rec, int n,
+                                        int split, float xi, float yi,
+                                        float zi, float hi, const float *L,
+                                        const float *iL, bool wx, bool wy,
+                                        bool wz, unsigned &lo,
+                                        unsigned &hi_bits) {
+  const int jmax = (n + kTmSplit - 1) / kTmSplit;  // wave-uniform
+  const int c0 = min(jmax, 32);
+  const bool WX = WMODE == 2 || (WMODE == 3 && wx);
+  const bool WY = WMODE == 2 || (WMODE == 3 && wy);
+  const bool WZ = WMODE == 1 || WMODE == 2 || (WMODE == 3 && wz);
+  auto test = [&](int j) -> unsigned {
+    const float4 r = rec[split + kTmSplit * j];
+    float dx = r.x - xi, dy = r.y - yi, dz = r.z - zi;
+    if (WX) dx = fmaf(-L[0], rintf(dx * iL[0]), dx);
+    if (WY) dy = fmaf(-L[1], rintf(dy * iL[1]), dy);
+    if (WZ) dz = fmaf(-L[2], rintf(dz * iL[2]), dz);
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    return (unsigned)(r2 <= hi);
+  };
+  unsigned m = 0u;
+#pragma unroll 4
+  for (int j = 0; j < c0; j++) m = (m << 1) + test(j);
+  lo = m;
+  m = 0u;
+#pragma unroll 4
+  for (int j = 32; j < jmax; j++) m = (m << 1) + test(j);
+  hi_bits = m;
+}
+
+template <bool F64, int ELEC, int ELSW, int BUF>
+__global__ void __launch_bounds__(kBlock)
+k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
+     const int4 *__restrict__ items, const int *__restrict__ nitems,
+     double *__restrict__ forces, double *__restrict__ partial, int pstride,
+     unsigned long long *__restrict__ npairs) {
+  __shared__ float4 recf[kTmWaves][BUF];
+  __shared__ int4 reci[kTmWaves][BUF];   // sorted index, type, mol, atom
+  __shared__ double4 recd[F64 ? kTmWaves : 1][F64 ? BUF : 1];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const F32Params &F = P.f;
+  const float hi = F.cut2 + F.band, lo = F.cut2 - F.band;
+  const bool vir = (P.flags & 3u) != 0;
+  const int n_items = *nitems;
+  unsigned cnt = 0;
+  for (int item_idx = blockIdx.x * kTmWaves + wave; item_idx < n_items;
+       item_idx += gridDim.x * kTmWaves) {
+    double acc[ACC_N];
+#pragma unroll
+    for (int k = 0; k < ACC_N; k++) acc[k] = 0.0;
+    const int4 item = items[item_idx];
+    const int a0 = uni(item.x), a1 = uni(item.y), col = uni(item.z);
+    const TileRep &R = reps[uni(item.w)];
+    const TileGrid G = R.tg;
+    const int coff = R.cell_off;
+    const bool wx = R.wrap[0] != 0, wy = R.wrap[1] != 0;
+    float L[3], iL[3];
+    for (int k = 0; k < 3; k++) {
+      L[k] = R.L[k];
+      iL[k] = R.iL[k];
+    }
+    const int cx = col / G.ny, cy = col % G.ny;
+    const int slot = lane & (kTileAtoms - 1), split = lane >> 4;
+    const int i = a0 + slot;
+    const bool active = i < a1;
+    const int ii = active ? i : a0;
+    const int zlo = uni((S.cell[a0] - coff) % G.nz);
+    const int zhi = uni((S.cell[a1 - 1] - coff) % G.nz);
+    const float4 li = S.loc[ii];
+    const float xi = li.x, yi = li.y;
+    const float zi = (float)((S.cell[ii] - coff) % G.nz - zlo) * G.csz + li.z;
+    const int ti = S.type[ii], mi = S.mol[ii], ai = S.atom[ii];
+    const double qi = S.q[ii];
+    const double xd = S.x[ii], yd = S.y[ii], zd = S.z[ii];
+    const float4 *ljrow = S.ljf + ti * P.n_types;
+    const double *ljrow64 = S.lj + 4 * (ti * P.n_types);
+    // bounding box of the wave's atoms in home-relative coordinates
+    const float bx0 = uni(wave_min(active ? xi : 1e30f)),
+                bx1 = uni(wave_max(active ? xi : -1e30f));
+    const float by0 = uni(wave_min(active ? yi : 1e30f)),
+                by1 = uni(wave_max(active ? yi : -1e30f));
+    const float bz0 = uni(wave_min(active ? zi : 1e30f)),
+                bz1 = uni(wave_max(active ? zi : -1e30f));
+    int zA = zlo - G.rz, zB = zhi + G.rz;
+    const bool wz = zB - zA + 1 > G.nz;  // all z cells: per-pair image
+    if (wz) zB = zA + G.nz - 1;
+    long long Fx = 0, Fy = 0, Fz = 0;
+    double fx64 = 0.0, fy64 = 0.0, fz64 = 0.0;
+    int nbuf = 0;
+
+    // FP32 partials of one buffer (flushed by process)
+    float px = 0.f, py = 0.f, pz = 0.f, pelj = 0.f, peel = 0.f;
+    float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
+
+    // One pair: candidate record k of this wave's buffer.
+    auto eval = [&](const float4 &r, const int4 &q4, int k) {
+      const int t = q4.x;
+      if (t == i) return;
+      float dx = r.x - xi, dy = r.y - yi, dz = r.z - zi;
+      if (wx) dx = fmaf(-L[0], rintf(dx * iL[0]), dx);
+      if (wy) dy = fmaf(-L[1], rintf(dy * iL[1]), dy);
+      if (wz) dz = fmaf(-L[2], rintf(dz * iL[2]), dz);
+      const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+      double ex = 0.0, ey = 0.0, ez = 0.0, e2d = 0.0;
+      if (F64) {
+        const double4 rd = recd[wave][k];
+        min_diff_exact(R.box, xd, yd, zd, rd.x, rd.y, rd.z, ex, ey, ez, e2d);
+        if (e2d > P.cut2) return;
+      } else if (r2 >= lo) {
+        // near the cutoff: decide with the reference's FP64 minimum image
+        min_diff_exact(rare_box(&R.box), xd, yd, zd, S.x[t], S.y[t], S.z[t],
+                       ex, ey, ez, e2d);
+        if (e2d > P.cut2) return;
+      }
+      const bool same = q4.z == mi;
+      const int excl =
+          same ? excl_class(S.xfull, S.xmod, P.max_excl_delta, ai, q4.w) : 0;
+      if (excl == 2) return;
+      if (i < t) cnt++;
+      const bool molv = (P.flags & 2u) && !same;
+      if (F64) {
+        const double *lp = ljrow64 + 4 * q4.y;
+        double ff, e1, e2;
+        pair_f64<ELEC, ELSW>(P, e2d, excl != 1 ? lp[0] : lp[2],
+                             excl != 1 ? lp[1] : lp[3], qi * recd[wave][k].w,
+                             excl, ff, e1, e2);
+        acc[ACC_ELJ] += 0.5 * e1;
+        acc[ACC_EEL] += 0.5 * e2;
+        const double gx = ex * ff, gy = ey * ff, gz = ez * ff;
+        fx64 -= gx;
+        fy64 -= gy;
+        fz64 -= gz;
+        if (molv || (P.flags & 1u))
+          add_virial(acc, ACC_VXX, gx, gy, gz, ex, ey, ez, 0.5);
+        if (molv) {
+          double cdx, cdy, cdz, cd2;
+          min_diff_exact(rare_box(&R.box), S.com[3 * mi], S.com[3 * mi + 1], S.com[3 * mi + 2], S.com[3 * q4.z], S.com[3 * q4.z + 1], S.com[3 * q4.z + 2],
+                         cdx, cdy, cdz, cd2);
+          add_virial(acc, ACC_MXX, gx, gy, gz, cdx, cdy, cdz, 0.5);
+        }
+      } else {
+        const float4 lp = ljrow[q4.y];
+        float ff, e1, e2;
+        pair_f32<ELEC>(F, r2, excl != 1 ? lp.x : lp.z, excl != 1 ? lp.y : lp.w,
+                       li.w * r.w, excl, ff, e1, e2);
+        pelj += e1;
+        peel += e2;
+        const float gx = dx * ff, gy = dy * ff, gz = dz * ff;
+        px -= gx;
+        py -= gy;
+        pz -= gz;
+        if (vir) {
+          if (molv || (P.flags & 1u)) {
+            v0 = fmaf(gx, dx, v0);
+            v1 = fmaf(gx, dy, v1);
+            v2 = fmaf(gx, dz, v2);
+            v3 = fmaf(gy, dy, v3);
+            v4 = fmaf(gy, dz, v4);
+            v5 = fmaf(gz, dz, v5);
+          }
+          if (molv) {
+            double cdx, cdy, cdz, cd2;
+            min_diff_exact(rare_box(&R.box), S.com[3 * mi], S.com[3 * mi + 1], S.com[3 * mi + 2], S.com[3 * q4.z], S.com[3 * q4.z + 1], S.com[3 * q4.z + 2],
+                           cdx, cdy, cdz, cd2);
+            add_virial(acc, ACC_MXX, gx, gy, gz, cdx, cdy, cdz, 0.5);
+          }
+        }
+      }
+    };
+
+    const int wmode = (wx ? 1 : 0) | (wy ? 2 : 0) | (wz ? 4 : 0);
+    // Tests the buffered candidates against this lane's atom (the atom
+    // itself has r2 == 0 and is left out), then evaluates the pairs that
+    // pass; the next pair's records are read from LDS before the current
+    // pair's arithmetic.
+    auto process = [&](int n) {
+      // records n .. n + 3 (the last group of 4) are made far away
+      if (lane < kTmSplit && n + lane < BUF)
+        recf[wave][n + lane] = make_float4(1e18f, 1e18f, 1e18f, 0.f);
+      wave_sync_lds();
+      unsigned m0 = 0u, m1 = 0u;
+      const float4 *rb = recf[wave];
+      if (wmode == 0)
+        tm_test<0>(rb, n, split, xi, yi, zi, hi, L, iL, wx, wy, wz, m0, m1);
+      else if (wmode == 4)
+        tm_test<1>(rb, n, split, xi, yi, zi, hi, L, iL, wx, wy, wz, m0, m1);
+      else if (wmode == 7)
+        tm_test<2>(rb, n, split, xi, yi, zi, hi, L, iL, wx, wy, wz, m0, m1);
+      else
+        tm_test<3>(rb, n, split, xi, yi, zi, hi, L, iL, wx, wy, wz, m0, m1);
+      if (!active) m0 = m1 = 0u;
+      const int jmax = (n + kTmSplit - 1) / kTmSplit;
+      const int c0 = min(jmax, 32), c1 = jmax - c0;
+      // record index of the lowest set bit (and clear it)
+      auto pop = [&]() -> int {
+        if (m0) {
+          const int b = __builtin_ctz(m0);
+          m0 &= m0 - 1u;
+          return split + kTmSplit * (c0 - 1 - b);
+        }
+        const int b = __builtin_ctz(m1);
+        m1 &= m1 - 1u;
+        return split + kTmSplit * (32 + c1 - 1 - b);
+      };
+      px = py = pz = pelj = peel = 0.f;
+      v0 = v1 = v2 = v3 = v4 = v5 = 0.f;
+      if (m0 | m1) {
+        int k = pop();
+        float4 r = recf[wave][k];
+        int4 q4 = reci[wave][k];
+        while (true) {
+          const bool more = (m0 | m1) != 0u;
+          int kn = k;
+          if (more) kn = pop();
+          const float4 rn = recf[wave][kn];
+          const int4 q4n = reci[wave][kn];
+          eval(r, q4, k);
+          if (!more) break;
+          r = rn;
+          q4 = q4n;
+          k = kn;
+        }
+      }
+      if (!F64) {
+        Fx += to_fixed((double)px);
+        Fy += to_fixed((double)py);
+        Fz += to_fixed((double)pz);
+        acc[ACC_ELJ] += 0.5 * (double)pelj;
+        acc[ACC_EEL] += 0.5 * (double)peel;
+        if (vir) {
+          acc[ACC_VXX] += 0.5 * (double)v0;
+          acc[ACC_VXY] += 0.5 * (double)v1;
+          acc[ACC_VXZ] += 0.5 * (double)v2;
+          acc[ACC_VYY] += 0.5 * (double)v3;
+          acc[ACC_VYZ] += 0.5 * (double)v4;
+          acc[ACC_VZZ] += 0.5 * (double)v5;
+        }
+      }
+      wave_sync_lds();  // buffer is refilled next
+    };
+
+    for (int oc = 0; oc < 9; oc++) {
+      const int ox = oc / 3 - 1, oy = oc % 3 - 1;
+      // an axis with a single column takes it once
+      if (wx && ox != 0) continue;
+      if (wy && oy != 0) continue;
+      // column prefilter: gap from the wave box to the neighbour column
+      const float gx = wx ? 0.f : fmaxf(0.f, fmaxf((float)ox * G.csx - bx1,
+                                                  bx0 - (float)(ox + 1) * G.csx));
+      const float gy = wy ? 0.f : fmaxf(0.f, fmaxf((float)oy * G.csy - by1,
+                                                  by0 - (float)(oy + 1) * G.csy));
+      if (fmaf(gx, gx, gy * gy) > hi) continue;
+      int tx = cx + ox, ty = cy + oy;
+      tx = tx < 0 ? tx + G.nx : (tx >= G.nx ? tx - G.nx : tx);
+      ty = ty < 0 ? ty + G.ny : (ty >= G.ny ? ty - G.ny : ty);
+      const int tbase = coff + (tx * G.ny + ty) * G.nz;
+      const float oxc = (float)ox * G.csx, oyc = (float)oy * G.csy;
+      // z cells zA..zB (unwrapped) as at most two contiguous sorted runs
+      const int wA = ((zA % G.nz) + G.nz) % G.nz;
+      const int len = zB - zA + 1;
+      for (int part = 0; part < 2; part++) {
+        int w0, w1, zshift;
+        if (part == 0) {
+          w0 = wA;
+          w1 = min(G.nz - 1, wA + len - 1);
+          zshift = zA - wA;
+        } else {
+          if (wA + len - 1 < G.nz) break;
+          w0 = 0;
+          w1 = wA + len - 1 - G.nz;
+          zshift = zA - wA + G.nz;
+        }
+        const int r0 = uni(S.cstart[tbase + w0]);
+        const int r1 = uni(S.cstart[tbase + w1] + S.ccount[tbase + w1]);
+        for (int base = r0; base < r1; base += 64) {
+          const int t = base + lane;
+          bool ok = t < r1;
+          float4 rf = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (ok) {
+            const float4 lt = S.loc[t];
+            const int zt = (S.cell[t] - coff) % G.nz + zshift;
+            rf.x = oxc + lt.x;
+            rf.y = oyc + lt.y;
+            rf.z = (float)(zt - zlo) * G.csz + lt.z;
+            rf.w = lt.w;
+            const float gzv = wz ? 0.f : fmaxf(0.f, fmaxf(bz0 - rf.z, rf.z - bz1));
+            const float gxv = wx ? 0.f : fmaxf(0.f, fmaxf(bx0 - rf.x, rf.x - bx1));
+            const float gyv = wy ? 0.f : fmaxf(0.f, fmaxf(by0 - rf.y, rf.y - by1));
+            ok = fmaf(gxv, gxv, fmaf(gyv, gyv, gzv * gzv)) <= hi;
+          }
+          const unsigned long long ball = __ballot(ok);
+          if (ok) {
+            const int pos = nbuf + lane_prefix(ball);
+            recf[wave][pos] = rf;
+            reci[wave][pos] = make_int4(t, S.type[t], S.mol[t], S.atom[t]);
+            if (F64)
+              recd[wave][pos] = make_double4(S.x[t], S.y[t], S.z[t], S.q[t]);
+          }
+          nbuf += __popcll(ball);
+          if (nbuf > BUF - 64) {
+            process(nbuf);
+            nbuf = 0;
+          }
+        }
+      }
+    }
+    if (nbuf > 0) process(nbuf);
+    // the 4 lanes of an atom add their partial forces
+    if (F64) {
+      fx64 += __shfl_down(fx64, 32, 64);
+      fy64 += __shfl_down(fy64, 32, 64);
+      fz64 += __shfl_down(fz64, 32, 64);
+      fx64 += __shfl_down(fx64, 16, 64);
+      fy64 += __shfl_down(fy64, 16, 64);
+      fz64 += __shfl_down(fz64, 16, 64);
+      if (active && split == 0) {
+        forces[3 * ai] += fx64;
+        forces[3 * ai + 1] += fy64;
+        forces[3 * ai + 2] += fz64;
+      }
+    } else {
+      Fx += __shfl_down(Fx, 32, 64);
+      Fy += __shfl_down(Fy, 32, 64);
+      Fz += __shfl_down(Fz, 32, 64);
+      Fx += __shfl_down(Fx, 16, 64);
+      Fy += __shfl_down(Fy, 16, 64);
+      Fz += __shfl_down(Fz, 16, 64);
+      if (active && split == 0) {
+        forces[3 * ai] += (double)Fx * kFixedInv;
+        forces[3 * ai + 1] += (double)Fy * kFixedInv;
+        forces[3 * ai + 2] += (double)Fz * kFixedInv;
+      }
+    }
+    // this item's energies and virials
+#pragma unroll
+    for (int k = 0; k < ACC_N; k++) {
+      double v = acc[k];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+      if (lane == 0) partial[(size_t)k * pstride + item_idx] = v;
+    }
+  }
+  unsigned long long tot = block_count<kTmWaves>(cnt);
+  if (threadIdx.x == 0 && tot) atomicAdd(npairs, tot);
+}
+
+// Single-system form of k_tm: one wave per item, the column grid as a
+// kernel argument, minimum image along z only when the cutoff reaches every
+// z cell (x and y need at least two columns).  Same per-lane work order and
+// per-item partial rows as k_tm, so both give the same bits; it keeps the
+// per-item values in vector registers (no replica table), which measures
+// faster for one large system.
+// Tests buffered records split, split + 4, ... (< n) against one atom.  The
+// result bits are shifted in, so record split + 4 j sits at bit
+// (c0 - 1 - j) of lo for j < c0 = min(32, jmax) and at bit
+// (c1 - 1 - (j - 32)) of hi for the rest (c1 = jmax - c0).  Records n ..
+// 4 jmax - 1 hold a far sentinel; the atom itself passes and is skipped by
+// the pair code.  WRAPZ: minimum image along z.
+template <bool WRAPZ>
+__device__ __forceinline__ void tile_test(const float4 *__restrict__ rec,
+                                          int n, int split, float xi,
+                                          float yi, float zi, float hi,
+                                          float Lz, float iLz, unsigned &lo,
+                                          unsigned &hi_bits) {
+  const int jmax = (n + kTmSplit - 1) / kTmSplit;  // wave-uniform
+  const int c0 = min(jmax, 32);
+  unsigned m = 0u;
+#pragma unroll 4
+  for (int j = 0; j < c0; j++) {
+    const int k = split + kTmSplit * j;
+    const float4 r = rec[k];
+    const float dx = r.x - xi, dy = r.y - yi;
+    float dz = r.z - zi;
+    if (WRAPZ) dz = fmaf(-Lz, rintf(dz * iLz), dz);
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    m = (m << 1) + (unsigned)(r2 <= hi);
+  }
+  lo = m;
+  m = 0u;
+#pragma unroll 4
+  for (int j = 32; j < jmax; j++) {
+    const int k = split + kTmSplit * j;
+    const float4 r = rec[k];
+    const float dx = r.x - xi, dy = r.y - yi;
+    float dz = r.z - zi;
+    if (WRAPZ) dz = fmaf(-Lz, rintf(dz * iLz), dz);
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    m = (m << 1) + (unsigned)(r2 <= hi);
+  }
+  hi_bits = m;
+}
+
+template <bool F64, int ELEC, int ELSW, int BUF>
+__global__ void __launch_bounds__(kBlock)
+k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
+       const int *__restrict__ nitems, double *__restrict__ forces,
+       double *__restrict__ partial, int pstride,
+       unsigned long long *__restrict__ npairs) {
+  __shared__ float4 recf[kTmWaves][BUF];
+  __shared__ int4 reci[kTmWaves][BUF];
+  __shared__ double4 recd[F64 ? kTmWaves : 1][F64 ? BUF : 1];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int item_idx = blockIdx.x * kTmWaves + wave;
+  double acc[ACC_N];
+#pragma unroll
+  for (int k = 0; k < ACC_N; k++) acc[k] = 0.0;
+  unsigned cnt = 0;
+  if (item_idx < *nitems) {
+    const int4 item = items[item_idx];
+    const int a0 = item.x, a1 = item.y, col = item.z;
+    const int cx = col / G.ny, cy = col % G.ny;
+    const int slot = lane & (kTileAtoms - 1), split = lane >> 4;
+    const int i = a0 + slot;
+    const bool active = i < a1;
+    const int ii = active ? i : a0;
+    const F32Params &F = P.f;
+    const float hi = F.cut2 + F.band, lo = F.cut2 - F.band;
+    const int zlo = S.cell[a0] % G.nz, zhi = S.cell[a1 - 1] % G.nz;
+    const float4 li = S.loc[ii];
+    const int zci = S.cell[ii] % G.nz;
+    const float xi = li.x, yi = li.y;
+    const float zi = (float)(zci - zlo) * G.csz + li.z;
+    const int ti = S.type[ii], mi = S.mol[ii], ai = S.atom[ii];
+    const double qi = S.q[ii];
+    const double xd = S.x[ii], yd = S.y[ii], zd = S.z[ii];
+    const float4 *ljrow = S.ljf + ti * P.n_types;
+    const double *ljrow64 = S.lj + 4 * (ti * P.n_types);
+    // bounding box of the wave's atoms in home-relative coordinates
+    const float bx0 = wave_min(active ? xi : 1e30f),
+                bx1 = wave_max(active ? xi : -1e30f);
+    const float by0 = wave_min(active ? yi : 1e30f),
+                by1 = wave_max(active ? yi : -1e30f);
+    const float bz0 = wave_min(active ? zi : 1e30f),
+                bz1 = wave_max(active ? zi : -1e30f);
+    int zA = zlo - G.rz, zB = zhi + G.rz;
+    const bool clampz = zB - zA + 1 > G.nz;
+    if (clampz) zB = zA + G.nz - 1;
+    const float Lz = F.Lz, iLz = F.iLz;
+    long long Fx = 0, Fy = 0, Fz = 0;
+    double fx64 = 0.0, fy64 = 0.0, fz64 = 0.0;
+    const bool vir = (P.flags & 3u) != 0;
+    int nbuf = 0;
+
+    // FP32 partials of one buffer (flushed by process)
+    float px = 0.f, py = 0.f, pz = 0.f, pelj = 0.f, peel = 0.f;
+    float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
+
+    // One pair: candidate record k of this wave's buffer.
+    auto eval = [&](const float4 &r, const int4 &q4, int k) {
+      const int t = q4.x;
+      if (t == i) return;
+      float dx = r.x - xi, dy = r.y - yi, dz = r.z - zi;
+      if (clampz) dz = fmaf(-Lz, rintf(dz * iLz), dz);
+      const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+      double ex = 0.0, ey = 0.0, ez = 0.0, e2d = 0.0;
+      if (F64) {
+        const double4 rd = recd[wave][k];
+        min_diff_exact(*S.box, xd, yd, zd, rd.x, rd.y, rd.z, ex, ey, ez, e2d);
+        if (e2d > P.cut2) return;
+      } else if (r2 >= lo) {
+        // near the cutoff: decide with the reference's FP64 minimum image
+        min_diff_exact(*S.box, xd, yd, zd, S.x[t], S.y[t], S.z[t], ex, ey, ez,
+                       e2d);
+        if (e2d > P.cut2) return;
+      }
+      const bool same = q4.z == mi;
+      const int excl =
+          same ? excl_class(S.xfull, S.xmod, P.max_excl_delta, ai, q4.w) : 0;
+      if (excl == 2) return;
+      if (i < t) cnt++;
+      const bool molv = (P.flags & 2u) && !same;
+      if (F64) {
+        const double *lp = ljrow64 + 4 * q4.y;
+        double ff, e1, e2;
+        pair_f64<ELEC, ELSW>(P, e2d, excl != 1 ? lp[0] : lp[2],
+                             excl != 1 ? lp[1] : lp[3], qi * recd[wave][k].w,
+                             excl, ff, e1, e2);
+        acc[ACC_ELJ] += 0.5 * e1;
+        acc[ACC_EEL] += 0.5 * e2;
+        const double gx = ex * ff, gy = ey * ff, gz = ez * ff;
+        fx64 -= gx;
+        fy64 -= gy;
+        fz64 -= gz;
+        if (molv || (P.flags & 1u))
+          add_virial(acc, ACC_VXX, gx, gy, gz, ex, ey, ez, 0.5);
+        if (molv) {
+          double cdx, cdy, cdz, cd2;
+          min_diff_exact(*S.box, S.com[3 * mi], S.com[3 * mi + 1],
+                         S.com[3 * mi + 2], S.com[3 * q4.z],
+                         S.com[3 * q4.z + 1], S.com[3 * q4.z + 2], cdx, cdy,
+                         cdz, cd2);
+          add_virial(acc, ACC_MXX, gx, gy, gz, cdx, cdy, cdz, 0.5);
+        }
+      } else {
+        const float4 lp = ljrow[q4.y];
+        float ff, e1, e2;
+        pair_f32<ELEC>(F, r2, excl != 1 ? lp.x : lp.z, excl != 1 ? lp.y : lp.w,
+                       li.w * r.w, excl, ff, e1, e2);
+        pelj += e1;
+        peel += e2;
+        const float gx = dx * ff, gy = dy * ff, gz = dz * ff;
+        px -= gx;
+        py -= gy;
+        pz -= gz;
+        if (vir) {
+          if (molv || (P.flags & 1u)) {
+            v0 = fmaf(gx, dx, v0);
+            v1 = fmaf(gx, dy, v1);
+            v2 = fmaf(gx, dz, v2);
+            v3 = fmaf(gy, dy, v3);
+            v4 = fmaf(gy, dz, v4);
+            v5 = fmaf(gz, dz, v5);
+          }
+          if (molv) {
+            double cdx, cdy, cdz, cd2;
+            min_diff_exact(*S.box, S.com[3 * mi], S.com[3 * mi + 1],
+                           S.com[3 * mi + 2], S.com[3 * q4.z],
+                           S.com[3 * q4.z + 1], S.com[3 * q4.z + 2], cdx, cdy,
+                           cdz, cd2);
+            add_virial(acc, ACC_MXX, gx, gy, gz, cdx, cdy, cdz, 0.5);
+          }
+        }
+      }
+    };
+
+    // Tests the buffered candidates against this lane's atom (the atom
+    // itself has r2 == 0 and is left out), then evaluates the pairs that
+    // pass; the next pair's records are read from LDS before the current
+    // pair's arithmetic.
+    auto process = [&](int n) {
+      // records n .. n + 3 (the last group of 4) are made far away
+      if (lane < kTmSplit && n + lane < BUF)
+        recf[wave][n + lane] = make_float4(1e18f, 1e18f, 1e18f, 0.f);
+      wave_sync_lds();
+      unsigned m0 = 0u, m1 = 0u;
+      if (clampz)
+        tile_test<true>(recf[wave], n, split, xi, yi, zi, hi, Lz, iLz, m0, m1);
+      else
+        tile_test<false>(recf[wave], n, split, xi, yi, zi, hi, Lz, iLz, m0, m1);
+      if (!active) m0 = m1 = 0u;
+      const int jmax = (n + kTmSplit - 1) / kTmSplit;
+      const int c0 = min(jmax, 32), c1 = jmax - c0;
+      // record index of the lowest set bit (and clear it)
+      auto pop = [&]() -> int {
+        if (m0) {
+          const int b = __builtin_ctz(m0);
+          m0 &= m0 - 1u;
+          return split + kTmSplit * (c0 - 1 - b);
+        }
+        const int b = __builtin_ctz(m1);
+        m1 &= m1 - 1u;
+        return split + kTmSplit * (32 + c1 - 1 - b);
+      };
+      px = py = pz = pelj = peel = 0.f;
+      v0 = v1 = v2 = v3 = v4 = v5 = 0.f;
+      if (m0 | m1) {
+        int k = pop();
+        float4 r = recf[wave][k];
+        int4 q4 = reci[wave][k];
+        while (true) {
+          const bool more = (m0 | m1) != 0u;
+          int kn = k;
+          if (more) kn = pop();
+          const float4 rn = recf[wave][kn];
+          const int4 q4n = reci[wave][kn];
+          eval(r, q4, k);
+          if (!more) break;
+          r = rn;
+          q4 = q4n;
+          k = kn;
+        }
+      }
+      if (!F64) {
+        Fx += to_fixed((double)px);
+        Fy += to_fixed((double)py);
+        Fz += to_fixed((double)pz);
+        acc[ACC_ELJ] += 0.5 * (double)pelj;
+        acc[ACC_EEL] += 0.5 * (double)peel;
+        if (vir) {
+          acc[ACC_VXX] += 0.5 * (double)v0;
+          acc[ACC_VXY] += 0.5 * (double)v1;
+          acc[ACC_VXZ] += 0.5 * (double)v2;
+          acc[ACC_VYY] += 0.5 * (double)v3;
+          acc[ACC_VYZ] += 0.5 * (double)v4;
+          acc[ACC_VZZ] += 0.5 * (double)v5;
+        }
+      }
+      wave_sync_lds();  // buffer is refilled next
+    };
+
+    for (int oc = 0; oc < 9; oc++) {
+      const int ox = oc / 3 - 1, oy = oc % 3 - 1;
+      // column prefilter: gap from the wave box to the neighbour column
+      const float gx = fmaxf(0.f, fmaxf((float)ox * G.csx - bx1,
+                                        bx0 - (float)(ox + 1) * G.csx));
+      const float gy = fmaxf(0.f, fmaxf((float)oy * G.csy - by1,
+                                        by0 - (float)(oy + 1) * G.csy));
+      if (fmaf(gx, gx, gy * gy) > hi) continue;
+      int tx = cx + ox, ty = cy + oy;
+      tx = tx < 0 ? tx + G.nx : (tx >= G.nx ? tx - G.nx : tx);
+      ty = ty < 0 ? ty + G.ny : (ty >= G.ny ? ty - G.ny : ty);
+      const int tbase = (tx * G.ny + ty) * G.nz;
+      const float oxc = (float)ox * G.csx, oyc = (float)oy * G.csy;
+      // z cells zA..zB (unwrapped) as at most two contiguous sorted runs
+      const int wA = ((zA % G.nz) + G.nz) % G.nz;
+      const int len = zB - zA + 1;
+      for (int part = 0; part < 2; part++) {
+        int w0, w1, zshift;
+        if (part == 0) {
+          w0 = wA;
+          w1 = min(G.nz - 1, wA + len - 1);
+          zshift = zA - wA;
+        } else {
+          if (wA + len - 1 < G.nz) break;
+          w0 = 0;
+          w1 = wA + len - 1 - G.nz;
+          zshift = zA - wA + G.nz;
+        }
+        const int r0 = S.cstart[tbase + w0];
+        const int r1 = S.cstart[tbase + w1] + S.ccount[tbase + w1];
+        for (int base = r0; base < r1; base += 64) {
+          const int t = base + lane;
+          bool ok = t < r1;
+          float4 rf = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (ok) {
+            const float4 lt = S.loc[t];
+            const int zt = S.cell[t] % G.nz + zshift;
+            rf.x = oxc + lt.x;
+            rf.y = oyc + lt.y;
+            rf.z = (float)(zt - zlo) * G.csz + lt.z;
+            rf.w = lt.w;
+            float gzv = fmaxf(0.f, fmaxf(bz0 - rf.z, rf.z - bz1));
+            if (clampz) gzv = 0.f;
+            const float gxv = fmaxf(0.f, fmaxf(bx0 - rf.x, rf.x - bx1));
+            const float gyv = fmaxf(0.f, fmaxf(by0 - rf.y, rf.y - by1));
+            ok = fmaf(gxv, gxv, fmaf(gyv, gyv, gzv * gzv)) <= hi;
+          }
+          const unsigned long long ball = __ballot(ok);
+          if (ok) {
+            const int pos = nbuf + lane_prefix(ball);
+            recf[wave][pos] = rf;
+            reci[wave][pos] = make_int4(t, S.type[t], S.mol[t], S.atom[t]);
+            if (F64) recd[wave][pos] = make_double4(S.x[t], S.y[t], S.z[t],
+                                                    S.q[t]);
+          }
+          nbuf += __popcll(ball);
+          if (nbuf > BUF - 64) {
+            process(nbuf);
+            nbuf = 0;
+          }
+        }
+      }
+    }
+    if (nbuf > 0) process(nbuf);
+    // the 4 lanes of an atom add their partial forces
+    if (F64) {
+      fx64 += __shfl_down(fx64, 32, 64);
+      fy64 += __shfl_down(fy64, 32, 64);
+      fz64 += __shfl_down(fz64, 32, 64);
+      fx64 += __shfl_down(fx64, 16, 64);
+      fy64 += __shfl_down(fy64, 16, 64);
+      fz64 += __shfl_down(fz64, 16, 64);
+      if (active && split == 0) {
+        forces[3 * ai] += fx64;
+        forces[3 * ai + 1] += fy64;
+        forces[3 * ai + 2] += fz64;
+      }
+    } else {
+      Fx += __shfl_down(Fx, 32, 64);
+      Fy += __shfl_down(Fy, 32, 64);
+      Fz += __shfl_down(Fz, 32, 64);
+      Fx += __shfl_down(Fx, 16, 64);
+      Fy += __shfl_down(Fy, 16, 64);
+      Fz += __shfl_down(Fz, 16, 64);
+      if (active && split == 0) {
+        forces[3 * ai] += (double)Fx * kFixedInv;
+        forces[3 * ai + 1] += (double)Fy * kFixedInv;
+        forces[3 * ai + 2] += (double)Fz * kFixedInv;
+      }
+    }
+  }
+  // this item's energies and virials (one row per item, as k_tm)
+  if (item_idx < *nitems) {
+#pragma unroll
+    for (int k = 0; k < ACC_N; k++) {
+      double v = acc[k];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+      if (lane == 0) partial[(size_t)k * pstride + item_idx] = v;
+    }
+  }
+  unsigned long long tot = block_count(cnt);
+  if (threadIdx.x == 0 && tot) atomicAdd(npairs, tot);
+}
+
+}  // namespace pmnb

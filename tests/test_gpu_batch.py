@@ -112,7 +112,7 @@ def test_autotune_keeps_accuracy(replicas):
                       device="cuda")
     shape, grid, ms = bf.autotune(pos, frc, out, 1)
     print("autotune shape", shape, "grid", grid, "ms", ms, flush=True)
-    assert 0 <= shape < 5 and grid >= 0 and ms > 0
+    assert 0 <= shape < 6 and grid >= 0 and ms > 0
     frc.zero_()
     out.zero_()
     bf.evaluate_device(pos, frc, out, 1)
