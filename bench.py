@@ -91,6 +91,178 @@ def cpu_baseline(w, budget_s):
                       "gcc -O2" % (len(times), t)}
 
 
+def bench_batch(args, torch):
+    """512 water replicas (workload 3): one graph replay per step against
+    per-replica sequential launches of the single-system force."""
+    from protomoladdon_amd import systems
+    from protomoladdon_amd.batch import BatchedCutoffForce, OUT_SLOTS
+    print("generating 512 replicas ...", file=sys.stderr, flush=True)
+    sizes = systems.config3_sizes()
+    ws = [systems.config3_replica(i, n) for i, n in enumerate(sizes)]
+    R = len(ws)
+    bf = BatchedCutoffForce.from_workloads(ws, precision="mixed")
+    dev = torch.device("cuda:0")
+    pos = torch.from_numpy(np.concatenate([w.positions for w in ws])).to(dev)
+    frc = torch.zeros_like(pos)
+    out = torch.zeros(R * OUT_SLOTS, dtype=torch.float64, device=dev)
+    bf.evaluate_device(pos, frc, out, 1)
+    torch.cuda.synchronize()
+    print("autotuning ...", file=sys.stderr, flush=True)
+    shape, grid, tune_ms = bf.autotune(pos, frc, out, 1)
+    bf.run_graph(pos, frc, out, 1, n_steps=args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bf.run_graph(pos, frc, out, 1, n_steps=args.steps)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    P = bf.pair_count()
+    fb = frc.cpu().numpy()
+    # per-replica sequential launches of the single-system force
+    print("sequential launches ...", file=sys.stderr, flush=True)
+    singles = [w.make_force("mixed") for w in ws]
+    off = bf.offsets
+    frs = torch.zeros_like(pos)
+    outs = torch.zeros_like(out)
+    sp = torch.cuda.current_stream(dev).cuda_stream
+
+    def seq_step():
+        frs.zero_()
+        outs.zero_()
+        for r, (w, f) in enumerate(zip(ws, singles)):
+            f.evaluate_device(w.topology, pos[off[r]:off[r + 1]],
+                              frs[off[r]:off[r + 1]],
+                              outs[OUT_SLOTS * r:OUT_SLOTS * (r + 1)], 1,
+                              stream=sp)
+    seq_step()
+    torch.cuda.synchronize()
+    nseq = max(3, args.steps // 50)
+    t0 = time.perf_counter()
+    for _ in range(nseq):
+        seq_step()
+    torch.cuda.synchronize()
+    dseq = (time.perf_counter() - t0) / nseq
+    diff = float(np.abs(frs.cpu().numpy() - fb).max())
+    for f in singles:
+        f.close()
+    bf.close()
+    return {
+        "metric": "batched replicas: ns/day per replica and aggregate pair "
+                  "evals/s, 512 waters boxes LJ+Ewald-real, one graph replay "
+                  "per step",
+        "value": ns_per_day(1.0 / dt), "unit": "ns/day per replica",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "512 periodic water replicas, 167-2667 "
+                               "molecules each (density 0.0334/A^3), LJ + "
+                               "Ewald real alpha 0.32, C2 7.5->9 A",
+                   "n_replicas": R, "n_atoms": int(off[-1]),
+                   "pairs_per_step": P},
+        "aggregate_ns_per_day": R * ns_per_day(1.0 / dt),
+        "pair_evals_per_s": P / dt,
+        "sequential_ms_per_step": 1e3 * dseq,
+        "speedup_vs_sequential": dseq / dt,
+        "max_abs_force_diff_vs_sequential": diff,
+        "autotune": {"shape": shape, "grid_blocks": grid, "ms": tune_ms},
+    }
+
+
+def bench_forcematch(args, torch):
+    """Force matching on the 250k-atom chain system (workload 4): loss and
+    its gradient with respect to positions, charges, sigma and epsilon."""
+    from protomoladdon_amd import grad as G
+    from protomoladdon_amd import systems
+    print("generating chains ...", file=sys.stderr, flush=True)
+    w = systems.config2()
+    t = w.topology
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(1234)
+    q0 = t.scaled_charges / 18.2226123264
+    s0 = np.array(systems.CHAIN_SIGMA)
+    e0 = np.array(systems.CHAIN_EPS)
+    qt = q0 * (1 + 0.05 * rng.normal(size=q0.shape))
+    st = s0 * (1 + 0.05 * rng.normal(size=4))
+    et = e0 * (1 + 0.05 * rng.normal(size=4))
+    T = lambda a: torch.as_tensor(a, dtype=torch.float64, device=dev)
+    pos = T(w.positions)
+    res = {}
+    grads = {}
+    for prec in ("mixed", "fp64"):
+        f = w.make_force(prec)
+        with torch.no_grad():
+            target = G.cutoff_forces(f, t, pos, T(qt), T(st), T(et), 0.5)
+
+        def step():
+            x = pos.clone().requires_grad_(True)
+            q = T(q0).requires_grad_(True)
+            s = T(s0).requires_grad_(True)
+            e = T(e0).requires_grad_(True)
+            F = G.cutoff_forces(f, t, x, q, s, e, 0.5)
+            L = G.force_matching_loss(F, target)
+            L.backward()
+            return L, x.grad, q.grad, s.grad, e.grad
+        for _ in range(2):
+            step()
+        torch.cuda.synchronize()
+        n = max(3, args.steps // 20)
+        t0 = time.perf_counter()
+        for _ in range(n):
+            out = step()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / n
+        res[prec] = {"ms_per_gradient": 1e3 * dt, "loss": float(out[0])}
+        grads[prec] = [g.cpu().numpy() for g in out[1:]]
+        f.close()
+
+    def rel(a, b):
+        return float(np.sqrt(np.mean((a - b) ** 2) / np.mean(b ** 2)))
+    errs = {k: rel(a, b) for k, a, b in zip(("x", "q", "sigma", "epsilon"),
+                                            grads["mixed"], grads["fp64"])}
+    # FP64 central differences (h = 1e-6) on 64 random parameters
+    f = w.make_force("fp64")
+    with torch.no_grad():
+        target = G.cutoff_forces(f, t, pos, T(qt), T(st), T(et), 0.5)
+
+    def loss(th):
+        with torch.no_grad():
+            F = G.cutoff_forces(f, t, pos, T(th[8:]), T(th[:4]), T(th[4:8]),
+                                0.5)
+            return float(G.force_matching_loss(F, target))
+    theta = np.concatenate([s0, e0, q0])
+    an = np.concatenate([grads["fp64"][2], grads["fp64"][3], grads["fp64"][1]])
+    pick = np.concatenate([np.arange(8), 8 + rng.choice(len(q0), 56,
+                                                        replace=False)])
+    h = 1e-6
+    worst = 0.0
+    gmax = np.abs(an[pick]).max()
+    for k in pick:
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd = (loss(tp) - loss(tm)) / (2 * h)
+        worst = max(worst, abs(fd - an[k]) / max(abs(fd), 1e-3 * gmax))
+    f.close()
+    ms = res["mixed"]["ms_per_gradient"]
+    return {
+        "metric": "force-matching gradients/s (loss + adjoint), 250k-atom "
+                  "chains LJ+Ewald-real",
+        "value": 1e3 / ms, "unit": "gradients/s", "n_gpus": 1,
+        "steps": max(3, args.steps // 20), "warmup": 2, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "12,500 chains x 20 atoms (250,000), 138 A "
+                               "box, LJ (4 types, LB, 1-4 x 0.5) + Ewald "
+                               "real alpha 0.34, C2 8.5->10 A; target from "
+                               "N(0, 0.05^2) relative parameter noise",
+                   "n_atoms": w.n_atoms},
+        "fp64": res["fp64"], "mixed": res["mixed"],
+        "speedup_vs_fp64": res["fp64"]["ms_per_gradient"] / ms,
+        "rms_rel_grad_err_mixed_vs_fp64": errs,
+        "fd_check_64_params_worst_rel_err": worst,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +272,11 @@ def main():
                     help="steps of the FP64 path (default: steps/4)")
     ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of CPU baseline sampling (0 = skip)")
+    ap.add_argument("--workload", default="water96k",
+                    choices=["water96k", "batch512", "forcematch250k"],
+                    help="water96k: the headline line; batch512 and "
+                         "forcematch250k: the batched-replica and "
+                         "differentiation workloads")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB "
                          "(3 times): the reference kernel that scales the "
@@ -122,6 +299,13 @@ def main():
         torch.cuda.synchronize()
         print("copied %d MiB x3" % args.copy_mib, file=sys.stderr, flush=True)
         del src, dst
+
+    if args.workload == "batch512":
+        print(json.dumps(bench_batch(args, torch)), flush=True)
+        return
+    if args.workload == "forcematch250k":
+        print(json.dumps(bench_forcematch(args, torch)), flush=True)
+        return
 
     print("generating workload ...", file=sys.stderr, flush=True)
     w = systems.config1()
@@ -181,7 +365,7 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "k_tq<mixed, Ewald-real, C2>",
+                     "kernel": "k_tile<mixed, Ewald-real, C2>",
                      "kernel_ms": mx["kernel_ms"],
                      "list_build_ms": mx["build_ms"],
                      "achieved_incl_list_build": P * FLOP_PER_PAIR / (

@@ -473,7 +473,7 @@ int ensure_eval_buffers(pmnb_ctx *ctx) {
   HIPCHK(ctx->d_cell_of.ensure(N * sizeof(int)));
   HIPCHK(ctx->d_count.ensure((size_t)nc * sizeof(int)));
   HIPCHK(ctx->d_start.ensure((size_t)nc * sizeof(int)));
-  HIPCHK(ctx->d_fill.ensure((size_t)nc * sizeof(int)));
+  HIPCHK(ctx->d_fill.ensure(N * sizeof(int)));  // per-atom rank in its cell
   HIPCHK(ctx->d_sorted.ensure(N * sizeof(int)));
   HIPCHK(ctx->d_sx.ensure(N * sizeof(double)));
   HIPCHK(ctx->d_sy.ensure(N * sizeof(double)));
@@ -693,24 +693,25 @@ int build_cells(pmnb_ctx *ctx, const double *d_pos, hipStream_t s) {
   const int n = ctx->n, nc = ctx->grid.ncell;
   const TileRep *reps = ctx->d_reps.as<TileRep>();
   const int *rep_of = ctx->batch ? ctx->d_rep_of.as<int>() : nullptr;
+  int *rank = ctx->d_fill.as<int>();
   hipLaunchKernelGGL(k_clear, dim3(blocks_for(nc)), dim3(kBlock), 0, s,
-                     ctx->d_count.as<int>(), ctx->d_fill.as<int>(), nc,
+                     ctx->d_count.as<int>(), nc,
                      ctx->d_npairs.as<unsigned long long>());
   if (n > 0 && ctx->use_tile) {
     hipLaunchKernelGGL(k_assign_r<TileRep>, dim3(blocks_for(n)), dim3(kBlock),
                        0, s, reps, rep_of, d_pos, n, ctx->d_cell_of.as<int>(),
-                       ctx->d_count.as<int>());
+                       ctx->d_count.as<int>(), rank);
   } else if (n > 0) {
     hipLaunchKernelGGL(k_assign, dim3(blocks_for(n)), dim3(kBlock), 0, s,
                        ctx->box, ctx->grid, d_pos, n, ctx->d_cell_of.as<int>(),
-                       ctx->d_count.as<int>());
+                       ctx->d_count.as<int>(), rank);
   }
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, ctx->d_count.as<int>(),
                      ctx->d_start.as<int>(), nc);
   if (n > 0) {
     hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, s,
                        ctx->d_cell_of.as<int>(), n, ctx->d_start.as<int>(),
-                       ctx->d_fill.as<int>(), ctx->d_sorted.as<int>());
+                       rank, ctx->d_sorted.as<int>());
     hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(nc, kBlock / 64)),
                        dim3(kBlock), 0, s,
                        ctx->d_start.as<int>(), ctx->d_count.as<int>(), nc,
@@ -822,7 +823,7 @@ int run_device(pmnb_ctx *ctx, const double *d_pos, double *d_forces,
       launch_pair_tq(mixed, el, sw, ctx->shape, a);
     }
     if (ev_stop) HIPCHK(hipEventRecord(ev_stop, s));
-    hipLaunchKernelGGL(k_finish_rep, dim3(nrep, ACC_N), dim3(256), 0, s, part,
+    hipLaunchKernelGGL(k_finish_rep, dim3(nrep, ACC_N), dim3(1024), 0, s, part,
                        max_items, rep_first, d_out);
     HIPCHK(hipGetLastError());
     return PMNB_OK;

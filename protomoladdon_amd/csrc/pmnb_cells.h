@@ -56,9 +56,11 @@ __device__ __forceinline__ void cell_coords(const Box &b, const Grid &g,
   else if (rz < 0) vz += b.H[2];
 }
 
+// Cell of every atom; rank[i] is the atom's arrival rank in its cell (any
+// order: k_sort_cells restores ascending atom order afterwards).
 __global__ void k_assign(Box b, Grid g, const double *__restrict__ pos,
                          int n, int *__restrict__ cell_of,
-                         int *__restrict__ count) {
+                         int *__restrict__ count, int *__restrict__ rank) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int cx, cy, cz;
@@ -67,7 +69,7 @@ __global__ void k_assign(Box b, Grid g, const double *__restrict__ pos,
               vx, vy, vz);
   int c = (cx * g.n[1] + cy) * g.n[2] + cz;
   cell_of[i] = c;
-  atomicAdd(&count[c], 1);
+  rank[i] = atomicAdd(&count[c], 1);
 }
 
 // Exclusive scan of count[0..n) into start[0..n) by one block of 1024.
@@ -96,12 +98,11 @@ k_scan(const int *__restrict__ count, int *__restrict__ start, int n) {
 
 __global__ void k_scatter(const int *__restrict__ cell_of, int n,
                           const int *__restrict__ start,
-                          int *__restrict__ fill, int *__restrict__ sorted) {
+                          const int *__restrict__ rank,
+                          int *__restrict__ sorted) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  int c = cell_of[i];
-  int slot = atomicAdd(&fill[c], 1);
-  sorted[start[c] + slot] = i;
+  sorted[start[cell_of[i]] + rank[i]] = i;
 }
 
 // Ascending atom order inside each cell (the reference's list order): one
@@ -141,13 +142,10 @@ k_sort_cells(const int *__restrict__ start, const int *__restrict__ count,
 }
 
 // Clears the per-cell counters and the pair counter in one launch.
-__global__ void k_clear(int *__restrict__ count, int *__restrict__ fill,
-                        int ncell, unsigned long long *__restrict__ npairs) {
+__global__ void k_clear(int *__restrict__ count, int ncell,
+                        unsigned long long *__restrict__ npairs) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < ncell) {
-    count[i] = 0;
-    fill[i] = 0;
-  }
+  if (i < ncell) count[i] = 0;
   if (i == 0) *npairs = 0ull;
 }
 
@@ -194,7 +192,8 @@ template <class RepT>
 __global__ void k_assign_r(const RepT *__restrict__ reps,
                            const int *__restrict__ rep_of,
                            const double *__restrict__ pos, int n,
-                           int *__restrict__ cell_of, int *__restrict__ count) {
+                           int *__restrict__ cell_of, int *__restrict__ count,
+                           int *__restrict__ rank) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const RepT &R = reps[rep_of ? rep_of[i] : 0];
@@ -204,7 +203,7 @@ __global__ void k_assign_r(const RepT *__restrict__ reps,
               cy, cz, vx, vy, vz);
   int c = R.cell_off + (cx * R.grid.n[1] + cy) * R.grid.n[2] + cz;
   cell_of[i] = c;
-  atomicAdd(&count[c], 1);
+  rank[i] = atomicAdd(&count[c], 1);
 }
 
 template <class RepT>

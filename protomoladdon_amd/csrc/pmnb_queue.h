@@ -146,21 +146,31 @@ k_items_r(const TileRep *__restrict__ reps, const int *__restrict__ col_rep,
 // partial[k * pstride + b]), one block per (replica, accumulator) with a
 // fixed reduction tree; out[20 r + ..]:
 // [0] LJ, [1] electrostatic, [2..10] virial 3x3, [11..19] molecular virial.
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 k_finish_rep(const double *__restrict__ partial, int pstride,
              const int *__restrict__ rep_first, double *__restrict__ out) {
-  __shared__ double red[4];
+  __shared__ double red[16];
   const int r = blockIdx.x, k = blockIdx.y;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i0 = rep_first[r], i1 = rep_first[r + 1];
-  double v = 0.0;
   const double *pk = partial + (size_t)k * pstride;
-  for (int b = i0 + (int)threadIdx.x; b < i1; b += 256) v += pk[b];
+  // four independent sums per thread, so the loads overlap
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+  int b = i0 + (int)threadIdx.x;
+  for (; b + 3072 < i1; b += 4096) {
+    v0 += pk[b];
+    v1 += pk[b + 1024];
+    v2 += pk[b + 2048];
+    v3 += pk[b + 3072];
+  }
+  for (; b < i1; b += 1024) v0 += pk[b];
+  double v = (v0 + v1) + (v2 + v3);
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   if (lane == 0) red[wave] = v;
   __syncthreads();
   if (threadIdx.x != 0) return;
-  v = (red[0] + red[1]) + (red[2] + red[3]);
+  v = 0.0;
+  for (int w = 0; w < 16; w++) v += red[w];
   double *o = out + 20 * r;
   if (k < 2) {
     o[k] += v;
