@@ -327,6 +327,14 @@ def main():
     achieved = P * FLOP_PER_PAIR / (mx["kernel_ms"] * 1e-3) / 1e12
     achieved64 = P * FLOP_PER_PAIR / (f64["kernel_ms"] * 1e-3) / 1e12
 
+    # HBM bytes per launch of the pair kernel, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE passes (scaled by a copy kernel of known size)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_tile_water96k_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+
     cpu = None
     if args.cpu_budget > 0:
         print("cpu baseline ...", file=sys.stderr, flush=True)
@@ -364,7 +372,8 @@ def main():
         "rms_rel_force_err_vs_fp64": rms,
         "roofline": {"bound": "valu", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "traffic_unit": "bytes per launch (HBM, rocprofv3)",
                      "kernel": "k_tile<mixed, Ewald-real, C2>",
                      "kernel_ms": mx["kernel_ms"],
                      "list_build_ms": mx["build_ms"],

@@ -105,18 +105,37 @@ __global__ void k_scatter(const int *__restrict__ cell_of, int n,
   sorted[start[cell_of[i]] + rank[i]] = i;
 }
 
+constexpr int kSortLds = 512;  // cell entries a wave ranks in LDS
+
 // Ascending atom order inside each cell (the reference's list order): one
 // wave per cell sorts up to 64 entries with a bitonic network across lanes;
+// up to kSortLds entries by rank (each lane counts, for its entries, the
+// smaller keys of the cell held in LDS; keys are distinct atom indices);
 // larger cells fall back to an insertion sort by one lane.
 __global__ void __launch_bounds__(kBlock)
 k_sort_cells(const int *__restrict__ start, const int *__restrict__ count,
              int ncell, int *__restrict__ sorted) {
-  const int c = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  __shared__ int keys[kBlock / 64][kSortLds];
+  const int w = threadIdx.x >> 6;
+  const int c = blockIdx.x * (kBlock / 64) + w;
   const int lane = threadIdx.x & 63;
   if (c >= ncell) return;
   const int m = count[c];
   if (m <= 1) return;
   int *seg = sorted + start[c];
+  if (m > 64 && m <= kSortLds) {
+    for (int k = lane; k < m; k += 64) keys[w][k] = seg[k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int k = lane; k < m; k += 64) {
+      const int v = keys[w][k];
+      int rank = 0;
+      for (int j = 0; j < m; j++) rank += keys[w][j] < v;
+      seg[rank] = v;
+    }
+    return;
+  }
   if (m <= 64) {
     int v = lane < m ? seg[lane] : 0x7fffffff;
     for (int k = 2; k <= 64; k <<= 1) {
