@@ -318,6 +318,20 @@ def main():
     f64 = time_gpu(w, "fp64", fp64_steps, max(2, args.warmup // 4), flags,
                    torch)
 
+    # the host-array entry point (pmnb_evaluate: positions in, forces out
+    # through PCIe, synchronous), for the record; never the headline value
+    from protomoladdon_amd.topology import ScalarStructure
+    fh = w.make_force("mixed")
+    fr = np.zeros((w.n_atoms, 3))
+    es = ScalarStructure(virial=True)
+    fh.evaluate(w.topology, w.positions, fr, es)
+    nh = max(5, args.steps // 20)
+    t0 = time.perf_counter()
+    for _ in range(nh):
+        fh.evaluate(w.topology, w.positions, fr, es)
+    host_ms = 1e3 * (time.perf_counter() - t0) / nh
+    fh.close()
+
     sps = 1e3 / mx["ms_per_step"]
     sps64 = 1e3 / f64["ms_per_step"]
     P = mx["pairs"]
@@ -369,6 +383,7 @@ def main():
                      "pair_kernel_tflops": achieved64,
                      "pairs_per_step": f64["pairs"]},
         "speedup_vs_fp64": sps / sps64,
+        "host_api_ms_per_step": host_ms,
         "rms_rel_force_err_vs_fp64": rms,
         "roofline": {"bound": "valu", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
