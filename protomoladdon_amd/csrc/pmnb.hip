@@ -78,7 +78,7 @@ struct pmnb_ctx {
   // device: per evaluation
   DevBuf d_pos, d_force, d_out, d_com, d_cell_of, d_count, d_start, d_fill,
       d_sorted, d_sx, d_sy, d_sz, d_loc, d_stype, d_sq, d_smol, d_scell,
-      d_shell, d_partial, d_npairs, d_pairs, d_list, d_ncount, d_partial2,
+      d_srec, d_shell, d_partial, d_npairs, d_pairs, d_list, d_ncount, d_partial2,
       d_shell_off, d_box;
   int shell_uploaded = 0;
   std::vector<float4> shell_off;  // FP32 cell offsets of the shell entries
@@ -483,6 +483,7 @@ int ensure_eval_buffers(pmnb_ctx *ctx) {
   HIPCHK(ctx->d_sq.ensure(N * sizeof(double)));
   HIPCHK(ctx->d_smol.ensure(N * sizeof(int)));
   HIPCHK(ctx->d_scell.ensure(N * sizeof(int)));
+  if (ctx->use_tile) HIPCHK(ctx->d_srec.ensure(N * sizeof(int4)));
   const int ncol_t = ctx->use_tile ? ctx->ncol_total : 0;
   const int tile_items = n / kTileAtoms + ncol_t + 1;  // partial row per item
   HIPCHK(ctx->d_partial.ensure((size_t)std::max(blocks_for(n), tile_items) *
@@ -574,7 +575,7 @@ template <bool F64, int EL, int SW>
 int launch_tq(int shape, const TqArgs &a) {
   if (shape == 0 && a.single) {
     const int nb = std::max(1, blocks_for(a.max_items, kTmWaves));
-    hipLaunchKernelGGL((k_tile<F64, EL, SW, F64 ? 128 : 256>), dim3(nb),
+    hipLaunchKernelGGL((k_tile<F64, EL, SW, F64 ? kTmBuf64 : kTmBuf32>), dim3(nb),
                        dim3(kBlock), 0, a.s, *a.P, *a.S, a.G, a.items,
                        a.nitems, a.forces, a.partial, a.pstride, a.npairs);
     return nb;
@@ -583,7 +584,7 @@ int launch_tq(int shape, const TqArgs &a) {
     int nb = blocks_for(a.max_items, kTmWaves);
     if (a.grid_cap > 0) nb = std::min(nb, a.grid_cap);
     nb = std::max(nb, 1);
-    hipLaunchKernelGGL((k_tm<F64, EL, SW, F64 ? 128 : 256>), dim3(nb),
+    hipLaunchKernelGGL((k_tm<F64, EL, SW, F64 ? kTmBuf64 : kTmBuf32>), dim3(nb),
                        dim3(kBlock), 0, a.s, *a.P, *a.S, a.reps, a.items,
                        a.nitems, a.forces, a.partial, a.pstride, a.npairs);
     return nb;
@@ -725,7 +726,7 @@ int build_cells(pmnb_ctx *ctx, const double *d_pos, hipStream_t s) {
                          ctx->d_sy.as<double>(), ctx->d_sz.as<double>(),
                          ctx->d_loc.as<float4>(), ctx->d_stype.as<int>(),
                          ctx->d_sq.as<double>(), ctx->d_smol.as<int>(),
-                         ctx->d_scell.as<int>());
+                         ctx->d_scell.as<int>(), ctx->d_srec.as<int4>());
     else
     hipLaunchKernelGGL(k_gather, dim3(blocks_for(n)), dim3(kBlock), 0, s,
                        ctx->box, ctx->grid, d_pos, ctx->d_sorted.as<int>(),
@@ -751,6 +752,7 @@ Sorted make_sorted(const pmnb_ctx *ctx, const double *d_com) {
   S.q = ctx->d_sq.as<double>();
   S.mol = ctx->d_smol.as<int>();
   S.cell = ctx->d_scell.as<int>();
+  S.rec = ctx->d_srec.as<int4>();
   S.cstart = ctx->d_start.as<int>();
   S.ccount = ctx->d_count.as<int>();
   S.delta = ctx->d_shell.as<int4>();
@@ -1254,7 +1256,7 @@ void pmnb_destroy(pmnb_ctx *ctx) {
                     &ctx->d_count, &ctx->d_start, &ctx->d_fill, &ctx->d_sorted,
                     &ctx->d_sx, &ctx->d_sy, &ctx->d_sz, &ctx->d_loc,
                     &ctx->d_stype, &ctx->d_sq, &ctx->d_smol, &ctx->d_scell,
-                    &ctx->d_shell, &ctx->d_partial, &ctx->d_npairs,
+                    &ctx->d_srec, &ctx->d_shell, &ctx->d_partial, &ctx->d_npairs,
                     &ctx->d_pairs, &ctx->d_list, &ctx->d_ncount,
                     &ctx->d_partial2, &ctx->d_shell_off, &ctx->d_box,
                     &ctx->d_items,

@@ -237,11 +237,13 @@ __global__ void k_gather_r(const RepT *__restrict__ reps,
                            double *__restrict__ sx, double *__restrict__ sy,
                            double *__restrict__ sz, float4 *__restrict__ sloc,
                            int *__restrict__ stype, double *__restrict__ sq,
-                           int *__restrict__ smol, int *__restrict__ scell) {
+                           int *__restrict__ smol, int *__restrict__ scell,
+                           int4 *__restrict__ srec) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   int a = sorted[s];
   const RepT &R = reps[rep_of ? rep_of[a] : 0];
+  srec[s] = make_int4(s, type[a], mol[a], a);
   double x = pos[3 * a], y = pos[3 * a + 1], z = pos[3 * a + 2];
   sx[s] = x;
   sy[s] = y;

@@ -25,23 +25,74 @@ namespace pmnb {
 
 constexpr int kTmSplit = 4;                 // lanes per atom
 constexpr int kTmWaves = kBlock / 64;       // waves per block
+// Waves per SIMD the register allocation must allow (FP32 / FP64 kernels).
+#ifndef PMNB_TM_OCC32
+#define PMNB_TM_OCC32 3
+#endif
+#ifndef PMNB_TM_OCC64
+#define PMNB_TM_OCC64 3
+#endif
+// Candidate records buffered per wave (FP32 / FP64 kernels): a larger
+// buffer evens out the lanes' pair counts per buffer; 3 blocks per CU must
+// still fit in LDS (32 / 64 bytes per record, 4 waves per block).
+#ifndef PMNB_TM_BUF32
+#define PMNB_TM_BUF32 256
+#endif
+#ifndef PMNB_TM_BUF64
+#define PMNB_TM_BUF64 192
+#endif
+constexpr int kTmBuf32 = PMNB_TM_BUF32, kTmBuf64 = PMNB_TM_BUF64;
 
-// Tests buffered records split, split + 4, ... (< n) against one atom.  The
-// result bits are shifted in, so record split + 4 j sits at bit
-// (c0 - 1 - j) of lo for j < c0 = min(32, jmax) and at bit (c1 - 1 - (j -
-// 32)) of hi for the rest (c1 = jmax - c0).  Records n .. 4 jmax - 1 hold a
-// far sentinel; the atom itself passes and is skipped by the pair code.
+// Per-lane pass masks over the buffered records split + 4 j, j < jmax, in
+// NW 32-bit words: word w holds j = 32 w .. 32 w + cw - 1 (cw = min(32,
+// jmax - 32 w)) with record j at bit cw - 1 - (j - 32 w), so the test
+// shifts its bits in.  Records n .. 4 jmax - 1 hold a far sentinel; the
+// atom itself passes and is skipped by the pair code.
+// (Both walk the words by compile-time recursion, so the masks stay in
+// registers.)
+template <int W, int NW, class Test>
+__device__ __forceinline__ void mask_test(int jmax, unsigned (&m)[NW],
+                                          Test &test) {
+  unsigned mm = 0u;
+  const int j1 = min(jmax, 32 * W + 32);
+#pragma unroll 4
+  for (int j = 32 * W; j < j1; j++) mm = (mm << 1) + test(j);
+  m[W] = mm;
+  if constexpr (W + 1 < NW) mask_test<W + 1, NW>(jmax, m, test);
+}
+
+// Record index of the lowest set bit of the first non-empty word, which is
+// cleared.  At least one bit must be set.
+template <int W, int NW>
+__device__ __forceinline__ int mask_pop(unsigned (&m)[NW], int split,
+                                        int jmax) {
+  if constexpr (W + 1 < NW) {
+    if (!m[W]) return mask_pop<W + 1, NW>(m, split, jmax);
+  }
+  const int b = __builtin_ctz(m[W]);
+  m[W] &= m[W] - 1u;
+  return split + kTmSplit * (32 * W + min(32, jmax - 32 * W) - 1 - b);
+}
+
+template <int NW>
+__device__ __forceinline__ bool mask_any(const unsigned (&m)[NW]) {
+  unsigned a = m[0];
+  if constexpr (NW > 1) a |= m[1];
+  if constexpr (NW > 2) a |= m[2];
+  if constexpr (NW > 3) a |= m[3];
+  static_assert(NW <= 4, "at most 4 mask words");
+  return a != 0u;
+}
+
+// Tests the buffered records against one atom (mask_test layout).
 // WMODE 0: no minimum image, 1: along z, 2: along x, y and z, 3: along the
 // axes whose runtime flag wx, wy, wz is set.
-template <int WMODE>
+template <int WMODE, int NW>
 __device__ __forceinline__ void tm_test(const float4 *__restrict__ rec, int n,
                                         int split, float xi, float yi,
                                         float zi, float hi, const float *L,
                                         const float *iL, bool wx, bool wy,
-                                        bool wz, unsigned &lo,
-                                        unsigned &hi_bits) {
-  const int jmax = (n + kTmSplit - 1) / kTmSplit;  // wave-uniform
-  const int c0 = min(jmax, 32);
+                                        bool wz, unsigned (&m)[NW]) {
   const bool WX = WMODE == 2 || (WMODE == 3 && wx);
   const bool WY = WMODE == 2 || (WMODE == 3 && wy);
   const bool WZ = WMODE == 1 || WMODE == 2 || (WMODE == 3 && wz);
@@ -54,22 +105,112 @@ __device__ __forceinline__ void tm_test(const float4 *__restrict__ rec, int n,
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     return (unsigned)(r2 <= hi);
   };
-  unsigned m = 0u;
-#pragma unroll 4
-  for (int j = 0; j < c0; j++) m = (m << 1) + test(j);
-  lo = m;
-  m = 0u;
-#pragma unroll 4
-  for (int j = 32; j < jmax; j++) m = (m << 1) + test(j);
-  hi_bits = m;
+  mask_test<0, NW>((n + kTmSplit - 1) / kTmSplit, m, test);
+}
+
+// Streams the candidate records of a wave's neighbour columns one z level at
+// a time, the wave's own levels first and then alternately one below and one
+// above.  A buffer then holds candidates from every direction around the
+// wave's atoms, so the atoms' pair counts per buffer are alike and the lanes
+// of the mask loop wait less for one another (a buffer filled column by
+// column holds the candidates of one side).  Lane l < 9 keeps neighbour
+// column (l / 3 - 1, l % 3 - 1); a level's cells of all passing columns are
+// one virtual run, walked 64 records per call of
+//   emit(ok, t, ox, oy, z0): sorted record t (valid if ok) with column image
+//   offsets ox, oy and level base z0, all relative to the wave's home cell.
+// wx / wy: single column on that axis (taken once, per-pair image); wz: the
+// levels zA..zB cover every z cell once (per-pair image along z).
+template <class Emit>
+__device__ __forceinline__ void stream_levels(
+    const Sorted &S, const TileGrid &G, int coff, int cx, int cy, bool wx,
+    bool wy, bool wz, int zlo, int zhi, int zA, int zB, float bx0, float bx1,
+    float by0, float by1, float bz0, float bz1, float hi, int lane,
+    Emit &&emit) {
+  const int ox = lane / 3 - 1, oy = lane % 3 - 1;
+  const float gx = wx ? 0.f
+                      : fmaxf(0.f, fmaxf((float)ox * G.csx - bx1,
+                                         bx0 - (float)(ox + 1) * G.csx));
+  const float gy = wy ? 0.f
+                      : fmaxf(0.f, fmaxf((float)oy * G.csy - by1,
+                                         by0 - (float)(oy + 1) * G.csy));
+  const float gxy2 = fmaf(gx, gx, gy * gy);
+  const bool cok = lane < 9 && !(wx && ox != 0) && !(wy && oy != 0) &&
+                   gxy2 <= hi;
+  int tx = cx + ox, ty = cy + oy;
+  tx = tx < 0 ? tx + G.nx : (tx >= G.nx ? tx - G.nx : tx);
+  ty = ty < 0 ? ty + G.ny : (ty >= G.ny ? ty - G.ny : ty);
+  const int tbase = coff + (tx * G.ny + ty) * G.nz;
+  const int ncz = zhi - zlo + 1;
+  const int nlev = ncz + 2 * G.rz;
+  auto level_z = [&](int q) {
+    if (q < ncz) return zlo + q;
+    const int d = q - ncz, k = (d >> 1) + 1;
+    return (d & 1) ? zhi + k : zlo - k;
+  };
+  // this lane's cell of level zc: record count and first sorted record
+  auto level_cell = [&](int zc, int &cnt, int &start) {
+    const int w = ((zc % G.nz) + G.nz) % G.nz;
+    const float z0 = (float)(zc - zlo) * G.csz;
+    // level gap to the wave box, less a margin for the rounding of the
+    // cell-local offsets, so the per-record test alone decides
+    float gz = wz ? 0.f : fmaxf(0.f, fmaxf(z0 - bz1, bz0 - (z0 + G.csz)));
+    gz = fmaxf(0.f, gz - 1e-3f);
+    const bool lok = cok && fmaf(gz, gz, gxy2) <= hi;
+    cnt = lok ? S.ccount[tbase + w] : 0;
+    start = lok ? S.cstart[tbase + w] : 0;
+  };
+  // level q = 0 (zlo) always lies in zA .. zB; the next level's cells are
+  // read before the current level is walked
+  int q = 0, zc = zlo, cnt, start;
+  level_cell(zc, cnt, start);
+  while (true) {
+    int qn = q + 1, zn = 0;
+    for (; qn < nlev; qn++) {
+      zn = level_z(qn);
+      if (zn >= zA && zn <= zB) break;
+    }
+    int cnt_n = 0, start_n = 0;
+    if (qn < nlev) level_cell(zn, cnt_n, start_n);
+    // inclusive prefix of the counts over lanes 0 .. 8 (row shifts within
+    // the first 16 lanes)
+    int inc = cnt;
+    inc += __builtin_amdgcn_update_dpp(0, inc, 0x111, 0xf, 0xf, false);
+    inc += __builtin_amdgcn_update_dpp(0, inc, 0x112, 0xf, 0xf, false);
+    inc += __builtin_amdgcn_update_dpp(0, inc, 0x114, 0xf, 0xf, false);
+    inc += __builtin_amdgcn_update_dpp(0, inc, 0x118, 0xf, 0xf, false);
+    const int total = __builtin_amdgcn_readlane(inc, 8);
+    if (total > 0) {
+      const float z0 = (float)(zc - zlo) * G.csz;
+      const int rel = start - (inc - cnt);  // record = run index + rel
+      int ends[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) ends[j] = __builtin_amdgcn_readlane(inc, j);
+      for (int base = 0; base < total; base += 64) {
+        const int idx = base + lane;
+        int k = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) k += idx >= ends[j] ? 1 : 0;
+        const int t = idx + __shfl(rel, k, 64);
+        emit(idx < total, t, (float)(k / 3 - 1) * G.csx,
+             (float)(k % 3 - 1) * G.csy, z0);
+      }
+    }
+    if (qn >= nlev) break;
+    q = qn;
+    zc = zn;
+    cnt = cnt_n;
+    start = start_n;
+  }
 }
 
 template <bool F64, int ELEC, int ELSW, int BUF>
 __global__ void __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(F64 ? PMNB_TM_OCC64 : PMNB_TM_OCC32)))
 k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
      const int4 *__restrict__ items, const int *__restrict__ nitems,
      double *__restrict__ forces, double *__restrict__ partial, int pstride,
      unsigned long long *__restrict__ npairs) {
+  constexpr int kNW = (BUF / kTmSplit + 31) / 32;  // mask words
   __shared__ float4 recf[kTmWaves][BUF];
   __shared__ int4 reci[kTmWaves][BUF];   // sorted index, type, mol, atom
   __shared__ double4 recd[F64 ? kTmWaves : 1][F64 ? BUF : 1];
@@ -215,40 +356,31 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
       if (lane < kTmSplit && n + lane < BUF)
         recf[wave][n + lane] = make_float4(1e18f, 1e18f, 1e18f, 0.f);
       wave_sync_lds();
-      unsigned m0 = 0u, m1 = 0u;
+      unsigned m[kNW];
       const float4 *rb = recf[wave];
       if (wmode == 0)
-        tm_test<0>(rb, n, split, xi, yi, zi, hi, L, iL, wx, wy, wz, m0, m1);
+        tm_test<0>(rb, n, split, xi, yi, zi, hi, L, iL, wx, wy, wz, m);
       else if (wmode == 4)
-        tm_test<1>(rb, n, split, xi, yi, zi, hi, L, iL, wx, wy, wz, m0, m1);
+        tm_test<1>(rb, n, split, xi, yi, zi, hi, L, iL, wx, wy, wz, m);
       else if (wmode == 7)
-        tm_test<2>(rb, n, split, xi, yi, zi, hi, L, iL, wx, wy, wz, m0, m1);
+        tm_test<2>(rb, n, split, xi, yi, zi, hi, L, iL, wx, wy, wz, m);
       else
-        tm_test<3>(rb, n, split, xi, yi, zi, hi, L, iL, wx, wy, wz, m0, m1);
-      if (!active) m0 = m1 = 0u;
+        tm_test<3>(rb, n, split, xi, yi, zi, hi, L, iL, wx, wy, wz, m);
+      if (!active) {
+#pragma unroll
+        for (int w = 0; w < kNW; w++) m[w] = 0u;
+      }
       const int jmax = (n + kTmSplit - 1) / kTmSplit;
-      const int c0 = min(jmax, 32), c1 = jmax - c0;
-      // record index of the lowest set bit (and clear it)
-      auto pop = [&]() -> int {
-        if (m0) {
-          const int b = __builtin_ctz(m0);
-          m0 &= m0 - 1u;
-          return split + kTmSplit * (c0 - 1 - b);
-        }
-        const int b = __builtin_ctz(m1);
-        m1 &= m1 - 1u;
-        return split + kTmSplit * (32 + c1 - 1 - b);
-      };
       px = py = pz = pelj = peel = 0.f;
       v0 = v1 = v2 = v3 = v4 = v5 = 0.f;
-      if (m0 | m1) {
-        int k = pop();
+      if (mask_any(m)) {
+        int k = mask_pop<0, kNW>(m, split, jmax);
         float4 r = recf[wave][k];
         int4 q4 = reci[wave][k];
         while (true) {
-          const bool more = (m0 | m1) != 0u;
+          const bool more = mask_any(m);
           int kn = k;
-          if (more) kn = pop();
+          if (more) kn = mask_pop<0, kNW>(m, split, jmax);
           const float4 rn = recf[wave][kn];
           const int4 q4n = reci[wave][kn];
           eval(r, q4, k);
@@ -276,71 +408,37 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
       wave_sync_lds();  // buffer is refilled next
     };
 
-    for (int oc = 0; oc < 9; oc++) {
-      const int ox = oc / 3 - 1, oy = oc % 3 - 1;
-      // an axis with a single column takes it once
-      if (wx && ox != 0) continue;
-      if (wy && oy != 0) continue;
-      // column prefilter: gap from the wave box to the neighbour column
-      const float gx = wx ? 0.f : fmaxf(0.f, fmaxf((float)ox * G.csx - bx1,
-                                                  bx0 - (float)(ox + 1) * G.csx));
-      const float gy = wy ? 0.f : fmaxf(0.f, fmaxf((float)oy * G.csy - by1,
-                                                  by0 - (float)(oy + 1) * G.csy));
-      if (fmaf(gx, gx, gy * gy) > hi) continue;
-      int tx = cx + ox, ty = cy + oy;
-      tx = tx < 0 ? tx + G.nx : (tx >= G.nx ? tx - G.nx : tx);
-      ty = ty < 0 ? ty + G.ny : (ty >= G.ny ? ty - G.ny : ty);
-      const int tbase = coff + (tx * G.ny + ty) * G.nz;
-      const float oxc = (float)ox * G.csx, oyc = (float)oy * G.csy;
-      // z cells zA..zB (unwrapped) as at most two contiguous sorted runs
-      const int wA = ((zA % G.nz) + G.nz) % G.nz;
-      const int len = zB - zA + 1;
-      for (int part = 0; part < 2; part++) {
-        int w0, w1, zshift;
-        if (part == 0) {
-          w0 = wA;
-          w1 = min(G.nz - 1, wA + len - 1);
-          zshift = zA - wA;
-        } else {
-          if (wA + len - 1 < G.nz) break;
-          w0 = 0;
-          w1 = wA + len - 1 - G.nz;
-          zshift = zA - wA + G.nz;
-        }
-        const int r0 = uni(S.cstart[tbase + w0]);
-        const int r1 = uni(S.cstart[tbase + w1] + S.ccount[tbase + w1]);
-        for (int base = r0; base < r1; base += 64) {
-          const int t = base + lane;
-          bool ok = t < r1;
-          float4 rf = make_float4(0.f, 0.f, 0.f, 0.f);
-          if (ok) {
-            const float4 lt = S.loc[t];
-            const int zt = (S.cell[t] - coff) % G.nz + zshift;
-            rf.x = oxc + lt.x;
-            rf.y = oyc + lt.y;
-            rf.z = (float)(zt - zlo) * G.csz + lt.z;
-            rf.w = lt.w;
-            const float gzv = wz ? 0.f : fmaxf(0.f, fmaxf(bz0 - rf.z, rf.z - bz1));
-            const float gxv = wx ? 0.f : fmaxf(0.f, fmaxf(bx0 - rf.x, rf.x - bx1));
-            const float gyv = wy ? 0.f : fmaxf(0.f, fmaxf(by0 - rf.y, rf.y - by1));
-            ok = fmaf(gxv, gxv, fmaf(gyv, gyv, gzv * gzv)) <= hi;
-          }
-          const unsigned long long ball = __ballot(ok);
-          if (ok) {
-            const int pos = nbuf + lane_prefix(ball);
-            recf[wave][pos] = rf;
-            reci[wave][pos] = make_int4(t, S.type[t], S.mol[t], S.atom[t]);
-            if (F64)
-              recd[wave][pos] = make_double4(S.x[t], S.y[t], S.z[t], S.q[t]);
-          }
-          nbuf += __popcll(ball);
-          if (nbuf > BUF - 64) {
-            process(nbuf);
-            nbuf = 0;
-          }
-        }
+    auto emit = [&](bool ok, int t, float oxc, float oyc, float z0) {
+      float4 rf = make_float4(0.f, 0.f, 0.f, 0.f);
+      int4 ri = make_int4(0, 0, 0, 0);
+      if (ok) {
+        const float4 lt = S.loc[t];
+        ri = S.rec[t];
+        rf.x = oxc + lt.x;
+        rf.y = oyc + lt.y;
+        rf.z = z0 + lt.z;
+        rf.w = lt.w;
+        const float gzv = wz ? 0.f : fmaxf(0.f, fmaxf(bz0 - rf.z, rf.z - bz1));
+        const float gxv = wx ? 0.f : fmaxf(0.f, fmaxf(bx0 - rf.x, rf.x - bx1));
+        const float gyv = wy ? 0.f : fmaxf(0.f, fmaxf(by0 - rf.y, rf.y - by1));
+        ok = fmaf(gxv, gxv, fmaf(gyv, gyv, gzv * gzv)) <= hi;
       }
-    }
+      const unsigned long long ball = __ballot(ok);
+      if (ok) {
+        const int pos = nbuf + lane_prefix(ball);
+        recf[wave][pos] = rf;
+        reci[wave][pos] = ri;
+        if (F64)
+          recd[wave][pos] = make_double4(S.x[t], S.y[t], S.z[t], S.q[t]);
+      }
+      nbuf += __popcll(ball);
+      if (nbuf > BUF - 64) {
+        process(nbuf);
+        nbuf = 0;
+      }
+    };
+    stream_levels(S, G, coff, cx, cy, wx, wy, wz, zlo, zhi, zA, zB, bx0, bx1,
+                  by0, by1, bz0, bz1, hi, lane, emit);
     if (nbuf > 0) process(nbuf);
     // the 4 lanes of an atom add their partial forces
     if (F64) {
@@ -386,52 +484,33 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
 // per-item partial rows as k_tm, so both give the same bits; it keeps the
 // per-item values in vector registers (no replica table), which measures
 // faster for one large system.
-// Tests buffered records split, split + 4, ... (< n) against one atom.  The
-// result bits are shifted in, so record split + 4 j sits at bit
-// (c0 - 1 - j) of lo for j < c0 = min(32, jmax) and at bit
-// (c1 - 1 - (j - 32)) of hi for the rest (c1 = jmax - c0).  Records n ..
-// 4 jmax - 1 hold a far sentinel; the atom itself passes and is skipped by
-// the pair code.  WRAPZ: minimum image along z.
-template <bool WRAPZ>
+// Tests the buffered records against one atom (mask_test layout).  WRAPZ:
+// minimum image along z.
+template <bool WRAPZ, int NW>
 __device__ __forceinline__ void tile_test(const float4 *__restrict__ rec,
                                           int n, int split, float xi,
                                           float yi, float zi, float hi,
-                                          float Lz, float iLz, unsigned &lo,
-                                          unsigned &hi_bits) {
-  const int jmax = (n + kTmSplit - 1) / kTmSplit;  // wave-uniform
-  const int c0 = min(jmax, 32);
-  unsigned m = 0u;
-#pragma unroll 4
-  for (int j = 0; j < c0; j++) {
-    const int k = split + kTmSplit * j;
-    const float4 r = rec[k];
+                                          float Lz, float iLz,
+                                          unsigned (&m)[NW]) {
+  auto test = [&](int j) -> unsigned {
+    const float4 r = rec[split + kTmSplit * j];
     const float dx = r.x - xi, dy = r.y - yi;
     float dz = r.z - zi;
     if (WRAPZ) dz = fmaf(-Lz, rintf(dz * iLz), dz);
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    m = (m << 1) + (unsigned)(r2 <= hi);
-  }
-  lo = m;
-  m = 0u;
-#pragma unroll 4
-  for (int j = 32; j < jmax; j++) {
-    const int k = split + kTmSplit * j;
-    const float4 r = rec[k];
-    const float dx = r.x - xi, dy = r.y - yi;
-    float dz = r.z - zi;
-    if (WRAPZ) dz = fmaf(-Lz, rintf(dz * iLz), dz);
-    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    m = (m << 1) + (unsigned)(r2 <= hi);
-  }
-  hi_bits = m;
+    return (unsigned)(r2 <= hi);
+  };
+  mask_test<0, NW>((n + kTmSplit - 1) / kTmSplit, m, test);
 }
 
 template <bool F64, int ELEC, int ELSW, int BUF>
 __global__ void __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(F64 ? PMNB_TM_OCC64 : PMNB_TM_OCC32)))
 k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
        const int *__restrict__ nitems, double *__restrict__ forces,
        double *__restrict__ partial, int pstride,
        unsigned long long *__restrict__ npairs) {
+  constexpr int kNW = (BUF / kTmSplit + 31) / 32;  // mask words
   __shared__ float4 recf[kTmWaves][BUF];
   __shared__ int4 reci[kTmWaves][BUF];
   __shared__ double4 recd[F64 ? kTmWaves : 1][F64 ? BUF : 1];
@@ -568,35 +647,26 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
       if (lane < kTmSplit && n + lane < BUF)
         recf[wave][n + lane] = make_float4(1e18f, 1e18f, 1e18f, 0.f);
       wave_sync_lds();
-      unsigned m0 = 0u, m1 = 0u;
+      unsigned m[kNW];
       if (clampz)
-        tile_test<true>(recf[wave], n, split, xi, yi, zi, hi, Lz, iLz, m0, m1);
+        tile_test<true>(recf[wave], n, split, xi, yi, zi, hi, Lz, iLz, m);
       else
-        tile_test<false>(recf[wave], n, split, xi, yi, zi, hi, Lz, iLz, m0, m1);
-      if (!active) m0 = m1 = 0u;
+        tile_test<false>(recf[wave], n, split, xi, yi, zi, hi, Lz, iLz, m);
+      if (!active) {
+#pragma unroll
+        for (int w = 0; w < kNW; w++) m[w] = 0u;
+      }
       const int jmax = (n + kTmSplit - 1) / kTmSplit;
-      const int c0 = min(jmax, 32), c1 = jmax - c0;
-      // record index of the lowest set bit (and clear it)
-      auto pop = [&]() -> int {
-        if (m0) {
-          const int b = __builtin_ctz(m0);
-          m0 &= m0 - 1u;
-          return split + kTmSplit * (c0 - 1 - b);
-        }
-        const int b = __builtin_ctz(m1);
-        m1 &= m1 - 1u;
-        return split + kTmSplit * (32 + c1 - 1 - b);
-      };
       px = py = pz = pelj = peel = 0.f;
       v0 = v1 = v2 = v3 = v4 = v5 = 0.f;
-      if (m0 | m1) {
-        int k = pop();
+      if (mask_any(m)) {
+        int k = mask_pop<0, kNW>(m, split, jmax);
         float4 r = recf[wave][k];
         int4 q4 = reci[wave][k];
         while (true) {
-          const bool more = (m0 | m1) != 0u;
+          const bool more = mask_any(m);
           int kn = k;
-          if (more) kn = pop();
+          if (more) kn = mask_pop<0, kNW>(m, split, jmax);
           const float4 rn = recf[wave][kn];
           const int4 q4n = reci[wave][kn];
           eval(r, q4, k);
@@ -624,69 +694,38 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
       wave_sync_lds();  // buffer is refilled next
     };
 
-    for (int oc = 0; oc < 9; oc++) {
-      const int ox = oc / 3 - 1, oy = oc % 3 - 1;
-      // column prefilter: gap from the wave box to the neighbour column
-      const float gx = fmaxf(0.f, fmaxf((float)ox * G.csx - bx1,
-                                        bx0 - (float)(ox + 1) * G.csx));
-      const float gy = fmaxf(0.f, fmaxf((float)oy * G.csy - by1,
-                                        by0 - (float)(oy + 1) * G.csy));
-      if (fmaf(gx, gx, gy * gy) > hi) continue;
-      int tx = cx + ox, ty = cy + oy;
-      tx = tx < 0 ? tx + G.nx : (tx >= G.nx ? tx - G.nx : tx);
-      ty = ty < 0 ? ty + G.ny : (ty >= G.ny ? ty - G.ny : ty);
-      const int tbase = (tx * G.ny + ty) * G.nz;
-      const float oxc = (float)ox * G.csx, oyc = (float)oy * G.csy;
-      // z cells zA..zB (unwrapped) as at most two contiguous sorted runs
-      const int wA = ((zA % G.nz) + G.nz) % G.nz;
-      const int len = zB - zA + 1;
-      for (int part = 0; part < 2; part++) {
-        int w0, w1, zshift;
-        if (part == 0) {
-          w0 = wA;
-          w1 = min(G.nz - 1, wA + len - 1);
-          zshift = zA - wA;
-        } else {
-          if (wA + len - 1 < G.nz) break;
-          w0 = 0;
-          w1 = wA + len - 1 - G.nz;
-          zshift = zA - wA + G.nz;
-        }
-        const int r0 = S.cstart[tbase + w0];
-        const int r1 = S.cstart[tbase + w1] + S.ccount[tbase + w1];
-        for (int base = r0; base < r1; base += 64) {
-          const int t = base + lane;
-          bool ok = t < r1;
-          float4 rf = make_float4(0.f, 0.f, 0.f, 0.f);
-          if (ok) {
-            const float4 lt = S.loc[t];
-            const int zt = S.cell[t] % G.nz + zshift;
-            rf.x = oxc + lt.x;
-            rf.y = oyc + lt.y;
-            rf.z = (float)(zt - zlo) * G.csz + lt.z;
-            rf.w = lt.w;
-            float gzv = fmaxf(0.f, fmaxf(bz0 - rf.z, rf.z - bz1));
-            if (clampz) gzv = 0.f;
-            const float gxv = fmaxf(0.f, fmaxf(bx0 - rf.x, rf.x - bx1));
-            const float gyv = fmaxf(0.f, fmaxf(by0 - rf.y, rf.y - by1));
-            ok = fmaf(gxv, gxv, fmaf(gyv, gyv, gzv * gzv)) <= hi;
-          }
-          const unsigned long long ball = __ballot(ok);
-          if (ok) {
-            const int pos = nbuf + lane_prefix(ball);
-            recf[wave][pos] = rf;
-            reci[wave][pos] = make_int4(t, S.type[t], S.mol[t], S.atom[t]);
-            if (F64) recd[wave][pos] = make_double4(S.x[t], S.y[t], S.z[t],
-                                                    S.q[t]);
-          }
-          nbuf += __popcll(ball);
-          if (nbuf > BUF - 64) {
-            process(nbuf);
-            nbuf = 0;
-          }
-        }
+    auto emit = [&](bool ok, int t, float oxc, float oyc, float z0) {
+      float4 rf = make_float4(0.f, 0.f, 0.f, 0.f);
+      int4 ri = make_int4(0, 0, 0, 0);
+      if (ok) {
+        const float4 lt = S.loc[t];
+        ri = S.rec[t];
+        rf.x = oxc + lt.x;
+        rf.y = oyc + lt.y;
+        rf.z = z0 + lt.z;
+        rf.w = lt.w;
+        float gzv = fmaxf(0.f, fmaxf(bz0 - rf.z, rf.z - bz1));
+        if (clampz) gzv = 0.f;
+        const float gxv = fmaxf(0.f, fmaxf(bx0 - rf.x, rf.x - bx1));
+        const float gyv = fmaxf(0.f, fmaxf(by0 - rf.y, rf.y - by1));
+        ok = fmaf(gxv, gxv, fmaf(gyv, gyv, gzv * gzv)) <= hi;
       }
-    }
+      const unsigned long long ball = __ballot(ok);
+      if (ok) {
+        const int pos = nbuf + lane_prefix(ball);
+        recf[wave][pos] = rf;
+        reci[wave][pos] = ri;
+        if (F64) recd[wave][pos] = make_double4(S.x[t], S.y[t], S.z[t],
+                                                S.q[t]);
+      }
+      nbuf += __popcll(ball);
+      if (nbuf > BUF - 64) {
+        process(nbuf);
+        nbuf = 0;
+      }
+    };
+    stream_levels(S, G, 0, cx, cy, false, false, clampz, zlo, zhi, zA, zB,
+                  bx0, bx1, by0, by1, bz0, bz1, hi, lane, emit);
     if (nbuf > 0) process(nbuf);
     // the 4 lanes of an atom add their partial forces
     if (F64) {
