@@ -246,34 +246,24 @@ __device__ __forceinline__ void pair_f32(const F32Params &F, float r2,
     e2 = poly * e * rr;
     f2 = fmaf(F.tapi, e, e2) * rd2;
   }
-  float sv = 1.f, sd = 0.f;
-  if (r2 >= F.on2l) {
-    const float c2 = F.cut2l - r2;
-    const float c4 = c2 * fmaf(2.f, r2, F.s2l);
-    sv = F.s1l * (c2 * c4);
-    sd = F.s3l * fmaf(c2, c2, -c4);
-    if (r2 > F.cut2l) {
-      sv = 0.f;
-      sd = 0.f;
-    }
-  }
+  // C2 switch without branches (most waves hold pairs on both sides of the
+  // switch-on distance): c2 = 0 beyond the cutoff gives S = S~ = 0
+  float c2 = fmaxf(F.cut2l - r2, 0.f);
+  float c4 = c2 * fmaf(2.f, r2, F.s2l);
+  const bool swl = r2 >= F.on2l;
+  const float sv = swl ? F.s1l * (c2 * c4) : 1.f;
+  const float sd = swl ? F.s3l * fmaf(c2, c2, -c4) : 0.f;
   f1 = fmaf(f1, sv, -e1 * sd);
   e1 = e1 * sv;
   if (ELEC != 0 && F.same_switch) {
     f2 = fmaf(f2, sv, -e2 * sd);
     e2 = e2 * sv;
   } else if (ELEC != 0) {
-    float se = 1.f, sde = 0.f;
-    if (r2 >= F.on2e) {
-      const float c2 = F.cut2e - r2;
-      const float c4 = c2 * fmaf(2.f, r2, F.s2e);
-      se = F.s1e * (c2 * c4);
-      sde = F.s3e * fmaf(c2, c2, -c4);
-      if (r2 > F.cut2e) {
-        se = 0.f;
-        sde = 0.f;
-      }
-    }
+    c2 = fmaxf(F.cut2e - r2, 0.f);
+    c4 = c2 * fmaf(2.f, r2, F.s2e);
+    const bool swe = r2 >= F.on2e;
+    const float se = swe ? F.s1e * (c2 * c4) : 1.f;
+    const float sde = swe ? F.s3e * fmaf(c2, c2, -c4) : 0.f;
     f2 = fmaf(f2, se, -e2 * sde);
     e2 = e2 * se;
   }

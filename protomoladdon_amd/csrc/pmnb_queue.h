@@ -76,8 +76,14 @@ __device__ __forceinline__ void lds_add(T *p, T v) {
 // The box of a rare path (pairs near the cutoff, molecule centres): the
 // pointer passes through an empty asm statement, so the loads of the box
 // fields stay in that branch instead of being hoisted into scalar
-// registers held across the whole work item.
+// registers held across the whole work item.  The pointer must be equal on
+// all lanes (one replica per work item); it is read from the first lane, so
+// it sits in scalar registers whatever the compiler infers.
 __device__ __forceinline__ const Box &rare_box(const Box *b) {
+  const unsigned long long v = (unsigned long long)b;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  b = (const Box *)(((unsigned long long)hi << 32) | lo);
   __asm__ volatile("" : "+s"(b));
   return *b;
 }

@@ -19,6 +19,8 @@
 // occupancy; the autotuner picks between them.
 #pragma once
 
+#include <type_traits>
+
 #include "pmnb_queue.h"
 
 namespace pmnb {
@@ -66,12 +68,23 @@ __device__ __forceinline__ void mask_test(int jmax, unsigned (&m)[NW],
 template <int W, int NW>
 __device__ __forceinline__ int mask_pop(unsigned (&m)[NW], int split,
                                         int jmax) {
-  if constexpr (W + 1 < NW) {
-    if (!m[W]) return mask_pop<W + 1, NW>(m, split, jmax);
+  if constexpr (W == 0 && NW == 2) {
+    // two words with selects instead of branches
+    const bool first = m[0] != 0u;
+    unsigned w = first ? m[0] : m[1];
+    const int b = __builtin_ctz(w);
+    w &= w - 1u;
+    m[0] = first ? w : m[0];
+    m[1] = first ? m[1] : w;
+    return split + kTmSplit * ((first ? min(32, jmax) : jmax) - 1 - b);
+  } else {
+    if constexpr (W + 1 < NW) {
+      if (!m[W]) return mask_pop<W + 1, NW>(m, split, jmax);
+    }
+    const int b = __builtin_ctz(m[W]);
+    m[W] &= m[W] - 1u;
+    return split + kTmSplit * (32 * W + min(32, jmax - 32 * W) - 1 - b);
   }
-  const int b = __builtin_ctz(m[W]);
-  m[W] &= m[W] - 1u;
-  return split + kTmSplit * (32 * W + min(32, jmax - 32 * W) - 1 - b);
 }
 
 template <int NW>
@@ -214,6 +227,10 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
   __shared__ float4 recf[kTmWaves][BUF];
   __shared__ int4 reci[kTmWaves][BUF];   // sorted index, type, mol, atom
   __shared__ double4 recd[F64 ? kTmWaves : 1][F64 ? BUF : 1];
+  __shared__ float4 ljs[kQLj];           // FP32 LJ table (T * T <= kQLj)
+  for (int k = threadIdx.x; k < P.n_types * P.n_types; k += kBlock)
+    ljs[k] = S.ljf[k];
+  __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const F32Params &F = P.f;
@@ -250,7 +267,7 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
     const int ti = S.type[ii], mi = S.mol[ii], ai = S.atom[ii];
     const double qi = S.q[ii];
     const double xd = S.x[ii], yd = S.y[ii], zd = S.z[ii];
-    const float4 *ljrow = S.ljf + ti * P.n_types;
+    const int ljbase = ti * P.n_types;  // row of this atom's type in ljs
     const double *ljrow64 = S.lj + 4 * (ti * P.n_types);
     // bounding box of the wave's atoms in home-relative coordinates
     const float bx0 = uni(wave_min(active ? xi : 1e30f)),
@@ -271,13 +288,18 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
     float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
 
     // One pair: candidate record k of this wave's buffer.
-    auto eval = [&](const float4 &r, const int4 &q4, int k) {
+    // WM: minimum-image mode as in tm_test (a compile-time tag, so the
+    // common modes carry no selects).
+    auto eval = [&](const float4 &r, const int4 &q4, int k,
+                    auto WM) __attribute__((always_inline)) {
+      constexpr int M = decltype(WM)::value;
       const int t = q4.x;
       if (t == i) return;
       float dx = r.x - xi, dy = r.y - yi, dz = r.z - zi;
-      if (wx) dx = fmaf(-L[0], rintf(dx * iL[0]), dx);
-      if (wy) dy = fmaf(-L[1], rintf(dy * iL[1]), dy);
-      if (wz) dz = fmaf(-L[2], rintf(dz * iL[2]), dz);
+      if (M == 2 || (M == 3 && wx)) dx = fmaf(-L[0], rintf(dx * iL[0]), dx);
+      if (M == 2 || (M == 3 && wy)) dy = fmaf(-L[1], rintf(dy * iL[1]), dy);
+      if (M == 1 || M == 2 || (M == 3 && wz))
+        dz = fmaf(-L[2], rintf(dz * iL[2]), dz);
       const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
       double ex = 0.0, ey = 0.0, ez = 0.0, e2d = 0.0;
       if (F64) {
@@ -317,7 +339,7 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
           add_virial(acc, ACC_MXX, gx, gy, gz, cdx, cdy, cdz, 0.5);
         }
       } else {
-        const float4 lp = ljrow[q4.y];
+        const float4 lp = ljs[ljbase + q4.y];
         float ff, e1, e2;
         pair_f32<ELEC>(F, r2, excl != 1 ? lp.x : lp.z, excl != 1 ? lp.y : lp.w,
                        li.w * r.w, excl, ff, e1, e2);
@@ -351,7 +373,7 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
     // itself has r2 == 0 and is left out), then evaluates the pairs that
     // pass; the next pair's records are read from LDS before the current
     // pair's arithmetic.
-    auto process = [&](int n) {
+    auto process = [&](int n) __attribute__((always_inline)) {
       // records n .. n + 3 (the last group of 4) are made far away
       if (lane < kTmSplit && n + lane < BUF)
         recf[wave][n + lane] = make_float4(1e18f, 1e18f, 1e18f, 0.f);
@@ -373,7 +395,8 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
       const int jmax = (n + kTmSplit - 1) / kTmSplit;
       px = py = pz = pelj = peel = 0.f;
       v0 = v1 = v2 = v3 = v4 = v5 = 0.f;
-      if (mask_any(m)) {
+      auto run = [&](auto WM) __attribute__((always_inline)) {
+        if (!mask_any(m)) return;
         int k = mask_pop<0, kNW>(m, split, jmax);
         float4 r = recf[wave][k];
         int4 q4 = reci[wave][k];
@@ -383,13 +406,22 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
           if (more) kn = mask_pop<0, kNW>(m, split, jmax);
           const float4 rn = recf[wave][kn];
           const int4 q4n = reci[wave][kn];
-          eval(r, q4, k);
+          eval(r, q4, k, WM);
           if (!more) break;
           r = rn;
           q4 = q4n;
           k = kn;
         }
-      }
+      };
+      // FP64 keeps one loop with runtime flags (fewer spilled registers)
+      if (F64 || (wmode != 0 && wmode != 4 && wmode != 7))
+        run(std::integral_constant<int, 3>{});
+      else if (wmode == 0)
+        run(std::integral_constant<int, 0>{});
+      else if (wmode == 4)
+        run(std::integral_constant<int, 1>{});
+      else
+        run(std::integral_constant<int, 2>{});
       if (!F64) {
         Fx += to_fixed((double)px);
         Fy += to_fixed((double)py);
@@ -408,7 +440,8 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
       wave_sync_lds();  // buffer is refilled next
     };
 
-    auto emit = [&](bool ok, int t, float oxc, float oyc, float z0) {
+    auto emit = [&](bool ok, int t, float oxc, float oyc,
+                    float z0) __attribute__((always_inline)) {
       float4 rf = make_float4(0.f, 0.f, 0.f, 0.f);
       int4 ri = make_int4(0, 0, 0, 0);
       if (ok) {
@@ -514,6 +547,10 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
   __shared__ float4 recf[kTmWaves][BUF];
   __shared__ int4 reci[kTmWaves][BUF];
   __shared__ double4 recd[F64 ? kTmWaves : 1][F64 ? BUF : 1];
+  __shared__ float4 ljs[kQLj];  // FP32 LJ table (T * T <= kQLj)
+  for (int k = threadIdx.x; k < P.n_types * P.n_types; k += kBlock)
+    ljs[k] = S.ljf[k];
+  __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int item_idx = blockIdx.x * kTmWaves + wave;
   double acc[ACC_N];
@@ -538,7 +575,7 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
     const int ti = S.type[ii], mi = S.mol[ii], ai = S.atom[ii];
     const double qi = S.q[ii];
     const double xd = S.x[ii], yd = S.y[ii], zd = S.z[ii];
-    const float4 *ljrow = S.ljf + ti * P.n_types;
+    const int ljbase = ti * P.n_types;  // row of this atom's type in ljs
     const double *ljrow64 = S.lj + 4 * (ti * P.n_types);
     // bounding box of the wave's atoms in home-relative coordinates
     const float bx0 = wave_min(active ? xi : 1e30f),
@@ -561,11 +598,15 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
     float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
 
     // One pair: candidate record k of this wave's buffer.
-    auto eval = [&](const float4 &r, const int4 &q4, int k) {
+    // WZ: minimum image along z, 0 no, 1 yes, 2 as the runtime flag (a
+    // compile-time tag, so the common case carries no select).
+    auto eval = [&](const float4 &r, const int4 &q4, int k,
+                    auto WZ) __attribute__((always_inline)) {
+      constexpr int M = decltype(WZ)::value;
       const int t = q4.x;
       if (t == i) return;
       float dx = r.x - xi, dy = r.y - yi, dz = r.z - zi;
-      if (clampz) dz = fmaf(-Lz, rintf(dz * iLz), dz);
+      if (M == 1 || (M == 2 && clampz)) dz = fmaf(-Lz, rintf(dz * iLz), dz);
       const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
       double ex = 0.0, ey = 0.0, ez = 0.0, e2d = 0.0;
       if (F64) {
@@ -607,7 +648,7 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
           add_virial(acc, ACC_MXX, gx, gy, gz, cdx, cdy, cdz, 0.5);
         }
       } else {
-        const float4 lp = ljrow[q4.y];
+        const float4 lp = ljs[ljbase + q4.y];
         float ff, e1, e2;
         pair_f32<ELEC>(F, r2, excl != 1 ? lp.x : lp.z, excl != 1 ? lp.y : lp.w,
                        li.w * r.w, excl, ff, e1, e2);
@@ -642,7 +683,7 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
     // itself has r2 == 0 and is left out), then evaluates the pairs that
     // pass; the next pair's records are read from LDS before the current
     // pair's arithmetic.
-    auto process = [&](int n) {
+    auto process = [&](int n) __attribute__((always_inline)) {
       // records n .. n + 3 (the last group of 4) are made far away
       if (lane < kTmSplit && n + lane < BUF)
         recf[wave][n + lane] = make_float4(1e18f, 1e18f, 1e18f, 0.f);
@@ -659,7 +700,8 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
       const int jmax = (n + kTmSplit - 1) / kTmSplit;
       px = py = pz = pelj = peel = 0.f;
       v0 = v1 = v2 = v3 = v4 = v5 = 0.f;
-      if (mask_any(m)) {
+      auto run = [&](auto WZ) __attribute__((always_inline)) {
+        if (!mask_any(m)) return;
         int k = mask_pop<0, kNW>(m, split, jmax);
         float4 r = recf[wave][k];
         int4 q4 = reci[wave][k];
@@ -669,13 +711,20 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
           if (more) kn = mask_pop<0, kNW>(m, split, jmax);
           const float4 rn = recf[wave][kn];
           const int4 q4n = reci[wave][kn];
-          eval(r, q4, k);
+          eval(r, q4, k, WZ);
           if (!more) break;
           r = rn;
           q4 = q4n;
           k = kn;
         }
-      }
+      };
+      // FP64 keeps one loop with a runtime flag (fewer spilled registers)
+      if (F64)
+        run(std::integral_constant<int, 2>{});
+      else if (clampz)
+        run(std::integral_constant<int, 1>{});
+      else
+        run(std::integral_constant<int, 0>{});
       if (!F64) {
         Fx += to_fixed((double)px);
         Fy += to_fixed((double)py);
@@ -694,7 +743,8 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
       wave_sync_lds();  // buffer is refilled next
     };
 
-    auto emit = [&](bool ok, int t, float oxc, float oyc, float z0) {
+    auto emit = [&](bool ok, int t, float oxc, float oyc,
+                    float z0) __attribute__((always_inline)) {
       float4 rf = make_float4(0.f, 0.f, 0.f, 0.f);
       int4 ri = make_int4(0, 0, 0, 0);
       if (ok) {
