@@ -65,10 +65,10 @@ __device__ __forceinline__ void mask_test(int jmax, unsigned (&m)[NW],
 
 // Record index of the lowest set bit of the first non-empty word, which is
 // cleared.  At least one bit must be set.
-template <int W, int NW>
+template <int W, int NW, bool SELECT = true>
 __device__ __forceinline__ int mask_pop(unsigned (&m)[NW], int split,
                                         int jmax) {
-  if constexpr (W == 0 && NW == 2) {
+  if constexpr (SELECT && W == 0 && NW == 2) {
     // two words with selects instead of branches
     const bool first = m[0] != 0u;
     unsigned w = first ? m[0] : m[1];
@@ -79,7 +79,7 @@ __device__ __forceinline__ int mask_pop(unsigned (&m)[NW], int split,
     return split + kTmSplit * ((first ? min(32, jmax) : jmax) - 1 - b);
   } else {
     if constexpr (W + 1 < NW) {
-      if (!m[W]) return mask_pop<W + 1, NW>(m, split, jmax);
+      if (!m[W]) return mask_pop<W + 1, NW, SELECT>(m, split, jmax);
     }
     const int b = __builtin_ctz(m[W]);
     m[W] &= m[W] - 1u;
@@ -227,10 +227,12 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
   __shared__ float4 recf[kTmWaves][BUF];
   __shared__ int4 reci[kTmWaves][BUF];   // sorted index, type, mol, atom
   __shared__ double4 recd[F64 ? kTmWaves : 1][F64 ? BUF : 1];
-  __shared__ float4 ljs[kQLj];           // FP32 LJ table (T * T <= kQLj)
-  for (int k = threadIdx.x; k < P.n_types * P.n_types; k += kBlock)
-    ljs[k] = S.ljf[k];
-  __syncthreads();
+  __shared__ float4 ljs[F64 ? 1 : kQLj];  // FP32 LJ table (T * T <= kQLj)
+  if (!F64) {
+    for (int k = threadIdx.x; k < P.n_types * P.n_types; k += kBlock)
+      ljs[k] = S.ljf[k];
+    __syncthreads();
+  }
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const F32Params &F = P.f;
@@ -397,13 +399,13 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
       v0 = v1 = v2 = v3 = v4 = v5 = 0.f;
       auto run = [&](auto WM) __attribute__((always_inline)) {
         if (!mask_any(m)) return;
-        int k = mask_pop<0, kNW>(m, split, jmax);
+        int k = mask_pop<0, kNW, !F64>(m, split, jmax);
         float4 r = recf[wave][k];
         int4 q4 = reci[wave][k];
         while (true) {
           const bool more = mask_any(m);
           int kn = k;
-          if (more) kn = mask_pop<0, kNW>(m, split, jmax);
+          if (more) kn = mask_pop<0, kNW, !F64>(m, split, jmax);
           const float4 rn = recf[wave][kn];
           const int4 q4n = reci[wave][kn];
           eval(r, q4, k, WM);
@@ -547,10 +549,12 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
   __shared__ float4 recf[kTmWaves][BUF];
   __shared__ int4 reci[kTmWaves][BUF];
   __shared__ double4 recd[F64 ? kTmWaves : 1][F64 ? BUF : 1];
-  __shared__ float4 ljs[kQLj];  // FP32 LJ table (T * T <= kQLj)
-  for (int k = threadIdx.x; k < P.n_types * P.n_types; k += kBlock)
-    ljs[k] = S.ljf[k];
-  __syncthreads();
+  __shared__ float4 ljs[F64 ? 1 : kQLj];  // FP32 LJ table (T * T <= kQLj)
+  if (!F64) {
+    for (int k = threadIdx.x; k < P.n_types * P.n_types; k += kBlock)
+      ljs[k] = S.ljf[k];
+    __syncthreads();
+  }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int item_idx = blockIdx.x * kTmWaves + wave;
   double acc[ACC_N];
@@ -702,13 +706,13 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
       v0 = v1 = v2 = v3 = v4 = v5 = 0.f;
       auto run = [&](auto WZ) __attribute__((always_inline)) {
         if (!mask_any(m)) return;
-        int k = mask_pop<0, kNW>(m, split, jmax);
+        int k = mask_pop<0, kNW, !F64>(m, split, jmax);
         float4 r = recf[wave][k];
         int4 q4 = reci[wave][k];
         while (true) {
           const bool more = mask_any(m);
           int kn = k;
-          if (more) kn = mask_pop<0, kNW>(m, split, jmax);
+          if (more) kn = mask_pop<0, kNW, !F64>(m, split, jmax);
           const float4 rn = recf[wave][kn];
           const int4 q4n = reci[wave][kn];
           eval(r, q4, k, WZ);
