@@ -618,14 +618,6 @@ void launch_pair_tq(bool mixed, int el, int sw, int shape, const TqArgs &a) {
   }
 }
 
-// Work items of the tile path from the sorted cells (one or many replicas).
-void launch_items(const TileRep *reps, const int *col_rep, const int *cstart,
-                  const int *ccount, int ncol, int nrep, int4 *items,
-                  int *nitems, int *rep_first, hipStream_t s) {
-  hipLaunchKernelGGL(k_items_r, dim3(1), dim3(1024), 0, s, reps, col_rep,
-                     cstart, ccount, ncol, nrep, items, nitems, rep_first);
-}
-
 // Cell-list build on the device from d_pos (original order).
 // FP32 copy of the LJ table (T*T records {A, B, A14, B14}).
 __global__ void k_lj_to_f32(const double *__restrict__ lj, int n,
@@ -707,27 +699,38 @@ int build_cells(pmnb_ctx *ctx, const double *d_pos, hipStream_t s) {
                        ctx->box, ctx->grid, d_pos, n, ctx->d_cell_of.as<int>(),
                        ctx->d_count.as<int>(), rank);
   }
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, ctx->d_count.as<int>(),
-                     ctx->d_start.as<int>(), nc);
+  if (ctx->use_tile) {
+    // starts and the work items of the tile path in one block
+    hipLaunchKernelGGL(k_scan_items, dim3(1), dim3(1024), 0, s,
+                       ctx->d_count.as<int>(), ctx->d_start.as<int>(), nc,
+                       reps, ctx->d_col_rep.as<int>(), ctx->ncol_total,
+                       (int)ctx->reps.size(), ctx->d_items.as<int4>(),
+                       ctx->d_nitems.as<int>(), ctx->d_rep_first.as<int>());
+  } else {
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s,
+                       ctx->d_count.as<int>(), ctx->d_start.as<int>(), nc);
+  }
   if (n > 0) {
     hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, s,
                        ctx->d_cell_of.as<int>(), n, ctx->d_start.as<int>(),
                        rank, ctx->d_sorted.as<int>());
+  }
+  if (n > 0 && ctx->use_tile) {
+    hipLaunchKernelGGL(k_sort_gather_r<TileRep>,
+                       dim3(blocks_for(nc, kBlock / 64)), dim3(kBlock), 0, s,
+                       reps, rep_of, d_pos, ctx->d_start.as<int>(),
+                       ctx->d_count.as<int>(), nc, ctx->d_type.as<int>(),
+                       ctx->d_q.as<double>(), ctx->d_mol.as<int>(),
+                       ctx->d_sorted.as<int>(), ctx->d_sx.as<double>(),
+                       ctx->d_sy.as<double>(), ctx->d_sz.as<double>(),
+                       ctx->d_loc.as<float4>(), ctx->d_stype.as<int>(),
+                       ctx->d_sq.as<double>(), ctx->d_smol.as<int>(),
+                       ctx->d_scell.as<int>(), ctx->d_srec.as<int4>());
+  } else if (n > 0) {
     hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(nc, kBlock / 64)),
                        dim3(kBlock), 0, s,
                        ctx->d_start.as<int>(), ctx->d_count.as<int>(), nc,
                        ctx->d_sorted.as<int>());
-    if (ctx->use_tile)
-      hipLaunchKernelGGL(k_gather_r<TileRep>, dim3(blocks_for(n)),
-                         dim3(kBlock), 0, s, reps, rep_of, d_pos,
-                         ctx->d_sorted.as<int>(), ctx->d_cell_of.as<int>(),
-                         ctx->d_type.as<int>(), ctx->d_q.as<double>(),
-                         ctx->d_mol.as<int>(), n, ctx->d_sx.as<double>(),
-                         ctx->d_sy.as<double>(), ctx->d_sz.as<double>(),
-                         ctx->d_loc.as<float4>(), ctx->d_stype.as<int>(),
-                         ctx->d_sq.as<double>(), ctx->d_smol.as<int>(),
-                         ctx->d_scell.as<int>(), ctx->d_srec.as<int4>());
-    else
     hipLaunchKernelGGL(k_gather, dim3(blocks_for(n)), dim3(kBlock), 0, s,
                        ctx->box, ctx->grid, d_pos, ctx->d_sorted.as<int>(),
                        ctx->d_cell_of.as<int>(), ctx->d_type.as<int>(),
@@ -814,8 +817,6 @@ int run_device(pmnb_ctx *ctx, const double *d_pos, double *d_forces,
     int4 *items = ctx->d_items.as<int4>();
     int *nitems = ctx->d_nitems.as<int>();
     int *rep_first = ctx->d_rep_first.as<int>();
-    launch_items(reps, ctx->d_col_rep.as<int>(), S.cstart, S.ccount, ncol,
-                 nrep, items, nitems, rep_first, s);
     if (ev_built) HIPCHK(hipEventRecord(ev_built, s));
     if (ctx->n > 0) {
       const bool single = !ctx->batch && nrep == 1 &&
@@ -1324,8 +1325,7 @@ int pmnb_adjoint_device(pmnb_ctx *ctx, const double *d_positions,
   const TileRep *reps = ctx->d_reps.as<TileRep>();
   int4 *items = ctx->d_items.as<int4>();
   int *nitems = ctx->d_nitems.as<int>();
-  launch_items(reps, ctx->d_col_rep.as<int>(), S.cstart, S.ccount, ncol, nrep,
-               items, nitems, ctx->d_rep_first.as<int>(), s);
+  // work items: built with the cell starts
   if (ctx->n > 0) {
     const bool f64 = ctx->prm.precision == PMNB_PREC_FP64;
     const int el = ctx->prm.elec, T = ctx->ntypes;

@@ -9,6 +9,7 @@
 #pragma once
 
 #include "pmnb_common.h"
+#include "pmnb_tile.h"
 
 namespace pmnb {
 
@@ -72,28 +73,58 @@ __global__ void k_assign(Box b, Grid g, const double *__restrict__ pos,
   rank[i] = atomicAdd(&count[c], 1);
 }
 
+// Exclusive prefix of v over the 1024 threads of a block (thread order);
+// *total receives the block sum.  Wave scans by shuffles, then one wave
+// scans the 16 wave sums.  Every thread of the block must call it.
+__device__ __forceinline__ int block_scan_1024(int v, int *total) {
+  __shared__ int wsum[16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  if (w == 0) {
+    int y = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+      const int z = __shfl_up(y, o, 64);
+      if (lane >= o) y += z;
+    }
+    if (lane < 16) wsum[lane] = y;
+  }
+  __syncthreads();
+  *total = wsum[15];
+  const int off = w > 0 ? wsum[w - 1] : 0;
+  __syncthreads();  // wsum may be reused by the next call
+  return off + x - v;
+}
+
 // Exclusive scan of count[0..n) into start[0..n) by one block of 1024.
+__device__ __forceinline__ void scan_counts(const int *__restrict__ count,
+                                            int *__restrict__ start, int n) {
+  const int t = threadIdx.x;
+  const int chunk = (n + 1023) / 1024;
+  const int b0 = min(n, t * chunk), b1 = min(n, b0 + chunk);
+  int s = 0;
+#pragma unroll 4
+  for (int i = b0; i < b1; i++) s += count[i];
+  int total;
+  int run = block_scan_1024(s, &total);  // exclusive prefix of this chunk
+#pragma unroll 4
+  for (int i = b0; i < b1; i++) {
+    const int c = count[i];
+    start[i] = run;
+    run += c;
+  }
+}
+
 __global__ void __launch_bounds__(1024)
 k_scan(const int *__restrict__ count, int *__restrict__ start, int n) {
-  __shared__ int part[1024];
-  int t = threadIdx.x;
-  int chunk = (n + 1023) / 1024;
-  int b0 = t * chunk, b1 = min(n, b0 + chunk);
-  int s = 0;
-  for (int i = b0; i < b1; i++) s += count[i];
-  part[t] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    int v = (t >= off) ? part[t - off] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  int run = part[t] - s;  // exclusive prefix of this chunk
-  for (int i = b0; i < b1; i++) {
-    start[i] = run;
-    run += count[i];
-  }
+  scan_counts(count, start, n);
 }
 
 __global__ void k_scatter(const int *__restrict__ cell_of, int n,
@@ -225,42 +256,95 @@ __global__ void k_assign_r(const RepT *__restrict__ reps,
   rank[i] = atomicAdd(&count[c], 1);
 }
 
+// Per-cell ordering and gather of the tile path in one pass: one wave per
+// cell puts the cell's entries in ascending atom order (as k_sort_cells)
+// and writes, for each sorted slot, the atom's FP64 position, the FP32
+// offset from its cell's lower corner (computed in FP64) with its charge,
+// type, charge, molecule, cell and packed record.
 template <class RepT>
-__global__ void k_gather_r(const RepT *__restrict__ reps,
-                           const int *__restrict__ rep_of,
-                           const double *__restrict__ pos,
-                           const int *__restrict__ sorted,
-                           const int *__restrict__ cell_of,
-                           const int *__restrict__ type,
-                           const double *__restrict__ q,
-                           const int *__restrict__ mol, int n,
-                           double *__restrict__ sx, double *__restrict__ sy,
-                           double *__restrict__ sz, float4 *__restrict__ sloc,
-                           int *__restrict__ stype, double *__restrict__ sq,
-                           int *__restrict__ smol, int *__restrict__ scell,
-                           int4 *__restrict__ srec) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  int a = sorted[s];
-  const RepT &R = reps[rep_of ? rep_of[a] : 0];
-  srec[s] = make_int4(s, type[a], mol[a], a);
-  double x = pos[3 * a], y = pos[3 * a + 1], z = pos[3 * a + 2];
-  sx[s] = x;
-  sy[s] = y;
-  sz[s] = z;
-  int cx, cy, cz;
-  double vx, vy, vz;
-  cell_coords(R.box, R.grid, x, y, z, cx, cy, cz, vx, vy, vz);
-  float4 l;
-  l.x = (float)(vx - (double)cx * R.grid.cs[0]);
-  l.y = (float)(vy - (double)cy * R.grid.cs[1]);
-  l.z = (float)(vz - (double)cz * R.grid.cs[2]);
-  l.w = (float)q[a];
-  sloc[s] = l;
-  stype[s] = type[a];
-  sq[s] = q[a];
-  smol[s] = mol[a];
-  scell[s] = cell_of[a];
+__global__ void __launch_bounds__(kBlock)
+k_sort_gather_r(const RepT *__restrict__ reps, const int *__restrict__ rep_of,
+                const double *__restrict__ pos, const int *__restrict__ start,
+                const int *__restrict__ count, int ncell,
+                const int *__restrict__ type, const double *__restrict__ q,
+                const int *__restrict__ mol, int *__restrict__ sorted,
+                double *__restrict__ sx, double *__restrict__ sy,
+                double *__restrict__ sz, float4 *__restrict__ sloc,
+                int *__restrict__ stype, double *__restrict__ sq,
+                int *__restrict__ smol, int *__restrict__ scell,
+                int4 *__restrict__ srec) {
+  __shared__ int keys[kBlock / 64][kSortLds];
+  const int w = threadIdx.x >> 6;
+  const int c = blockIdx.x * (kBlock / 64) + w;
+  const int lane = threadIdx.x & 63;
+  if (c >= ncell) return;
+  const int m = count[c];
+  if (m == 0) return;
+  const int s0 = start[c];
+  int *seg = sorted + s0;
+  // slot s0 + k receives atom a
+  auto put = [&](int k, int a) {
+    const int s = s0 + k;
+    seg[k] = a;
+    const RepT &R = reps[rep_of ? rep_of[a] : 0];
+    const double x = pos[3 * a], y = pos[3 * a + 1], z = pos[3 * a + 2];
+    sx[s] = x;
+    sy[s] = y;
+    sz[s] = z;
+    int cx, cy, cz;
+    double vx, vy, vz;
+    cell_coords(R.box, R.grid, x, y, z, cx, cy, cz, vx, vy, vz);
+    float4 l;
+    l.x = (float)(vx - (double)cx * R.grid.cs[0]);
+    l.y = (float)(vy - (double)cy * R.grid.cs[1]);
+    l.z = (float)(vz - (double)cz * R.grid.cs[2]);
+    l.w = (float)q[a];
+    sloc[s] = l;
+    stype[s] = type[a];
+    sq[s] = q[a];
+    smol[s] = mol[a];
+    scell[s] = c;
+    srec[s] = make_int4(s, type[a], mol[a], a);
+  };
+  if (m <= 64) {
+    int v = lane < m ? seg[lane] : 0x7fffffff;
+    for (int k = 2; k <= 64; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const int other = __shfl_xor(v, j, 64);
+        const bool up = (lane & k) == 0;
+        const bool lower = (lane & j) == 0;
+        v = (lower == up) ? min(v, other) : max(v, other);
+      }
+    }
+    if (lane < m) put(lane, v);
+  } else if (m <= kSortLds) {
+    for (int k = lane; k < m; k += 64) keys[w][k] = seg[k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int k = lane; k < m; k += 64) {
+      const int v = keys[w][k];
+      int rank = 0;
+      for (int j = 0; j < m; j++) rank += keys[w][j] < v;
+      put(rank, v);
+    }
+  } else {
+    if (lane == 0) {
+      for (int i = 1; i < m; i++) {
+        int v = seg[i];
+        int j = i - 1;
+        while (j >= 0 && seg[j] > v) {
+          seg[j + 1] = seg[j];
+          j--;
+        }
+        seg[j + 1] = v;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    for (int k = lane; k < m; k += 64) put(k, seg[k]);
+  }
 }
 
 // Sets n doubles to zero (a kernel node, not a memset, inside graphs).

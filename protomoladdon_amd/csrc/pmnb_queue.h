@@ -38,6 +38,7 @@
 
 #include <type_traits>
 
+#include "pmnb_cells.h"
 #include "pmnb_list.h"
 #include "pmnb_tile.h"
 
@@ -102,12 +103,11 @@ __device__ __forceinline__ float uni(float v) {
 // then every thread writes the items {first atom, end atom, local column,
 // replica} of its columns.  rep_first[r] = first item of replica r;
 // rep_first[nrep] and *nitems = the item count.
-__global__ void __launch_bounds__(1024)
-k_items_r(const TileRep *__restrict__ reps, const int *__restrict__ col_rep,
-          const int *__restrict__ cstart, const int *__restrict__ ccount,
-          int ncol, int nrep, int4 *__restrict__ items,
-          int *__restrict__ nitems, int *__restrict__ rep_first) {
-  __shared__ int part[1024];
+__device__ __forceinline__ void build_items(
+    const TileRep *__restrict__ reps, const int *__restrict__ col_rep,
+    const int *__restrict__ cstart, const int *__restrict__ ccount, int ncol,
+    int nrep, int4 *__restrict__ items, int *__restrict__ nitems,
+    int *__restrict__ rep_first) {
   const int t = threadIdx.x;
   const int chunk = (ncol + 1023) / 1024;
   const int b0 = min(ncol, t * chunk), b1 = min(ncol, b0 + chunk);
@@ -126,15 +126,8 @@ k_items_r(const TileRep *__restrict__ reps, const int *__restrict__ col_rep,
     span(gc, r, lc, a0, a1);
     sum += (a1 - a0 + kTileAtoms - 1) / kTileAtoms;
   }
-  part[t] = sum;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int v = t >= off ? part[t - off] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  int o = part[t] - sum;  // first item of this thread's columns
+  int total;
+  int o = block_scan_1024(sum, &total);  // first item of this thread's columns
   for (int gc = b0; gc < b1; gc++) {
     int r, lc, a0, a1;
     span(gc, r, lc, a0, a1);
@@ -143,9 +136,23 @@ k_items_r(const TileRep *__restrict__ reps, const int *__restrict__ col_rep,
       items[o++] = make_int4(a, min(a + kTileAtoms, a1), lc, r);
   }
   if (t == 1023) {
-    *nitems = part[1023];
-    rep_first[nrep] = part[1023];
+    *nitems = total;
+    rep_first[nrep] = total;
   }
+}
+
+// Cell starts (exclusive scan of the counts) and then the work items, in
+// one block: the items read the starts this block has just written.
+__global__ void __launch_bounds__(1024)
+k_scan_items(const int *__restrict__ count, int *__restrict__ start,
+             int ncell, const TileRep *__restrict__ reps,
+             const int *__restrict__ col_rep, int ncol, int nrep,
+             int4 *__restrict__ items, int *__restrict__ nitems,
+             int *__restrict__ rep_first) {
+  scan_counts(count, start, ncell);
+  __syncthreads();
+  build_items(reps, col_rep, start, count, ncol, nrep, items, nitems,
+              rep_first);
 }
 
 // Per-replica sums of the item partials (accumulator k of item b at
