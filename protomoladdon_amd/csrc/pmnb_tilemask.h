@@ -64,7 +64,9 @@ __device__ __forceinline__ void mask_test(int jmax, unsigned (&m)[NW],
 }
 
 // Record index of the lowest set bit of the first non-empty word, which is
-// cleared.  At least one bit must be set.
+// cleared.  At least one bit must be set, except in the two- and
+// three-word SELECT forms, which leave an empty mask empty and return an
+// unusable index.
 template <int W, int NW, bool SELECT = true>
 __device__ __forceinline__ int mask_pop(unsigned (&m)[NW], int split,
                                         int jmax) {
@@ -77,6 +79,16 @@ __device__ __forceinline__ int mask_pop(unsigned (&m)[NW], int split,
     m[0] = first ? w : m[0];
     m[1] = first ? m[1] : w;
     return split + kTmSplit * ((first ? min(32, jmax) : jmax) - 1 - b);
+  } else if constexpr (SELECT && W == 0 && NW == 3) {
+    const bool w0 = m[0] != 0u, w1 = !w0 && m[1] != 0u, w2 = !w0 && !w1;
+    unsigned w = w0 ? m[0] : (w1 ? m[1] : m[2]);
+    const int b = __builtin_ctz(w);
+    w &= w - 1u;
+    m[0] = w0 ? w : m[0];
+    m[1] = w1 ? w : m[1];
+    m[2] = w2 ? w : m[2];
+    const int top = w0 ? min(32, jmax) : (w1 ? min(64, jmax) : jmax);
+    return split + kTmSplit * (top - 1 - b);
   } else {
     if constexpr (W + 1 < NW) {
       if (!m[W]) return mask_pop<W + 1, NW, SELECT>(m, split, jmax);
@@ -399,20 +411,45 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
       v0 = v1 = v2 = v3 = v4 = v5 = 0.f;
       auto run = [&](auto WM) __attribute__((always_inline)) {
         if (!mask_any(m)) return;
-        int k = mask_pop<0, kNW, !F64>(m, split, jmax);
-        float4 r = recf[wave][k];
-        int4 q4 = reci[wave][k];
-        while (true) {
-          const bool more = mask_any(m);
-          int kn = k;
-          if (more) kn = mask_pop<0, kNW, !F64>(m, split, jmax);
-          const float4 rn = recf[wave][kn];
-          const int4 q4n = reci[wave][kn];
-          eval(r, q4, k, WM);
-          if (!more) break;
-          r = rn;
-          q4 = q4n;
-          k = kn;
+        if constexpr (F64) {
+          int k = mask_pop<0, kNW, false>(m, split, jmax);
+          float4 r = recf[wave][k];
+          int4 q4 = reci[wave][k];
+          while (true) {
+            const bool more = mask_any(m);
+            int kn = k;
+            if (more) kn = mask_pop<0, kNW, false>(m, split, jmax);
+            const float4 rn = recf[wave][kn];
+            const int4 q4n = reci[wave][kn];
+            eval(r, q4, k, WM);
+            if (!more) break;
+            r = rn;
+            q4 = q4n;
+            k = kn;
+          }
+        } else {
+          // two record sets in turn (no register moves); the next pop is
+          // taken unconditionally and its index replaced by 0 when the
+          // mask was empty, so every LDS read stays in the buffer
+          int k0 = mask_pop<0, kNW, true>(m, split, jmax);
+          float4 r0 = recf[wave][k0];
+          int4 q0 = reci[wave][k0];
+          while (true) {
+            const bool more1 = mask_any(m);
+            int k1 = mask_pop<0, kNW, true>(m, split, jmax);
+            k1 = more1 ? k1 : 0;
+            const float4 r1 = recf[wave][k1];
+            const int4 q1 = reci[wave][k1];
+            eval(r0, q0, k0, WM);
+            if (!more1) break;
+            const bool more0 = mask_any(m);
+            k0 = mask_pop<0, kNW, true>(m, split, jmax);
+            k0 = more0 ? k0 : 0;
+            r0 = recf[wave][k0];
+            q0 = reci[wave][k0];
+            eval(r1, q1, k1, WM);
+            if (!more0) break;
+          }
         }
       };
       // FP64 keeps one loop with runtime flags (fewer spilled registers)
@@ -555,7 +592,12 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
       ljs[k] = S.ljf[k];
     __syncthreads();
   }
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // FP64: values equal on all lanes of the wave are marked uniform (uni), so
+  // the work item, its z range and box live in scalar registers (in the
+  // FP32 kernel the extra scalar registers spill and cost more than they
+  // save)
+  auto U = [](auto v) { return F64 ? uni(v) : v; };
+  const int wave = U((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
   const int item_idx = blockIdx.x * kTmWaves + wave;
   double acc[ACC_N];
 #pragma unroll
@@ -563,7 +605,7 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
   unsigned cnt = 0;
   if (item_idx < *nitems) {
     const int4 item = items[item_idx];
-    const int a0 = item.x, a1 = item.y, col = item.z;
+    const int a0 = U(item.x), a1 = U(item.y), col = U(item.z);
     const int cx = col / G.ny, cy = col % G.ny;
     const int slot = lane & (kTileAtoms - 1), split = lane >> 4;
     const int i = a0 + slot;
@@ -571,7 +613,7 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
     const int ii = active ? i : a0;
     const F32Params &F = P.f;
     const float hi = F.cut2 + F.band, lo = F.cut2 - F.band;
-    const int zlo = S.cell[a0] % G.nz, zhi = S.cell[a1 - 1] % G.nz;
+    const int zlo = U(S.cell[a0] % G.nz), zhi = U(S.cell[a1 - 1] % G.nz);
     const float4 li = S.loc[ii];
     const int zci = S.cell[ii] % G.nz;
     const float xi = li.x, yi = li.y;
@@ -582,12 +624,12 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
     const int ljbase = ti * P.n_types;  // row of this atom's type in ljs
     const double *ljrow64 = S.lj + 4 * (ti * P.n_types);
     // bounding box of the wave's atoms in home-relative coordinates
-    const float bx0 = wave_min(active ? xi : 1e30f),
-                bx1 = wave_max(active ? xi : -1e30f);
-    const float by0 = wave_min(active ? yi : 1e30f),
-                by1 = wave_max(active ? yi : -1e30f);
-    const float bz0 = wave_min(active ? zi : 1e30f),
-                bz1 = wave_max(active ? zi : -1e30f);
+    const float bx0 = U(wave_min(active ? xi : 1e30f)),
+                bx1 = U(wave_max(active ? xi : -1e30f));
+    const float by0 = U(wave_min(active ? yi : 1e30f)),
+                by1 = U(wave_max(active ? yi : -1e30f));
+    const float bz0 = U(wave_min(active ? zi : 1e30f)),
+                bz1 = U(wave_max(active ? zi : -1e30f));
     int zA = zlo - G.rz, zB = zhi + G.rz;
     const bool clampz = zB - zA + 1 > G.nz;
     if (clampz) zB = zA + G.nz - 1;
@@ -706,20 +748,45 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
       v0 = v1 = v2 = v3 = v4 = v5 = 0.f;
       auto run = [&](auto WZ) __attribute__((always_inline)) {
         if (!mask_any(m)) return;
-        int k = mask_pop<0, kNW, !F64>(m, split, jmax);
-        float4 r = recf[wave][k];
-        int4 q4 = reci[wave][k];
-        while (true) {
-          const bool more = mask_any(m);
-          int kn = k;
-          if (more) kn = mask_pop<0, kNW, !F64>(m, split, jmax);
-          const float4 rn = recf[wave][kn];
-          const int4 q4n = reci[wave][kn];
-          eval(r, q4, k, WZ);
-          if (!more) break;
-          r = rn;
-          q4 = q4n;
-          k = kn;
+        if constexpr (F64) {
+          int k = mask_pop<0, kNW, false>(m, split, jmax);
+          float4 r = recf[wave][k];
+          int4 q4 = reci[wave][k];
+          while (true) {
+            const bool more = mask_any(m);
+            int kn = k;
+            if (more) kn = mask_pop<0, kNW, false>(m, split, jmax);
+            const float4 rn = recf[wave][kn];
+            const int4 q4n = reci[wave][kn];
+            eval(r, q4, k, WZ);
+            if (!more) break;
+            r = rn;
+            q4 = q4n;
+            k = kn;
+          }
+        } else {
+          // two record sets in turn (no register moves); the next pop is
+          // taken unconditionally and its index replaced by 0 when the
+          // mask was empty, so every LDS read stays in the buffer
+          int k0 = mask_pop<0, kNW, true>(m, split, jmax);
+          float4 r0 = recf[wave][k0];
+          int4 q0 = reci[wave][k0];
+          while (true) {
+            const bool more1 = mask_any(m);
+            int k1 = mask_pop<0, kNW, true>(m, split, jmax);
+            k1 = more1 ? k1 : 0;
+            const float4 r1 = recf[wave][k1];
+            const int4 q1 = reci[wave][k1];
+            eval(r0, q0, k0, WZ);
+            if (!more1) break;
+            const bool more0 = mask_any(m);
+            k0 = mask_pop<0, kNW, true>(m, split, jmax);
+            k0 = more0 ? k0 : 0;
+            r0 = recf[wave][k0];
+            q0 = reci[wave][k0];
+            eval(r1, q1, k1, WZ);
+            if (!more0) break;
+          }
         }
       };
       // FP64 keeps one loop with a runtime flag (fewer spilled registers)
