@@ -784,6 +784,20 @@ int run_device(pmnb_ctx *ctx, const double *d_pos, double *d_forces,
   if (rc) return rc;
   if ((flags & PMNB_FLAG_MOLVIRIAL) && !d_com)
     return fail(ctx, PMNB_ERR_ARG, "molecular virial needs molecule centres");
+  // timing: [start, built) = cell build (+ work items or neighbour lists),
+  // [built, stop) = pair kernel
+  hipEvent_t ev_built = nullptr, ev_stop = nullptr;
+  if (ctx->timing) {
+    while (ctx->ev_pool.size() < ctx->ev_used + 3) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      ctx->ev_pool.push_back(e);
+    }
+    HIPCHK(hipEventRecord(ctx->ev_pool[ctx->ev_used], s));
+    ev_built = ctx->ev_pool[ctx->ev_used + 1];
+    ev_stop = ctx->ev_pool[ctx->ev_used + 2];
+    ctx->ev_used += 3;
+  }
   rc = build_cells(ctx, d_pos, s);
   if (rc) return rc;
   const ForceParams P = make_params(ctx, flags);
@@ -798,18 +812,6 @@ int run_device(pmnb_ctx *ctx, const double *d_pos, double *d_forces,
   const int cap = ctx->cap;
   unsigned *lst = ctx->d_list.as<unsigned>();
   int *cnt = ctx->d_ncount.as<int>();
-  hipEvent_t ev_built = nullptr, ev_stop = nullptr;
-  if (ctx->timing) {
-    while (ctx->ev_pool.size() < ctx->ev_used + 3) {
-      hipEvent_t e;
-      HIPCHK(hipEventCreate(&e));
-      ctx->ev_pool.push_back(e);
-    }
-    HIPCHK(hipEventRecord(ctx->ev_pool[ctx->ev_used], s));
-    ev_built = ctx->ev_pool[ctx->ev_used + 1];
-    ev_stop = ctx->ev_pool[ctx->ev_used + 2];
-    ctx->ev_used += 3;
-  }
   if (ctx->use_tile) {
     const int ncol = ctx->ncol_total, nrep = (int)ctx->reps.size();
     const int max_items = ctx->n / kTileAtoms + ncol + 1;
