@@ -74,7 +74,7 @@ struct pmnb_ctx {
   int per_pair_wrap = 0;
   long long last_pairs = 0;
   // device: topology (original order)
-  DevBuf d_type, d_q, d_mol, d_xfull, d_xmod, d_lj, d_ljf;
+  DevBuf d_type, d_q, d_mol, d_xfull, d_xmod, d_xown, d_lj, d_ljf;
   // device: per evaluation
   DevBuf d_pos, d_force, d_out, d_com, d_cell_of, d_count, d_start, d_fill,
       d_sorted, d_sx, d_sy, d_sz, d_loc, d_stype, d_sq, d_smol, d_scell,
@@ -761,6 +761,7 @@ Sorted make_sorted(const pmnb_ctx *ctx, const double *d_com) {
   S.delta = ctx->d_shell.as<int4>();
   S.xfull = ctx->d_xfull.as<unsigned long long>();
   S.xmod = ctx->d_xmod.as<unsigned long long>();
+  S.xown = ctx->d_xown.as<unsigned long long>();
   S.lj = ctx->d_lj.as<double>();
   S.ljf = ctx->d_ljf.as<float4>();
   S.com = d_com;
@@ -1009,6 +1010,21 @@ int pmnb_set_topology(pmnb_ctx *ctx, int32_t n_atoms, const int32_t *type,
     else if (cls == PMNB_EXCL_MODIFIED) mod[a] |= bit;
     maxd = std::max(maxd, d);
   }
+  // Each atom's own view of its exclusions: forward masks (partners a + d)
+  // and reverse masks (partners a - d, bit d - 1), so the tile kernels read
+  // only the masks of their own atoms.
+  std::vector<unsigned long long> xown((size_t)std::max(n_atoms, 1) * 4, 0ull);
+  for (int a = 0; a < n_atoms; a++) {
+    xown[4 * (size_t)a] = full[a];
+    xown[4 * (size_t)a + 1] = mod[a];
+    const unsigned long long fb = full[a], mb = mod[a];
+    for (unsigned long long r = fb | mb; r; r &= r - 1) {
+      const int d = __builtin_ctzll(r) + 1;
+      const unsigned long long bit = 1ull << (d - 1);
+      if (fb & bit) xown[4 * (size_t)(a + d) + 2] |= bit;
+      if (mb & bit) xown[4 * (size_t)(a + d) + 3] |= bit;
+    }
+  }
   // LJ table: entries (i <= j) mirrored, as LennardJonesParameterTable::set
   // holds one record per unordered type pair.
   const int T = n_types;
@@ -1028,6 +1044,7 @@ int pmnb_set_topology(pmnb_ctx *ctx, int32_t n_atoms, const int32_t *type,
   HIPCHK(ctx->d_mol.ensure(N * sizeof(int)));
   HIPCHK(ctx->d_xfull.ensure(N * sizeof(unsigned long long)));
   HIPCHK(ctx->d_xmod.ensure(N * sizeof(unsigned long long)));
+  HIPCHK(ctx->d_xown.ensure(N * 4 * sizeof(unsigned long long)));
   HIPCHK(ctx->d_lj.ensure(ljd.size() * sizeof(double)));
   HIPCHK(ctx->d_ljf.ensure(ljf.size() * sizeof(float4)));
   if (n_atoms > 0) {
@@ -1040,6 +1057,8 @@ int pmnb_set_topology(pmnb_ctx *ctx, int32_t n_atoms, const int32_t *type,
     HIPCHK(hipMemcpy(ctx->d_xfull.p, full.data(), n_atoms * 8,
                      hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->d_xmod.p, mod.data(), n_atoms * 8,
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_xown.p, xown.data(), (size_t)n_atoms * 32,
                      hipMemcpyHostToDevice));
   }
   HIPCHK(hipMemcpy(ctx->d_lj.p, ljd.data(), ljd.size() * sizeof(double),
@@ -1254,7 +1273,7 @@ void pmnb_destroy(pmnb_ctx *ctx) {
   (void)hipSetDevice(ctx->prm.device);
   (void)hipStreamSynchronize(ctx->stream);
   DevBuf *bufs[] = {&ctx->d_type, &ctx->d_q, &ctx->d_mol, &ctx->d_xfull,
-                    &ctx->d_xmod, &ctx->d_lj, &ctx->d_ljf, &ctx->d_pos,
+                    &ctx->d_xmod, &ctx->d_xown, &ctx->d_lj, &ctx->d_ljf, &ctx->d_pos,
                     &ctx->d_force, &ctx->d_out, &ctx->d_com, &ctx->d_cell_of,
                     &ctx->d_count, &ctx->d_start, &ctx->d_fill, &ctx->d_sorted,
                     &ctx->d_sx, &ctx->d_sy, &ctx->d_sz, &ctx->d_loc,

@@ -184,6 +184,20 @@ __device__ __forceinline__ double poly5_exact(double ar) {
 
 // Exclusion class of atoms a, b of the same molecule from the per-atom
 // masks (bit d-1 of full[min]/mod[min] holds pair (min, min + d)).
+// Exclusion class of the pair (a, a + d) from atom a's own masks m =
+// {forward FULL, forward MODIFIED, reverse FULL, reverse MODIFIED} (bit
+// |d| - 1; forward for d > 0, reverse for d < 0): the same answer as
+// excl_class, without reading the partner's masks.
+__device__ __forceinline__ int excl_own(const unsigned long long *m, int d) {
+  const int ad = d < 0 ? -d : d;
+  if (ad < 1 || ad > 64) return 0;
+  const unsigned long long bit = 1ull << (ad - 1);
+  const int o = d > 0 ? 0 : 2;
+  if (m[o] & bit) return 2;
+  if (m[o + 1] & bit) return 1;
+  return 0;
+}
+
 __device__ __forceinline__ int excl_class(const unsigned long long *full,
                                           const unsigned long long *mod,
                                           int max_delta, int a, int b) {

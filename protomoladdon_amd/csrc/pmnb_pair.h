@@ -30,6 +30,8 @@ struct Sorted {
   const int *cstart, *ccount;    // per-cell segment of the sorted order
   const int4 *delta;             // shell offsets, (0,0,0) first
   const unsigned long long *xfull, *xmod;  // exclusion masks (original idx)
+  const unsigned long long *xown;  // per atom {forward FULL, forward MODIFIED,
+                                   // reverse FULL, reverse MODIFIED}
   const double *lj;              // T*T*4 {A, B, A14, B14}
   const float4 *ljf;             // same in FP32
   const double *com;             // molecule centres (molecular virial)

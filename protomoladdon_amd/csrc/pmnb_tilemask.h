@@ -239,6 +239,8 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
   __shared__ float4 recf[kTmWaves][BUF];
   __shared__ int4 reci[kTmWaves][BUF];   // sorted index, type, mol, atom
   __shared__ double4 recd[F64 ? kTmWaves : 1][F64 ? BUF : 1];
+  // exclusion masks of the wave's own atoms (excl_own layout)
+  __shared__ unsigned long long xm[kTmWaves][kTileAtoms][4];
   __shared__ float4 ljs[F64 ? 1 : kQLj];  // FP32 LJ table (T * T <= kQLj)
   if (!F64) {
     for (int k = threadIdx.x; k < P.n_types * P.n_types; k += kBlock)
@@ -279,17 +281,22 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
     const float xi = li.x, yi = li.y;
     const float zi = (float)((S.cell[ii] - coff) % G.nz - zlo) * G.csz + li.z;
     const int ti = S.type[ii], mi = S.mol[ii], ai = S.atom[ii];
+    if (split == 0) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) xm[wave][slot][k] = S.xown[4 * ai + k];
+    }
+    wave_sync_lds();
     const double qi = S.q[ii];
     const double xd = S.x[ii], yd = S.y[ii], zd = S.z[ii];
     const int ljbase = ti * P.n_types;  // row of this atom's type in ljs
     const double *ljrow64 = S.lj + 4 * (ti * P.n_types);
     // bounding box of the wave's atoms in home-relative coordinates
-    const float bx0 = uni(wave_min(active ? xi : 1e30f)),
-                bx1 = uni(wave_max(active ? xi : -1e30f));
-    const float by0 = uni(wave_min(active ? yi : 1e30f)),
-                by1 = uni(wave_max(active ? yi : -1e30f));
-    const float bz0 = uni(wave_min(active ? zi : 1e30f)),
-                bz1 = uni(wave_max(active ? zi : -1e30f));
+    const float bx0 = (wave_min(active ? xi : 1e30f)),
+                bx1 = (wave_max(active ? xi : -1e30f));
+    const float by0 = (wave_min(active ? yi : 1e30f)),
+                by1 = (wave_max(active ? yi : -1e30f));
+    const float bz0 = (wave_min(active ? zi : 1e30f)),
+                bz1 = (wave_max(active ? zi : -1e30f));
     int zA = zlo - G.rz, zB = zhi + G.rz;
     const bool wz = zB - zA + 1 > G.nz;  // all z cells: per-pair image
     if (wz) zB = zA + G.nz - 1;
@@ -328,7 +335,7 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
       }
       const bool same = q4.z == mi;
       const int excl =
-          same ? excl_class(S.xfull, S.xmod, P.max_excl_delta, ai, q4.w) : 0;
+          same ? excl_own(xm[wave][slot], q4.w - ai) : 0;
       if (excl == 2) return;
       if (i < t) cnt++;
       const bool molv = (P.flags & 2u) && !same;
@@ -586,6 +593,8 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
   __shared__ float4 recf[kTmWaves][BUF];
   __shared__ int4 reci[kTmWaves][BUF];
   __shared__ double4 recd[F64 ? kTmWaves : 1][F64 ? BUF : 1];
+  // exclusion masks of the wave's own atoms (excl_own layout)
+  __shared__ unsigned long long xm[kTmWaves][kTileAtoms][4];
   __shared__ float4 ljs[F64 ? 1 : kQLj];  // FP32 LJ table (T * T <= kQLj)
   if (!F64) {
     for (int k = threadIdx.x; k < P.n_types * P.n_types; k += kBlock)
@@ -619,6 +628,11 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
     const float xi = li.x, yi = li.y;
     const float zi = (float)(zci - zlo) * G.csz + li.z;
     const int ti = S.type[ii], mi = S.mol[ii], ai = S.atom[ii];
+    if (split == 0) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) xm[wave][slot][k] = S.xown[4 * ai + k];
+    }
+    wave_sync_lds();
     const double qi = S.q[ii];
     const double xd = S.x[ii], yd = S.y[ii], zd = S.z[ii];
     const int ljbase = ti * P.n_types;  // row of this atom's type in ljs
@@ -667,7 +681,7 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
       }
       const bool same = q4.z == mi;
       const int excl =
-          same ? excl_class(S.xfull, S.xmod, P.max_excl_delta, ai, q4.w) : 0;
+          same ? excl_own(xm[wave][slot], q4.w - ai) : 0;
       if (excl == 2) return;
       if (i < t) cnt++;
       const bool molv = (P.flags & 2u) && !same;
