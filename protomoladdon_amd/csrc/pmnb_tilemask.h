@@ -2,21 +2,24 @@
 // replicas (TileRep, pmnb_queue.h).
 //
 // One wave handles 16 consecutive sorted atoms of one column, 4 lanes per
-// atom (lane = atom slot + 16 * split).  Candidates of the neighbour columns
-// stream into an LDS buffer as in k_tq.  When the buffer is nearly full each
-// lane tests every 4th buffered candidate against its own atom into a 64-bit
-// mask, then evaluates the set bits one by one, reading the candidate from
-// LDS (the next candidate is read before the current pair's arithmetic).
+// atom (lane = atom slot + 16 * split).  Candidates of the 3 x 3 neighbour
+// columns stream into an LDS buffer one z level at a time (stream_levels),
+// after a test of their distance to the wave's bounding box.  When the
+// buffer is nearly full each lane tests every 4th buffered candidate against
+// its own atom into a pass mask, then evaluates the set bits one by one,
+// reading each candidate from LDS one pair ahead of its arithmetic.  The LJ
+// table and the exclusion masks of the wave's own atoms are held in LDS.
 // Forces stay in registers: FP32 partials per buffer, added into 64-bit
 // fixed point (mixed) or FP64 sums, and the 4 lanes of an atom combine by
 // shuffles at the end.  Pairs within `band` of the cutoff are re-decided
 // with the reference's FP64 minimum image.  Every pair is visited from both
-// atoms; energies and virials take half.  Work items are walked with a grid
-// stride; each item writes one partial row of energies and virials.
+// atoms; energies and virials take half.  Each item writes one partial row
+// of energies and virials.
 //
-// Against k_tq this trades lane idling (a lane with few pairs waits for the
-// lane of its wave with most) for no LDS traffic per pair and higher
-// occupancy; the autotuner picks between them.
+// k_tm walks the work items of any number of replicas with a grid stride;
+// k_tile is its single-system form.  Against k_tq this trades lane idling (a
+// lane with few pairs waits for the lane of its wave with most) for no LDS
+// traffic per pair and higher occupancy; the autotuner picks between them.
 #pragma once
 
 #include <type_traits>
@@ -557,12 +560,6 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
   if (threadIdx.x == 0 && tot) atomicAdd(npairs, tot);
 }
 
-// Single-system form of k_tm: one wave per item, the column grid as a
-// kernel argument, minimum image along z only when the cutoff reaches every
-// z cell (x and y need at least two columns).  Same per-lane work order and
-// per-item partial rows as k_tm, so both give the same bits; it keeps the
-// per-item values in vector registers (no replica table), which measures
-// faster for one large system.
 // Tests the buffered records against one atom (mask_test layout).  WRAPZ:
 // minimum image along z.
 template <bool WRAPZ, int NW>
@@ -582,6 +579,12 @@ __device__ __forceinline__ void tile_test(const float4 *__restrict__ rec,
   mask_test<0, NW>((n + kTmSplit - 1) / kTmSplit, m, test);
 }
 
+// Single-system form of k_tm: one wave per item, the column grid as a
+// kernel argument, minimum image along z only when the cutoff reaches every
+// z cell (x and y need at least two columns).  Same per-lane work order and
+// per-item partial rows as k_tm, so both give the same bits; without a
+// replica table it needs fewer scalar registers, which measures faster for
+// one large system.
 template <bool F64, int ELEC, int ELSW, int BUF>
 __global__ void __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(F64 ? PMNB_TM_OCC64 : PMNB_TM_OCC32)))
