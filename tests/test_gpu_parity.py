@@ -76,6 +76,21 @@ def check_mixed(w, flags=1):
     return err
 
 
+def slab_water():
+    """600 waters with z compressed to 0.7 of the box: two columns in x and
+    y, and a z axis the cutoff spans completely (the single-system tile
+    kernel then takes the minimum image along z per pair)."""
+    from protomoladdon_amd.topology import PeriodicBoundaryConditions
+    w = systems.small_water(n_mol=600)
+    L = float(w.topology.boundary_conditions.lengths()[0])
+    Lz = 0.7 * L
+    w.positions = w.positions * np.array([1.0, 1.0, 0.7])
+    w.topology.boundary_conditions = PeriodicBoundaryConditions(
+        (L, 0, 0), (0, L, 0), (0, 0, Lz), (0.5 * L, 0.5 * L, 0.5 * Lz))
+    w.name = "water_slab"
+    return w
+
+
 # ------------------------------------------------------------------ FP64
 @pytest.mark.parametrize("make", [
     systems.config0,
@@ -83,7 +98,9 @@ def check_mixed(w, flags=1):
     lambda: systems.small_chains(n_chains=120),
     lambda: systems.config3_replica(0, 167),
     lambda: systems.config3_replica(7, 2667),
-], ids=["config0", "water_ewald", "chains", "replica_small", "replica_large"])
+    slab_water,
+], ids=["config0", "water_ewald", "chains", "replica_small", "replica_large",
+        "water_slab"])
 def test_fp64_matches_oracle(make):
     check_fp64(make())
 
@@ -182,7 +199,8 @@ def test_fp64_is_deterministic_and_accumulates():
     lambda: systems.small_water(n_mol=400),
     lambda: systems.small_chains(n_chains=120),
     lambda: systems.config3_replica(0, 167),
-], ids=["config0", "water_ewald", "chains", "replica_small"])
+    slab_water,
+], ids=["config0", "water_ewald", "chains", "replica_small", "water_slab"])
 def test_mixed_matches_oracle(make):
     check_mixed(make())
 
