@@ -657,7 +657,7 @@ void launch_adj(int max_items, hipStream_t s, const ForceParams &P,
   hipLaunchKernelGGL((k_tq_adj<F64, EL, NW, BUF, MK>), dim3(nb),
                      dim3(64 * NW), 0, s, P, make_adj_consts<float>(P),
                      make_adj_consts<double>(P), S, reps, items, nitems, gf,
-                     gx, gq, plj, np);
+                     gx, gq, plj, max_items, np);
 }
 
 template <bool F64, int EL>
@@ -669,14 +669,14 @@ void launch_adj_t(int T, int max_items, hipStream_t s, const ForceParams &P,
     launch_adj<F64, EL, 3>(max_items, s, P, S, reps, items, nitems, gf, gx,
                            gq, plj, np);
     if (glj)
-      hipLaunchKernelGGL(k_adj_lj_finish<3>, dim3(1), dim3(256), 0, s, plj,
-                         nitems, T, glj);
+      hipLaunchKernelGGL(k_adj_lj_finish<3>, dim3(3 * 4), dim3(1024), 0, s,
+                         plj, max_items, nitems, T, glj);
   } else {
     launch_adj<F64, EL, 10>(max_items, s, P, S, reps, items, nitems, gf, gx,
                             gq, plj, np);
     if (glj)
-      hipLaunchKernelGGL(k_adj_lj_finish<10>, dim3(1), dim3(256), 0, s, plj,
-                         nitems, T, glj);
+      hipLaunchKernelGGL(k_adj_lj_finish<10>, dim3(10 * 4), dim3(1024), 0, s,
+                         plj, max_items, nitems, T, glj);
   }
 }
 

@@ -135,7 +135,8 @@ k_tq_adj(ForceParams P, AdjC<float> C32, AdjC<double> C64, Sorted S,
          const TileRep *__restrict__ reps, const int4 *__restrict__ items,
          const int *__restrict__ nitems, const double *__restrict__ grad_f,
          double *__restrict__ grad_x, double *__restrict__ grad_q,
-         double *__restrict__ partial_lj, unsigned long long *__restrict__ npairs) {
+         double *__restrict__ partial_lj, int pstride,
+         unsigned long long *__restrict__ npairs) {
   using Real = typename std::conditional<F64, double, float>::type;
   constexpr int KS = kTileAtoms * kAdjSlots;
   __shared__ float4 recf[NW][BUF];       // home-relative x, y, z; charge
@@ -446,7 +447,7 @@ k_tq_adj(ForceParams P, AdjC<float> C32, AdjC<double> C64, Sorted S,
       for (int c = 0; c < 4; c++) {
         double v = (double)pa[k][c];
         for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-        if (lane == 0) partial_lj[((size_t)item_idx * MK + k) * 4 + c] = v;
+        if (lane == 0) partial_lj[(size_t)(k * 4 + c) * pstride + item_idx] = v;
       }
     wave_sync_lds();
   }
@@ -454,36 +455,43 @@ k_tq_adj(ForceParams P, AdjC<float> C32, AdjC<double> C64, Sorted S,
   if (threadIdx.x == 0 && tot) atomicAdd(npairs, tot);
 }
 
-// Sums the item partials of the LJ-parameter gradients in item order and
-// adds them to grad_lj (T x T x 4, {A, B, A14, B14}), half of an unordered
-// off-diagonal pair's gradient to each of its two entries.
+// Sums the item partials of the LJ-parameter gradients (accumulator kc of
+// item b at partial_lj[kc * pstride + b]) with a fixed reduction tree, one
+// block of 1024 threads per accumulator, and adds them to grad_lj (T x T x
+// 4, {A, B, A14, B14}), half of an unordered off-diagonal pair's gradient to
+// each of its two entries.
 template <int MK>
-__global__ void __launch_bounds__(256)
-k_adj_lj_finish(const double *__restrict__ partial_lj,
+__global__ void __launch_bounds__(1024)
+k_adj_lj_finish(const double *__restrict__ partial_lj, int pstride,
                 const int *__restrict__ nitems, int T,
                 double *__restrict__ grad_lj) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __shared__ double red[16];
+  const int kc = blockIdx.x;  // key * 4 + component
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int n = *nitems;
-  for (int kc = wave; kc < MK * 4; kc += 4) {
-    double v = 0.0;
-    for (int it = lane; it < n; it += 64) v += partial_lj[(size_t)it * MK * 4 + kc];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if (lane != 0) continue;
-    const int key = kc / 4, c = kc % 4;
-    // key -> (lo, hi)
-    int lo = 0, rem = key;
-    while (lo < T && rem >= T - lo) {
-      rem -= T - lo;
-      lo++;
-    }
-    if (lo >= T) continue;
-    const int hi = lo + rem;
-    if (lo == hi) {
-      grad_lj[4 * (lo * T + hi) + c] += v;
-    } else {
-      grad_lj[4 * (lo * T + hi) + c] += 0.5 * v;
-      grad_lj[4 * (hi * T + lo) + c] += 0.5 * v;
-    }
+  const double *col = partial_lj + (size_t)kc * pstride;
+  double v = 0.0;
+  for (int it = t; it < n; it += 1024) v += col[it];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  if (t != 0) return;
+  double sum = 0.0;
+  for (int k = 0; k < 16; k++) sum += red[k];
+  const int key = kc / 4, c = kc % 4;
+  // key -> (lo, hi)
+  int lo = 0, rem = key;
+  while (lo < T && rem >= T - lo) {
+    rem -= T - lo;
+    lo++;
+  }
+  if (lo >= T) return;
+  const int hi = lo + rem;
+  if (lo == hi) {
+    grad_lj[4 * (lo * T + hi) + c] += sum;
+  } else {
+    grad_lj[4 * (lo * T + hi) + c] += 0.5 * sum;
+    grad_lj[4 * (hi * T + lo) + c] += 0.5 * sum;
   }
 }
 

@@ -220,7 +220,10 @@ __device__ __forceinline__ void pair_f32(const F32Params &F, float r2,
                                          float A, float B, float qq,
                                          int excl, float &ff, float &e1,
                                          float &e2) {
-  const float rd2 = __builtin_amdgcn_rcpf(r2);
+  // with electrostatics 1/r^2 is the square of the 1/r the term needs (one
+  // transcendental instead of two)
+  const float rr = ELEC != 0 ? __builtin_amdgcn_rsqf(r2) : 0.f;
+  const float rd2 = ELEC != 0 ? rr * rr : __builtin_amdgcn_rcpf(r2);
   const float r6 = rd2 * rd2 * rd2;
   const float r6B = B * r6;
   const float r12A = A * r6 * r6;
@@ -230,11 +233,10 @@ __device__ __forceinline__ void pair_f32(const F32Params &F, float r2,
   e2 = 0.f;
   if (ELEC == 1) {
     if (excl == 1) qq *= F.csf;
-    e2 = qq * __builtin_amdgcn_rsqf(r2);
+    e2 = qq * rr;
     f2 = e2 * rd2;
   } else if (ELEC == 2) {
     if (excl == 1) qq *= F.csf;
-    const float rr = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * rr;
     const float ar = F.alpha * r;
     // Abramowitz & Stegun 7.1.26 (base/MathUtilities.cpp:39-49)
