@@ -238,22 +238,45 @@ __global__ void k_gather(Box b, Grid g, const double *__restrict__ pos,
 // to replica rep_of[i] (replica 0 when rep_of is null), whose box and
 // column grid place it; cells are numbered globally from the replica's
 // cell offset.  RepT provides box, grid and cell_off.
+// Cells of a block's atoms are few (consecutive atoms are neighbours): the
+// block counts its atoms per cell in an LDS table (open addressing on the
+// cell index), takes one global atomic add per distinct cell, all issued
+// together, and gives each atom the cell's base plus its rank in the block.
+// Requires blockDim.x == kBlock.
 template <class RepT>
-__global__ void k_assign_r(const RepT *__restrict__ reps,
-                           const int *__restrict__ rep_of,
-                           const double *__restrict__ pos, int n,
-                           int *__restrict__ cell_of, int *__restrict__ count,
-                           int *__restrict__ rank) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const RepT &R = reps[rep_of ? rep_of[i] : 0];
-  int cx, cy, cz;
-  double vx, vy, vz;
-  cell_coords(R.box, R.grid, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], cx,
-              cy, cz, vx, vy, vz);
-  int c = R.cell_off + (cx * R.grid.n[1] + cy) * R.grid.n[2] + cz;
-  cell_of[i] = c;
-  rank[i] = atomicAdd(&count[c], 1);
+__global__ void __launch_bounds__(kBlock)
+k_assign_r(const RepT *__restrict__ reps, const int *__restrict__ rep_of,
+           const double *__restrict__ pos, int n, int *__restrict__ cell_of,
+           int *__restrict__ count, int *__restrict__ rank) {
+  __shared__ int key[kBlock], lcount[kBlock], base[kBlock];
+  const int t = threadIdx.x;
+  const int i = blockIdx.x * kBlock + t;
+  key[t] = -1;
+  lcount[t] = 0;
+  __syncthreads();
+  int slot = -1, lrank = 0;
+  if (i < n) {
+    const RepT &R = reps[rep_of ? rep_of[i] : 0];
+    int cx, cy, cz;
+    double vx, vy, vz;
+    cell_coords(R.box, R.grid, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2],
+                cx, cy, cz, vx, vy, vz);
+    const int c = R.cell_off + (cx * R.grid.n[1] + cy) * R.grid.n[2] + cz;
+    cell_of[i] = c;
+    // at most kBlock distinct cells in kBlock slots: the probe ends
+    int h = c & (kBlock - 1);
+    while (true) {
+      const int old = atomicCAS(&key[h], -1, c);
+      if (old == -1 || old == c) break;
+      h = (h + 1) & (kBlock - 1);
+    }
+    slot = h;
+    lrank = atomicAdd(&lcount[h], 1);
+  }
+  __syncthreads();
+  if (key[t] >= 0) base[t] = atomicAdd(&count[key[t]], lcount[t]);
+  __syncthreads();
+  if (i < n) rank[i] = base[slot] + lrank;
 }
 
 // Per-cell ordering and gather of the tile path in one pass: one wave per
