@@ -574,7 +574,9 @@ int launch_tq_shape(const TqArgs &a) {
 template <bool F64, int EL, int SW>
 int launch_tq(int shape, const TqArgs &a) {
   if (shape == 0 && a.single) {
-    const int nb = std::max(1, blocks_for(a.max_items, kTmWaves));
+    // a multiple of 8 blocks (k_tile maps blocks to XCD-contiguous items)
+    const int nb = 8 * blocks_for(std::max(1, blocks_for(a.max_items,
+                                                         kTmWaves)), 8);
     hipLaunchKernelGGL((k_tile<F64, EL, SW, F64 ? kTmBuf64 : kTmBuf32>), dim3(nb),
                        dim3(kBlock), 0, a.s, *a.P, *a.S, a.G, a.items,
                        a.nitems, a.forces, a.partial, a.pstride, a.npairs);

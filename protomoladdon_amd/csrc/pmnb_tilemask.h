@@ -610,7 +610,15 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
   // save)
   auto U = [](auto v) { return F64 ? uni(v) : v; };
   const int wave = U((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
-  const int item_idx = blockIdx.x * kTmWaves + wave;
+  // FP64: blocks go to the 8 XCDs in turn; block b takes the items of block
+  // position (b % 8) * per + b / 8 (the grid is a multiple of 8), so each
+  // XCD works through one contiguous range of items and the candidates that
+  // neighbouring items share stay in that XCD's L2 (measured faster for the
+  // FP64 kernel, slightly slower for the FP32 one).
+  const int per = gridDim.x / 8;
+  const int blk =
+      F64 ? (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8) : blockIdx.x;
+  const int item_idx = blk * kTmWaves + wave;
   double acc[ACC_N];
 #pragma unroll
   for (int k = 0; k < ACC_N; k++) acc[k] = 0.0;
