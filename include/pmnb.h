@@ -269,6 +269,68 @@ const char *pmnb_batch_last_error(const pmnb_batch *b);
 /* Frees the batch and its device memory. */
 void pmnb_batch_destroy(pmnb_batch *b);
 
+/* ------------------------------------------------------------ leapfrog
+   Device-resident LeapfrogIntegrator::run (integrator/leapfrog/
+   LeapfrogIntegrator.cpp:85-99) whose force group is this context's cutoff
+   force, with the modifiers ModifierShake (attached post-drift,
+   modifier/ModifierShake.cpp:16-89) and ModifierRattle (attached post-step,
+   modifier/ModifierRattle.cpp:17-71) as module/ModifierModule.cpp:104-127
+   attaches them.  The step sequence, the kicks and drifts
+   (integrator/StandardIntegrator.cpp:91-116, STSIntegrator.cpp:44-48) and
+   each SHAKE/RATTLE sweep follow the reference operation for operation in
+   FP64; the sweep count is the reference's (mean error over all
+   constraints <= epsilon, at most max_iter + 1 sweeps).  The constraint
+   virial the modifiers add (ScalarStructure::addVirial) is not computed. */
+typedef struct pmnb_leapfrog pmnb_leapfrog;
+
+typedef struct {
+  double timestep;        /* fs (STSIntegrator::myTimestep) */
+  int32_t shake;          /* ModifierShake attached */
+  int32_t rattle;         /* ModifierRattle attached */
+  double shake_epsilon;   /* shakeEpsilon (default 1e-5) */
+  int32_t shake_max_iter; /* shakeMaxIter (default 30) */
+  int32_t rattle_max_iter;/* rattleMaxIter (default 30) */
+  double rattle_epsilon;  /* rattleEpsilon (default 1e-5) */
+} pmnb_lf_params;
+
+/* Creates the integrator over ctx's system: mass[n_atoms] is
+   Atom::scaledMass; constraints are (atom1, atom2) pairs with rest lengths
+   in the order of GenericTopology::bondRattleShakeConstraints
+   (topology/TopologyUtilities.cpp:694-775).  A connected set of
+   constraints may span at most 4 atoms and 6 constraints
+   (PMNB_ERR_UNSUPPORTED otherwise).  SHAKE's previous positions start as
+   the positions given to the first run (ModifierMetaRattleShake::
+   doInitialize). */
+int pmnb_lf_create(pmnb_ctx *ctx, const pmnb_lf_params *params,
+                   int32_t n_atoms, const double *mass, int32_t n_constraints,
+                   const int32_t *constraint_atoms, const double *rest_length,
+                   pmnb_leapfrog **out);
+
+/* LeapfrogIntegrator::run(n_steps) on device arrays, enqueued on `stream`
+   (NULL = the context's stream) without host synchronisation.
+   d_positions, d_velocities: 3N doubles, updated.  d_forces: the forces at
+   d_positions on entry (as after initializeForces), at the final positions
+   on exit.  d_out: the 20-double block of pmnb_evaluate_device, holding the
+   last evaluation's energies on exit (calculateForces clears them before
+   every evaluation).  flags: as pmnb_evaluate_device, without
+   PMNB_FLAG_MOLVIRIAL. */
+int pmnb_lf_run_device(pmnb_leapfrog *lf, double *d_positions,
+                       double *d_velocities, double *d_forces, double *d_out,
+                       int32_t n_steps, uint32_t flags, void *stream);
+
+/* Starts (capacity > 0) or stops (0) recording the sweep count of every
+   SHAKE and RATTLE call, and returns what was recorded since the last
+   start: up to `capacity_out` counts into iters (may be NULL), the number
+   of calls and the number that reached max_iter + 1 sweeps (the
+   reference's "not converged" warning).  Waits for the device. */
+int pmnb_lf_iterations(pmnb_leapfrog *lf, int64_t capacity, int32_t *iters,
+                       int64_t capacity_out, int64_t *n_calls,
+                       int64_t *n_unconverged);
+
+/* Releases the integrator; may be called before or after pmnb_destroy of
+   its context (which must outlive every other pmnb_lf_* call). */
+void pmnb_lf_destroy(pmnb_leapfrog *lf);
+
 #ifdef __cplusplus
 }
 #endif

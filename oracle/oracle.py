@@ -295,3 +295,67 @@ def run_ref(system):
     off += 8
     r.pairs = np.frombuffer(raw, np.int32, 3 * r.n_pairs, off).reshape(-1, 3).copy()
     return r
+
+
+# ------------------------------------------------------------------ leapfrog
+class OrcLf(ctypes.Structure):
+    _fields_ = [
+        ("n_cons", ctypes.c_int32), ("cons", ctypes.c_void_p),
+        ("rest", ctypes.c_void_p), ("mass", ctypes.c_void_p),
+        ("timestep", ctypes.c_double), ("shake_eps", ctypes.c_double),
+        ("rattle_eps", ctypes.c_double), ("shake_max_iter", ctypes.c_int32),
+        ("rattle_max_iter", ctypes.c_int32), ("shake", ctypes.c_int32),
+        ("rattle", ctypes.c_int32),
+    ]
+
+
+class MDState:
+    """Positions, velocities, forces at the positions and SHAKE's previous
+    positions, as LeapfrogIntegrator and its modifiers keep them."""
+
+    def __init__(self, x, v, f, last=None, energies=None):
+        self.x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3).copy()
+        self.v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3).copy()
+        self.f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1, 3).copy()
+        self.last = self.x.copy() if last is None else np.array(
+            last, dtype=np.float64).reshape(-1, 3)
+        self.energies = np.zeros(2) if energies is None else np.array(energies)
+
+
+def leapfrog_run(system, state, n_steps, masses, constraints, rest, timestep,
+                 shake=None, rattle=None, max_calls=1 << 16):
+    """LeapfrogIntegrator::run(n_steps) on `state` (updated in place) with
+    the cutoff force of `system` as the only force.  shake / rattle are
+    (epsilon, max_iter) or None.  Returns the sweep count of every SHAKE
+    and RATTLE call in call order."""
+    lib = load()
+    if not hasattr(lib, "_lf_ready"):
+        lib.orc_leapfrog_run.argtypes = [
+            ctypes.POINTER(OrcSystem), ctypes.POINTER(OrcLf)] + \
+            [ctypes.c_void_p] * 5 + [ctypes.c_int32, ctypes.c_void_p,
+                                     ctypes.c_int64]
+        lib.orc_leapfrog_run.restype = ctypes.c_int64
+        lib._lf_ready = True
+    st = system.struct()
+    cons = np.ascontiguousarray(constraints, dtype=np.int32).reshape(-1, 2)
+    rest = np.ascontiguousarray(rest, dtype=np.float64)
+    mass = np.ascontiguousarray(masses, dtype=np.float64)
+    lf = OrcLf()
+    lf.n_cons = len(cons)
+    lf.cons = cons.ctypes.data if len(cons) else None
+    lf.rest = rest.ctypes.data if len(cons) else None
+    lf.mass = mass.ctypes.data
+    lf.timestep = float(timestep)
+    lf.shake = int(shake is not None)
+    lf.rattle = int(rattle is not None)
+    lf.shake_eps, lf.shake_max_iter = shake if shake else (0.0, 0)
+    lf.rattle_eps, lf.rattle_max_iter = rattle if rattle else (0.0, 0)
+    iters = np.zeros(max_calls, dtype=np.int32)
+    n = lib.orc_leapfrog_run(ctypes.byref(st), ctypes.byref(lf),
+                             state.x.ctypes.data, state.v.ctypes.data,
+                             state.f.ctypes.data, state.last.ctypes.data,
+                             state.energies.ctypes.data, int(n_steps),
+                             iters.ctypes.data, max_calls)
+    if n < 0:
+        raise RuntimeError("oracle leapfrog failed")
+    return iters[:min(n, max_calls)].copy()

@@ -626,3 +626,217 @@ void orc_switch_c2(double switchon, double cutoff, double d2, double *v, double 
   sw_init_c2(&s, switchon, cutoff);
   sw_eval(&s, d2, v, dv);
 }
+
+/* ------------------------------------------------------------ leapfrog
+ * LeapfrogIntegrator::run with the SHAKE (post-drift) and RATTLE
+ * (post-step) modifiers, restated operation for operation:
+ *   integrator/leapfrog/LeapfrogIntegrator.cpp:33-99   run, doHalfKickdoDrift,
+ *                                                      doKickdoDrift
+ *   integrator/StandardIntegrator.cpp:91-116           doHalfKick, doKick
+ *   integrator/STSIntegrator.cpp:28-51                 doDrift, calculateForces
+ *   modifier/ModifierShake.cpp:16-89                   SHAKE sweeps
+ *   modifier/ModifierRattle.cpp:17-71                  RATTLE sweeps
+ *   modifier/ModifierMetaShake.cpp:11-33, ModifierMetaRattle.cpp:11-30  errors
+ *   module/ModifierModule.cpp:104-127                  which hook each uses
+ * The constraint virial (ScalarStructure::addVirial in both modifiers) is
+ * not accumulated: calculateForces clears the energies after every SHAKE
+ * and every RATTLE but the last one.
+ */
+#define ORC_TIMEFACTOR 48.88821290839616       /* base/PMConstants.cpp:50 */
+#define ORC_INV_TIMEFACTOR 0.02045482828087295 /* base/PMConstants.cpp:51 */
+
+typedef struct {
+  int32_t n_cons;
+  const int32_t *cons;  /* 2 per constraint: atom1, atom2, in list order */
+  const double *rest;   /* rest length per constraint */
+  const double *mass;   /* scaledMass per atom */
+  double timestep;      /* fs */
+  double shake_eps, rattle_eps;
+  int32_t shake_max_iter, rattle_max_iter;
+  int32_t shake, rattle; /* modifier present (1) or not (0) */
+} orc_lf;
+
+typedef struct {
+  const orc_system *s;
+  const orc_lf *lf;
+  double *x, *v, *f, *last, *energies2;
+  int32_t *iters;
+  int64_t iters_cap, n_calls;
+} orc_md;
+
+static void md_record(orc_md *m, int iters) {
+  if (m->iters && m->n_calls < m->iters_cap) m->iters[m->n_calls] = iters;
+  m->n_calls++;
+}
+
+/* ModifierMetaShake::calcError */
+static double shake_error(const orc_md *m) {
+  const orc_lf *lf = m->lf;
+  double error = 0;
+  int i, k;
+  for (i = 0; i < lf->n_cons; i++) {
+    int a1 = lf->cons[2 * i], a2 = lf->cons[2 * i + 1];
+    double r2 = lf->rest[i] * lf->rest[i], d[3], n2;
+    for (k = 0; k < 3; k++) d[k] = m->x[3 * a2 + k] - m->x[3 * a1 + k];
+    n2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    error += fabs(n2 - r2) / (2.0 * r2);
+  }
+  return error / lf->n_cons;
+}
+
+/* ModifierMetaRattle::calcError */
+static double rattle_error(const orc_md *m) {
+  const orc_lf *lf = m->lf;
+  double error = 0;
+  int i, k;
+  for (i = 0; i < lf->n_cons; i++) {
+    int a1 = lf->cons[2 * i], a2 = lf->cons[2 * i + 1];
+    double vab[3], rab[3];
+    for (k = 0; k < 3; k++) {
+      vab[k] = m->v[3 * a1 + k] - m->v[3 * a2 + k];
+      rab[k] = m->x[3 * a1 + k] - m->x[3 * a2 + k];
+    }
+    error += fabs(rab[0] * vab[0] + rab[1] * vab[1] + rab[2] * vab[2]);
+  }
+  return error / lf->n_cons;
+}
+
+/* ModifierShake::doExecute (virial term left out, see above) */
+static void md_shake(orc_md *m) {
+  const orc_lf *lf = m->lf;
+  double error, dt = lf->timestep / ORC_TIMEFACTOR;
+  int iter = 0, i, k;
+  if (!lf->shake || lf->n_cons == 0) return;
+  error = shake_error(m);
+  while (error > lf->shake_eps) {
+    for (i = 0; i < lf->n_cons; i++) {
+      int a1 = lf->cons[2 * i], a2 = lf->cons[2 * i + 1];
+      double pab[3], rab[3], dp[3], pabsq, rabsq, diffsq, rpab, rM1, rM2, gab;
+      for (k = 0; k < 3; k++) pab[k] = m->x[3 * a1 + k] - m->x[3 * a2 + k];
+      pabsq = pab[0] * pab[0] + pab[1] * pab[1] + pab[2] * pab[2];
+      rabsq = lf->rest[i] * lf->rest[i];
+      diffsq = rabsq - pabsq;
+      for (k = 0; k < 3; k++) rab[k] = m->last[3 * a1 + k] - m->last[3 * a2 + k];
+      rpab = rab[0] * pab[0] + rab[1] * pab[1] + rab[2] * pab[2];
+      rM1 = 1 / lf->mass[a1];
+      rM2 = 1 / lf->mass[a2];
+      gab = diffsq / (2 * (rM1 + rM2) * rpab);
+      for (k = 0; k < 3; k++) {
+        dp[k] = rab[k] * gab;
+        m->x[3 * a1 + k] += dp[k] * rM1;
+        m->x[3 * a2 + k] -= dp[k] * rM2;
+        dp[k] /= dt;
+        m->v[3 * a1 + k] += dp[k] * rM1;
+        m->v[3 * a2 + k] -= dp[k] * rM2;
+      }
+    }
+    error = shake_error(m);
+    iter++;
+    if (iter > lf->shake_max_iter) break;
+  }
+  md_record(m, iter);
+  memcpy(m->last, m->x, sizeof(double) * 3 * m->s->n_atoms);
+}
+
+/* ModifierRattle::doExecute (virial term left out, see above) */
+static void md_rattle(orc_md *m) {
+  const orc_lf *lf = m->lf;
+  double error, dt = lf->timestep / ORC_TIMEFACTOR;
+  int iter = 0, i, k;
+  if (!lf->rattle || lf->n_cons == 0) return;
+  error = rattle_error(m);
+  while (error > lf->rattle_eps) {
+    for (i = 0; i < lf->n_cons; i++) {
+      int a1 = lf->cons[2 * i], a2 = lf->cons[2 * i + 1];
+      double rab[3], vab[3], dp[3], rabsq, rvab, rM1, rM2, gab;
+      rM1 = 1 / lf->mass[a1];
+      rM2 = 1 / lf->mass[a2];
+      for (k = 0; k < 3; k++) rab[k] = m->x[3 * a1 + k] - m->x[3 * a2 + k];
+      rabsq = rab[0] * rab[0] + rab[1] * rab[1] + rab[2] * rab[2];
+      for (k = 0; k < 3; k++) vab[k] = m->v[3 * a1 + k] - m->v[3 * a2 + k];
+      rvab = rab[0] * vab[0] + rab[1] * vab[1] + rab[2] * vab[2];
+      gab = -rvab / (dt * (rM1 + rM2) * rabsq);
+      for (k = 0; k < 3; k++) {
+        dp[k] = rab[k] * gab;
+        m->v[3 * a1 + k] += dp[k] * dt * rM1;
+        m->v[3 * a2 + k] -= dp[k] * dt * rM2;
+      }
+    }
+    error = rattle_error(m);
+    iter++;
+    if (iter > lf->rattle_max_iter) break;
+  }
+  md_record(m, iter);
+}
+
+/* v += F * w / m  (w = h/2 for a half kick, h for a full kick) */
+static void md_kick(orc_md *m, double w) {
+  int i, k;
+  for (i = 0; i < m->s->n_atoms; i++)
+    for (k = 0; k < 3; k++)
+      m->v[3 * i + k] += m->f[3 * i + k] * w / m->lf->mass[i];
+}
+
+/* x += v * h */
+static void md_drift(orc_md *m) {
+  int i;
+  double h = m->lf->timestep * ORC_INV_TIMEFACTOR;
+  for (i = 0; i < 3 * m->s->n_atoms; i++) m->x[i] += m->v[i] * h;
+}
+
+/* StandardIntegrator::calculateForces + STSIntegrator::calculateForces:
+   energies and forces cleared, then this force alone evaluated. */
+static int md_forces(orc_md *m) {
+  orc_system sys = *m->s;
+  memset(m->f, 0, sizeof(double) * 3 * sys.n_atoms);
+  m->energies2[0] = m->energies2[1] = 0;
+  sys.pos = m->x;
+  return orc_evaluate(&sys, m->f, m->energies2, NULL, NULL, NULL, 0, NULL);
+}
+
+/*
+ * LeapfrogIntegrator::run(n_steps).  x, v: positions and velocities (in
+ * and out); f: the forces at x on entry (initializeForces), at the final x
+ * on exit; last: SHAKE's previous positions (ModifierMetaRattleShake keeps
+ * them across calls; initialise them to x before the first call);
+ * energies2: LJ and electrostatic energy of the last force evaluation.
+ * iters (optional): the sweep count of every SHAKE and RATTLE call in call
+ * order.  Returns the number of modifier calls, or -1 on failure.
+ */
+int64_t orc_leapfrog_run(const orc_system *s, const orc_lf *lf, double *x,
+                         double *v, double *f, double *last, double *energies2,
+                         int32_t n_steps, int32_t *iters, int64_t iters_cap) {
+  orc_md m;
+  double h = lf->timestep * ORC_INV_TIMEFACTOR;
+  double h2 = 0.5 * lf->timestep * ORC_INV_TIMEFACTOR;
+  int i;
+  m.s = s; m.lf = lf; m.x = x; m.v = v; m.f = f; m.last = last;
+  m.energies2 = energies2; m.iters = iters; m.iters_cap = iters_cap;
+  m.n_calls = 0;
+  if (n_steps < 1) return 0;
+  /* doHalfKickdoDrift (no pre-drift modifier): v += F*h*0.5/m, x += v*h */
+  {
+    int a, k;
+    for (a = 0; a < s->n_atoms; a++)
+      for (k = 0; k < 3; k++)
+        v[3 * a + k] += f[3 * a + k] * h * 0.5 / lf->mass[a];
+  }
+  md_drift(&m);
+  md_shake(&m);
+  if (md_forces(&m)) return -1;
+  for (i = 1; i < n_steps; i++) {
+    if (lf->rattle) {   /* doKickdoDrift with a post-step modifier */
+      md_kick(&m, h2);
+      md_rattle(&m);
+      md_kick(&m, h2);
+    } else {
+      md_kick(&m, h);
+    }
+    md_drift(&m);
+    md_shake(&m);
+    if (md_forces(&m)) return -1;
+  }
+  md_kick(&m, h2);
+  md_rattle(&m);
+  return m.n_calls;
+}

@@ -37,6 +37,8 @@ EXPORTED = (
     "pmnb_batch_autotune", "pmnb_batch_set_shape", "pmnb_batch_pair_count",
     "pmnb_batch_last_error", "pmnb_batch_destroy",
     "pmnb_set_parameters_device", "pmnb_adjoint_device",
+    "pmnb_lf_create", "pmnb_lf_run_device", "pmnb_lf_iterations",
+    "pmnb_lf_destroy",
 )
 
 
@@ -53,6 +55,18 @@ class PmnbParams(ctypes.Structure):
         ("algorithm_cutoff", ctypes.c_double),
         ("cell_size", ctypes.c_double),
         ("ewald_alpha", ctypes.c_double),
+    ]
+
+
+class PmnbLfParams(ctypes.Structure):
+    _fields_ = [
+        ("timestep", ctypes.c_double),
+        ("shake", ctypes.c_int32),
+        ("rattle", ctypes.c_int32),
+        ("shake_epsilon", ctypes.c_double),
+        ("shake_max_iter", ctypes.c_int32),
+        ("rattle_max_iter", ctypes.c_int32),
+        ("rattle_epsilon", ctypes.c_double),
     ]
 
 
@@ -135,6 +149,17 @@ def load():
     lib.pmnb_set_parameters_device.restype = ctypes.c_int
     lib.pmnb_adjoint_device.argtypes = [_p, _p, _p, _p, _p, _p, _p]
     lib.pmnb_adjoint_device.restype = ctypes.c_int
+    lib.pmnb_lf_create.argtypes = [_p, ctypes.POINTER(PmnbLfParams), _i32, _p,
+                                   _i32, _p, _p, ctypes.POINTER(_p)]
+    lib.pmnb_lf_create.restype = ctypes.c_int
+    lib.pmnb_lf_run_device.argtypes = [_p, _p, _p, _p, _p, _i32, _u32, _p]
+    lib.pmnb_lf_run_device.restype = ctypes.c_int
+    lib.pmnb_lf_iterations.argtypes = [_p, _i64, _p, _i64,
+                                       ctypes.POINTER(_i64),
+                                       ctypes.POINTER(_i64)]
+    lib.pmnb_lf_iterations.restype = ctypes.c_int
+    lib.pmnb_lf_destroy.argtypes = [_p]
+    lib.pmnb_lf_destroy.restype = None
     _LIB = lib
     return lib
 

@@ -24,6 +24,7 @@
 #include "pmnb_queue.h"
 #include "pmnb_adjoint.h"
 #include "pmnb_tilemask.h"
+#include "pmnb_leapfrog.h"
 
 using namespace pmnb;
 
@@ -1679,6 +1680,390 @@ void pmnb_batch_destroy(pmnb_batch *b) {
   if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
   pmnb_destroy(b->ctx);
   delete b;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ leapfrog
+struct pmnb_leapfrog {
+  pmnb_ctx *ctx = nullptr;
+  int device = 0;
+  pmnb_lf_params p{};
+  bool shake = false, rattle = false;  // attached and effective
+  int n = 0, ng = 0, nf = 0, ngb = 0, nfb = 0, n_cons = 0, shape = 0;
+  DevBuf mass, rmass, g_atoms, g_cons, g_rest, g_rinv, free_atoms, last, xs,
+      vs, partial_s, partial_r, iters, nonconv;
+  bool last_ready = false;
+  long long calls = 0, iters_cap = 0;
+};
+
+namespace {
+
+constexpr double kTimeFactor = 48.88821290839616;      // base/PMConstants.cpp:50
+constexpr double kInvTimeFactor = 0.02045482828087295; // base/PMConstants.cpp:51
+
+LfDev lf_dev(pmnb_leapfrog *lf, double *x, double *v, double *f,
+             double *out) {
+  LfDev d{};
+  d.x = x;
+  d.v = v;
+  d.f = f;
+  d.out = out;
+  d.last = lf->last.as<double>();
+  d.mass = lf->mass.as<double>();
+  d.rmass = lf->rmass.as<double>();
+  d.g_atoms = lf->g_atoms.as<int>();
+  d.g_cons = lf->g_cons.as<int>();
+  d.g_rest = lf->g_rest.as<double>();
+  d.g_rinv = lf->g_rinv.as<double>();
+  d.free_atoms = lf->free_atoms.as<int>();
+  d.n_groups = lf->ng;
+  d.n_free = lf->nf;
+  d.ngb = lf->ngb;
+  d.n_cons = lf->n_cons;
+  // LeapfrogIntegrator / StandardIntegrator use timestep * INV_TIMEFACTOR,
+  // the modifiers timestep / TIMEFACTOR.
+  d.h = lf->p.timestep * kInvTimeFactor;
+  d.h2 = 0.5 * lf->p.timestep * kInvTimeFactor;
+  d.dtc = lf->p.timestep / kTimeFactor;
+  d.eps[kStShake] = lf->p.shake_epsilon;
+  d.eps[kStRattle] = lf->p.rattle_epsilon;
+  d.maxit[kStShake] = lf->p.shake_max_iter;
+  d.maxit[kStRattle] = lf->p.rattle_max_iter;
+  d.partial[kStShake] = lf->partial_s.as<double>();
+  d.partial[kStRattle] = lf->partial_r.as<double>();
+  d.xs = lf->xs.as<double>();
+  d.vs = lf->vs.as<double>();
+  d.iters = lf->iters_cap > 0 ? lf->iters.as<int>() : nullptr;
+  d.iters_cap = lf->iters_cap;
+  d.nonconv = lf->nonconv.as<int>();
+  return d;
+}
+
+struct LfKernel {
+  int apply = kStNone, pre = kPreNone, probe = kStNone;
+  bool zero = false;
+  bool empty() const {
+    return apply == kStNone && pre == kPreNone && probe == kStNone && !zero;
+  }
+};
+
+#define PMNB_LF_CASE(A, P, Q, Z)                                            \
+  if (k.apply == A && k.pre == P && k.probe == Q && k.zero == Z) {          \
+    hipLaunchKernelGGL((k_lf_step<A, P, Q, Z, NA, NC>), dim3(grid),          \
+                       dim3(kLfBlock), 0, s, d, call);                       \
+    return hipGetLastError();                                                \
+  }
+
+template <int NA, int NC>
+hipError_t lf_launch_step(const LfKernel &k, const LfDev &d, int grid,
+                          long long call, hipStream_t s) {
+  PMNB_LF_CASE(kStNone, kPreFirst, kStShake, false)
+  PMNB_LF_CASE(kStShake, kPreNone, kStNone, true)
+  PMNB_LF_CASE(kStNone, kPreFirst, kStNone, true)
+  PMNB_LF_CASE(kStNone, kPreHalf, kStRattle, false)
+  PMNB_LF_CASE(kStRattle, kPreHalfDrift, kStShake, false)
+  PMNB_LF_CASE(kStRattle, kPreHalfDrift, kStNone, true)
+  PMNB_LF_CASE(kStNone, kPreHalf, kStNone, false)
+  PMNB_LF_CASE(kStNone, kPreHalfDrift, kStNone, true)
+  PMNB_LF_CASE(kStNone, kPreFullDrift, kStShake, false)
+  PMNB_LF_CASE(kStNone, kPreFullDrift, kStNone, true)
+  PMNB_LF_CASE(kStRattle, kPreNone, kStNone, false)
+  return hipErrorInvalidValue;  // not a sequence run() produces
+}
+#undef PMNB_LF_CASE
+
+template <int NA, int NC>
+hipError_t lf_launch_probe(int stage, const LfDev &d, int p, hipStream_t s) {
+  if (stage == kStShake)
+    hipLaunchKernelGGL((k_lf_probe<kStShake, NA, NC>), dim3(d.ngb),
+                       dim3(kLfBlock), 0, s, d, p);
+  else
+    hipLaunchKernelGGL((k_lf_probe<kStRattle, NA, NC>), dim3(d.ngb),
+                       dim3(kLfBlock), 0, s, d, p);
+  return hipGetLastError();
+}
+
+// Launches one fused kernel, then the further probe windows of the stage it
+// probes.
+int lf_flush(pmnb_leapfrog *lf, LfKernel &k, const LfDev &d, hipStream_t s) {
+  pmnb_ctx *ctx = lf->ctx;
+  if (k.empty()) return PMNB_OK;
+  const long long call = k.apply != kStNone ? lf->calls++ : -1;
+  const int grid = lf->ngb + lf->nfb;
+  if (grid > 0) {
+    hipError_t e = lf->shape == 0 ? lf_launch_step<3, 3>(k, d, grid, call, s)
+                                  : lf_launch_step<4, 6>(k, d, grid, call, s);
+    if (e != hipSuccess)
+      return fail(ctx, PMNB_ERR_HIP, "leapfrog step launch failed: %s",
+                  hipGetErrorString(e));
+    if (k.probe != kStNone && lf->ngb > 0) {
+      const int nwin = (d.maxit[k.probe] + kLfWindow) / kLfWindow;
+      for (int p = 1; p < nwin; p++) {
+        e = lf->shape == 0 ? lf_launch_probe<3, 3>(k.probe, d, p, s)
+                           : lf_launch_probe<4, 6>(k.probe, d, p, s);
+        if (e != hipSuccess)
+          return fail(ctx, PMNB_ERR_HIP, "leapfrog probe launch failed: %s",
+                      hipGetErrorString(e));
+      }
+    }
+  }
+  k = LfKernel{};
+  return PMNB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pmnb_lf_create(pmnb_ctx *ctx, const pmnb_lf_params *params,
+                   int32_t n_atoms, const double *mass, int32_t n_constraints,
+                   const int32_t *constraint_atoms, const double *rest_length,
+                   pmnb_leapfrog **out) {
+  if (!ctx || !params || !out) return PMNB_ERR_ARG;
+  *out = nullptr;
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (n_atoms != ctx->n)
+    return fail(ctx, PMNB_ERR_ARG, "n_atoms %d differs from the topology's %d",
+                n_atoms, ctx->n);
+  if (n_atoms > 0 && !mass) return fail(ctx, PMNB_ERR_ARG, "null mass");
+  if (n_constraints < 0 ||
+      (n_constraints > 0 && (!constraint_atoms || !rest_length)))
+    return fail(ctx, PMNB_ERR_ARG, "bad constraint arrays");
+  const pmnb_lf_params &p = *params;
+  if (!(p.timestep > 0))
+    return fail(ctx, PMNB_ERR_ARG, "timestep must be positive");
+  for (int i = 0; i < n_atoms; i++)
+    if (!(mass[i] > 0))
+      return fail(ctx, PMNB_ERR_ARG, "mass of atom %d is not positive", i);
+  // union-find over constraint atoms
+  std::vector<int> parent(n_atoms);
+  for (int i = 0; i < n_atoms; i++) parent[i] = i;
+  auto root = [&](int a) {
+    while (parent[a] != a) a = parent[a] = parent[parent[a]];
+    return a;
+  };
+  for (int c = 0; c < n_constraints; c++) {
+    const int a = constraint_atoms[2 * c], b = constraint_atoms[2 * c + 1];
+    if (a < 0 || b < 0 || a >= n_atoms || b >= n_atoms || a == b)
+      return fail(ctx, PMNB_ERR_ARG, "constraint %d has atoms (%d, %d)", c, a, b);
+    if (!(rest_length[c] > 0))
+      return fail(ctx, PMNB_ERR_ARG, "constraint %d has rest length <= 0", c);
+    parent[root(a)] = root(b);
+  }
+  std::vector<char> constrained(n_atoms, 0);
+  for (int c = 0; c < 2 * n_constraints; c++) constrained[constraint_atoms[c]] = 1;
+  // groups in order of their lowest atom; atoms ascending within a group;
+  // constraints in list order
+  std::vector<int> gid(n_atoms, -1), first_of_root(n_atoms, -1);
+  std::vector<std::vector<int>> gat;
+  for (int i = 0; i < n_atoms; i++) {
+    if (!constrained[i]) continue;
+    const int r = root(i);
+    if (first_of_root[r] < 0) {
+      first_of_root[r] = (int)gat.size();
+      gat.emplace_back();
+    }
+    gid[i] = first_of_root[r];
+    gat[gid[i]].push_back(i);
+  }
+  const int ng = (int)gat.size();
+  std::vector<std::vector<int>> gcon(ng);
+  for (int c = 0; c < n_constraints; c++)
+    gcon[gid[constraint_atoms[2 * c]]].push_back(c);
+  int max_a = 0, max_c = 0;
+  for (int g = 0; g < ng; g++) {
+    max_a = std::max(max_a, (int)gat[g].size());
+    max_c = std::max(max_c, (int)gcon[g].size());
+  }
+  if (max_a > kLfMaxAtoms || max_c > kLfMaxCons)
+    return fail(ctx, PMNB_ERR_UNSUPPORTED,
+                "a connected set of constraints spans %d atoms and %d "
+                "constraints (at most %d and %d supported)",
+                max_a, max_c, kLfMaxAtoms, kLfMaxCons);
+  const int shape = (max_a <= 3 && max_c <= 3) ? 0 : 1;
+  const int NA = shape == 0 ? 3 : 4, NC = shape == 0 ? 3 : 6;
+  std::vector<int> h_atoms((size_t)std::max(ng, 1) * NA, -1);
+  std::vector<int> h_cons((size_t)std::max(ng, 1) * NC, -1);
+  std::vector<double> h_rest((size_t)std::max(ng, 1) * NC, 1.0);
+  std::vector<double> h_rinv((size_t)std::max(ng, 1) * NC, 0.0);
+  for (int g = 0; g < ng; g++) {
+    for (size_t k = 0; k < gat[g].size(); k++) h_atoms[(size_t)g * NA + k] = gat[g][k];
+    for (size_t k = 0; k < gcon[g].size(); k++) {
+      const int c = gcon[g][k];
+      auto local = [&](int atom) {
+        return (int)(std::find(gat[g].begin(), gat[g].end(), atom) - gat[g].begin());
+      };
+      h_cons[(size_t)g * NC + k] =
+          local(constraint_atoms[2 * c]) | (local(constraint_atoms[2 * c + 1]) << 4);
+      const double r = rest_length[c];
+      h_rest[(size_t)g * NC + k] = r;
+      h_rinv[(size_t)g * NC + k] = 1.0 / (2.0 * (r * r));
+    }
+  }
+  std::vector<int> h_free;
+  for (int i = 0; i < n_atoms; i++)
+    if (!constrained[i]) h_free.push_back(i);
+  std::vector<double> h_rmass(n_atoms);
+  for (int i = 0; i < n_atoms; i++) h_rmass[i] = 1 / mass[i];
+
+  pmnb_leapfrog *lf = new pmnb_leapfrog;
+  lf->ctx = ctx;
+  lf->device = ctx->prm.device;
+  lf->p = p;
+  // module/ModifierModule.cpp:110-125: a modifier is attached only with a
+  // positive epsilon and max_iter
+  lf->shake = p.shake && p.shake_epsilon > 0.0 && p.shake_max_iter > 0;
+  lf->rattle = p.rattle && p.rattle_epsilon > 0.0 && p.rattle_max_iter > 0;
+  lf->n = n_atoms;
+  lf->ng = ng;
+  lf->nf = (int)h_free.size();
+  lf->ngb = blocks_for(ng, kLfBlock);
+  lf->nfb = blocks_for(lf->nf, kLfBlock);
+  lf->n_cons = n_constraints;
+  lf->shape = shape;
+  auto up = [&](DevBuf &b, const void *src, size_t bytes) -> hipError_t {
+    hipError_t e = b.ensure(bytes);
+    if (e == hipSuccess && bytes && src)
+      e = hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice);
+    return e;
+  };
+  const size_t N = (size_t)std::max(n_atoms, 1);
+  const size_t ngb = (size_t)std::max(lf->ngb, 1);
+  hipError_t e = hipSetDevice(ctx->prm.device);
+  if (e == hipSuccess) e = up(lf->mass, mass, n_atoms * sizeof(double));
+  if (e == hipSuccess) e = up(lf->rmass, h_rmass.data(), n_atoms * sizeof(double));
+  if (e == hipSuccess) e = up(lf->g_atoms, h_atoms.data(), h_atoms.size() * sizeof(int));
+  if (e == hipSuccess) e = up(lf->g_cons, h_cons.data(), h_cons.size() * sizeof(int));
+  if (e == hipSuccess) e = up(lf->g_rest, h_rest.data(), h_rest.size() * sizeof(double));
+  if (e == hipSuccess) e = up(lf->g_rinv, h_rinv.data(), h_rinv.size() * sizeof(double));
+  if (e == hipSuccess)
+    e = up(lf->free_atoms, h_free.data(), h_free.size() * sizeof(int));
+  if (e == hipSuccess) e = lf->last.ensure(N * 3 * sizeof(double));
+  if (e == hipSuccess) e = lf->xs.ensure(N * 3 * sizeof(double));
+  if (e == hipSuccess) e = lf->vs.ensure(N * 3 * sizeof(double));
+  if (e == hipSuccess)
+    e = lf->partial_s.ensure((size_t)(std::max(p.shake_max_iter, 0) + 1) * ngb *
+                             sizeof(double));
+  if (e == hipSuccess)
+    e = lf->partial_r.ensure((size_t)(std::max(p.rattle_max_iter, 0) + 1) * ngb *
+                             sizeof(double));
+  if (e == hipSuccess) e = lf->nonconv.ensure(4 * sizeof(int));
+  if (e == hipSuccess) e = hipMemset(lf->nonconv.p, 0, 4 * sizeof(int));
+  if (e != hipSuccess) {
+    pmnb_lf_destroy(lf);
+    return fail(ctx, PMNB_ERR_HIP, "leapfrog set-up failed: %s",
+                hipGetErrorString(e));
+  }
+  *out = lf;
+  return PMNB_OK;
+}
+
+int pmnb_lf_run_device(pmnb_leapfrog *lf, double *d_positions,
+                       double *d_velocities, double *d_forces, double *d_out,
+                       int32_t n_steps, uint32_t flags, void *stream) {
+  if (!lf) return PMNB_ERR_ARG;
+  pmnb_ctx *ctx = lf->ctx;
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (ctx->n != lf->n)
+    return fail(ctx, PMNB_ERR_STATE, "topology changed since pmnb_lf_create");
+  if (flags & PMNB_FLAG_MOLVIRIAL)
+    return fail(ctx, PMNB_ERR_UNSUPPORTED,
+                "the molecular virial needs molecule centres per step");
+  if (n_steps < 1) return PMNB_OK;  // LeapfrogIntegrator::run returns
+  if (lf->n > 0 && (!d_positions || !d_velocities || !d_forces || !d_out))
+    return fail(ctx, PMNB_ERR_ARG, "null device pointer");
+  HIPCHK(hipSetDevice(ctx->prm.device));
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  if (!lf->last_ready) {
+    if (lf->n > 0)
+      HIPCHK(hipMemcpyAsync(lf->last.p, d_positions, (size_t)lf->n * 3 * sizeof(double),
+                            hipMemcpyDeviceToDevice, s));
+    lf->last_ready = true;
+  }
+  if (lf->n == 0) return PMNB_OK;
+  const LfDev d = lf_dev(lf, d_positions, d_velocities, d_forces, d_out);
+  const bool S = lf->shake && lf->n_cons > 0;
+  const bool R = lf->rattle && lf->n_cons > 0;
+  LfKernel k;
+  auto pre = [&](int op) -> int {
+    if (k.pre != kPreNone || k.probe != kStNone || k.zero) {
+      int r = lf_flush(lf, k, d, s);
+      if (r) return r;
+    }
+    k.pre = op;
+    return PMNB_OK;
+  };
+  auto stage = [&](int st) -> int {
+    k.probe = st;
+    int r = lf_flush(lf, k, d, s);
+    k.apply = st;
+    return r;
+  };
+  auto force = [&]() -> int {
+    k.zero = true;
+    int r = lf_flush(lf, k, d, s);
+    if (r) return r;
+    return run_device(ctx, d_positions, d_forces, d_out, nullptr, flags, s);
+  };
+  // doHalfKickdoDrift (LeapfrogIntegrator.cpp:33-58), SHAKE post-drift
+  if ((rc = pre(kPreFirst))) return rc;
+  if (S && (rc = stage(kStShake))) return rc;
+  if ((rc = force())) return rc;
+  for (int i = 1; i < n_steps; i++) {
+    // doKickdoDrift (LeapfrogIntegrator.cpp:60-83)
+    if (lf->rattle) {
+      if ((rc = pre(kPreHalf))) return rc;
+      if (R && (rc = stage(kStRattle))) return rc;
+      if ((rc = pre(kPreHalfDrift))) return rc;
+    } else {
+      if ((rc = pre(kPreFullDrift))) return rc;
+    }
+    if (S && (rc = stage(kStShake))) return rc;
+    if ((rc = force())) return rc;
+  }
+  // doHalfKick and postStepModify (LeapfrogIntegrator.cpp:96-97)
+  if ((rc = pre(kPreHalf))) return rc;
+  if (R && (rc = stage(kStRattle))) return rc;
+  return lf_flush(lf, k, d, s);
+}
+
+int pmnb_lf_iterations(pmnb_leapfrog *lf, int64_t capacity, int32_t *iters,
+                       int64_t capacity_out, int64_t *n_calls,
+                       int64_t *n_unconverged) {
+  if (!lf) return PMNB_ERR_ARG;
+  pmnb_ctx *ctx = lf->ctx;
+  HIPCHK(hipSetDevice(ctx->prm.device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipDeviceSynchronize());
+  if (n_calls) *n_calls = lf->calls;
+  const long long have = std::min<long long>(lf->calls, lf->iters_cap);
+  const long long n_out = std::min<long long>(have, std::max<int64_t>(capacity_out, 0));
+  if (iters && n_out > 0)
+    HIPCHK(hipMemcpy(iters, lf->iters.p, n_out * sizeof(int), hipMemcpyDeviceToHost));
+  int nc[4] = {0, 0, 0, 0};
+  HIPCHK(hipMemcpy(nc, lf->nonconv.p, sizeof nc, hipMemcpyDeviceToHost));
+  if (n_unconverged) *n_unconverged = (int64_t)nc[kStShake] + nc[kStRattle];
+  if (capacity > 0) HIPCHK(lf->iters.ensure((size_t)capacity * sizeof(int)));
+  lf->iters_cap = capacity > 0 ? capacity : 0;
+  lf->calls = 0;
+  HIPCHK(hipMemset(lf->nonconv.p, 0, 4 * sizeof(int)));
+  return PMNB_OK;
+}
+
+void pmnb_lf_destroy(pmnb_leapfrog *lf) {
+  if (!lf) return;
+  // the context may already be destroyed: only the device is used here
+  (void)hipSetDevice(lf->device);
+  (void)hipDeviceSynchronize();
+  DevBuf *bufs[] = {&lf->mass, &lf->rmass, &lf->g_atoms, &lf->g_cons,
+                    &lf->g_rest, &lf->g_rinv, &lf->free_atoms, &lf->last,
+                    &lf->xs, &lf->vs, &lf->partial_s, &lf->partial_r,
+                    &lf->iters, &lf->nonconv};
+  for (DevBuf *b : bufs) b->release();
+  delete lf;
 }
 
 }  // extern "C"

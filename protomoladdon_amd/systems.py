@@ -158,8 +158,16 @@ def water_topology(n_mol, L, cell_size):
     excl = build_exclusion_table(n, EXCLUDE_SCALED14, bonds, angles, [])
     lj = lorentz_berthelot_table([WATER_SIGMA_O, 0.4], [WATER_EPS_O, 0.0])
     bc = PeriodicBoundaryConditions.cubic(L)
-    return Topology(types, q, mol, lj, excl, bc, CubicCellManager(cell_size),
+    topo = Topology(types, q, mol, lj, excl, bc, CubicCellManager(cell_size),
                     coulomb_scaling_factor=1.0, masses=masses)
+    # TIP3 atom names and the bond / angle terms with their rest values, as
+    # the SHAKE / RATTLE constraint list reads them
+    topo.atom_names = np.tile(np.array(["OH2", "H1", "H2"]), n_mol)
+    topo.bonds = bonds
+    topo.bond_rest = np.full(len(bonds), WATER_R_OH)
+    topo.angles = angles
+    topo.angle_rest = np.full(n_mol, WATER_ANGLE_HOH * math.pi / 180.0)
+    return topo
 
 
 def water_workload(name, n_mol, L, seed, rc, rs, elec, alpha=0.0,

@@ -1,0 +1,199 @@
+"""Parity of the device LeapfrogIntegrator / SHAKE / RATTLE (through the C
+ABI, pmnb_lf_*) with the oracle's restatement of LeapfrogIntegrator::run.
+
+Bars: FP64 positions and velocities after 10 steps within 1e-9 of the
+largest oracle value (the force sums differ in order only; every integrator
+operation is the reference's), forces within 1e-9 relative, energies within
+1e-9 relative, and the sweep count of every SHAKE and RATTLE call equal to
+the oracle's.  Mixed precision: positions within 1e-6 A RMS of the FP64
+device run after 10 steps.  Full-size water (96,000 atoms): constraints
+met to epsilon after every run, no call at max_iter."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from protomoladdon_amd import systems
+from protomoladdon_amd.integrator import (LeapfrogIntegrator,
+                                          kinetic_energy, topology_constraints)
+
+pytestmark = pytest.mark.gpu
+
+KB = 0.001987191
+DEFAULT = (1e-5, 30)
+
+
+def maxwell(masses, seed, temperature=300.0):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((len(masses), 3)) * np.sqrt(
+        KB * temperature / np.asarray(masses))[:, None]
+
+
+class Device:
+    def __init__(self, w, precision, constraints, shake, rattle, v0):
+        import torch
+        self.torch = torch
+        self.w = w
+        self.constraints = constraints
+        self.force = w.make_force(precision)
+        self.lf = LeapfrogIntegrator(1.0, self.force, shake=shake, rattle=rattle)
+        dev = torch.device("cuda:0")
+        self.x = torch.tensor(w.positions, dtype=torch.float64, device=dev)
+        self.v = torch.tensor(v0, dtype=torch.float64, device=dev)
+        self.f = torch.zeros_like(self.x)
+        self.out = torch.zeros(20, dtype=torch.float64, device=dev)
+        self.lf.initialize(w.topology, self.x, self.f, self.out, constraints)
+        self.lf.record_iterations(1 << 16)
+
+    def run(self, n):
+        self.lf.run(self.w.topology, self.x, self.v, self.f, self.out, n,
+                    self.constraints)
+        self.torch.cuda.synchronize()
+
+    def iterations(self):
+        its, n, unc = self.lf.record_iterations(1 << 16)
+        return its[:n], unc
+
+    def host(self):
+        return (self.x.cpu().numpy(), self.v.cpu().numpy(),
+                self.f.cpu().numpy(), self.out.cpu().numpy())
+
+    def close(self):
+        self.lf.close()
+        self.force.close()
+
+
+def oracle_state(w, v0):
+    s = O.System.from_workload(w)
+    r0 = O.evaluate(s)
+    return s, O.MDState(w.positions, v0, r0.forces, energies=r0.energies)
+
+
+def compare(dev, st, its_dev, its_ref):
+    x, v, f, out = dev.host()
+    assert np.abs(x - st.x).max() <= 1e-9 * max(1.0, np.abs(st.x).max())
+    assert np.abs(v - st.v).max() <= 1e-9 * np.abs(st.v).max()
+    assert np.abs(f - st.f).max() <= 1e-9 * np.abs(st.f).max()
+    np.testing.assert_allclose(out[:2], st.energies, rtol=1e-9, atol=1e-9)
+    assert list(its_dev) == list(its_ref)
+
+
+def water_case(elec="coulomb", n_mol=300):
+    w = systems.small_water(n_mol=n_mol, elec=elec)
+    return w, topology_constraints(w.topology)
+
+
+@pytest.mark.parametrize("elec,shake,rattle", [
+    ("coulomb", DEFAULT, DEFAULT),
+    ("ewald", DEFAULT, DEFAULT),
+    ("ewald", DEFAULT, None),
+    ("ewald", None, DEFAULT),
+    ("ewald", None, None),
+    ("coulomb", (1e-12, 30), (1e-12, 30)),
+    ("coulomb", (1e-30, 2), (1e-30, 4)),
+], ids=["shake_rattle_coulomb", "shake_rattle_ewald", "shake_only",
+        "rattle_only", "no_modifier", "tight_epsilon", "max_iter"])
+def test_water_trajectory_fp64(elec, shake, rattle):
+    w, cons = water_case(elec)
+    t = w.topology
+    v0 = maxwell(t.masses, 7)
+    print("water300", elec, shake, rattle, flush=True)
+    dev = Device(w, "fp64", cons, shake, rattle, v0)
+    s, st = oracle_state(w, v0)
+    dev.run(10)
+    its_dev, unc = dev.iterations()
+    its_ref = O.leapfrog_run(s, st, 10, t.masses, cons[0], cons[1], 1.0,
+                             shake=shake, rattle=rattle)
+    compare(dev, st, its_dev, its_ref)
+    # stage of each call: SHAKE, then (RATTLE, SHAKE) per later step, RATTLE
+    seq = ["S"] if shake else []
+    for _ in range(1, 10):
+        seq += (["R"] if rattle else []) + (["S"] if shake else [])
+    seq += ["R"] if rattle else []
+    maxit = {"S": shake[1] if shake else 0, "R": rattle[1] if rattle else 0}
+    assert len(seq) == len(its_ref)
+    assert unc == sum(1 for k, st_ in zip(its_ref, seq) if k > maxit[st_])
+    dev.close()
+
+
+def test_two_runs_keep_shake_state():
+    w, cons = water_case("ewald")
+    t = w.topology
+    v0 = maxwell(t.masses, 8)
+    print("water300 two runs", flush=True)
+    dev = Device(w, "fp64", cons, DEFAULT, DEFAULT, v0)
+    s, st = oracle_state(w, v0)
+    dev.run(4)
+    dev.run(6)
+    its_dev, _ = dev.iterations()
+    its_ref = list(O.leapfrog_run(s, st, 4, t.masses, cons[0], cons[1], 1.0,
+                                  shake=DEFAULT, rattle=DEFAULT))
+    its_ref += list(O.leapfrog_run(s, st, 6, t.masses, cons[0], cons[1], 1.0,
+                                   shake=DEFAULT, rattle=DEFAULT))
+    compare(dev, st, its_dev, its_ref)
+    dev.close()
+
+
+def test_four_atom_groups_and_free_atoms():
+    """Groups of 4 atoms and 4 constraints (a water plus the O of the next
+    water, at its current distance) with the other atoms unconstrained:
+    the general group shape and the free-atom threads."""
+    w = systems.small_water(n_mol=300, elec="ewald")
+    t = w.topology
+    pos = w.positions
+    pairs, rest = [], []
+    for m in range(0, 300, 2):
+        o, h1, h2, o2 = 3 * m, 3 * m + 1, 3 * m + 2, 3 * m + 3
+        for a, b in ((h1, h2), (h1, o), (h2, o), (o, o2)):
+            pairs.append((a, b))
+            rest.append(float(np.linalg.norm(pos[a] - pos[b])))
+    cons = (np.array(pairs, np.int32), np.array(rest))
+    v0 = maxwell(t.masses, 9)
+    print("water300 4-atom groups", flush=True)
+    dev = Device(w, "fp64", cons, DEFAULT, DEFAULT, v0)
+    s, st = oracle_state(w, v0)
+    dev.run(10)
+    its_dev, _ = dev.iterations()
+    its_ref = O.leapfrog_run(s, st, 10, t.masses, cons[0], cons[1], 1.0,
+                             shake=DEFAULT, rattle=DEFAULT)
+    compare(dev, st, its_dev, its_ref)
+    dev.close()
+
+
+def test_mixed_precision_trajectory():
+    w, cons = water_case("ewald")
+    v0 = maxwell(w.topology.masses, 10)
+    print("water300 mixed vs fp64", flush=True)
+    a = Device(w, "fp64", cons, DEFAULT, DEFAULT, v0)
+    b = Device(w, "mixed", cons, DEFAULT, DEFAULT, v0)
+    a.run(10)
+    b.run(10)
+    xa, xb = a.host()[0], b.host()[0]
+    rms = float(np.sqrt(np.mean(np.sum((xa - xb) ** 2, axis=1))))
+    print("rms dx", rms)
+    assert rms <= 1e-6
+    a.close()
+    b.close()
+
+
+def test_full_size_water_constraints():
+    w = systems.config1()
+    t = w.topology
+    pairs, rest = topology_constraints(t)
+    v0 = maxwell(t.masses, 12)
+    print("water96k mixed 3 x 10 steps", flush=True)
+    dev = Device(w, "mixed", (pairs, rest), DEFAULT, DEFAULT, v0)
+    for _ in range(3):
+        dev.run(10)
+        x, v, f, out = dev.host()
+        d = x[pairs[:, 0]] - x[pairs[:, 1]]
+        serr = np.mean(np.abs((d * d).sum(1) - rest ** 2) / (2 * rest ** 2))
+        vab = v[pairs[:, 0]] - v[pairs[:, 1]]
+        rerr = np.mean(np.abs((d * vab).sum(1)))
+        its, unc = dev.iterations()
+        print("shake err", serr, "rattle err", rerr, "sweeps", its.min(),
+              its.max(), "E", out[:2], "KE", kinetic_energy(t.masses, v))
+        assert serr <= 1e-5 * (1 + 1e-9)
+        assert rerr <= 1e-5 * (1 + 1e-9)
+        assert unc == 0 and len(its) == 20
+        assert np.isfinite(out[:2]).all()
+    dev.close()
