@@ -67,6 +67,45 @@ def time_gpu(w, precision, steps, warmup, flags, torch):
             "energies": out[:2].cpu().numpy()}
 
 
+def time_md(w, precision, steps, warmup, flags, torch):
+    """LeapfrogIntegrator::run on the device: 1 fs steps, SHAKE and RATTLE
+    at their defaults (epsilon 1e-5, 30 iterations), the cutoff force as the
+    force group; velocities Maxwell-Boltzmann at 300 K (seeded)."""
+    from protomoladdon_amd.integrator import (LeapfrogIntegrator,
+                                              topology_constraints)
+    t = w.topology
+    cons = topology_constraints(t)
+    rng = np.random.default_rng(1234)
+    v0 = rng.standard_normal((w.n_atoms, 3)) * np.sqrt(
+        0.001987191 * 300.0 / t.masses)[:, None]
+    force = w.make_force(precision)
+    lf = LeapfrogIntegrator(DT_FS, force, shake=(1e-5, 30), rattle=(1e-5, 30))
+    dev = torch.device("cuda:0")
+    x = torch.from_numpy(np.ascontiguousarray(w.positions)).to(dev)
+    v = torch.from_numpy(v0).to(dev)
+    f = torch.zeros_like(x)
+    out = torch.zeros(20, dtype=torch.float64, device=dev)
+    sp = torch.cuda.current_stream(dev).cuda_stream
+    lf.initialize(t, x, f, out, cons, flags=flags, stream=sp)
+    lf.run(t, x, v, f, out, warmup, cons, flags=flags, stream=sp)
+    torch.cuda.synchronize()
+    lf.record_iterations(1 << 16)
+    t0 = time.perf_counter()
+    lf.run(t, x, v, f, out, steps, cons, flags=flags, stream=sp)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    its, n_calls, unconverged = lf.record_iterations(0)
+    # call order: SHAKE, then (RATTLE, SHAKE) per later step, then RATTLE
+    shake_its, rattle_its = its[0::2], its[1::2]
+    lf.close()
+    force.close()
+    return {"value": ns_per_day(steps / dt), "unit": "ns/day",
+            "ms_per_step": 1e3 * dt / steps, "steps": steps,
+            "mean_shake_sweeps": float(shake_its.mean()),
+            "mean_rattle_sweeps": float(rattle_its.mean()),
+            "unconverged_calls": int(unconverged)}
+
+
 def cpu_baseline(w, budget_s):
     from oracle import oracle as O
     O.build()
@@ -318,6 +357,11 @@ def main():
     f64 = time_gpu(w, "fp64", fp64_steps, max(2, args.warmup // 4), flags,
                    torch)
 
+    print("dynamics ...", file=sys.stderr, flush=True)
+    md_mx = time_md(w, "mixed", args.steps, max(2, args.warmup // 4), flags,
+                    torch)
+    md_64 = time_md(w, "fp64", fp64_steps, 2, flags, torch)
+
     # the host-array entry point (pmnb_evaluate: positions in, forces out
     # through PCIe, synchronous), for the record; never the headline value
     from protomoladdon_amd.topology import ScalarStructure
@@ -383,6 +427,9 @@ def main():
                      "pair_kernel_tflops": achieved64,
                      "pairs_per_step": f64["pairs"]},
         "speedup_vs_fp64": sps / sps64,
+        "md": {"integrator": "Leapfrog 1 fs + SHAKE/RATTLE (1e-5, 30), "
+                             "whole step on the device, virial on",
+               "mixed": md_mx, "fp64": md_64},
         "host_api_ms_per_step": host_ms,
         "rms_rel_force_err_vs_fp64": rms,
         "roofline": {"bound": "valu", "achieved": achieved,

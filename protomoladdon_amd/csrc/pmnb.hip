@@ -1692,9 +1692,10 @@ struct pmnb_leapfrog {
   bool shake = false, rattle = false;  // attached and effective
   int n = 0, ng = 0, nf = 0, ngb = 0, nfb = 0, n_cons = 0, shape = 0;
   DevBuf mass, rmass, g_atoms, g_cons, g_rest, g_rinv, free_atoms, last, xs,
-      vs, partial_s, partial_r, iters, nonconv;
+      vs, partial_s, partial_r, iters, nonconv, hint;
   bool last_ready = false;
   long long calls = 0, iters_cap = 0;
+  long long scalls[3] = {0, 0, 0};  // calls per stage
 };
 
 namespace {
@@ -1732,6 +1733,7 @@ LfDev lf_dev(pmnb_leapfrog *lf, double *x, double *v, double *f,
   d.maxit[kStRattle] = lf->p.rattle_max_iter;
   d.partial[kStShake] = lf->partial_s.as<double>();
   d.partial[kStRattle] = lf->partial_r.as<double>();
+  d.hint = lf->hint.as<int>();
   d.xs = lf->xs.as<double>();
   d.vs = lf->vs.as<double>();
   d.iters = lf->iters_cap > 0 ? lf->iters.as<int>() : nullptr;
@@ -1750,13 +1752,14 @@ struct LfKernel {
 
 #define PMNB_LF_CASE(A, P, Q, Z)                                            \
   if (k.apply == A && k.pre == P && k.probe == Q && k.zero == Z) {          \
-    hipLaunchKernelGGL((k_lf_step<A, P, Q, Z, NA, NC>), dim3(grid),          \
-                       dim3(kLfBlock), 0, s, d, call);                       \
+    hipLaunchKernelGGL((k_lf_step<A, P, Q, Z, SHAPE>), dim3(grid),           \
+                       dim3(kLfBlock), 0, s, d, sc_apply, sc_probe, call);   \
     return hipGetLastError();                                                \
   }
 
-template <int NA, int NC>
+template <int SHAPE>
 hipError_t lf_launch_step(const LfKernel &k, const LfDev &d, int grid,
+                          long long sc_apply, long long sc_probe,
                           long long call, hipStream_t s) {
   PMNB_LF_CASE(kStNone, kPreFirst, kStShake, false)
   PMNB_LF_CASE(kStShake, kPreNone, kStNone, true)
@@ -1773,41 +1776,47 @@ hipError_t lf_launch_step(const LfKernel &k, const LfDev &d, int grid,
 }
 #undef PMNB_LF_CASE
 
-template <int NA, int NC>
-hipError_t lf_launch_probe(int stage, const LfDev &d, int p, hipStream_t s) {
+template <int SHAPE>
+hipError_t lf_launch_probe(int stage, const LfDev &d, long long scall,
+                           hipStream_t s) {
   if (stage == kStShake)
-    hipLaunchKernelGGL((k_lf_probe<kStShake, NA, NC>), dim3(d.ngb),
-                       dim3(kLfBlock), 0, s, d, p);
+    hipLaunchKernelGGL((k_lf_probe<kStShake, SHAPE>), dim3(d.ngb),
+                       dim3(kLfBlock), 0, s, d, scall);
   else
-    hipLaunchKernelGGL((k_lf_probe<kStRattle, NA, NC>), dim3(d.ngb),
-                       dim3(kLfBlock), 0, s, d, p);
+    hipLaunchKernelGGL((k_lf_probe<kStRattle, SHAPE>), dim3(d.ngb),
+                       dim3(kLfBlock), 0, s, d, scall);
   return hipGetLastError();
 }
 
-// Launches one fused kernel, then the further probe windows of the stage it
+// Launches one fused kernel, then the continuation probe of the stage it
 // probes.
 int lf_flush(pmnb_leapfrog *lf, LfKernel &k, const LfDev &d, hipStream_t s) {
   pmnb_ctx *ctx = lf->ctx;
   if (k.empty()) return PMNB_OK;
   const long long call = k.apply != kStNone ? lf->calls++ : -1;
+  const long long sc_apply = lf->scalls[k.apply];
+  const long long sc_probe = lf->scalls[k.probe];
   const int grid = lf->ngb + lf->nfb;
   if (grid > 0) {
-    hipError_t e = lf->shape == 0 ? lf_launch_step<3, 3>(k, d, grid, call, s)
-                                  : lf_launch_step<4, 6>(k, d, grid, call, s);
+    hipError_t e =
+        lf->shape == kShapeWater
+            ? lf_launch_step<kShapeWater>(k, d, grid, sc_apply, sc_probe, call, s)
+        : lf->shape == kShape33
+            ? lf_launch_step<kShape33>(k, d, grid, sc_apply, sc_probe, call, s)
+            : lf_launch_step<kShape46>(k, d, grid, sc_apply, sc_probe, call, s);
     if (e != hipSuccess)
       return fail(ctx, PMNB_ERR_HIP, "leapfrog step launch failed: %s",
                   hipGetErrorString(e));
     if (k.probe != kStNone && lf->ngb > 0) {
-      const int nwin = (d.maxit[k.probe] + kLfWindow) / kLfWindow;
-      for (int p = 1; p < nwin; p++) {
-        e = lf->shape == 0 ? lf_launch_probe<3, 3>(k.probe, d, p, s)
-                           : lf_launch_probe<4, 6>(k.probe, d, p, s);
-        if (e != hipSuccess)
-          return fail(ctx, PMNB_ERR_HIP, "leapfrog probe launch failed: %s",
-                      hipGetErrorString(e));
-      }
+      e = lf->shape == kShapeWater ? lf_launch_probe<kShapeWater>(k.probe, d, sc_probe, s)
+          : lf->shape == kShape33  ? lf_launch_probe<kShape33>(k.probe, d, sc_probe, s)
+                                   : lf_launch_probe<kShape46>(k.probe, d, sc_probe, s);
+      if (e != hipSuccess)
+        return fail(ctx, PMNB_ERR_HIP, "leapfrog probe launch failed: %s",
+                    hipGetErrorString(e));
     }
   }
+  if (k.apply != kStNone) lf->scalls[k.apply]++;
   k = LfKernel{};
   return PMNB_OK;
 }
@@ -1882,8 +1891,8 @@ int pmnb_lf_create(pmnb_ctx *ctx, const pmnb_lf_params *params,
                 "a connected set of constraints spans %d atoms and %d "
                 "constraints (at most %d and %d supported)",
                 max_a, max_c, kLfMaxAtoms, kLfMaxCons);
-  const int shape = (max_a <= 3 && max_c <= 3) ? 0 : 1;
-  const int NA = shape == 0 ? 3 : 4, NC = shape == 0 ? 3 : 6;
+  int shape = (max_a <= 3 && max_c <= 3) ? kShape33 : kShape46;
+  const int NA = shape == kShape33 ? 3 : 4, NC = shape == kShape33 ? 3 : 6;
   std::vector<int> h_atoms((size_t)std::max(ng, 1) * NA, -1);
   std::vector<int> h_cons((size_t)std::max(ng, 1) * NC, -1);
   std::vector<double> h_rest((size_t)std::max(ng, 1) * NC, 1.0);
@@ -1901,6 +1910,17 @@ int pmnb_lf_create(pmnb_ctx *ctx, const pmnb_lf_params *params,
       h_rest[(size_t)g * NC + k] = r;
       h_rinv[(size_t)g * NC + k] = 1.0 / (2.0 * (r * r));
     }
+  }
+  // every group a water in the reference's constraint order (atoms O < H1 <
+  // H2, constraints (H1,H2), (H1,O), (H2,O)): the compile-time shape
+  if (shape == kShape33 && ng > 0) {
+    bool water = true;
+    const int want[3] = {1 | (2 << 4), 1 | (0 << 4), 2 | (0 << 4)};
+    for (int g = 0; g < ng && water; g++) {
+      water = gat[g].size() == 3 && gcon[g].size() == 3;
+      for (int c = 0; c < 3 && water; c++) water = h_cons[(size_t)g * 3 + c] == want[c];
+    }
+    if (water) shape = kShapeWater;
   }
   std::vector<int> h_free;
   for (int i = 0; i < n_atoms; i++)
@@ -1951,6 +1971,8 @@ int pmnb_lf_create(pmnb_ctx *ctx, const pmnb_lf_params *params,
                              sizeof(double));
   if (e == hipSuccess) e = lf->nonconv.ensure(4 * sizeof(int));
   if (e == hipSuccess) e = hipMemset(lf->nonconv.p, 0, 4 * sizeof(int));
+  if (e == hipSuccess) e = lf->hint.ensure(8 * sizeof(int));
+  if (e == hipSuccess) e = hipMemset(lf->hint.p, 0xff, 8 * sizeof(int));  // -1
   if (e != hipSuccess) {
     pmnb_lf_destroy(lf);
     return fail(ctx, PMNB_ERR_HIP, "leapfrog set-up failed: %s",
@@ -2061,7 +2083,7 @@ void pmnb_lf_destroy(pmnb_leapfrog *lf) {
   DevBuf *bufs[] = {&lf->mass, &lf->rmass, &lf->g_atoms, &lf->g_cons,
                     &lf->g_rest, &lf->g_rinv, &lf->free_atoms, &lf->last,
                     &lf->xs, &lf->vs, &lf->partial_s, &lf->partial_r,
-                    &lf->iters, &lf->nonconv};
+                    &lf->iters, &lf->nonconv, &lf->hint};
   for (DevBuf *b : bufs) b->release();
   delete lf;
 }

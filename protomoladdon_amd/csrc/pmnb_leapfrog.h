@@ -22,14 +22,16 @@
 //
 // Sweep count.  The reference sweeps every constraint until the mean error
 // over all constraints is <= epsilon (at most max_iter + 1 sweeps), a
-// decision over the whole system.  A "probe" records, per block, the summed
-// group errors after 0, 1, ..., W-1 sweeps without keeping the swept state;
-// the kernel that applies the stage reduces these partials in a fixed order
-// (every block computes the same value), finds the reference's sweep count
-// k, and sweeps k times from the stage input.  When the error can stay
-// above epsilon past W levels, further probe launches continue from the
-// state stored after W sweeps; they return at once when an earlier window
-// already decided.  No host synchronisation, no atomics on the decision.
+// decision over the whole system.  A "probe" records the summed group
+// errors after 0, 1, ..., L-1 sweeps (one partial per block) without keeping
+// the swept state; the kernel that applies the stage reduces these partials
+// in a fixed order (every block computes the same value), finds the
+// reference's sweep count k, and sweeps k times from the stage input.  L is
+// the previous call's sweep count of the same stage plus a margin; when the
+// error is still above epsilon after L levels, one continuation launch
+// probes the remaining levels from the state stored after L sweeps (it
+// returns at once otherwise).  No host synchronisation, no atomics on the
+// decision.
 //
 // The global error is summed per group then per block rather than in the
 // reference's constraint order, and each constraint's relative error is
@@ -42,9 +44,33 @@
 namespace pmnb {
 
 constexpr int kLfBlock = 256;
-constexpr int kLfWindow = 8;     // error levels one probe launch records
-constexpr int kLfMaxAtoms = 4;   // largest group
-constexpr int kLfMaxCons = 6;    // most constraints in a group
+constexpr int kLfWaves = kLfBlock / 64;
+constexpr int kLfFirstLevels = 16;  // first probe's levels with no history
+constexpr int kLfMargin = 2;        // levels past the previous sweep count
+constexpr int kLfMaxAtoms = 4;      // largest group
+constexpr int kLfMaxCons = 6;       // most constraints in a group
+
+// Group shapes: the register layout (atoms, constraints) and whether the
+// constraint pattern is fixed at compile time.  kShapeWater: atoms O < H1 <
+// H2 with the constraints (H1,H2), (H1,O), (H2,O) of
+// topology/TopologyUtilities.cpp:732-737.
+enum LfShapeId { kShape33 = 0, kShape46 = 1, kShapeWater = 2 };
+template <int S> struct LfShape;
+template <> struct LfShape<kShape33> {
+  static constexpr int NA = 3, NC = 3;
+  static constexpr bool fixed = false;
+  static constexpr int A[3] = {0, 0, 0}, B[3] = {0, 0, 0};
+};
+template <> struct LfShape<kShape46> {
+  static constexpr int NA = 4, NC = 6;
+  static constexpr bool fixed = false;
+  static constexpr int A[6] = {0, 0, 0, 0, 0, 0}, B[6] = {0, 0, 0, 0, 0, 0};
+};
+template <> struct LfShape<kShapeWater> {
+  static constexpr int NA = 3, NC = 3;
+  static constexpr bool fixed = true;
+  static constexpr int A[3] = {1, 1, 2}, B[3] = {2, 0, 0};
+};
 
 enum LfStage { kStNone = 0, kStShake = 1, kStRattle = 2 };
 enum LfPre {
@@ -70,7 +96,8 @@ struct LfDev {
   double eps[3];            // by stage
   int maxit[3];
   double *partial[3];       // by stage: (maxit + 1) levels x ngb blocks
-  double *xs, *vs;          // state after a probe window (per atom)
+  int *hint;                // [stage * 2 + call parity]: last sweep count
+  double *xs, *vs;          // state after the first probe (per atom)
   int *iters;               // sweep count per stage call (optional)
   long long iters_cap;
   int *nonconv;             // [3] stage calls that hit max_iter
@@ -137,87 +164,107 @@ __device__ __forceinline__ void upd3(D3 (&a)[NA], int i, D3 d, double s) {
   put3_r<NA - 1>(a, i, n);
 }
 
-template <int NA, int NC>
+template <class SH>
 struct LfGroup {
-  D3 x[NA], v[NA], l[NA];
-  double rm[NA];
-  int idx[NA];
-  int cn[NC];
-  double rest[NC], rinv[NC];
+  D3 x[SH::NA], v[SH::NA], l[SH::NA];
+  double rm[SH::NA];
+  int idx[SH::NA];
+  int cn[SH::NC];
+  double rest[SH::NC], rinv[SH::NC];
 };
 
-// ModifierShake::doExecute, one sweep over the group's constraints.  With
-// VEL false the velocity updates are left out (they never feed back into
-// positions or the error; the probe uses this).
-template <bool VEL, int NA, int NC>
-__device__ __forceinline__ void shake_sweep(LfGroup<NA, NC> &g, double dt) {
-#pragma unroll
-  for (int c = 0; c < NC; c++) {
-    if (g.cn[c] < 0) continue;
-    const int a = g.cn[c] & 15, b = g.cn[c] >> 4;
-    const D3 pab = d3_sub(pick3(g.x, a), pick3(g.x, b));
-    const double pabsq = d3_dot(pab, pab);
-    const double rabsq = g.rest[c] * g.rest[c];
-    const double diffsq = rabsq - pabsq;
-    const D3 rab = d3_sub(pick3(g.l, a), pick3(g.l, b));
-    const double rpab = d3_dot(rab, pab);
-    const double rM1 = pick1(g.rm, a), rM2 = pick1(g.rm, b);
-    const double gab = diffsq / (2 * (rM1 + rM2) * rpab);
-    D3 dp = {rab.x * gab, rab.y * gab, rab.z * gab};
-    upd3<1>(g.x, a, dp, rM1);
-    upd3<-1>(g.x, b, dp, rM2);
-    if (VEL) {
-      dp = {dp.x / dt, dp.y / dt, dp.z / dt};
-      upd3<1>(g.v, a, dp, rM1);
-      upd3<-1>(g.v, b, dp, rM2);
-    }
+// Local atom indices of constraint C: compile-time constants for a fixed
+// shape, otherwise from the group's table (-1: padding).
+template <class SH, int C>
+__device__ __forceinline__ bool cons_ab(const LfGroup<SH> &g, int &a, int &b) {
+  if constexpr (SH::fixed) {
+    a = SH::A[C];
+    b = SH::B[C];
+    return true;
+  } else {
+    a = g.cn[C] & 15;
+    b = g.cn[C] >> 4;
+    return g.cn[C] >= 0;
   }
 }
 
-// ModifierRattle::doExecute, one sweep (positions are not changed).
-template <int NA, int NC>
-__device__ __forceinline__ void rattle_sweep(LfGroup<NA, NC> &g, double dt) {
-#pragma unroll
-  for (int c = 0; c < NC; c++) {
-    if (g.cn[c] < 0) continue;
-    const int a = g.cn[c] & 15, b = g.cn[c] >> 4;
-    const double rM1 = pick1(g.rm, a), rM2 = pick1(g.rm, b);
-    const D3 rab = d3_sub(pick3(g.x, a), pick3(g.x, b));
-    const double rabsq = d3_dot(rab, rab);
-    const D3 vab = d3_sub(pick3(g.v, a), pick3(g.v, b));
-    const double rvab = d3_dot(rab, vab);
-    const double gab = -rvab / (dt * (rM1 + rM2) * rabsq);
-    const D3 dp = {rab.x * gab, rab.y * gab, rab.z * gab};
-    const D3 dpt = {dp.x * dt, dp.y * dt, dp.z * dt};
-    upd3<1>(g.v, a, dpt, rM1);
-    upd3<-1>(g.v, b, dpt, rM2);
+// ModifierShake::doExecute for constraint C onwards (one sweep).  With VEL
+// false the velocity updates are left out (they never feed back into
+// positions or the error; the probe uses this).
+template <bool VEL, class SH, int C = 0>
+__device__ __forceinline__ void shake_sweep(LfGroup<SH> &g, double dt) {
+  if constexpr (C < SH::NC) {
+    int a, b;
+    if (cons_ab<SH, C>(g, a, b)) {
+      const D3 pab = d3_sub(pick3(g.x, a), pick3(g.x, b));
+      const double pabsq = d3_dot(pab, pab);
+      const double rabsq = g.rest[C] * g.rest[C];
+      const double diffsq = rabsq - pabsq;
+      const D3 rab = d3_sub(pick3(g.l, a), pick3(g.l, b));
+      const double rpab = d3_dot(rab, pab);
+      const double rM1 = pick1(g.rm, a), rM2 = pick1(g.rm, b);
+      const double gab = diffsq / (2 * (rM1 + rM2) * rpab);
+      D3 dp = {rab.x * gab, rab.y * gab, rab.z * gab};
+      upd3<1>(g.x, a, dp, rM1);
+      upd3<-1>(g.x, b, dp, rM2);
+      if (VEL) {
+        dp = {dp.x / dt, dp.y / dt, dp.z / dt};
+        upd3<1>(g.v, a, dp, rM1);
+        upd3<-1>(g.v, b, dp, rM2);
+      }
+    }
+    shake_sweep<VEL, SH, C + 1>(g, dt);
+  }
+}
+
+// ModifierRattle::doExecute for constraint C onwards (one sweep; positions
+// are not changed).
+template <class SH, int C = 0>
+__device__ __forceinline__ void rattle_sweep(LfGroup<SH> &g, double dt) {
+  if constexpr (C < SH::NC) {
+    int a, b;
+    if (cons_ab<SH, C>(g, a, b)) {
+      const double rM1 = pick1(g.rm, a), rM2 = pick1(g.rm, b);
+      const D3 rab = d3_sub(pick3(g.x, a), pick3(g.x, b));
+      const double rabsq = d3_dot(rab, rab);
+      const D3 vab = d3_sub(pick3(g.v, a), pick3(g.v, b));
+      const double rvab = d3_dot(rab, vab);
+      const double gab = -rvab / (dt * (rM1 + rM2) * rabsq);
+      const D3 dp = {rab.x * gab, rab.y * gab, rab.z * gab};
+      const D3 dpt = {dp.x * dt, dp.y * dt, dp.z * dt};
+      upd3<1>(g.v, a, dpt, rM1);
+      upd3<-1>(g.v, b, dpt, rM2);
+    }
+    rattle_sweep<SH, C + 1>(g, dt);
   }
 }
 
 // Sum over the group's constraints of the reference's per-constraint error
 // (ModifierMetaShake::calcError / ModifierMetaRattle::calcError).
-template <int STAGE, int NA, int NC>
-__device__ __forceinline__ double group_error(const LfGroup<NA, NC> &g) {
-  double e = 0;
-#pragma unroll
-  for (int c = 0; c < NC; c++) {
-    if (g.cn[c] < 0) continue;
-    const int a = g.cn[c] & 15, b = g.cn[c] >> 4;
-    if (STAGE == kStShake) {
-      const D3 d = d3_sub(pick3(g.x, b), pick3(g.x, a));
-      const double r2 = g.rest[c] * g.rest[c];
-      e += fabs(d3_dot(d, d) - r2) * g.rinv[c];
-    } else {
-      const D3 rab = d3_sub(pick3(g.x, a), pick3(g.x, b));
-      const D3 vab = d3_sub(pick3(g.v, a), pick3(g.v, b));
-      e += fabs(d3_dot(rab, vab));
+template <int STAGE, class SH, int C = 0>
+__device__ __forceinline__ double group_error(const LfGroup<SH> &g) {
+  if constexpr (C < SH::NC) {
+    double e = 0;
+    int a, b;
+    if (cons_ab<SH, C>(g, a, b)) {
+      if (STAGE == kStShake) {
+        const D3 d = d3_sub(pick3(g.x, b), pick3(g.x, a));
+        const double r2 = g.rest[C] * g.rest[C];
+        e = fabs(d3_dot(d, d) - r2) * g.rinv[C];
+      } else {
+        const D3 rab = d3_sub(pick3(g.x, a), pick3(g.x, b));
+        const D3 vab = d3_sub(pick3(g.v, a), pick3(g.v, b));
+        e = fabs(d3_dot(rab, vab));
+      }
     }
+    return e + group_error<STAGE, SH, C + 1>(g);
+  } else {
+    return 0.0;
   }
-  return e;
 }
 
-template <int STAGE, int NA, int NC>
-__device__ __forceinline__ void sweep(LfGroup<NA, NC> &g, double dt, bool vel) {
+template <int STAGE, class SH>
+__device__ __forceinline__ void sweep(LfGroup<SH> &g, double dt, bool vel) {
   if (STAGE == kStShake) {
     if (vel) shake_sweep<true>(g, dt);
     else shake_sweep<false>(g, dt);
@@ -250,39 +297,58 @@ __device__ __forceinline__ void pre_op(D3 &x, D3 &v, D3 f, double m,
   }
 }
 
-// Sweep count of a stage: the first error level j (0..maxit) whose mean
-// error over all constraints is <= epsilon, else maxit + 1.  Every block
-// reduces the same partials in the same order.  Returns -1 when the
-// windows before `upto` (exclusive) did not decide (only when upto is
-// given, for the continuation probes).
-__device__ __forceinline__ int lf_decide(const LfDev &d, int stage,
-                                          int upto_window) {
-  __shared__ double red[kLfWindow][16];
+// Levels the first probe of stage call `scall` records: the previous call's
+// sweep count plus a margin (the same value in every kernel of the call:
+// the apply kernel of call c writes slot c & 1, calls read slot (c + 1) & 1).
+__device__ __forceinline__ int lf_first_levels(const LfDev &d, int stage,
+                                               long long scall) {
+  const int hint = d.hint[stage * 2 + (int)((scall + 1) & 1)];
+  const int L = hint < 0 ? kLfFirstLevels : hint + kLfMargin + 1;
+  return min(max(L, 1), d.maxit[stage] + 1);
+}
+
+// First level in [j0, j1) whose mean error over all constraints is <=
+// epsilon, or -1.  Block-wide; every block sums the per-block partials in
+// the same order, so every block returns the same level.  A thread sums
+// blocks sub, sub + 8, ... of one level; its loads are issued in batches
+// before the (ordered) additions so that they overlap.
+__device__ __forceinline__ int lf_scan(const LfDev &d, int stage, int j0,
+                                       int j1) {
+  constexpr int kSub = 8, kChunk = kLfBlock / kSub, kBatch = 8;
+  __shared__ double red[kChunk][kSub];
   __shared__ int hit;
   const int t = threadIdx.x;
-  const int maxit = d.maxit[stage];
   const double *P = d.partial[stage];
-  const int nwin = (maxit + 1 + kLfWindow - 1) / kLfWindow;
-  const int last_window = upto_window >= 0 ? upto_window : nwin;
-  for (int p = 0; p < last_window; p++) {
-    const int j0 = p * kLfWindow;
-    const int wn = min(kLfWindow, maxit + 1 - j0);
+  const int nb = d.ngb;
+  for (int c0 = j0; c0 < j1; c0 += kChunk) {
+    const int n = min(kChunk, j1 - c0);
     __syncthreads();
-    if (t < kLfWindow * 16) {
-      const int jj = t / 16, sub = t % 16;
+    {
+      const int jj = t % kChunk, sub = t / kChunk;
       double s = 0;
-      if (jj < wn)
-        for (int b = sub; b < d.ngb; b += 16) s += P[(size_t)(j0 + jj) * d.ngb + b];
+      if (jj < n) {
+        const double *row = P + (size_t)(c0 + jj) * nb;
+        for (int q0 = sub; q0 < nb; q0 += kSub * kBatch) {
+          double v[kBatch];
+#pragma unroll
+          for (int u = 0; u < kBatch; u++) {
+            const int q = q0 + u * kSub;
+            v[u] = q < nb ? row[q] : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < kBatch; u++) s += v[u];
+        }
+      }
       red[jj][sub] = s;
     }
     if (t == 0) hit = -1;
     __syncthreads();
     if (t == 0) {
-      for (int jj = 0; jj < wn; jj++) {
+      for (int jj = 0; jj < n; jj++) {
         double s = 0;
-        for (int k = 0; k < 16; k++) s += red[jj][k];
+        for (int k = 0; k < kSub; k++) s += red[jj][k];
         if (s / d.n_cons <= d.eps[stage]) {
-          hit = j0 + jj;
+          hit = c0 + jj;
           break;
         }
       }
@@ -291,59 +357,72 @@ __device__ __forceinline__ int lf_decide(const LfDev &d, int stage,
     const int hv = hit;
     if (hv >= 0) return hv;
   }
-  return upto_window >= 0 ? -1 : maxit + 1;
+  return -1;
 }
 
-// Records W error levels of a stage for this block's groups, starting at
-// level j0 from the state in g (active threads only), and leaves the state
-// after the window in xs / vs when a further window follows.
-template <int STAGE, int NA, int NC>
-__device__ __forceinline__ void lf_probe(const LfDev &d, LfGroup<NA, NC> &g,
-                                         bool act, int j0) {
-  __shared__ double wred[kLfBlock / 64][kLfWindow];
-  const int maxit = d.maxit[STAGE];
-  const int wn = min(kLfWindow, maxit + 1 - j0);
-  const bool more = j0 + wn <= maxit;
-  double e[kLfWindow];
+// Sweep count of a stage call: the first level j (0..maxit) whose mean
+// error is <= epsilon, else maxit + 1.
+__device__ __forceinline__ int lf_decide(const LfDev &d, int stage,
+                                         long long scall) {
+  const int L = lf_first_levels(d, stage, scall);
+  const int k = lf_scan(d, stage, 0, L);
+  if (k >= 0) return k;
+  const int k2 = lf_scan(d, stage, L, d.maxit[stage] + 1);
+  return k2 >= 0 ? k2 : d.maxit[stage] + 1;
+}
+
+__device__ __forceinline__ double wave_sum(double s) {
 #pragma unroll
-  for (int jj = 0; jj < kLfWindow; jj++) {
-    e[jj] = 0;
-    if (jj < wn && act) {
-      e[jj] = group_error<STAGE>(g);
-      if (jj < wn - 1 || more) sweep<STAGE>(g, d.dtc, false);
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  return s;
+}
+
+// Records the error levels [j0, j1) of a stage for this block's groups from
+// the state in g (active threads only): one partial per block and level,
+// the four wave sums added in wave order.  With `store`, the state after
+// j1 - j0 sweeps is left in xs / vs.  Block-wide (all threads call it).
+template <int STAGE, class SH>
+__device__ __forceinline__ void lf_probe(const LfDev &d, LfGroup<SH> &g,
+                                         bool act, int j0, int j1, bool store) {
+  constexpr int kLevels = 32;
+  __shared__ double wred[kLfWaves][kLevels];
+  const int wave = threadIdx.x >> 6;
+  double *P = d.partial[STAGE] + blockIdx.x;
+  for (int c0 = j0; c0 < j1; c0 += kLevels) {
+    const int n = min(kLevels, j1 - c0);
+    // branch-free level body (inactive threads hold a copy of a real
+    // group and only contribute 0): the wave sum of one level overlaps the
+    // next sweep.  The sweep after the last level is only used by `store`.
+    for (int jj = 0; jj < n; jj++) {
+      const double e = group_error<STAGE>(g);
+      sweep<STAGE>(g, d.dtc, false);
+      wred[wave][jj] = wave_sum(act ? e : 0.0);
     }
-  }
-  if (more && act) {
+    __syncthreads();
+    if ((int)threadIdx.x < n) {
+      double s = 0;
 #pragma unroll
-    for (int k = 0; k < NA; k++) {
+      for (int w = 0; w < kLfWaves; w++) s += wred[w][threadIdx.x];
+      P[(size_t)(c0 + threadIdx.x) * d.ngb] = s;
+    }
+    __syncthreads();
+  }
+  if (store && act) {
+#pragma unroll
+    for (int k = 0; k < SH::NA; k++) {
       if (g.idx[k] < 0) continue;
       if (STAGE == kStShake) st3(d.xs, g.idx[k], g.x[k]);
       else st3(d.vs, g.idx[k], g.v[k]);
     }
   }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int jj = 0; jj < kLfWindow; jj++) {
-    double s = e[jj];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (lane == 0) wred[wave][jj] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x < wn) {
-    double s = 0;
-#pragma unroll
-    for (int w = 0; w < kLfBlock / 64; w++) s += wred[w][threadIdx.x];
-    d.partial[STAGE][(size_t)(j0 + threadIdx.x) * d.ngb + blockIdx.x] = s;
-  }
 }
 
-template <int NA, int NC>
+template <class SH>
 __device__ __forceinline__ void load_group(const LfDev &d, int gi,
-                                           LfGroup<NA, NC> &g, bool need_last) {
+                                           LfGroup<SH> &g, bool need_last) {
 #pragma unroll
-  for (int k = 0; k < NA; k++) {
-    const int i = d.g_atoms[(size_t)gi * NA + k];
+  for (int k = 0; k < SH::NA; k++) {
+    const int i = d.g_atoms[(size_t)gi * SH::NA + k];
     g.idx[k] = i;
     const int j = i < 0 ? 0 : i;  // padding slots read atom 0, never written
     g.x[k] = ld3(d.x, j);
@@ -352,22 +431,27 @@ __device__ __forceinline__ void load_group(const LfDev &d, int gi,
     g.rm[k] = d.rmass[j];
   }
 #pragma unroll
-  for (int c = 0; c < NC; c++) {
-    g.cn[c] = d.g_cons[(size_t)gi * NC + c];
-    g.rest[c] = d.g_rest[(size_t)gi * NC + c];
-    g.rinv[c] = d.g_rinv[(size_t)gi * NC + c];
+  for (int c = 0; c < SH::NC; c++) {
+    g.cn[c] = d.g_cons[(size_t)gi * SH::NC + c];
+    g.rest[c] = d.g_rest[(size_t)gi * SH::NC + c];
+    g.rinv[c] = d.g_rinv[(size_t)gi * SH::NC + c];
   }
 }
 
 // One fused launch: apply stage APPLY (sweep count from its probe), then the
-// per-atom update PRE, store, then probe stage PROBE's first window; with
+// per-atom update PRE, store, then the first probe of stage PROBE; with
 // ZERO, clear the forces and the energy block for the next evaluation.
-template <int APPLY, int PRE, int PROBE, bool ZERO, int NA, int NC>
-__global__ __launch_bounds__(kLfBlock) void k_lf_step(LfDev d, long long call) {
+// sc_apply / sc_probe: the per-stage call numbers; call: the global one.
+template <int APPLY, int PRE, int PROBE, bool ZERO, int SHAPE>
+__global__ __launch_bounds__(kLfBlock) void k_lf_step(LfDev d, long long sc_apply,
+                                                      long long sc_probe,
+                                                      long long call) {
+  using SH = LfShape<SHAPE>;
   int k = 0;
   if (APPLY != kStNone) {
-    k = lf_decide(d, APPLY, -1);
+    k = lf_decide(d, APPLY, sc_apply);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
+      d.hint[APPLY * 2 + (int)(sc_apply & 1)] = k;
       if (d.iters && call < d.iters_cap) d.iters[call] = k;
       if (k > d.maxit[APPLY]) atomicAdd(&d.nonconv[APPLY], 1);
     }
@@ -388,12 +472,14 @@ __global__ __launch_bounds__(kLfBlock) void k_lf_step(LfDev d, long long call) {
   }
   const int gi = blockIdx.x * kLfBlock + threadIdx.x;
   const bool act = gi < d.n_groups;
-  LfGroup<NA, NC> g;
+  LfGroup<SH> g;
+  // inactive threads hold a copy of the last group (read only)
+  load_group(d, act ? gi : d.n_groups - 1, g,
+             APPLY == kStShake || PROBE == kStShake);
   if (act) {
-    load_group(d, gi, g, APPLY == kStShake || PROBE == kStShake);
     for (int s = 0; s < k; s++) sweep<APPLY>(g, d.dtc, true);
 #pragma unroll
-    for (int a = 0; a < NA; a++) {
+    for (int a = 0; a < SH::NA; a++) {
       const int i = g.idx[a];
       if (i < 0) continue;
       if (APPLY == kStShake) {
@@ -408,26 +494,31 @@ __global__ __launch_bounds__(kLfBlock) void k_lf_step(LfDev d, long long call) {
       if (ZERO) st3(d.f, i, D3{0, 0, 0});
     }
   }
-  if (PROBE != kStNone) lf_probe<PROBE>(d, g, act, 0);
+  if (PROBE != kStNone) {
+    const int L = lf_first_levels(d, PROBE, sc_probe);
+    lf_probe<PROBE>(d, g, act, 0, L, L <= d.maxit[PROBE]);
+  }
 }
 
-// Window p >= 1 of a stage's probe, from the state the previous window left.
-template <int STAGE, int NA, int NC>
-__global__ __launch_bounds__(kLfBlock) void k_lf_probe(LfDev d, int p) {
-  if (lf_decide(d, STAGE, p) >= 0) return;
+// The levels after the first probe's, from the state it stored; returns at
+// once when the first probe already decided.
+template <int STAGE, int SHAPE>
+__global__ __launch_bounds__(kLfBlock) void k_lf_probe(LfDev d, long long scall) {
+  using SH = LfShape<SHAPE>;
+  const int L = lf_first_levels(d, STAGE, scall);
+  if (L > d.maxit[STAGE] || lf_scan(d, STAGE, 0, L) >= 0) return;
   const int gi = blockIdx.x * kLfBlock + threadIdx.x;
   const bool act = gi < d.n_groups;
-  LfGroup<NA, NC> g;
-  if (act) {
-    load_group(d, gi, g, STAGE == kStShake);
+  LfGroup<SH> g;
+  // inactive threads hold a copy of the last group (read only)
+  load_group(d, act ? gi : d.n_groups - 1, g, STAGE == kStShake);
 #pragma unroll
-    for (int a = 0; a < NA; a++) {
-      if (g.idx[a] < 0) continue;
-      if (STAGE == kStShake) g.x[a] = ld3(d.xs, g.idx[a]);
-      else g.v[a] = ld3(d.vs, g.idx[a]);
-    }
+  for (int a = 0; a < SH::NA; a++) {
+    if (g.idx[a] < 0) continue;
+    if (STAGE == kStShake) g.x[a] = ld3(d.xs, g.idx[a]);
+    else g.v[a] = ld3(d.vs, g.idx[a]);
   }
-  lf_probe<STAGE>(d, g, act, p * kLfWindow);
+  lf_probe<STAGE>(d, g, act, L, d.maxit[STAGE] + 1, false);
 }
 
 }  // namespace pmnb
