@@ -310,13 +310,37 @@ int pmnb_lf_create(pmnb_ctx *ctx, const pmnb_lf_params *params,
    (NULL = the context's stream) without host synchronisation.
    d_positions, d_velocities: 3N doubles, updated.  d_forces: the forces at
    d_positions on entry (as after initializeForces), at the final positions
-   on exit.  d_out: the 20-double block of pmnb_evaluate_device, holding the
-   last evaluation's energies on exit (calculateForces clears them before
-   every evaluation).  flags: as pmnb_evaluate_device, without
+   on exit.  d_out: the 20-double block of pmnb_evaluate_device (22
+   doubles when bonded terms are set: [20] bond, [21] angle energy),
+   holding the last evaluation's energies on exit (calculateForces clears
+   them before every evaluation).  flags: as pmnb_evaluate_device, without
    PMNB_FLAG_MOLVIRIAL. */
 int pmnb_lf_run_device(pmnb_leapfrog *lf, double *d_positions,
                        double *d_velocities, double *d_forces, double *d_out,
                        int32_t n_steps, uint32_t flags, void *stream);
+
+/* Bonded terms evaluated with the cutoff force in every calculateForces of
+   the integrator: BondSystemForce (force/bonded/BondSystemForce.h:89-122,
+   E = k (r - r0)^2) over bond_atoms (2 per bond) and AngleSystemForce
+   (force/bonded/AngleSystemForce.h:86-150, E = k (theta - theta0)^2 plus
+   Urey-Bradley kub (d13 - r_ub)^2) over angle_atoms (3 per angle, theta0
+   in radians; ub_k / ub_r0 may be NULL for none), with the minimum image of
+   the context's box.  Replaces any earlier set; n = 0 for none.  Their
+   virial is not computed. */
+int pmnb_lf_set_bonded(pmnb_leapfrog *lf, int32_t n_bonds,
+                       const int32_t *bond_atoms, const double *bond_k,
+                       const double *bond_r0, int32_t n_angles,
+                       const int32_t *angle_atoms, const double *angle_k,
+                       const double *angle_theta0, const double *ub_k,
+                       const double *ub_r0);
+
+/* StandardIntegrator::calculateForces of the integrator's force group:
+   clears d_forces and d_out (20 or 22 doubles, as pmnb_lf_run_device),
+   then evaluates the cutoff force and the bonded terms at d_positions.
+   This is what initializeForces computes before the first run. */
+int pmnb_lf_forces_device(pmnb_leapfrog *lf, const double *d_positions,
+                          double *d_forces, double *d_out, uint32_t flags,
+                          void *stream);
 
 /* Starts (capacity > 0) or stops (0) recording the sweep count of every
    SHAKE and RATTLE call, and returns what was recorded since the last

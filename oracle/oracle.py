@@ -306,6 +306,11 @@ class OrcLf(ctypes.Structure):
         ("rattle_eps", ctypes.c_double), ("shake_max_iter", ctypes.c_int32),
         ("rattle_max_iter", ctypes.c_int32), ("shake", ctypes.c_int32),
         ("rattle", ctypes.c_int32),
+        ("n_bonds", ctypes.c_int32), ("n_angles", ctypes.c_int32),
+        ("bonds", ctypes.c_void_p), ("bond_k", ctypes.c_void_p),
+        ("bond_r0", ctypes.c_void_p), ("angles", ctypes.c_void_p),
+        ("angle_k", ctypes.c_void_p), ("angle_t0", ctypes.c_void_p),
+        ("ub_k", ctypes.c_void_p), ("ub_r0", ctypes.c_void_p),
     ]
 
 
@@ -313,33 +318,22 @@ class MDState:
     """Positions, velocities, forces at the positions and SHAKE's previous
     positions, as LeapfrogIntegrator and its modifiers keep them."""
 
-    def __init__(self, x, v, f, last=None, energies=None):
+    def __init__(self, x, v, f, last=None, energies=None, bonded=None):
         self.x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3).copy()
         self.v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3).copy()
         self.f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1, 3).copy()
         self.last = self.x.copy() if last is None else np.array(
             last, dtype=np.float64).reshape(-1, 3)
         self.energies = np.zeros(2) if energies is None else np.array(energies)
+        self.bonded = np.zeros(2) if bonded is None else np.array(bonded)
 
 
-def leapfrog_run(system, state, n_steps, masses, constraints, rest, timestep,
-                 shake=None, rattle=None, max_calls=1 << 16):
-    """LeapfrogIntegrator::run(n_steps) on `state` (updated in place) with
-    the cutoff force of `system` as the only force.  shake / rattle are
-    (epsilon, max_iter) or None.  Returns the sweep count of every SHAKE
-    and RATTLE call in call order."""
-    lib = load()
-    if not hasattr(lib, "_lf_ready"):
-        lib.orc_leapfrog_run.argtypes = [
-            ctypes.POINTER(OrcSystem), ctypes.POINTER(OrcLf)] + \
-            [ctypes.c_void_p] * 5 + [ctypes.c_int32, ctypes.c_void_p,
-                                     ctypes.c_int64]
-        lib.orc_leapfrog_run.restype = ctypes.c_int64
-        lib._lf_ready = True
-    st = system.struct()
+def _lf_struct(masses, constraints, rest, timestep, shake, rattle, bonded,
+               keep):
     cons = np.ascontiguousarray(constraints, dtype=np.int32).reshape(-1, 2)
     rest = np.ascontiguousarray(rest, dtype=np.float64)
     mass = np.ascontiguousarray(masses, dtype=np.float64)
+    keep += [cons, rest, mass]
     lf = OrcLf()
     lf.n_cons = len(cons)
     lf.cons = cons.ctypes.data if len(cons) else None
@@ -350,11 +344,71 @@ def leapfrog_run(system, state, n_steps, masses, constraints, rest, timestep,
     lf.rattle = int(rattle is not None)
     lf.shake_eps, lf.shake_max_iter = shake if shake else (0.0, 0)
     lf.rattle_eps, lf.rattle_max_iter = rattle if rattle else (0.0, 0)
+    if bonded is not None:
+        def arr(name, dt=np.float64):
+            a = np.ascontiguousarray(bonded[name], dtype=dt)
+            keep.append(a)
+            return a.ctypes.data
+        lf.n_bonds = len(bonded["bond_k"])
+        lf.n_angles = len(bonded["angle_k"])
+        lf.bonds = arr("bonds", np.int32)
+        lf.bond_k, lf.bond_r0 = arr("bond_k"), arr("bond_r0")
+        lf.angles = arr("angles", np.int32)
+        lf.angle_k, lf.angle_t0 = arr("angle_k"), arr("angle_theta0")
+        if "ub_k" in bonded:
+            lf.ub_k, lf.ub_r0 = arr("ub_k"), arr("ub_r0")
+    return lf
+
+
+def lf_forces(system, positions, masses, bonded=None):
+    """calculateForces of the leapfrog's force group (cleared forces and
+    energies, then the bonded terms and the cutoff force): returns an
+    MDState-ready (forces, energies (LJ, electrostatic), bonded (bond,
+    angle))."""
+    lib = load()
+    if not hasattr(lib, "_lff_ready"):
+        lib.orc_lf_forces.argtypes = [ctypes.POINTER(OrcSystem),
+                                      ctypes.POINTER(OrcLf)] + [ctypes.c_void_p] * 4
+        lib._lff_ready = True
+    keep = []
+    lf = _lf_struct(masses, np.zeros((0, 2)), np.zeros(0), 1.0, None, None,
+                    bonded, keep)
+    x = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3).copy()
+    f = np.zeros_like(x)
+    e, b = np.zeros(2), np.zeros(2)
+    st = system.struct()
+    if lib.orc_lf_forces(ctypes.byref(st), ctypes.byref(lf), x.ctypes.data,
+                         f.ctypes.data, e.ctypes.data, b.ctypes.data):
+        raise RuntimeError("oracle force evaluation failed")
+    return f, e, b
+
+
+def leapfrog_run(system, state, n_steps, masses, constraints, rest, timestep,
+                 shake=None, rattle=None, bonded=None, max_calls=1 << 16):
+    """LeapfrogIntegrator::run(n_steps) on `state` (updated in place) with
+    the cutoff force of `system` and the optional bonded terms as forces.
+    shake / rattle are (epsilon, max_iter) or None; bonded is a dict with
+    bonds, bond_k, bond_r0, angles, angle_k, angle_theta0 (radians) and
+    optionally ub_k, ub_r0.  Returns the sweep count of every SHAKE and
+    RATTLE call in call order."""
+    lib = load()
+    if not hasattr(lib, "_lf_ready"):
+        lib.orc_leapfrog_run.argtypes = [
+            ctypes.POINTER(OrcSystem), ctypes.POINTER(OrcLf)] + \
+            [ctypes.c_void_p] * 6 + [ctypes.c_int32, ctypes.c_void_p,
+                                     ctypes.c_int64]
+        lib.orc_leapfrog_run.restype = ctypes.c_int64
+        lib._lf_ready = True
+    st = system.struct()
+    keep = []
+    lf = _lf_struct(masses, constraints, rest, timestep, shake, rattle,
+                    bonded, keep)
     iters = np.zeros(max_calls, dtype=np.int32)
     n = lib.orc_leapfrog_run(ctypes.byref(st), ctypes.byref(lf),
                              state.x.ctypes.data, state.v.ctypes.data,
                              state.f.ctypes.data, state.last.ctypes.data,
-                             state.energies.ctypes.data, int(n_steps),
+                             state.energies.ctypes.data,
+                             state.bonded.ctypes.data, int(n_steps),
                              iters.ctypes.data, max_calls)
     if n < 0:
         raise RuntimeError("oracle leapfrog failed")

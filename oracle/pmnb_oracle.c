@@ -638,6 +638,8 @@ void orc_switch_c2(double switchon, double cutoff, double d2, double *v, double 
  *   modifier/ModifierRattle.cpp:17-71                  RATTLE sweeps
  *   modifier/ModifierMetaShake.cpp:11-33, ModifierMetaRattle.cpp:11-30  errors
  *   module/ModifierModule.cpp:104-127                  which hook each uses
+ *   force/bonded/BondSystemForce.h:89-122              harmonic bond
+ *   force/bonded/AngleSystemForce.h:86-150             angle + Urey-Bradley
  * The constraint virial (ScalarStructure::addVirial in both modifiers) is
  * not accumulated: calculateForces clears the energies after every SHAKE
  * and every RATTLE but the last one.
@@ -654,12 +656,19 @@ typedef struct {
   double shake_eps, rattle_eps;
   int32_t shake_max_iter, rattle_max_iter;
   int32_t shake, rattle; /* modifier present (1) or not (0) */
+  /* optional bonded terms evaluated with the cutoff force */
+  int32_t n_bonds, n_angles;
+  const int32_t *bonds;       /* 2 per bond */
+  const double *bond_k, *bond_r0;
+  const int32_t *angles;      /* 3 per angle */
+  const double *angle_k, *angle_t0, *ub_k, *ub_r0;  /* t0 in radians */
 } orc_lf;
 
 typedef struct {
   const orc_system *s;
   const orc_lf *lf;
-  double *x, *v, *f, *last, *energies2;
+  orc_pbc pbc;
+  double *x, *v, *f, *last, *energies2, *bonded2;
   int32_t *iters;
   int64_t iters_cap, n_calls;
 } orc_md;
@@ -784,12 +793,75 @@ static void md_drift(orc_md *m) {
   for (i = 0; i < 3 * m->s->n_atoms; i++) m->x[i] += m->v[i] * h;
 }
 
+static double norm3(const double *a) {
+  return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+}
+
+/* BondSystemForce::calcBond and AngleSystemForce::calcAngle over all terms
+   (forces += , bonded2[0] += bond energy, bonded2[1] += angle energy). */
+static void md_bonded(orc_md *m) {
+  const orc_lf *lf = m->lf;
+  double *x = m->x, *f = m->f;
+  int i, k;
+  for (i = 0; i < lf->n_bonds; i++) {
+    int a1 = lf->bonds[2 * i], a2 = lf->bonds[2 * i + 1];
+    double r12[3], d2, r, dpotdr, f1[3];
+    orc_pbc_min_diff(&m->pbc, x + 3 * a2, x + 3 * a1, r12, &d2);
+    r = norm3(r12);
+    dpotdr = 2.0 * lf->bond_k[i] * (r - lf->bond_r0[i]);
+    for (k = 0; k < 3; k++) {
+      f1[k] = r12[k] * (-dpotdr / r);
+      f[3 * a1 + k] += f1[k];
+      f[3 * a2 + k] -= f1[k];
+    }
+    m->bonded2[0] += lf->bond_k[i] * (r - lf->bond_r0[i]) * (r - lf->bond_r0[i]);
+  }
+  for (i = 0; i < lf->n_angles; i++) {
+    int a1 = lf->angles[3 * i], a2 = lf->angles[3 * i + 1],
+        a3 = lf->angles[3 * i + 2];
+    double t0 = lf->angle_t0[i], ka = lf->angle_k[i];
+    double kub = lf->ub_k ? lf->ub_k[i] : 0.0, rub = lf->ub_r0 ? lf->ub_r0[i] : 0.0;
+    double r12[3], r32[3], r13[3], d2, d12, d32, d13, c[3], theta, sn, cs,
+        dpot, dr12[3], dr32[3], dr13[3], dt1[3], dt3[3], ub1[3], f1[3], f3[3];
+    orc_pbc_min_diff(&m->pbc, x + 3 * a2, x + 3 * a1, r12, &d2);
+    orc_pbc_min_diff(&m->pbc, x + 3 * a2, x + 3 * a3, r32, &d2);
+    orc_pbc_min_diff(&m->pbc, x + 3 * a3, x + 3 * a1, r13, &d2);
+    d12 = norm3(r12);
+    d32 = norm3(r32);
+    d13 = norm3(r13);
+    cross3(r12, r32, c);
+    theta = atan2(norm3(c), dot3(r12, r32));
+    sn = sin(theta);
+    cs = cos(theta);
+    dpot = 2.0 * ka * (theta - t0);
+    for (k = 0; k < 3; k++) {
+      dr12[k] = r12[k] / d12;
+      dr32[k] = r32[k] / d32;
+      dr13[k] = r13[k] / d13;
+    }
+    for (k = 0; k < 3; k++) {
+      dt1[k] = (dr12[k] * cs - dr32[k]) / (sn * d12);
+      dt3[k] = (dr32[k] * cs - dr12[k]) / (sn * d32);
+      ub1[k] = dr13[k] * (2.0 * kub * (d13 - rub));
+      f1[k] = -dt1[k] * dpot - ub1[k];
+      f3[k] = -dt3[k] * dpot - (-ub1[k]);
+      f[3 * a1 + k] += f1[k];
+      f[3 * a2 + k] += -f1[k] - f3[k];
+      f[3 * a3 + k] += f3[k];
+    }
+    m->bonded2[1] += ka * (theta - t0) * (theta - t0) +
+                     kub * (d13 - rub) * (d13 - rub);
+  }
+}
+
 /* StandardIntegrator::calculateForces + STSIntegrator::calculateForces:
-   energies and forces cleared, then this force alone evaluated. */
+   energies and forces cleared, then the bonded terms and this force. */
 static int md_forces(orc_md *m) {
   orc_system sys = *m->s;
   memset(m->f, 0, sizeof(double) * 3 * sys.n_atoms);
   m->energies2[0] = m->energies2[1] = 0;
+  m->bonded2[0] = m->bonded2[1] = 0;
+  md_bonded(m);
   sys.pos = m->x;
   return orc_evaluate(&sys, m->f, m->energies2, NULL, NULL, NULL, 0, NULL);
 }
@@ -799,20 +871,23 @@ static int md_forces(orc_md *m) {
  * and out); f: the forces at x on entry (initializeForces), at the final x
  * on exit; last: SHAKE's previous positions (ModifierMetaRattleShake keeps
  * them across calls; initialise them to x before the first call);
- * energies2: LJ and electrostatic energy of the last force evaluation.
+ * energies2: LJ and electrostatic energy of the last force evaluation;
+ * bonded2: its bond and angle energies (the terms in lf, may be none).
  * iters (optional): the sweep count of every SHAKE and RATTLE call in call
  * order.  Returns the number of modifier calls, or -1 on failure.
  */
 int64_t orc_leapfrog_run(const orc_system *s, const orc_lf *lf, double *x,
                          double *v, double *f, double *last, double *energies2,
-                         int32_t n_steps, int32_t *iters, int64_t iters_cap) {
+                         double *bonded2, int32_t n_steps, int32_t *iters,
+                         int64_t iters_cap) {
   orc_md m;
   double h = lf->timestep * ORC_INV_TIMEFACTOR;
   double h2 = 0.5 * lf->timestep * ORC_INV_TIMEFACTOR;
   int i;
   m.s = s; m.lf = lf; m.x = x; m.v = v; m.f = f; m.last = last;
-  m.energies2 = energies2; m.iters = iters; m.iters_cap = iters_cap;
-  m.n_calls = 0;
+  m.energies2 = energies2; m.bonded2 = bonded2; m.iters = iters;
+  m.iters_cap = iters_cap; m.n_calls = 0;
+  orc_pbc_set(&m.pbc, s->e1, s->e2, s->e3, s->origin);
   if (n_steps < 1) return 0;
   /* doHalfKickdoDrift (no pre-drift modifier): v += F*h*0.5/m, x += v*h */
   {
@@ -839,4 +914,16 @@ int64_t orc_leapfrog_run(const orc_system *s, const orc_lf *lf, double *x,
   md_kick(&m, h2);
   md_rattle(&m);
   return m.n_calls;
+}
+
+/* calculateForces alone (forces and energies cleared, then the bonded terms
+   of lf and the cutoff force at x): the forces initializeForces gives. */
+int orc_lf_forces(const orc_system *s, const orc_lf *lf, double *x, double *f,
+                  double *energies2, double *bonded2) {
+  orc_md m;
+  memset(&m, 0, sizeof m);
+  m.s = s; m.lf = lf; m.x = x; m.f = f;
+  m.energies2 = energies2; m.bonded2 = bonded2;
+  orc_pbc_set(&m.pbc, s->e1, s->e2, s->e3, s->origin);
+  return md_forces(&m);
 }

@@ -37,8 +37,8 @@ EXPORTED = (
     "pmnb_batch_autotune", "pmnb_batch_set_shape", "pmnb_batch_pair_count",
     "pmnb_batch_last_error", "pmnb_batch_destroy",
     "pmnb_set_parameters_device", "pmnb_adjoint_device",
-    "pmnb_lf_create", "pmnb_lf_run_device", "pmnb_lf_iterations",
-    "pmnb_lf_destroy",
+    "pmnb_lf_create", "pmnb_lf_run_device", "pmnb_lf_set_bonded",
+    "pmnb_lf_forces_device", "pmnb_lf_iterations", "pmnb_lf_destroy",
 )
 
 
@@ -158,6 +158,11 @@ def load():
                                        ctypes.POINTER(_i64),
                                        ctypes.POINTER(_i64)]
     lib.pmnb_lf_iterations.restype = ctypes.c_int
+    lib.pmnb_lf_set_bonded.argtypes = [_p, _i32, _p, _p, _p, _i32, _p, _p, _p,
+                                       _p, _p]
+    lib.pmnb_lf_set_bonded.restype = ctypes.c_int
+    lib.pmnb_lf_forces_device.argtypes = [_p, _p, _p, _p, _u32, _p]
+    lib.pmnb_lf_forces_device.restype = ctypes.c_int
     lib.pmnb_lf_destroy.argtypes = [_p]
     lib.pmnb_lf_destroy.restype = None
     _LIB = lib

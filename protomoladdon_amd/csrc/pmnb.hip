@@ -1696,6 +1696,9 @@ struct pmnb_leapfrog {
   bool last_ready = false;
   long long calls = 0, iters_cap = 0;
   long long scalls[3] = {0, 0, 0};  // calls per stage
+  // bonded terms evaluated with the force (pmnb_lf_set_bonded)
+  int n_bonds = 0, n_angles = 0;
+  DevBuf bptr, bent, aptr, aent, bonds, bk, br0, angles, ak, at0, ubk, ubr0;
 };
 
 namespace {
@@ -1722,6 +1725,7 @@ LfDev lf_dev(pmnb_leapfrog *lf, double *x, double *v, double *f,
   d.n_free = lf->nf;
   d.ngb = lf->ngb;
   d.n_cons = lf->n_cons;
+  d.n_out = (lf->n_bonds + lf->n_angles) > 0 ? 22 : 20;
   // LeapfrogIntegrator / StandardIntegrator use timestep * INV_TIMEFACTOR,
   // the modifiers timestep / TIMEFACTOR.
   d.h = lf->p.timestep * kInvTimeFactor;
@@ -1818,6 +1822,48 @@ int lf_flush(pmnb_leapfrog *lf, LfKernel &k, const LfDev &d, hipStream_t s) {
   }
   if (k.apply != kStNone) lf->scalls[k.apply]++;
   k = LfKernel{};
+  return PMNB_OK;
+}
+
+LfBonded lf_bonded(pmnb_leapfrog *lf) {
+  LfBonded B{};
+  B.bptr = lf->bptr.as<int>();
+  B.bent = lf->bent.as<int>();
+  B.aptr = lf->aptr.as<int>();
+  B.aent = lf->aent.as<int>();
+  B.bonds = lf->bonds.as<int>();
+  B.bk = lf->bk.as<double>();
+  B.br0 = lf->br0.as<double>();
+  B.angles = lf->angles.as<int>();
+  B.ak = lf->ak.as<double>();
+  B.at0 = lf->at0.as<double>();
+  B.ubk = lf->ubk.as<double>();
+  B.ubr0 = lf->ubr0.as<double>();
+  B.n_atoms = lf->n;
+  B.n_bonds = lf->n_bonds;
+  B.n_angles = lf->n_angles;
+  return B;
+}
+
+// calculateForces of the integrator's force group after the forces and the
+// energy block were cleared: the cutoff force, then the bonded terms (their
+// energies into out[20], out[21] when `energies`).
+int lf_calc_forces(pmnb_leapfrog *lf, const double *d_pos, double *d_force,
+                   double *d_out, uint32_t flags, bool energies,
+                   hipStream_t s) {
+  pmnb_ctx *ctx = lf->ctx;
+  int rc = run_device(ctx, d_pos, d_force, d_out, nullptr, flags, s);
+  if (rc) return rc;
+  if (lf->n_bonds + lf->n_angles == 0) return PMNB_OK;
+  const LfBonded B = lf_bonded(lf);
+  hipLaunchKernelGGL(k_lf_bonded, dim3(blocks_for(lf->n, kLfBlock)),
+                     dim3(kLfBlock), 0, s, ctx->box, d_pos, d_force, B);
+  HIPCHK(hipGetLastError());
+  if (energies) {
+    hipLaunchKernelGGL(k_lf_bonded_energy, dim3(1), dim3(kLfBlock), 0, s,
+                       ctx->box, d_pos, B, d_out + 20);
+    HIPCHK(hipGetLastError());
+  }
   return PMNB_OK;
 }
 
@@ -2024,16 +2070,16 @@ int pmnb_lf_run_device(pmnb_leapfrog *lf, double *d_positions,
     k.apply = st;
     return r;
   };
-  auto force = [&]() -> int {
+  auto force = [&](bool last) -> int {
     k.zero = true;
     int r = lf_flush(lf, k, d, s);
     if (r) return r;
-    return run_device(ctx, d_positions, d_forces, d_out, nullptr, flags, s);
+    return lf_calc_forces(lf, d_positions, d_forces, d_out, flags, last, s);
   };
   // doHalfKickdoDrift (LeapfrogIntegrator.cpp:33-58), SHAKE post-drift
   if ((rc = pre(kPreFirst))) return rc;
   if (S && (rc = stage(kStShake))) return rc;
-  if ((rc = force())) return rc;
+  if ((rc = force(n_steps == 1))) return rc;
   for (int i = 1; i < n_steps; i++) {
     // doKickdoDrift (LeapfrogIntegrator.cpp:60-83)
     if (lf->rattle) {
@@ -2044,7 +2090,7 @@ int pmnb_lf_run_device(pmnb_leapfrog *lf, double *d_positions,
       if ((rc = pre(kPreFullDrift))) return rc;
     }
     if (S && (rc = stage(kStShake))) return rc;
-    if ((rc = force())) return rc;
+    if ((rc = force(i == n_steps - 1))) return rc;
   }
   // doHalfKick and postStepModify (LeapfrogIntegrator.cpp:96-97)
   if ((rc = pre(kPreHalf))) return rc;
@@ -2083,9 +2129,97 @@ void pmnb_lf_destroy(pmnb_leapfrog *lf) {
   DevBuf *bufs[] = {&lf->mass, &lf->rmass, &lf->g_atoms, &lf->g_cons,
                     &lf->g_rest, &lf->g_rinv, &lf->free_atoms, &lf->last,
                     &lf->xs, &lf->vs, &lf->partial_s, &lf->partial_r,
-                    &lf->iters, &lf->nonconv, &lf->hint};
+                    &lf->iters, &lf->nonconv, &lf->hint, &lf->bptr,
+                    &lf->bent, &lf->aptr, &lf->aent, &lf->bonds, &lf->bk,
+                    &lf->br0, &lf->angles, &lf->ak, &lf->at0, &lf->ubk,
+                    &lf->ubr0};
   for (DevBuf *b : bufs) b->release();
   delete lf;
+}
+
+int pmnb_lf_set_bonded(pmnb_leapfrog *lf, int32_t n_bonds,
+                       const int32_t *bond_atoms, const double *bond_k,
+                       const double *bond_r0, int32_t n_angles,
+                       const int32_t *angle_atoms, const double *angle_k,
+                       const double *angle_theta0, const double *ub_k,
+                       const double *ub_r0) {
+  if (!lf) return PMNB_ERR_ARG;
+  pmnb_ctx *ctx = lf->ctx;
+  const int n = lf->n;
+  if (n_bonds < 0 || n_angles < 0 ||
+      (n_bonds > 0 && (!bond_atoms || !bond_k || !bond_r0)) ||
+      (n_angles > 0 && (!angle_atoms || !angle_k || !angle_theta0)))
+    return fail(ctx, PMNB_ERR_ARG, "bad bonded term arrays");
+  if (n_bonds > (1 << 29) || n_angles > (1 << 28))
+    return fail(ctx, PMNB_ERR_ARG, "too many bonded terms");
+  for (int i = 0; i < 2 * n_bonds; i++)
+    if (bond_atoms[i] < 0 || bond_atoms[i] >= n)
+      return fail(ctx, PMNB_ERR_ARG, "bond atom %d out of range", bond_atoms[i]);
+  for (int i = 0; i < 3 * n_angles; i++)
+    if (angle_atoms[i] < 0 || angle_atoms[i] >= n)
+      return fail(ctx, PMNB_ERR_ARG, "angle atom %d out of range", angle_atoms[i]);
+  // per-atom entry lists in term order
+  std::vector<int> bcnt(n + 1, 0), acnt(n + 1, 0);
+  for (int i = 0; i < 2 * n_bonds; i++) bcnt[bond_atoms[i] + 1]++;
+  for (int i = 0; i < 3 * n_angles; i++) acnt[angle_atoms[i] + 1]++;
+  for (int i = 0; i < n; i++) {
+    bcnt[i + 1] += bcnt[i];
+    acnt[i + 1] += acnt[i];
+  }
+  std::vector<int> bent(std::max(2 * n_bonds, 1)), aent(std::max(3 * n_angles, 1));
+  {
+    std::vector<int> fill(bcnt.begin(), bcnt.end() - 1);
+    for (int i = 0; i < n_bonds; i++)
+      for (int r = 0; r < 2; r++) bent[fill[bond_atoms[2 * i + r]]++] = (i << 1) | r;
+    std::vector<int> filla(acnt.begin(), acnt.end() - 1);
+    for (int i = 0; i < n_angles; i++)
+      for (int r = 0; r < 3; r++) aent[filla[angle_atoms[3 * i + r]]++] = (i << 2) | r;
+  }
+  std::vector<double> zeros(std::max(n_angles, 1), 0.0);
+  auto up = [&](DevBuf &b, const void *src, size_t bytes) -> hipError_t {
+    hipError_t e = b.ensure(std::max<size_t>(bytes, 8));
+    if (e == hipSuccess && bytes && src)
+      e = hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice);
+    return e;
+  };
+  HIPCHK(hipSetDevice(ctx->prm.device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(up(lf->bptr, bcnt.data(), (n + 1) * sizeof(int)));
+  HIPCHK(up(lf->aptr, acnt.data(), (n + 1) * sizeof(int)));
+  HIPCHK(up(lf->bent, bent.data(), 2 * (size_t)n_bonds * sizeof(int)));
+  HIPCHK(up(lf->aent, aent.data(), 3 * (size_t)n_angles * sizeof(int)));
+  HIPCHK(up(lf->bonds, bond_atoms, 2 * (size_t)n_bonds * sizeof(int)));
+  HIPCHK(up(lf->bk, bond_k, (size_t)n_bonds * sizeof(double)));
+  HIPCHK(up(lf->br0, bond_r0, (size_t)n_bonds * sizeof(double)));
+  HIPCHK(up(lf->angles, angle_atoms, 3 * (size_t)n_angles * sizeof(int)));
+  HIPCHK(up(lf->ak, angle_k, (size_t)n_angles * sizeof(double)));
+  HIPCHK(up(lf->at0, angle_theta0, (size_t)n_angles * sizeof(double)));
+  HIPCHK(up(lf->ubk, ub_k ? ub_k : zeros.data(), (size_t)n_angles * sizeof(double)));
+  HIPCHK(up(lf->ubr0, ub_r0 ? ub_r0 : zeros.data(), (size_t)n_angles * sizeof(double)));
+  lf->n_bonds = n_bonds;
+  lf->n_angles = n_angles;
+  return PMNB_OK;
+}
+
+int pmnb_lf_forces_device(pmnb_leapfrog *lf, const double *d_positions,
+                          double *d_forces, double *d_out, uint32_t flags,
+                          void *stream) {
+  if (!lf) return PMNB_ERR_ARG;
+  pmnb_ctx *ctx = lf->ctx;
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (lf->n > 0 && (!d_positions || !d_forces || !d_out))
+    return fail(ctx, PMNB_ERR_ARG, "null device pointer");
+  if (flags & PMNB_FLAG_MOLVIRIAL)
+    return fail(ctx, PMNB_ERR_UNSUPPORTED,
+                "the molecular virial needs molecule centres per step");
+  HIPCHK(hipSetDevice(ctx->prm.device));
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  if (lf->n == 0) return PMNB_OK;
+  launch_fill_i32(reinterpret_cast<int *>(d_forces), lf->n * 6, 0, s);
+  launch_fill_i32(reinterpret_cast<int *>(d_out),
+                  (lf->n_bonds + lf->n_angles) > 0 ? 44 : 40, 0, s);
+  return lf_calc_forces(lf, d_positions, d_forces, d_out, flags, true, s);
 }
 
 }  // extern "C"

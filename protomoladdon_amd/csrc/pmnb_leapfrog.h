@@ -92,6 +92,7 @@ struct LfDev {
   const int *free_atoms;
   int n_groups, n_free, ngb;  // ngb: blocks of group threads
   int n_cons;
+  int n_out;                // doubles of the energy block cleared with forces
   double h, h2, dtc;        // ts/TIMEFACTOR-scaled steps (see host)
   double eps[3];            // by stage
   int maxit[3];
@@ -456,7 +457,7 @@ __global__ __launch_bounds__(kLfBlock) void k_lf_step(LfDev d, long long sc_appl
       if (k > d.maxit[APPLY]) atomicAdd(&d.nonconv[APPLY], 1);
     }
   }
-  if (ZERO && blockIdx.x == 0 && threadIdx.x < 20) d.out[threadIdx.x] = 0;
+  if (ZERO && blockIdx.x == 0 && (int)threadIdx.x < d.n_out) d.out[threadIdx.x] = 0;
   if ((int)blockIdx.x >= d.ngb) {
     const int t = ((int)blockIdx.x - d.ngb) * kLfBlock + threadIdx.x;
     if (t >= d.n_free) return;
@@ -519,6 +520,138 @@ __global__ __launch_bounds__(kLfBlock) void k_lf_probe(LfDev d, long long scall)
     else g.v[a] = ld3(d.vs, g.idx[a]);
   }
   lf_probe<STAGE>(d, g, act, L, d.maxit[STAGE] + 1, false);
+}
+
+// ------------------------------------------------------------ bonded terms
+// BondSystemForce::calcBond (force/bonded/BondSystemForce.h:89-122) and
+// AngleSystemForce::calcAngle (force/bonded/AngleSystemForce.h:86-150) with
+// Urey-Bradley, evaluated per atom: a thread sums, in list order, the force
+// its atom receives from each term it belongs to (each term is recomputed by
+// each of its atoms), so no atomics are needed and the sum order is fixed.
+struct LfBonded {
+  const int *bptr, *bent;  // per atom: entries bond << 1 | role (0 atom1)
+  const int *aptr, *aent;  // per atom: entries angle << 2 | role (0, 1, 2)
+  const int *bonds;        // 2 per bond
+  const double *bk, *br0;
+  const int *angles;       // 3 per angle
+  const double *ak, *at0, *ubk, *ubr0;
+  int n_atoms, n_bonds, n_angles;
+};
+
+__device__ __forceinline__ D3 min_diff3(const Box &b, D3 c1, D3 c2) {
+  D3 r;
+  double d2;
+  min_diff_exact(b, c1.x, c1.y, c1.z, c2.x, c2.y, c2.z, r.x, r.y, r.z, d2);
+  return r;
+}
+__device__ __forceinline__ double d3_norm(D3 a) {
+  return sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+}
+
+// force on atom1 of bond i; energy k (r - r0)^2
+__device__ __forceinline__ D3 bond_f1(const Box &box, const double *x,
+                                      const LfBonded &B, int i, double &e) {
+  const int a1 = B.bonds[2 * i], a2 = B.bonds[2 * i + 1];
+  const double r0 = B.br0[i], k = B.bk[i];
+  const D3 r12 = min_diff3(box, ld3(x, a2), ld3(x, a1));
+  const double r = d3_norm(r12);
+  const double dpotdr = 2.0 * k * (r - r0);
+  e = k * (r - r0) * (r - r0);
+  const double s = -dpotdr / r;
+  return {r12.x * s, r12.y * s, r12.z * s};
+}
+
+// forces on atoms 1 and 3 of angle i; energy k (theta - theta0)^2 + UB
+__device__ __forceinline__ void angle_f13(const Box &box, const double *x,
+                                          const LfBonded &B, int i, D3 &f1,
+                                          D3 &f3, double &e) {
+  const int a1 = B.angles[3 * i], a2 = B.angles[3 * i + 1],
+            a3 = B.angles[3 * i + 2];
+  const double t0 = B.at0[i], ka = B.ak[i];
+  const double kub = B.ubk[i], rub = B.ubr0[i];
+  const D3 p1 = ld3(x, a1), p2 = ld3(x, a2), p3 = ld3(x, a3);
+  const D3 r12 = min_diff3(box, p2, p1);
+  const D3 r32 = min_diff3(box, p2, p3);
+  const D3 r13 = min_diff3(box, p3, p1);
+  const double d12 = d3_norm(r12), d32 = d3_norm(r32), d13 = d3_norm(r13);
+  const D3 c = {r12.y * r32.z - r12.z * r32.y, r12.z * r32.x - r12.x * r32.z,
+                r12.x * r32.y - r12.y * r32.x};
+  const double theta = atan2(d3_norm(c), d3_dot(r12, r32));
+  const double sn = sin(theta), cs = cos(theta);
+  const double dpot = 2.0 * ka * (theta - t0);
+  const D3 dr12 = {r12.x / d12, r12.y / d12, r12.z / d12};
+  const D3 dr32 = {r32.x / d32, r32.y / d32, r32.z / d32};
+  const D3 dr13 = {r13.x / d13, r13.y / d13, r13.z / d13};
+  const double s1 = sn * d12, s3 = sn * d32;
+  const D3 dt1 = {(dr12.x * cs - dr32.x) / s1, (dr12.y * cs - dr32.y) / s1,
+                  (dr12.z * cs - dr32.z) / s1};
+  const D3 dt3 = {(dr32.x * cs - dr12.x) / s3, (dr32.y * cs - dr12.y) / s3,
+                  (dr32.z * cs - dr12.z) / s3};
+  const double u = 2.0 * kub * (d13 - rub);
+  const D3 ub1 = {dr13.x * u, dr13.y * u, dr13.z * u};
+  f1 = {-dt1.x * dpot - ub1.x, -dt1.y * dpot - ub1.y, -dt1.z * dpot - ub1.z};
+  f3 = {-dt3.x * dpot - -ub1.x, -dt3.y * dpot - -ub1.y, -dt3.z * dpot - -ub1.z};
+  e = ka * (theta - t0) * (theta - t0) + kub * (d13 - rub) * (d13 - rub);
+}
+
+__global__ __launch_bounds__(kLfBlock) void k_lf_bonded(Box box,
+                                                        const double *x,
+                                                        double *f, LfBonded B) {
+  const int i = blockIdx.x * kLfBlock + threadIdx.x;
+  if (i >= B.n_atoms) return;
+  D3 acc = {0, 0, 0};
+  double e;
+  for (int q = B.bptr[i]; q < B.bptr[i + 1]; q++) {
+    const int ent = B.bent[q];
+    const D3 f1 = bond_f1(box, x, B, ent >> 1, e);
+    if (ent & 1) acc = {acc.x - f1.x, acc.y - f1.y, acc.z - f1.z};
+    else acc = {acc.x + f1.x, acc.y + f1.y, acc.z + f1.z};
+  }
+  for (int q = B.aptr[i]; q < B.aptr[i + 1]; q++) {
+    const int ent = B.aent[q], role = ent & 3;
+    D3 f1, f3;
+    angle_f13(box, x, B, ent >> 2, f1, f3, e);
+    const D3 fr = role == 0 ? f1
+                  : role == 2 ? f3
+                              : D3{-f1.x - f3.x, -f1.y - f3.y, -f1.z - f3.z};
+    acc = {acc.x + fr.x, acc.y + fr.y, acc.z + fr.z};
+  }
+  if (B.bptr[i + 1] > B.bptr[i] || B.aptr[i + 1] > B.aptr[i]) {
+    const D3 fi = ld3(f, i);
+    st3(f, i, D3{fi.x + acc.x, fi.y + acc.y, fi.z + acc.z});
+  }
+}
+
+// Bond and angle energies into out[0], out[1] (+=): one block, terms
+// strided over threads, a fixed-order tree across threads.
+__global__ __launch_bounds__(kLfBlock) void k_lf_bonded_energy(
+    Box box, const double *x, LfBonded B, double *out) {
+  __shared__ double red[2][kLfBlock];
+  const int t = threadIdx.x;
+  double eb = 0, ea = 0, e;
+  for (int i = t; i < B.n_bonds; i += kLfBlock) {
+    bond_f1(box, x, B, i, e);
+    eb += e;
+  }
+  for (int i = t; i < B.n_angles; i += kLfBlock) {
+    D3 f1, f3;
+    angle_f13(box, x, B, i, f1, f3, e);
+    ea += e;
+  }
+  red[0][t] = eb;
+  red[1][t] = ea;
+  __syncthreads();
+  for (int o = kLfBlock / 2; o > 0; o >>= 1) {
+    if (t < o) {
+      red[0][t] += red[0][t + o];
+      red[1][t] += red[1][t + o];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[0] += red[0][0];
+    out[1] += red[1][0];
+  }
 }
 
 }  // namespace pmnb

@@ -97,22 +97,32 @@ def kinetic_energy(masses, velocities):
 
 
 class LeapfrogIntegrator:
-    """Leapfrog (velocity Verlet split) over one NonbondedCutoffSystemForce.
+    """Leapfrog (velocity Verlet split) over one NonbondedCutoffSystemForce
+    and, optionally, harmonic bonds and angles (the "force Bond, Angle"
+    lines of a configuration).
 
     shake / rattle: (epsilon, max_iter) to attach the modifier, or None.
+    bonded: dict with bonds (B, 2), bond_k, bond_r0, angles (A, 3), angle_k,
+    angle_theta0 (radians) and optionally ub_k, ub_r0; or None.  With
+    bonded terms the energy block has 22 slots ([20] bond, [21] angle).
     """
 
     keyword = "Leapfrog"
 
-    def __init__(self, timestep, force, shake=None, rattle=None):
+    def __init__(self, timestep, force, shake=None, rattle=None, bonded=None):
         if not timestep > 0:
             raise ValueError("Leapfrog: timestep > 0")
         self.timestep = float(timestep)
         self.force = force
         self.shake = shake
         self.rattle = rattle
+        self.bonded = bonded
         self._lf = None
         self._key = None
+
+    @property
+    def out_slots(self):
+        return 22 if self.bonded is not None else 20
 
     def _params(self):
         p = _lib.PmnbLfParams()
@@ -150,6 +160,19 @@ class LeapfrogIntegrator:
         self._lf = lf
         self._key = key
         self._constraints = constraints
+        if self.bonded is not None:
+            b = self.bonded
+            arr = {k: np.ascontiguousarray(b[k], dtype=np.int32 if k in (
+                "bonds", "angles") else np.float64) for k in b}
+            self._bonded_arrays = arr
+            ub = "ub_k" in arr
+            _lib.check(lib, ctx, lib.pmnb_lf_set_bonded(
+                lf, len(arr["bond_k"]), arr["bonds"].ctypes.data,
+                arr["bond_k"].ctypes.data, arr["bond_r0"].ctypes.data,
+                len(arr["angle_k"]), arr["angles"].ctypes.data,
+                arr["angle_k"].ctypes.data, arr["angle_theta0"].ctypes.data,
+                arr["ub_k"].ctypes.data if ub else None,
+                arr["ub_r0"].ctypes.data if ub else None))
 
     def close(self):
         if self._lf is not None:
@@ -168,10 +191,11 @@ class LeapfrogIntegrator:
         d_out) at the start positions; d_forces and d_out are cleared
         first (calculateForces)."""
         self._ensure(topology, constraints)
-        d_forces.zero_()
-        d_out.zero_()
-        self.force.evaluate_device(topology, d_positions, d_forces, d_out,
-                                   flags=flags, stream=stream)
+        lib = self.force._lib
+        _lib.check(lib, self.force._ctx, lib.pmnb_lf_forces_device(
+            self._lf, d_positions.data_ptr(), d_forces.data_ptr(),
+            d_out.data_ptr(), int(flags),
+            ctypes.c_void_p(stream) if stream is not None else None))
 
     def run(self, topology, d_positions, d_velocities, d_forces, d_out,
             n_steps, constraints=None, flags=0, stream=None):

@@ -7,7 +7,11 @@ ref`).
    (data/water_423.{psf,pos.pdb}), CHARMM parameters for OT/HT
    (data/par_all27_prot_lipid.inp), the step-0 row of
    expected/water_CHARMM_PERIODIC_L.energy and the first frame of
-   expected/water_CHARMM_PERIODIC_L.forces.
+   expected/water_CHARMM_PERIODIC_L.forces; for the 200-step run of the
+   same configuration, the start velocities (data/water_423.vel.pdb, as
+   read: the reference divides them by PDBVELSCALINGFACTOR), the energy row
+   at 100 fs and the final positions and velocities
+   (expected/water_CHARMM_PERIODIC_L.{energy,pos,vel}).
 2. pentane320.npz -- inputs and the step-0 VdW energy of the feature test
    protomol-test/ConfigTests/data/EnergiesTest.conf (VelacTest.{psf,coor,par},
    EnergiesTest.energies.orig).
@@ -108,8 +112,18 @@ def water423():
     names = ["OT", "HT"]
     types = np.array([names.index(t) for t in psf["atom_type"]], dtype=np.int32)
     lj = np.array([par[n] for n in names])  # eps, rmin/2, eps14, rmin14/2
-    energy = np.array([float(v) for v in open(os.path.join(
-        WATER_DIR, "expected/water_CHARMM_PERIODIC_L.energy")).readline().split()])
+    rows = open(os.path.join(
+        WATER_DIR, "expected/water_CHARMM_PERIODIC_L.energy")).read().splitlines()
+    energy = np.array([float(v) for v in rows[0].split()])
+    energy100 = np.array([float(v) for v in rows[1].split()])
+    vel_pdb = read_pdb(os.path.join(WATER_DIR, "data/water_423.vel.pdb"))
+
+    def read_xyz(name):
+        lines = open(os.path.join(WATER_DIR, "expected", name)).read().splitlines()
+        cnt = int(lines[0].split()[0])
+        rows_ = [l.split() for l in lines[1:] if len(l.split()) == 4]
+        assert len(rows_) == cnt
+        return np.array([[float(v) for v in r[1:]] for r in rows_])
     fl = open(os.path.join(WATER_DIR, "expected/water_CHARMM_PERIODIC_L.forces")
               ).read().splitlines()
     n = int(fl[1].split()[0])
@@ -124,7 +138,12 @@ def water423():
         angle_k_theta0=np.array([55.0, 104.52]),
         box=np.array([28.0, 28.0, 28.0]), origin=np.zeros(3), cell_size=6.5,
         lj_switch=np.array([0.1, 6.5]), coulomb_c1_cutoff=6.5,
-        algorithm_cutoff=6.5, energy_row0=energy, forces_frame0=forces)
+        algorithm_cutoff=6.5, energy_row0=energy, forces_frame0=forces,
+        velocities_pdb=vel_pdb, energy_row100=energy100,
+        final_positions=read_xyz("water_CHARMM_PERIODIC_L.pos"),
+        final_velocities=read_xyz("water_CHARMM_PERIODIC_L.vel"),
+        # Leapfrog timestep 0.5 fs, numsteps 200 (water_CHARMM_PERIODIC_L.conf)
+        timestep=0.5, n_steps=200)
     print("water423.npz", pos.shape, energy[9:11])
 
 

@@ -197,3 +197,65 @@ def test_full_size_water_constraints():
         assert unc == 0 and len(its) == 20
         assert np.isfinite(out[:2]).all()
     dev.close()
+
+
+def test_bonded_forces_match_oracle():
+    """calculateForces with bonds and angles (pmnb_lf_forces_device) on the
+    423-atom water of the reference's regression case."""
+    import torch
+    from tests.test_oracle_leapfrog import water423_md
+    w, d, bonded, v0 = water423_md()
+    print("water423 bonded forces", flush=True)
+    force = w.make_force("fp64")
+    lf = LeapfrogIntegrator(0.5, force, bonded=bonded)
+    dev = torch.device("cuda:0")
+    x = torch.tensor(w.positions, dtype=torch.float64, device=dev)
+    f = torch.zeros_like(x)
+    out = torch.zeros(lf.out_slots, dtype=torch.float64, device=dev)
+    lf.initialize(w.topology, x, f, out)
+    torch.cuda.synchronize()
+    fr, er, br = O.lf_forces(O.System.from_workload(w), w.positions,
+                             w.topology.masses, bonded)
+    fd, od = f.cpu().numpy(), out.cpu().numpy()
+    assert np.abs(fd - fr).max() <= 1e-12 * np.abs(fr).max()
+    np.testing.assert_allclose(od[:2], er, rtol=1e-12)
+    np.testing.assert_allclose(od[20:22], br, rtol=1e-12)
+    lf.close()
+    force.close()
+
+
+def test_water423_200_steps_match_reference_outputs():
+    """The reference's regression test/tests/water_CHARMM_PERIODIC_L.conf on
+    the device: 200 leapfrog steps of 0.5 fs with bonds, angles, C1 Coulomb
+    and C2 LJ; final positions and velocities and the 100 fs energies
+    against its expected outputs, and against the oracle's run."""
+    import torch
+    from tests.test_oracle_leapfrog import check_water423_outputs, water423_md
+    w, d, bonded, v0 = water423_md()
+    print("water423 200 steps", flush=True)
+    force = w.make_force("fp64")
+    lf = LeapfrogIntegrator(float(d["timestep"]), force, bonded=bonded)
+    dev = torch.device("cuda:0")
+    x = torch.tensor(w.positions, dtype=torch.float64, device=dev)
+    v = torch.tensor(v0, dtype=torch.float64, device=dev)
+    f = torch.zeros_like(x)
+    out = torch.zeros(lf.out_slots, dtype=torch.float64, device=dev)
+    lf.initialize(w.topology, x, f, out)
+    lf.run(w.topology, x, v, f, out, int(d["n_steps"]))
+    torch.cuda.synchronize()
+    xd, vd, od = x.cpu().numpy(), v.cpu().numpy(), out.cpu().numpy()
+    print("max |x - expected| %.3g, max |v - expected| %.3g" % (
+        np.abs(xd - d["final_positions"]).max(),
+        np.abs(vd - d["final_velocities"]).max()))
+    check_water423_outputs(xd, vd, od[:2], od[20:22], w.topology.masses, d,
+                           tol=1e-10)
+    s = O.System.from_workload(w)
+    f0, e0, b0 = O.lf_forces(s, w.positions, w.topology.masses, bonded)
+    st = O.MDState(w.positions, v0, f0, energies=e0, bonded=b0)
+    O.leapfrog_run(s, st, int(d["n_steps"]), w.topology.masses,
+                   np.zeros((0, 2)), np.zeros(0), float(d["timestep"]),
+                   bonded=bonded)
+    assert np.abs(xd - st.x).max() <= 1e-11
+    assert np.abs(vd - st.v).max() <= 1e-10
+    lf.close()
+    force.close()

@@ -186,3 +186,59 @@ def test_energy_conservation(elec):
     spread = (max(energies) - min(energies)) / kin
     print(elec, "E", energies[0], energies[-1], "spread/KE", spread)
     assert spread < 5e-3
+
+
+PDB_VEL_SCALING = 20.45482706  # base/PMConstants.cpp:68 (PDBVELSCALINGFACTOR)
+
+
+def water423_md():
+    """The reference's regression test/tests/water_CHARMM_PERIODIC_L.conf:
+    141 flexible TIP3 waters, Leapfrog 0.5 fs x 200, forces Bond, Angle,
+    Coulomb (C1, 6.5) and LennardJones (C2 0.1 -> 6.5), periodic 28 A box."""
+    from tests import fixtures_lib as FX
+    w, d = FX.water423()
+    kb, r0 = d["bond_k_r0"]
+    ka, t0 = d["angle_k_theta0"]
+    bonded = dict(bonds=d["bonds"], bond_k=np.full(len(d["bonds"]), kb),
+                  bond_r0=np.full(len(d["bonds"]), r0), angles=d["angles"],
+                  angle_k=np.full(len(d["angles"]), ka),
+                  angle_theta0=np.full(len(d["angles"]), t0 * math.pi / 180.0))
+    v0 = d["velocities_pdb"] / PDB_VEL_SCALING
+    return w, d, bonded, v0
+
+
+def check_water423_outputs(x, v, energies, bonded_e, masses, d, tol):
+    """Final positions / velocities (expected/*.pos, *.vel) and the 100 fs
+    energy row: potential, kinetic, total, bond, angle, LJ, Coulomb."""
+    np.testing.assert_allclose(x, d["final_positions"], rtol=0, atol=tol)
+    np.testing.assert_allclose(v, d["final_velocities"], rtol=0, atol=tol)
+    row = d["energy_row100"]
+    ke = kinetic_energy(masses, v)
+    pe = bonded_e[0] + bonded_e[1] + energies[0] + energies[1]
+    # columns (output/OutputEnergies.cpp): time, potential, kinetic, total,
+    # temperature, bond, angle, dihedral, improper, LJ, Coulomb, ...
+    got = [pe, ke, ke + pe, bonded_e[0], bonded_e[1], energies[0], energies[1]]
+    want = [row[1], row[2], row[3], row[5], row[6], row[9], row[10]]
+    print("100 fs energies", got, "expected", want)
+    # the expected file prints 8 significant digits
+    np.testing.assert_allclose(got, want, rtol=2e-7, atol=2e-4)
+
+
+def test_water423_200_steps_match_reference_outputs():
+    w, d, bonded, v0 = water423_md()
+    s = O.System.from_workload(w)
+    f0, e0, b0 = O.lf_forces(s, w.positions, w.topology.masses, bonded)
+    # the step-0 row of the expected energies
+    row0 = d["energy_row0"]
+    np.testing.assert_allclose([b0[0], b0[1], e0[0], e0[1]],
+                               [row0[5], row0[6], row0[9], row0[10]],
+                               rtol=2e-7, atol=2e-4)
+    st = O.MDState(w.positions, v0, f0, energies=e0, bonded=b0)
+    O.leapfrog_run(s, st, int(d["n_steps"]), w.topology.masses,
+                   np.zeros((0, 2)), np.zeros(0), float(d["timestep"]),
+                   bonded=bonded)
+    err_x = np.abs(st.x - d["final_positions"]).max()
+    err_v = np.abs(st.v - d["final_velocities"]).max()
+    print("max |x - expected| %.3g, max |v - expected| %.3g" % (err_x, err_v))
+    check_water423_outputs(st.x, st.v, st.energies, st.bonded,
+                           w.topology.masses, d, tol=1e-10)
