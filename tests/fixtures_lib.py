@@ -102,3 +102,33 @@ def harmonic_bonded_forces(pos, bonds, angles, kb, r0, ka, theta0_deg, L):
         f[c] -= dE * dtv
         f[b] += dE * (dtu + dtv)
     return f, e_b, e_a
+
+
+DATA = os.path.join(GOLDEN, "data")
+
+
+def comparator(expected_path, new_path, epsilon=1e-5, scalar=1.0):
+    """The reference's test/comparator.py rule: the same number of fields
+    per line; fields that parse as integers are skipped; other numbers
+    must agree within an absolute epsilon (test/test.py:14 default 1e-5).
+    Returns the number of differing lines or fields."""
+    diffs = 0
+    with open(expected_path) as fe, open(new_path) as fn:
+        for le in fe:
+            a, b = le.split(), fn.readline().split()
+            if len(a) != len(b):
+                diffs += 1
+                continue
+            for x, y in zip(a, b):
+                try:
+                    int(x)
+                    int(y)
+                    continue
+                except ValueError:
+                    pass
+                try:
+                    if abs(float(x) * scalar - float(y)) > epsilon:
+                        diffs += 1
+                except ValueError:
+                    pass
+    return diffs

@@ -259,3 +259,39 @@ def test_water423_200_steps_match_reference_outputs():
     assert np.abs(vd - st.v).max() <= 1e-10
     lf.close()
     force.close()
+
+
+def test_device_run_through_io_passes_reference_comparator(tmp_path):
+    """Readers (PSF / PDB / PAR), the device leapfrog with bonds, angles and
+    the cutoff force, writers (XYZ, energies): the output files pass the
+    reference's comparator rule against its expected files."""
+    import torch
+    from protomoladdon_amd import io
+    from tests.test_io import (CONF, check_run_files, energy_row, load_system,
+                               write_outputs)
+    w, bonded, v0 = load_system()
+    t = w.topology
+    print("water423 via io, 200 steps", flush=True)
+    force = w.make_force("fp64")
+    lf = LeapfrogIntegrator(CONF["timestep"], force, bonded=bonded)
+    dev = torch.device("cuda:0")
+    x = torch.tensor(w.positions, dtype=torch.float64, device=dev)
+    v = torch.tensor(v0, dtype=torch.float64, device=dev)
+    f = torch.zeros_like(x)
+    out = torch.zeros(lf.out_slots, dtype=torch.float64, device=dev)
+    lf.initialize(t, x, f, out)
+    vol = float(np.prod(CONF["box"]))
+    o = out.cpu().numpy()
+    rows = [energy_row(0.0, t.masses, v0, o[:2], o[20:22],
+                       t.degrees_of_freedom, vol)]
+    lf.run(t, x, v, f, out, CONF["n_steps"])
+    torch.cuda.synchronize()
+    o = out.cpu().numpy()
+    vd = v.cpu().numpy()
+    rows.append(energy_row(CONF["n_steps"] * CONF["timestep"], t.masses, vd,
+                           o[:2], o[20:22], t.degrees_of_freedom, vol))
+    names = [t.type_names[k] for k in t.types]
+    write_outputs(str(tmp_path), names, x.cpu().numpy(), vd, rows)
+    check_run_files(str(tmp_path))
+    lf.close()
+    force.close()
