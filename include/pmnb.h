@@ -355,6 +355,42 @@ int pmnb_lf_iterations(pmnb_leapfrog *lf, int64_t capacity, int32_t *iters,
    its context (which must outlive every other pmnb_lf_* call). */
 void pmnb_lf_destroy(pmnb_leapfrog *lf);
 
+/* ------------------------------------------------------- Ewald k-space
+   The reciprocal-space and correction terms of
+   NonbondedFullEwaldSystemForce<PeriodicBoundaryConditions, ...,
+   real = false, reciprocal = true, correction = true>
+   (force/nonbonded/NonbondedFullEwaldSystemForce.h:324-398): the
+   reciprocal sum over the k-vectors |k| < kc of the h >= 0 half space
+   (:884-1086), the intramolecular correction over the exclusion table
+   (:1092-1130), the point self energy and the charged-system energy
+   (:680-700), all added to the electrostatic energy.  With the cutoff
+   force's Ewald real-space functor it completes the Ewald sum.
+   alpha <= 0 selects the reference's default alpha (:603-604); kc =
+   2 alpha sqrt(-ln accuracy).  The box is the context's (orthogonal).
+   scaled_charge, coulomb_scaling_factor and the exclusion rows are those
+   given to pmnb_set_topology.  The work is atoms x k-vectors, as in the
+   reference. */
+typedef struct pmnb_ewald pmnb_ewald;
+
+int pmnb_ewald_create(pmnb_ctx *ctx, double alpha, double accuracy,
+                      int32_t n_atoms, const double *scaled_charge,
+                      double coulomb_scaling_factor, int32_t n_excl,
+                      const int32_t *excl, pmnb_ewald **out);
+
+/* alpha, real-space cutoff sqrt(-ln accuracy)/alpha, kc and the number of
+   k-vectors (pointers may be NULL). */
+int pmnb_ewald_info(const pmnb_ewald *ew, double *alpha, double *rc,
+                    double *kc, int32_t *n_kvectors);
+
+/* Enqueued on `stream` (NULL = the context's stream): forces += ,
+   d_out[1] (electrostatic energy) += , d_out[2..10] (atomic virial) +=
+   with PMNB_FLAG_VIRIAL.  PMNB_FLAG_MOLVIRIAL is not supported. */
+int pmnb_ewald_evaluate_device(pmnb_ewald *ew, const double *d_positions,
+                               double *d_forces, double *d_out,
+                               uint32_t flags, void *stream);
+
+void pmnb_ewald_destroy(pmnb_ewald *ew);
+
 #ifdef __cplusplus
 }
 #endif

@@ -413,3 +413,25 @@ def leapfrog_run(system, state, n_steps, masses, constraints, rest, timestep,
     if n < 0:
         raise RuntimeError("oracle leapfrog failed")
     return iters[:min(n, max_calls)].copy()
+
+
+# ------------------------------------------------------------------ Ewald
+def ewald_kspace(system, alpha, accuracy, virial=True):
+    """Reciprocal + correction + self + charged-system terms of
+    NonbondedFullEwaldSystemForce: returns (forces, electrostatic energy,
+    virial[9], info) with info = (alpha, rc, kc, number of k-vectors)."""
+    lib = load()
+    if not hasattr(lib, "_ew_ready"):
+        lib.orc_ewald_eval.argtypes = [ctypes.POINTER(OrcSystem), ctypes.c_double,
+                                  ctypes.c_double] + [ctypes.c_void_p] * 4
+        lib._ew_ready = True
+    st = system.struct()
+    f = np.zeros((system.n, 3))
+    e = np.zeros(2)
+    vir = np.zeros(9)
+    info = np.zeros(4)
+    if lib.orc_ewald_eval(ctypes.byref(st), float(alpha), float(accuracy),
+                     f.ctypes.data, e.ctypes.data,
+                     vir.ctypes.data if virial else None, info.ctypes.data):
+        raise RuntimeError("oracle Ewald evaluation failed")
+    return f, e[1], vir, info

@@ -927,3 +927,223 @@ int orc_lf_forces(const orc_system *s, const orc_lf *lf, double *x, double *f,
   orc_pbc_set(&m.pbc, s->e1, s->e2, s->e3, s->origin);
   return md_forces(&m);
 }
+
+/* ------------------------------------------------------- Ewald k-space
+ * Reciprocal and correction terms of NonbondedFullEwaldSystemForce
+ * (force/nonbonded/NonbondedFullEwaldSystemForce.h) for an orthogonal
+ * periodic box, restated in its operation order:
+ *   :597-652   alpha default, rc, kc, k-vector list (h >= 0 half space)
+ *   :680-700   point self energy, charged system energy
+ *   :884-1086  reciprocal term with the sin/cos tables built by the
+ *              addition theorem, energy, atomic virial, forces
+ *   :1092-1130 intramolecular correction over the exclusion table
+ * All terms go into the electrostatic energy, as evaluate does (:376-384).
+ */
+typedef struct {
+  double alpha, alpha2, alpha2r, two_alpha_pi, rc, kc, fac, V, Vr;
+  double L[3], Lr[3];
+  int hmax, kmax, lmax, hklmax, nk;
+  int *hkl;         /* 3 per k-vector */
+  double *k, *k2;   /* 3 per k-vector, |k|^2 */
+  double self_energy, charged_energy;
+} orc_ewald;
+
+/* alpha <= 0 selects the default alpha of initialize (:603-604). */
+int orc_ewald_init(orc_ewald *ew, const orc_system *s, double alpha,
+                   double accuracy) {
+  orc_pbc p;
+  double t_rate = 5.5, pp, kc2, q, q2;
+  int h, k, l, i, cap;
+  orc_pbc_set(&p, s->e1, s->e2, s->e3, s->origin);
+  memset(ew, 0, sizeof(*ew));
+  ew->L[0] = s->e1[0]; ew->L[1] = s->e2[1]; ew->L[2] = s->e3[2];
+  ew->V = p.V;
+  ew->Lr[0] = p.e1r[0]; ew->Lr[1] = p.e2r[1]; ew->Lr[2] = p.e3r[2];
+  ew->Vr = 1.0 / ew->V;
+  ew->alpha = alpha > 0 ? alpha
+                        : sqrt(ORC_PI) * pow(t_rate * s->n_atoms / (ew->V * ew->V),
+                                             1.0 / 6.0);
+  ew->alpha2 = ew->alpha * ew->alpha;
+  ew->alpha2r = 1.0 / ew->alpha2;
+  ew->two_alpha_pi = 2.0 * ew->alpha / sqrt(ORC_PI);
+  pp = -log(accuracy);
+  ew->rc = sqrt(pp) / ew->alpha;
+  ew->kc = 2.0 * ew->alpha * sqrt(pp);
+  ew->fac = 1.0 / (4.0 * ew->alpha * ew->alpha);
+  ew->hmax = (int)floor(ew->kc / (2.0 * ORC_PI) * ew->L[0]);
+  ew->kmax = (int)floor(ew->kc / (2.0 * ORC_PI) * ew->L[1]);
+  ew->lmax = (int)floor(ew->kc / (2.0 * ORC_PI) * ew->L[2]);
+  ew->hklmax = ew->hmax > ew->kmax ? ew->hmax : ew->kmax;
+  if (ew->lmax > ew->hklmax) ew->hklmax = ew->lmax;
+  ew->hklmax += 1;
+  if (ew->hklmax < 2) ew->hklmax = 2;
+  cap = (ew->hmax + 1) * (2 * ew->kmax + 1) * (2 * ew->lmax + 1);
+  ew->hkl = (int *)malloc(sizeof(int) * 3 * (cap > 0 ? cap : 1));
+  ew->k = (double *)malloc(sizeof(double) * 3 * (cap > 0 ? cap : 1));
+  ew->k2 = (double *)malloc(sizeof(double) * (cap > 0 ? cap : 1));
+  if (!ew->hkl || !ew->k || !ew->k2) return -1;
+  kc2 = ew->kc * ew->kc;
+  for (h = 0; h <= ew->hmax; h++) {
+    double kx = 2.0 * ORC_PI * h * ew->Lr[0];
+    for (k = (h == 0 ? 0 : -ew->kmax); k <= ew->kmax; k++) {
+      double ky = 2.0 * ORC_PI * k * ew->Lr[1];
+      for (l = (h == 0 && k == 0 ? 1 : -ew->lmax); l <= ew->lmax; l++) {
+        double kz = 2.0 * ORC_PI * l * ew->Lr[2];
+        if (kx * kx + ky * ky + kz * kz < kc2) {
+          int n = ew->nk++;
+          ew->hkl[3 * n] = h; ew->hkl[3 * n + 1] = k; ew->hkl[3 * n + 2] = l;
+          ew->k[3 * n] = kx; ew->k[3 * n + 1] = ky; ew->k[3 * n + 2] = kz;
+          ew->k2[n] = kx * kx + ky * ky + kz * kz;
+        }
+      }
+    }
+  }
+  q2 = 0.0;
+  for (i = 0; i < s->n_atoms; i++) q2 += s->q[i] * s->q[i];
+  ew->self_energy = -q2 * ew->alpha / sqrt(ORC_PI);
+  q = 0.0;
+  for (i = 0; i < s->n_atoms; i++) q += s->q[i];
+  ew->charged_energy = 0.0;
+  if (fabs(q * 0.00268283) > 1.0e-5)
+    ew->charged_energy = -ORC_PI / (2.0 * ew->V * ew->alpha2) * q * q;
+  return 0;
+}
+
+void orc_ewald_free(orc_ewald *ew) {
+  free(ew->hkl); free(ew->k); free(ew->k2);
+}
+
+/* forces +=, energies2[1] += reciprocal + correction + self + charged,
+   vir9 += atomic virial (when not NULL). */
+int orc_ewald_kspace(const orc_system *s, const orc_ewald *ew, double *forces,
+                     double *energies2, double *vir9) {
+  orc_pbc p;
+  int N = s->n_atoms, H = ew->hklmax, i, n, j;
+  double *tab, *sca, *last, energy = 0.0, intra = 0.0, c;
+  double vxx = 0, vxy = 0, vxz = 0, vyy = 0, vyz = 0, vzz = 0;
+  int lastH = 2147483647, lastK = 2147483647;
+  orc_pbc_set(&p, s->e1, s->e2, s->e3, s->origin);
+  tab = (double *)malloc(sizeof(double) * 3 * 2 * (size_t)H * (N ? N : 1));
+  sca = (double *)malloc(sizeof(double) * 2 * (N ? N : 1));
+  last = (double *)malloc(sizeof(double) * 2 * (N ? N : 1));
+  if (!tab || !sca || !last) return -1;
+  /* tables: tab[((j*H + m)*2 + sc)*3 + dim], sc 0 sin, 1 cos (:887-934) */
+#define TAB(j, m, sc, d) tab[((((size_t)(j) * H + (m)) * 2 + (sc)) * 3) + (d)]
+  for (j = 0; j < N; j++) {
+    double r[3], x[3], sn[3], cs[3], qi = s->q[j];
+    int d, m;
+    orc_pbc_min_pos(&p, s->pos + 3 * j, r);
+    for (d = 0; d < 3; d++) {
+      x[d] = r[d] * 2.0 * ORC_PI * ew->Lr[d];
+      sn[d] = sin(x[d]);
+      cs[d] = cos(x[d]);
+    }
+    for (d = 0; d < 3; d++) {
+      TAB(j, 0, 0, d) = 0.0;
+      TAB(j, 0, 1, d) = d == 0 ? qi * 1.0 : 1.0;
+      TAB(j, 1, 0, d) = d == 0 ? qi * sn[d] : sn[d];
+      TAB(j, 1, 1, d) = d == 0 ? qi * cs[d] : cs[d];
+    }
+    for (m = 2; m < H; m++)
+      for (d = 0; d < 3; d++) {
+        TAB(j, m, 0, d) = sn[d] * TAB(j, m - 1, 1, d) + cs[d] * TAB(j, m - 1, 0, d);
+        TAB(j, m, 1, d) = cs[d] * TAB(j, m - 1, 1, d) - sn[d] * TAB(j, m - 1, 0, d);
+      }
+  }
+  for (n = 0; n < ew->nk; n++) {
+    const double *kv = ew->k + 3 * n;
+    double sumSin = 0.0, sumCos = 0.0, b, e, a;
+    int ih = ew->hkl[3 * n], ik = ew->hkl[3 * n + 1], il = ew->hkl[3 * n + 2];
+    int ika = ik < 0 ? -ik : ik, iks = ik < 0 ? -1 : 1;
+    int ila = il < 0 ? -il : il, ils = il < 0 ? -1 : 1;
+    if (ih != lastH || ik != lastK) {
+      for (i = 0; i < N; i++) {
+        double xs = TAB(i, ih, 0, 0), xc = TAB(i, ih, 1, 0);
+        double ys = iks * TAB(i, ika, 0, 1), yc = TAB(i, ika, 1, 1);
+        last[2 * i] = xs * yc + xc * ys;
+        last[2 * i + 1] = xc * yc - xs * ys;
+      }
+      lastH = ih; lastK = ik;
+    }
+    for (i = 0; i < N; i++) {
+      double xys = last[2 * i], xyc = last[2 * i + 1];
+      double zs = ils * TAB(i, ila, 0, 2), zc = TAB(i, ila, 1, 2);
+      double sA = xys * zc + xyc * zs, cA = xyc * zc - xys * zs;
+      sca[2 * i] = sA;
+      sca[2 * i + 1] = cA;
+      sumSin += sA;
+      sumCos += cA;
+    }
+    b = 1.0 / ew->k2[n] * exp(-ew->k2[n] * ew->alpha2r / 4.0);
+    e = b * (sumSin * sumSin + sumCos * sumCos);
+    energy += e;
+    if (vir9) {
+      double cc = 2.0 * (1.0 / ew->k2[n] + ew->fac);
+      vxx += e * (1.0 - cc * kv[0] * kv[0]);
+      vxy -= e * cc * kv[0] * kv[1];
+      vxz -= e * cc * kv[0] * kv[2];
+      vyy += e * (1.0 - cc * kv[1] * kv[1]);
+      vyz -= e * cc * kv[1] * kv[2];
+      vzz += e * (1.0 - cc * kv[2] * kv[2]);
+    }
+    a = 8.0 * ORC_PI * ew->Vr * b;
+    for (i = 0; i < N; i++) {
+      double w = a * (sca[2 * i] * sumCos - sca[2 * i + 1] * sumSin);
+      forces[3 * i] += kv[0] * w;
+      forces[3 * i + 1] += kv[1] * w;
+      forces[3 * i + 2] += kv[2] * w;
+    }
+  }
+#undef TAB
+  c = 4.0 * ORC_PI * ew->Vr;
+  if (vir9) {
+    vir9[0] += c * vxx; vir9[1] += c * vxy; vir9[2] += c * vxz;
+    vir9[3] += c * vxy; vir9[4] += c * vyy; vir9[5] += c * vyz;
+    vir9[6] += c * vxz; vir9[7] += c * vyz; vir9[8] += c * vzz;
+    vir9[0] += ew->charged_energy;
+    vir9[4] += ew->charged_energy;
+    vir9[8] += ew->charged_energy;
+  }
+  /* intramolecular correction over the exclusion table rows (:1105-1129) */
+  for (i = 0; i < s->n_excl; i++) {
+    int a1 = s->excl[3 * i], a2 = s->excl[3 * i + 1], cls = s->excl[3 * i + 2];
+    double rij[3], r2, qq, r, rr, e, g;
+    orc_pbc_min_diff(&p, s->pos + 3 * a1, s->pos + 3 * a2, rij, &r2);
+    qq = s->q[a1] * s->q[a2];
+    if (cls == ORC_MODIFIED) qq *= 1 - s->csf;
+    r = sqrt(r2);
+    rr = 1 / r;
+    e = erf(ew->alpha * r) * rr;
+    intra -= qq * e;
+    g = qq * (ew->two_alpha_pi * exp(-ew->alpha2 * r2) - e) * rr * rr;
+    for (n = 0; n < 3; n++) {
+      forces[3 * a1 + n] -= rij[n] * g;
+      forces[3 * a2 + n] += rij[n] * g;
+    }
+    if (vir9) { /* ScalarStructure::addVirial(fij, rij) */
+      double f0 = rij[0] * g, f1 = rij[1] * g, f2 = rij[2] * g;
+      double xy = f0 * rij[1], xz = f0 * rij[2], yz = f1 * rij[2];
+      vir9[0] += f0 * rij[0]; vir9[1] += xy; vir9[2] += xz;
+      vir9[3] += xy; vir9[4] += f1 * rij[1]; vir9[5] += yz;
+      vir9[6] += xz; vir9[7] += yz; vir9[8] += f2 * rij[2];
+    }
+  }
+  energies2[1] += c * energy + intra + ew->self_energy + ew->charged_energy;
+  free(tab); free(sca); free(last);
+  return 0;
+}
+
+/* One k-space evaluation: info4 = (alpha, rc, kc, number of k-vectors). */
+int orc_ewald_eval(const orc_system *s, double alpha, double accuracy,
+                   double *forces, double *energies2, double *vir9,
+                   double *info4) {
+  orc_ewald ew;
+  int rc;
+  if (orc_ewald_init(&ew, s, alpha, accuracy)) return -1;
+  rc = orc_ewald_kspace(s, &ew, forces, energies2, vir9);
+  if (info4) {
+    info4[0] = ew.alpha; info4[1] = ew.rc; info4[2] = ew.kc; info4[3] = ew.nk;
+  }
+  orc_ewald_free(&ew);
+  return rc;
+}

@@ -39,6 +39,8 @@ EXPORTED = (
     "pmnb_set_parameters_device", "pmnb_adjoint_device",
     "pmnb_lf_create", "pmnb_lf_run_device", "pmnb_lf_set_bonded",
     "pmnb_lf_forces_device", "pmnb_lf_iterations", "pmnb_lf_destroy",
+    "pmnb_ewald_create", "pmnb_ewald_info", "pmnb_ewald_evaluate_device",
+    "pmnb_ewald_destroy",
 )
 
 
@@ -163,6 +165,15 @@ def load():
     lib.pmnb_lf_set_bonded.restype = ctypes.c_int
     lib.pmnb_lf_forces_device.argtypes = [_p, _p, _p, _p, _u32, _p]
     lib.pmnb_lf_forces_device.restype = ctypes.c_int
+    lib.pmnb_ewald_create.argtypes = [_p, _dbl, _dbl, _i32, _p, _dbl, _i32, _p,
+                                      ctypes.POINTER(_p)]
+    lib.pmnb_ewald_create.restype = ctypes.c_int
+    lib.pmnb_ewald_info.argtypes = [_p, _p, _p, _p, _p]
+    lib.pmnb_ewald_info.restype = ctypes.c_int
+    lib.pmnb_ewald_evaluate_device.argtypes = [_p, _p, _p, _p, _u32, _p]
+    lib.pmnb_ewald_evaluate_device.restype = ctypes.c_int
+    lib.pmnb_ewald_destroy.argtypes = [_p]
+    lib.pmnb_ewald_destroy.restype = None
     lib.pmnb_lf_destroy.argtypes = [_p]
     lib.pmnb_lf_destroy.restype = None
     _LIB = lib

@@ -25,6 +25,7 @@
 #include "pmnb_adjoint.h"
 #include "pmnb_tilemask.h"
 #include "pmnb_leapfrog.h"
+#include "pmnb_ewald.h"
 
 using namespace pmnb;
 
@@ -2220,6 +2221,197 @@ int pmnb_lf_forces_device(pmnb_leapfrog *lf, const double *d_positions,
   launch_fill_i32(reinterpret_cast<int *>(d_out),
                   (lf->n_bonds + lf->n_angles) > 0 ? 44 : 40, 0, s);
   return lf_calc_forces(lf, d_positions, d_forces, d_out, flags, true, s);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ Ewald
+struct pmnb_ewald {
+  pmnb_ctx *ctx = nullptr;
+  int device = 0, n = 0, nk = 0, H = 0, n_excl = 0;
+  double alpha = 0, alpha2 = 0, rc = 0, kc = 0, fac = 0, V = 0, Vr = 0;
+  double csf = 1.0, e_self = 0, e_charged = 0;
+  DevBuf kv, hkl, q, tab, sk, xptr, xent, xrows;
+};
+
+extern "C" {
+
+int pmnb_ewald_create(pmnb_ctx *ctx, double alpha, double accuracy,
+                      int32_t n_atoms, const double *scaled_charge,
+                      double coulomb_scaling_factor, int32_t n_excl,
+                      const int32_t *excl, pmnb_ewald **out) {
+  if (!ctx || !out) return PMNB_ERR_ARG;
+  *out = nullptr;
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (n_atoms != ctx->n)
+    return fail(ctx, PMNB_ERR_ARG, "n_atoms %d differs from the topology's %d",
+                n_atoms, ctx->n);
+  if (n_atoms > 0 && !scaled_charge)
+    return fail(ctx, PMNB_ERR_ARG, "null charges");
+  if (!(accuracy > 0 && accuracy < 1))
+    return fail(ctx, PMNB_ERR_ARG, "accuracy must lie in (0, 1)");
+  if (n_excl < 0 || (n_excl > 0 && !excl))
+    return fail(ctx, PMNB_ERR_ARG, "bad exclusion table");
+  for (int i = 0; i < n_excl; i++)
+    for (int k = 0; k < 2; k++)
+      if (excl[3 * i + k] < 0 || excl[3 * i + k] >= n_atoms)
+        return fail(ctx, PMNB_ERR_ARG, "exclusion row %d out of range", i);
+  const Box &b = ctx->box;
+  pmnb_ewald *ew = new pmnb_ewald;
+  ew->ctx = ctx;
+  ew->device = ctx->prm.device;
+  ew->n = n_atoms;
+  ew->n_excl = n_excl;
+  ew->csf = coulomb_scaling_factor;
+  // NonbondedFullEwaldSystemForce::initialize (:597-700), orthogonal box
+  const double L[3] = {b.e[0], b.e[1], b.e[2]};
+  ew->V = std::fabs((L[0] * L[1]) * L[2]);
+  ew->Vr = 1.0 / ew->V;
+  ew->alpha = alpha > 0 ? alpha
+                        : std::sqrt(M_PI) * std::pow(5.5 * n_atoms / (ew->V * ew->V),
+                                                     1.0 / 6.0);
+  ew->alpha2 = ew->alpha * ew->alpha;
+  const double alpha2r = 1.0 / ew->alpha2;
+  const double p = -std::log(accuracy);
+  ew->rc = std::sqrt(p) / ew->alpha;
+  ew->kc = 2.0 * ew->alpha * std::sqrt(p);
+  ew->fac = 1.0 / (4.0 * ew->alpha * ew->alpha);
+  const int hmax = (int)std::floor(ew->kc / (2.0 * M_PI) * L[0]);
+  const int kmax = (int)std::floor(ew->kc / (2.0 * M_PI) * L[1]);
+  const int lmax = (int)std::floor(ew->kc / (2.0 * M_PI) * L[2]);
+  ew->H = std::max(2, std::max(hmax, std::max(kmax, lmax)) + 1);
+  std::vector<double4> kv;
+  std::vector<int4> hkl;
+  const double kc2 = ew->kc * ew->kc;
+  for (int h = 0; h <= hmax; h++) {
+    const double kx = 2.0 * M_PI * h * b.er[0];
+    for (int k = (h == 0 ? 0 : -kmax); k <= kmax; k++) {
+      const double ky = 2.0 * M_PI * k * b.er[1];
+      for (int l = (h == 0 && k == 0 ? 1 : -lmax); l <= lmax; l++) {
+        const double kz = 2.0 * M_PI * l * b.er[2];
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        if (k2 < kc2) {
+          const double bk = 1.0 / k2 * std::exp(-k2 * alpha2r / 4.0);
+          kv.push_back(make_double4(kx, ky, kz, bk));
+          hkl.push_back(make_int4(h, k, l, 0));
+        }
+      }
+    }
+  }
+  ew->nk = (int)kv.size();
+  double q2 = 0.0, qs = 0.0;
+  for (int i = 0; i < n_atoms; i++) q2 += scaled_charge[i] * scaled_charge[i];
+  for (int i = 0; i < n_atoms; i++) qs += scaled_charge[i];
+  ew->e_self = -q2 * ew->alpha / std::sqrt(M_PI);
+  ew->e_charged = 0.0;
+  if (std::fabs(qs * 0.00268283) > 1.0e-5)
+    ew->e_charged = -M_PI / (2.0 * ew->V * ew->alpha2) * qs * qs;
+  // per-atom exclusion rows in row order
+  std::vector<int> cnt(n_atoms + 1, 0);
+  for (int i = 0; i < n_excl; i++) {
+    cnt[excl[3 * i] + 1]++;
+    cnt[excl[3 * i + 1] + 1]++;
+  }
+  for (int i = 0; i < n_atoms; i++) cnt[i + 1] += cnt[i];
+  std::vector<int> ent(std::max(2 * n_excl, 1)), fill(cnt.begin(), cnt.end() - 1);
+  for (int i = 0; i < n_excl; i++) {
+    ent[fill[excl[3 * i]]++] = i << 1;
+    ent[fill[excl[3 * i + 1]]++] = (i << 1) | 1;
+  }
+  auto up = [&](DevBuf &d, const void *src, size_t bytes) -> hipError_t {
+    hipError_t e = d.ensure(std::max<size_t>(bytes, 16));
+    if (e == hipSuccess && bytes && src)
+      e = hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice);
+    return e;
+  };
+  hipError_t e = hipSetDevice(ew->device);
+  if (e == hipSuccess) e = up(ew->kv, kv.data(), kv.size() * sizeof(double4));
+  if (e == hipSuccess) e = up(ew->hkl, hkl.data(), hkl.size() * sizeof(int4));
+  if (e == hipSuccess) e = up(ew->q, scaled_charge, (size_t)n_atoms * sizeof(double));
+  if (e == hipSuccess)
+    e = ew->tab.ensure((size_t)3 * ew->H * 2 * std::max(n_atoms, 1) * sizeof(double));
+  if (e == hipSuccess) e = ew->sk.ensure((size_t)std::max(ew->nk, 1) * sizeof(double2));
+  if (e == hipSuccess) e = up(ew->xptr, cnt.data(), (n_atoms + 1) * sizeof(int));
+  if (e == hipSuccess) e = up(ew->xent, ent.data(), 2 * (size_t)n_excl * sizeof(int));
+  if (e == hipSuccess) e = up(ew->xrows, excl, 3 * (size_t)n_excl * sizeof(int));
+  if (e != hipSuccess) {
+    pmnb_ewald_destroy(ew);
+    return fail(ctx, PMNB_ERR_HIP, "Ewald set-up failed: %s", hipGetErrorString(e));
+  }
+  *out = ew;
+  return PMNB_OK;
+}
+
+int pmnb_ewald_info(const pmnb_ewald *ew, double *alpha, double *rc,
+                    double *kc, int32_t *n_kvectors) {
+  if (!ew) return PMNB_ERR_ARG;
+  if (alpha) *alpha = ew->alpha;
+  if (rc) *rc = ew->rc;
+  if (kc) *kc = ew->kc;
+  if (n_kvectors) *n_kvectors = ew->nk;
+  return PMNB_OK;
+}
+
+int pmnb_ewald_evaluate_device(pmnb_ewald *ew, const double *d_positions,
+                               double *d_forces, double *d_out, uint32_t flags,
+                               void *stream) {
+  if (!ew) return PMNB_ERR_ARG;
+  pmnb_ctx *ctx = ew->ctx;
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (ctx->n != ew->n)
+    return fail(ctx, PMNB_ERR_STATE, "topology changed since pmnb_ewald_create");
+  if (flags & PMNB_FLAG_MOLVIRIAL)
+    return fail(ctx, PMNB_ERR_UNSUPPORTED,
+                "the molecular virial of the k-space term is not built");
+  if (ew->n > 0 && (!d_positions || !d_forces || !d_out))
+    return fail(ctx, PMNB_ERR_ARG, "null device pointer");
+  HIPCHK(hipSetDevice(ew->device));
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  if (ew->n == 0) return PMNB_OK;
+  EwDev d{};
+  d.box = ctx->box;
+  d.n = ew->n;
+  d.nk = ew->nk;
+  d.H = ew->H;
+  d.kv = ew->kv.as<double4>();
+  d.hkl = ew->hkl.as<int4>();
+  d.q = ew->q.as<double>();
+  d.tab = ew->tab.as<double>();
+  d.sk = ew->sk.as<double2>();
+  d.xptr = ew->xptr.as<int>();
+  d.xent = ew->xent.as<int>();
+  d.xrows = ew->xrows.as<int>();
+  d.n_excl = ew->n_excl;
+  d.alpha = ew->alpha;
+  d.alpha2 = ew->alpha2;
+  d.two_alpha_pi = 2.0 * ew->alpha / std::sqrt(M_PI);
+  d.csf = ew->csf;
+  d.Vr = ew->Vr;
+  d.fac = ew->fac;
+  const int nb = blocks_for(ew->n, kEwBlock);
+  hipLaunchKernelGGL(k_ew_tables, dim3(nb), dim3(kEwBlock), 0, s, d, d_positions);
+  if (ew->nk > 0)
+    hipLaunchKernelGGL(k_ew_sk, dim3(blocks_for(ew->nk, kEwBlock / 64)),
+                       dim3(kEwBlock), 0, s, d);
+  hipLaunchKernelGGL(k_ew_force, dim3(nb), dim3(kEwBlock), 0, s, d, d_positions,
+                     d_forces);
+  hipLaunchKernelGGL(k_ew_finish, dim3(1), dim3(kEwBlock), 0, s, d, d_positions,
+                     d_out, (int)((flags & PMNB_FLAG_VIRIAL) != 0),
+                     ew->e_self + ew->e_charged, ew->e_charged);
+  HIPCHK(hipGetLastError());
+  return PMNB_OK;
+}
+
+void pmnb_ewald_destroy(pmnb_ewald *ew) {
+  if (!ew) return;
+  (void)hipSetDevice(ew->device);
+  (void)hipDeviceSynchronize();
+  DevBuf *bufs[] = {&ew->kv, &ew->hkl, &ew->q, &ew->tab, &ew->sk,
+                    &ew->xptr, &ew->xent, &ew->xrows};
+  for (DevBuf *b : bufs) b->release();
+  delete ew;
 }
 
 }  // extern "C"

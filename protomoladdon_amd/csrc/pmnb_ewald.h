@@ -1,0 +1,242 @@
+// pmnb_ewald.h -- reciprocal-space and correction terms of ProtoMol's
+// NonbondedFullEwaldSystemForce for an orthogonal periodic box.
+//
+// Reference (force/nonbonded/NonbondedFullEwaldSystemForce.h):
+//   :597-652   alpha, rc, kc and the k-vector list (h >= 0 half space),
+//              built on the host in the same order
+//   :680-700   point self energy and charged-system energy (host)
+//   :884-1086  reciprocal term: per-atom sin/cos tables by the addition
+//              theorem, structure factor, energy, atomic virial, forces
+//   :1092-1130 intramolecular correction over the exclusion table
+//
+// Kernels: k_ew_tables (one thread per atom: the sin/cos tables of the
+// three axes, charge folded into x as the reference does, stored
+// [axis][order][sin|cos][atom] so a wave reads consecutive atoms);
+// k_ew_sk (one wave per k-vector: the structure factor, lanes over atoms,
+// a fixed shuffle tree); k_ew_force (one thread per atom: the force from
+// every k-vector, recomputing exp(i k.r) from the tables, plus the
+// correction of its excluded pairs, each pair evaluated by both atoms so no
+// atomics are needed); k_ew_finish (one block: energy and virial sums in a
+// fixed order).  Work is N x K (atoms x k-vectors), as in the reference.
+#pragma once
+
+namespace pmnb {
+
+constexpr int kEwBlock = 256;
+
+struct EwDev {
+  Box box;
+  int n, nk, H;               // atoms, k-vectors, table orders
+  const double4 *kv;          // kx, ky, kz, b = exp(-k^2/(4 alpha^2))/k^2
+  const int4 *hkl;            // h, k, l
+  const double *q;            // scaled charges
+  double *tab;                // 3 * H * 2 * n
+  double2 *sk;                // per k: (sum sin, sum cos)
+  const int *xptr, *xent;     // per atom: exclusion rows (row << 1 | role)
+  const int *xrows;           // 3 per row: a1, a2, class
+  int n_excl;
+  double alpha, alpha2, two_alpha_pi, csf, Vr, fac;
+};
+
+__device__ __forceinline__ double &ew_tab(const EwDev &d, int axis, int m,
+                                          int sc, int i) {
+  return d.tab[((size_t)(axis * d.H + m) * 2 + sc) * d.n + i];
+}
+
+__global__ __launch_bounds__(kEwBlock) void k_ew_tables(EwDev d,
+                                                        const double *x) {
+  const int i = blockIdx.x * kEwBlock + threadIdx.x;
+  if (i >= d.n) return;
+  double r[3];
+  min_pos_exact(d.box, x[3 * i], x[3 * i + 1], x[3 * i + 2], r[0], r[1], r[2]);
+  const double qi = d.q[i];
+  for (int a = 0; a < 3; a++) {
+    const double xa = r[a] * 2.0 * M_PI * d.box.er[a];
+    const double sn = sin(xa), cs = cos(xa);
+    double s0 = 0.0, c0 = a == 0 ? qi * 1.0 : 1.0;
+    double s1 = a == 0 ? qi * sn : sn, c1 = a == 0 ? qi * cs : cs;
+    ew_tab(d, a, 0, 0, i) = s0;
+    ew_tab(d, a, 0, 1, i) = c0;
+    ew_tab(d, a, 1, 0, i) = s1;
+    ew_tab(d, a, 1, 1, i) = c1;
+    for (int m = 2; m < d.H; m++) {
+      const double s2 = sn * c1 + cs * s1;
+      const double c2 = cs * c1 - sn * s1;
+      ew_tab(d, a, m, 0, i) = s2;
+      ew_tab(d, a, m, 1, i) = c2;
+      s1 = s2;
+      c1 = c2;
+    }
+  }
+}
+
+// q_i sin(k.r_i), q_i cos(k.r_i) from the tables, in the reference's order
+__device__ __forceinline__ void ew_phase(const EwDev &d, int4 t, int i,
+                                         double &sA, double &cA) {
+  const int ka = t.y < 0 ? -t.y : t.y, la = t.z < 0 ? -t.z : t.z;
+  const double xs = ew_tab(d, 0, t.x, 0, i), xc = ew_tab(d, 0, t.x, 1, i);
+  const double ys = (t.y < 0 ? -1 : 1) * ew_tab(d, 1, ka, 0, i);
+  const double yc = ew_tab(d, 1, ka, 1, i);
+  const double xys = xs * yc + xc * ys, xyc = xc * yc - xs * ys;
+  const double zs = (t.z < 0 ? -1 : 1) * ew_tab(d, 2, la, 0, i);
+  const double zc = ew_tab(d, 2, la, 1, i);
+  sA = xys * zc + xyc * zs;
+  cA = xyc * zc - xys * zs;
+}
+
+__global__ __launch_bounds__(kEwBlock) void k_ew_sk(EwDev d) {
+  const int lane = threadIdx.x & 63;
+  const int kk = blockIdx.x * (kEwBlock / 64) + (threadIdx.x >> 6);
+  if (kk >= d.nk) return;
+  const int4 t = d.hkl[kk];
+  double ss = 0.0, sc = 0.0;
+  for (int i = lane; i < d.n; i += 64) {
+    double sA, cA;
+    ew_phase(d, t, i, sA, cA);
+    ss += sA;
+    sc += cA;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    ss += __shfl_xor(ss, o);
+    sc += __shfl_xor(sc, o);
+  }
+  if (lane == 0) d.sk[kk] = make_double2(ss, sc);
+}
+
+// Force on atom i: every k-vector, then the correction of its excluded
+// pairs (the pair force of row (a1, a2) acts as -f on a1 and +f on a2).
+__global__ __launch_bounds__(kEwBlock) void k_ew_force(EwDev d,
+                                                       const double *x,
+                                                       double *f) {
+  const int i = blockIdx.x * kEwBlock + threadIdx.x;
+  if (i >= d.n) return;
+  double fx = 0, fy = 0, fz = 0;
+  const double a0 = 8.0 * M_PI * d.Vr;
+  for (int kk = 0; kk < d.nk; kk++) {
+    const double4 k = d.kv[kk];
+    const double2 s = d.sk[kk];
+    double sA, cA;
+    ew_phase(d, d.hkl[kk], i, sA, cA);
+    const double w = a0 * k.w * (sA * s.y - cA * s.x);
+    fx += k.x * w;
+    fy += k.y * w;
+    fz += k.z * w;
+  }
+  for (int e = d.xptr[i]; e < d.xptr[i + 1]; e++) {
+    const int row = d.xent[e] >> 1, role = d.xent[e] & 1;
+    const int a1 = d.xrows[3 * row], a2 = d.xrows[3 * row + 1],
+              cls = d.xrows[3 * row + 2];
+    double rx, ry, rz, r2;
+    min_diff_exact(d.box, x[3 * a1], x[3 * a1 + 1], x[3 * a1 + 2], x[3 * a2],
+                   x[3 * a2 + 1], x[3 * a2 + 2], rx, ry, rz, r2);
+    double qq = d.q[a1] * d.q[a2];
+    if (cls == 1) qq *= 1 - d.csf;  // EXCLUSION_MODIFIED
+    const double r = sqrt(r2), rr = 1 / r;
+    const double e1 = erf(d.alpha * r) * rr;
+    const double g = qq * (d.two_alpha_pi * exp(-d.alpha2 * r2) - e1) * rr * rr;
+    if (role == 0) {
+      fx -= rx * g;
+      fy -= ry * g;
+      fz -= rz * g;
+    } else {
+      fx += rx * g;
+      fy += ry * g;
+      fz += rz * g;
+    }
+  }
+  f[3 * i] += fx;
+  f[3 * i + 1] += fy;
+  f[3 * i + 2] += fz;
+}
+
+// out[1] += reciprocal + correction + constant energies; with the virial
+// flag out[2..10] += reciprocal and correction virial + charged-system
+// diagonal.  consts: (self + charged energy, charged energy).
+__global__ __launch_bounds__(kEwBlock) void k_ew_finish(EwDev d,
+                                                        const double *x,
+                                                        double *out,
+                                                        int virial,
+                                                        double e_const,
+                                                        double e_charged) {
+  __shared__ double red[kEwBlock];
+  const int t = threadIdx.x;
+  double acc[8];  // energy, xx, xy, xz, yy, yz, zz, intra energy
+  for (int c = 0; c < 8; c++) acc[c] = 0;
+  for (int kk = t; kk < d.nk; kk += kEwBlock) {
+    const double4 k = d.kv[kk];
+    const double2 s = d.sk[kk];
+    const double k2 = k.x * k.x + k.y * k.y + k.z * k.z;
+    const double e = k.w * (s.x * s.x + s.y * s.y);
+    acc[0] += e;
+    const double c = 2.0 * (1.0 / k2 + d.fac);
+    acc[1] += e * (1.0 - c * k.x * k.x);
+    acc[2] -= e * c * k.x * k.y;
+    acc[3] -= e * c * k.x * k.z;
+    acc[4] += e * (1.0 - c * k.y * k.y);
+    acc[5] -= e * c * k.y * k.z;
+    acc[6] += e * (1.0 - c * k.z * k.z);
+  }
+  // correction: energy (acc[7]) and virial of each excluded pair, scaled
+  // into the same accumulators after the reciprocal factor is applied
+  double cv[6] = {0, 0, 0, 0, 0, 0};
+  for (int row = t; row < d.n_excl; row += kEwBlock) {
+    const int a1 = d.xrows[3 * row], a2 = d.xrows[3 * row + 1],
+              cls = d.xrows[3 * row + 2];
+    double rx, ry, rz, r2;
+    min_diff_exact(d.box, x[3 * a1], x[3 * a1 + 1], x[3 * a1 + 2], x[3 * a2],
+                   x[3 * a2 + 1], x[3 * a2 + 2], rx, ry, rz, r2);
+    double qq = d.q[a1] * d.q[a2];
+    if (cls == 1) qq *= 1 - d.csf;
+    const double r = sqrt(r2), rr = 1 / r;
+    const double e1 = erf(d.alpha * r) * rr;
+    acc[7] -= qq * e1;
+    const double g = qq * (d.two_alpha_pi * exp(-d.alpha2 * r2) - e1) * rr * rr;
+    const double f0 = rx * g, f1 = ry * g, f2 = rz * g;
+    cv[0] += f0 * rx;
+    cv[1] += f0 * ry;
+    cv[2] += f0 * rz;
+    cv[3] += f1 * ry;
+    cv[4] += f1 * rz;
+    cv[5] += f2 * rz;
+  }
+  double tot[14];
+  for (int c = 0; c < 8; c++) {
+    red[t] = acc[c];
+    __syncthreads();
+    for (int o = kEwBlock / 2; o > 0; o >>= 1) {
+      if (t < o) red[t] += red[t + o];
+      __syncthreads();
+    }
+    tot[c] = red[0];
+    __syncthreads();
+  }
+  for (int c = 0; c < 6; c++) {
+    red[t] = cv[c];
+    __syncthreads();
+    for (int o = kEwBlock / 2; o > 0; o >>= 1) {
+      if (t < o) red[t] += red[t + o];
+      __syncthreads();
+    }
+    tot[8 + c] = red[0];
+    __syncthreads();
+  }
+  if (t == 0) {
+    const double c = 4.0 * M_PI * d.Vr;
+    out[1] += c * tot[0] + tot[7] + e_const;
+    if (virial) {
+      // slots XX XY XZ YX YY YZ ZX ZY ZZ (ScalarStructure::addVirial)
+      const double xx = c * tot[1] + tot[8] + e_charged;
+      const double xy = c * tot[2] + tot[9];
+      const double xz = c * tot[3] + tot[10];
+      const double yy = c * tot[4] + tot[11] + e_charged;
+      const double yz = c * tot[5] + tot[12];
+      const double zz = c * tot[6] + tot[13] + e_charged;
+      out[2] += xx; out[3] += xy; out[4] += xz;
+      out[5] += xy; out[6] += yy; out[7] += yz;
+      out[8] += xz; out[9] += yz; out[10] += zz;
+    }
+  }
+}
+
+}  // namespace pmnb

@@ -1,0 +1,153 @@
+"""Pins the oracle's restatement of the k-space terms of
+NonbondedFullEwaldSystemForce (reciprocal sum, exclusion correction, point
+self and charged-system energies) by what the mathematics fixes:
+
+* rock salt: the full Ewald sum (an exact real-space image sum written
+  here with scipy's erfc, plus the oracle's k-space terms) gives the
+  Madelung constant 1.747564594633 for every alpha;
+* alpha independence of real + k-space with FULL and MODIFIED (scaled by
+  coulombScalingFactor) exclusions and a net charge: the correction, the
+  1 - csf factor and the charged-system term each cancel an alpha
+  dependence, so a dropped or mis-signed term breaks it;
+* forces are minus the finite-difference gradient of the k-space energy.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy.special import erfc
+
+from oracle import oracle as O
+from protomoladdon_amd.systems import Workload
+from protomoladdon_amd.topology import (CubicCellManager,
+                                        PeriodicBoundaryConditions, Topology,
+                                        build_exclusion_table,
+                                        lorentz_berthelot_table)
+
+MADELUNG_NACL = 1.747564594633182
+
+
+def make_system(pos, q, L, bonds=(), angles=(), dihedrals=(), mol=None,
+                csf=1.0):
+    n = len(pos)
+    excl = build_exclusion_table(n, "scaled1-4",
+                                 np.asarray(bonds, int).reshape(-1, 2),
+                                 np.asarray(angles, int).reshape(-1, 3),
+                                 np.asarray(dihedrals, int).reshape(-1, 4))
+    topo = Topology(np.zeros(n, np.int32), q,
+                    np.arange(n, dtype=np.int32) if mol is None else mol,
+                    lorentz_berthelot_table([1.0], [0.0]), excl,
+                    PeriodicBoundaryConditions.cubic(L), CubicCellManager(2.0),
+                    coulomb_scaling_factor=csf, masses=np.ones(n))
+    w = Workload("ewald", topo, np.asarray(pos, float), 4.0, (3.0, 4.0),
+                 "coulomb", (3.0, 4.0))
+    return w, O.System.from_workload(w)
+
+
+def real_space_exact(w, alpha, rc):
+    """Sum over images of q_i q_j erfc(alpha r)/r (i != j or a non-zero
+    image); at the minimum image FULL pairs are left out and MODIFIED pairs
+    scaled by coulombScalingFactor, as the cutoff force treats them."""
+    t = w.topology
+    pos, qs, L = w.positions, t.scaled_charges, t.boundary_conditions.e1[0]
+    cls = {}
+    for a, b, c in t.exclusions.table():
+        cls[(min(a, b), max(a, b))] = c
+    nimg = int(math.ceil(rc / L)) + 1
+    sh = np.array([[i, j, k] for i in range(-nimg, nimg + 1)
+                   for j in range(-nimg, nimg + 1)
+                   for k in range(-nimg, nimg + 1)], float) * L
+    e = 0.0
+    n = len(pos)
+    for i in range(n):
+        for j in range(i, n):
+            d0 = pos[j] - pos[i]
+            dmin = d0 - L * np.rint(d0 / L)
+            d = dmin[None, :] + sh
+            r = np.linalg.norm(d, axis=1)
+            keep = (r > 1e-12) & (r < rc)
+            fac = np.ones(len(sh))
+            c = cls.get((i, j))
+            if c is not None:
+                at_min = np.all(np.abs(sh) < 1e-9, axis=1)
+                fac[at_min] = 0.0 if c == 2 else t.coulomb_scaling_factor
+            v = np.sum(fac[keep] * erfc(alpha * r[keep]) / r[keep])
+            e += (0.5 if i == j else 1.0) * qs[i] * qs[j] * v
+    return e
+
+
+def rock_salt(ncell, a):
+    base = np.array([[0, 0, 0], [.5, .5, 0], [.5, 0, .5], [0, .5, .5]])
+    pos, q = [], []
+    for o in np.ndindex(ncell, ncell, ncell):
+        for b in base:
+            pos.append((np.array(o) + b) * a)
+            q.append(1.0)
+            pos.append((np.array(o) + b + [.5, 0, 0]) * a)
+            q.append(-1.0)
+    return np.array(pos), np.array(q)
+
+
+@pytest.mark.parametrize("alpha", [0.8, 1.0, 1.3])
+def test_rock_salt_madelung_constant(alpha):
+    a = 5.64
+    pos, q = rock_salt(2, a)
+    w, s = make_system(pos, q, 2 * a)
+    f, e_k, vir, info = O.ewald_kspace(s, alpha, 1e-14)
+    e = real_space_exact(w, alpha, info[1]) + e_k
+    n_pairs = len(pos) / 2
+    madelung = -e / n_pairs * (a / 2) / 332.0636
+    assert abs(madelung - MADELUNG_NACL) < 1e-9
+    assert np.abs(f).max() < 1e-9
+
+
+def chains(seed=3, n_mol=4, L=13.0):
+    """Four 5-atom chains (1-2, 1-3 FULL, 1-4 MODIFIED) with random charges
+    summing to a non-zero total."""
+    rng = np.random.default_rng(seed)
+    pos, bonds, angles, dih, mol = [], [], [], [], []
+    for m in range(n_mol):
+        start = rng.uniform(0, L, 3)
+        d = rng.normal(size=3)
+        for k in range(5):
+            step = rng.normal(size=3) * 0.3 + d / np.linalg.norm(d)
+            start = start + 1.5 * step / np.linalg.norm(step)
+            pos.append(start.copy())
+            mol.append(m)
+        b = 5 * m
+        bonds += [(b + k, b + k + 1) for k in range(4)]
+        angles += [(b + k, b + k + 1, b + k + 2) for k in range(3)]
+        dih += [(b + k, b + k + 1, b + k + 2, b + k + 3) for k in range(2)]
+    q = rng.normal(0, 0.4, len(pos))
+    q[0] += 0.3  # net charge
+    return (np.mod(np.array(pos), L), q, L, bonds, angles, dih,
+            np.array(mol, np.int32))
+
+
+def test_alpha_independence_with_exclusions_and_net_charge():
+    pos, q, L, bonds, angles, dih, mol = chains()
+    w, s = make_system(pos, q, L, bonds, angles, dih, mol, csf=0.5)
+    cls = w.topology.exclusions.table()[:, 2]
+    assert (cls == 2).any() and (cls == 1).any()
+    totals = []
+    for alpha in (0.55, 0.8):
+        _, e_k, _, info = O.ewald_kspace(s, alpha, 1e-14)
+        totals.append(real_space_exact(w, alpha, info[1]) + e_k)
+    assert abs(totals[0] - totals[1]) < 1e-8 * abs(totals[0])
+
+
+def test_kspace_forces_are_energy_gradient():
+    pos, q, L, bonds, angles, dih, mol = chains(seed=5)
+    w, s = make_system(pos, q, L, bonds, angles, dih, mol, csf=0.5)
+    f, _, _, _ = O.ewald_kspace(s, 0.7, 1e-12)
+    h = 1e-5
+    for i in (0, 7, 13):
+        for k in range(3):
+            ep = []
+            for sgn in (1, -1):
+                p = pos.copy()
+                p[i, k] += sgn * h
+                _, sp = make_system(p, q, L, bonds, angles, dih, mol, csf=0.5)
+                ep.append(O.ewald_kspace(sp, 0.7, 1e-12)[1])
+            fd = -(ep[0] - ep[1]) / (2 * h)
+            assert abs(fd - f[i, k]) <= 1e-6 * max(1.0, abs(f[i, k]))
