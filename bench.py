@@ -106,6 +106,33 @@ def time_md(w, precision, steps, warmup, flags, torch):
             "unconverged_calls": int(unconverged)}
 
 
+def time_ewald_kspace(w, alpha, accuracy, reps, torch):
+    """The k-space terms of NonbondedFullEwaldSystemForce (reciprocal sum,
+    exclusion correction, self and charged-system energies) on the
+    device; returns ms per evaluation and the k-vector count."""
+    from protomoladdon_amd.ewald import NonbondedFullEwaldKSpace
+    force = w.make_force("fp64")
+    ks = NonbondedFullEwaldKSpace(force, alpha, accuracy)
+    dev = torch.device("cuda:0")
+    x = torch.from_numpy(np.ascontiguousarray(w.positions)).to(dev)
+    f = torch.zeros_like(x)
+    out = torch.zeros(20, dtype=torch.float64, device=dev)
+    sp = torch.cuda.current_stream(dev).cuda_stream
+    ks.evaluate_device(w.topology, x, f, out, 1, stream=sp)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ks.evaluate_device(w.topology, x, f, out, 1, stream=sp)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / reps
+    a, rc, kc, nk = ks.info(w.topology)
+    ks.close()
+    force.close()
+    return {"ms": ms, "alpha": a, "accuracy": accuracy, "kc": kc,
+            "n_kvectors": nk, "n_atoms": w.n_atoms,
+            "atom_kvector_terms_per_s": w.n_atoms * nk / (ms * 1e-3)}
+
+
 def cpu_baseline(w, budget_s):
     from oracle import oracle as O
     O.build()
@@ -362,6 +389,11 @@ def main():
                     torch)
     md_64 = time_md(w, "fp64", fp64_steps, 2, flags, torch)
 
+    print("ewald k-space ...", file=sys.stderr, flush=True)
+    ewald = {"water3k": time_ewald_kspace(systems.config0(), -1.0, 1e-5, 20,
+                                          torch),
+             "water96k": time_ewald_kspace(w, 0.32, 1e-5, 3, torch)}
+
     # the host-array entry point (pmnb_evaluate: positions in, forces out
     # through PCIe, synchronous), for the record; never the headline value
     from protomoladdon_amd.topology import ScalarStructure
@@ -430,6 +462,7 @@ def main():
         "md": {"integrator": "Leapfrog 1 fs + SHAKE/RATTLE (1e-5, 30), "
                              "whole step on the device, virial on",
                "mixed": md_mx, "fp64": md_64},
+        "ewald_kspace": ewald,
         "host_api_ms_per_step": host_ms,
         "rms_rel_force_err_vs_fp64": rms,
         "roofline": {"bound": "valu", "achieved": achieved,

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libpmnb.so")
 SOURCES = ["pmnb.hip"]
 HEADERS = ["pmnb_common.h", "pmnb_cells.h", "pmnb_pair.h", "pmnb_list.h",
            "pmnb_tile.h", "pmnb_queue.h", "pmnb_adjoint.h",
-           "pmnb_tilemask.h", "pmnb_leapfrog.h"]
+           "pmnb_tilemask.h", "pmnb_leapfrog.h", "pmnb_ewald.h"]
 
 # -ffp-contract=off: FP64 expressions round exactly as the reference's x86-64
 # build (no FMA); FP32 kernels ask for fused multiply-adds explicitly.
