@@ -2231,7 +2231,8 @@ struct pmnb_ewald {
   int device = 0, n = 0, nk = 0, H = 0, n_excl = 0;
   double alpha = 0, alpha2 = 0, rc = 0, kc = 0, fac = 0, V = 0, Vr = 0;
   double csf = 1.0, e_self = 0, e_charged = 0;
-  DevBuf kv, hkl, q, tab, sk, xptr, xent, xrows;
+  int n_runs = 0, n_chunks = 0;
+  DevBuf kv, q, tab, sk, xptr, xent, xrows, runs, chunks;
 };
 
 extern "C" {
@@ -2300,6 +2301,25 @@ int pmnb_ewald_create(pmnb_ctx *ctx, double alpha, double accuracy,
     }
   }
   ew->nk = (int)kv.size();
+  // runs of equal (h, k) (contiguous l) and their chunks of l values
+  std::vector<EwRun> runs;
+  std::vector<EwChunk> chunks;
+  for (int i = 0; i < ew->nk; i++) {
+    if (runs.empty() || runs.back().h != hkl[i].x || runs.back().k != hkl[i].y)
+      runs.push_back(EwRun{hkl[i].x, hkl[i].y, hkl[i].z, hkl[i].z, i});
+    else
+      runs.back().l_hi = hkl[i].z;
+  }
+  // chunks of one sign of l each (|l| monotonic inside a chunk)
+  for (int r = 0; r < (int)runs.size(); r++) {
+    const int parts[2][2] = {{runs[r].l_lo, std::min(runs[r].l_hi, -1)},
+                             {std::max(runs[r].l_lo, 0), runs[r].l_hi}};
+    for (const auto &pt : parts)
+      for (int l0 = pt[0]; l0 <= pt[1]; l0 += kEwChunk)
+        chunks.push_back(EwChunk{r, l0, std::min(l0 + kEwChunk - 1, pt[1])});
+  }
+  ew->n_runs = (int)runs.size();
+  ew->n_chunks = (int)chunks.size();
   double q2 = 0.0, qs = 0.0;
   for (int i = 0; i < n_atoms; i++) q2 += scaled_charge[i] * scaled_charge[i];
   for (int i = 0; i < n_atoms; i++) qs += scaled_charge[i];
@@ -2327,7 +2347,6 @@ int pmnb_ewald_create(pmnb_ctx *ctx, double alpha, double accuracy,
   };
   hipError_t e = hipSetDevice(ew->device);
   if (e == hipSuccess) e = up(ew->kv, kv.data(), kv.size() * sizeof(double4));
-  if (e == hipSuccess) e = up(ew->hkl, hkl.data(), hkl.size() * sizeof(int4));
   if (e == hipSuccess) e = up(ew->q, scaled_charge, (size_t)n_atoms * sizeof(double));
   if (e == hipSuccess)
     e = ew->tab.ensure((size_t)3 * ew->H * 2 * std::max(n_atoms, 1) * sizeof(double));
@@ -2335,6 +2354,8 @@ int pmnb_ewald_create(pmnb_ctx *ctx, double alpha, double accuracy,
   if (e == hipSuccess) e = up(ew->xptr, cnt.data(), (n_atoms + 1) * sizeof(int));
   if (e == hipSuccess) e = up(ew->xent, ent.data(), 2 * (size_t)n_excl * sizeof(int));
   if (e == hipSuccess) e = up(ew->xrows, excl, 3 * (size_t)n_excl * sizeof(int));
+  if (e == hipSuccess) e = up(ew->runs, runs.data(), runs.size() * sizeof(EwRun));
+  if (e == hipSuccess) e = up(ew->chunks, chunks.data(), chunks.size() * sizeof(EwChunk));
   if (e != hipSuccess) {
     pmnb_ewald_destroy(ew);
     return fail(ctx, PMNB_ERR_HIP, "Ewald set-up failed: %s", hipGetErrorString(e));
@@ -2376,7 +2397,10 @@ int pmnb_ewald_evaluate_device(pmnb_ewald *ew, const double *d_positions,
   d.nk = ew->nk;
   d.H = ew->H;
   d.kv = ew->kv.as<double4>();
-  d.hkl = ew->hkl.as<int4>();
+  d.runs = ew->runs.as<EwRun>();
+  d.chunks = ew->chunks.as<EwChunk>();
+  d.n_runs = ew->n_runs;
+  d.n_chunks = ew->n_chunks;
   d.q = ew->q.as<double>();
   d.tab = ew->tab.as<double>();
   d.sk = ew->sk.as<double2>();
@@ -2392,9 +2416,8 @@ int pmnb_ewald_evaluate_device(pmnb_ewald *ew, const double *d_positions,
   d.fac = ew->fac;
   const int nb = blocks_for(ew->n, kEwBlock);
   hipLaunchKernelGGL(k_ew_tables, dim3(nb), dim3(kEwBlock), 0, s, d, d_positions);
-  if (ew->nk > 0)
-    hipLaunchKernelGGL(k_ew_sk, dim3(blocks_for(ew->nk, kEwBlock / 64)),
-                       dim3(kEwBlock), 0, s, d);
+  if (ew->n_chunks > 0)
+    hipLaunchKernelGGL(k_ew_sk, dim3(ew->n_chunks), dim3(kEwBlock), 0, s, d);
   hipLaunchKernelGGL(k_ew_force, dim3(nb), dim3(kEwBlock), 0, s, d, d_positions,
                      d_forces);
   hipLaunchKernelGGL(k_ew_finish, dim3(1), dim3(kEwBlock), 0, s, d, d_positions,
@@ -2408,8 +2431,8 @@ void pmnb_ewald_destroy(pmnb_ewald *ew) {
   if (!ew) return;
   (void)hipSetDevice(ew->device);
   (void)hipDeviceSynchronize();
-  DevBuf *bufs[] = {&ew->kv, &ew->hkl, &ew->q, &ew->tab, &ew->sk,
-                    &ew->xptr, &ew->xent, &ew->xrows};
+  DevBuf *bufs[] = {&ew->kv, &ew->q, &ew->tab, &ew->sk,
+                    &ew->xptr, &ew->xent, &ew->xrows, &ew->runs, &ew->chunks};
   for (DevBuf *b : bufs) b->release();
   delete ew;
 }

@@ -9,26 +9,43 @@
 //              theorem, structure factor, energy, atomic virial, forces
 //   :1092-1130 intramolecular correction over the exclusion table
 //
+// The k-vector list is a sequence of runs: for each (h, k) the l values
+// form one contiguous interval (the reference's loop order).
 // Kernels: k_ew_tables (one thread per atom: the sin/cos tables of the
 // three axes, charge folded into x as the reference does, stored
 // [axis][order][sin|cos][atom] so a wave reads consecutive atoms);
-// k_ew_sk (one wave per k-vector: the structure factor, lanes over atoms,
-// a fixed shuffle tree); k_ew_force (one thread per atom: the force from
-// every k-vector, recomputing exp(i k.r) from the tables, plus the
-// correction of its excluded pairs, each pair evaluated by both atoms so no
-// atomics are needed); k_ew_finish (one block: energy and virial sums in a
-// fixed order).  Work is N x K (atoms x k-vectors), as in the reference.
+// k_ew_sk (one block per chunk of at most kEwChunk l values of one sign of
+// a run: the structure factors, threads over atoms, the x-y product formed
+// once per atom and chunk and exp(i |l| z) stepped from one table value,
+// fixed-order reductions); k_ew_force (one thread per atom:
+// per run the x-y product once, then exp(i l z) for l = 0, 1, ... by the
+// same addition-theorem step that built the tables, so the values are the
+// tables' without reading them, and the terms of +l and -l; then the
+// correction of the atom's excluded pairs, each pair evaluated by both of
+// its atoms so no atomics are needed); k_ew_finish (one block: energy and
+// virial sums in a fixed order).  Work is N x K (atoms x k-vectors), as in
+// the reference.
 #pragma once
 
 namespace pmnb {
 
 constexpr int kEwBlock = 256;
+constexpr int kEwChunk = 16;
+
+struct EwRun {
+  int h, k, l_lo, l_hi, start;  // l in [l_lo, l_hi] at list index start + l - l_lo
+};
+struct EwChunk {
+  int run, l0, l1;              // l in [l0, l1] of run
+};
 
 struct EwDev {
   Box box;
   int n, nk, H;               // atoms, k-vectors, table orders
   const double4 *kv;          // kx, ky, kz, b = exp(-k^2/(4 alpha^2))/k^2
-  const int4 *hkl;            // h, k, l
+  const EwRun *runs;
+  const EwChunk *chunks;
+  int n_runs, n_chunks;
   const double *q;            // scaled charges
   double *tab;                // 3 * H * 2 * n
   double2 *sk;                // per k: (sum sin, sum cos)
@@ -70,38 +87,77 @@ __global__ __launch_bounds__(kEwBlock) void k_ew_tables(EwDev d,
   }
 }
 
-// q_i sin(k.r_i), q_i cos(k.r_i) from the tables, in the reference's order
-__device__ __forceinline__ void ew_phase(const EwDev &d, int4 t, int i,
-                                         double &sA, double &cA) {
-  const int ka = t.y < 0 ? -t.y : t.y, la = t.z < 0 ? -t.z : t.z;
-  const double xs = ew_tab(d, 0, t.x, 0, i), xc = ew_tab(d, 0, t.x, 1, i);
-  const double ys = (t.y < 0 ? -1 : 1) * ew_tab(d, 1, ka, 0, i);
-  const double yc = ew_tab(d, 1, ka, 1, i);
-  const double xys = xs * yc + xc * ys, xyc = xc * yc - xs * ys;
-  const double zs = (t.z < 0 ? -1 : 1) * ew_tab(d, 2, la, 0, i);
-  const double zc = ew_tab(d, 2, la, 1, i);
-  sA = xys * zc + xyc * zs;
-  cA = xyc * zc - xys * zs;
+__device__ __forceinline__ double ew_wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
 }
 
+// Chunks never contain both signs of l, so |l| is monotonic in a chunk:
+// slot jj of the accumulators holds l = l0 + jj (l0 >= 0) or l = l1 - jj
+// (l1 < 0), and |l| grows with jj, so exp(i |l| z) is stepped from the
+// table value of the chunk's smallest |l| by the table's own step.
 __global__ __launch_bounds__(kEwBlock) void k_ew_sk(EwDev d) {
-  const int lane = threadIdx.x & 63;
-  const int kk = blockIdx.x * (kEwBlock / 64) + (threadIdx.x >> 6);
-  if (kk >= d.nk) return;
-  const int4 t = d.hkl[kk];
-  double ss = 0.0, sc = 0.0;
-  for (int i = lane; i < d.n; i += 64) {
-    double sA, cA;
-    ew_phase(d, t, i, sA, cA);
-    ss += sA;
-    sc += cA;
-  }
+  __shared__ double red[kEwBlock / 64][2 * kEwChunk];
+  const EwChunk c = d.chunks[blockIdx.x];
+  const EwRun r = d.runs[c.run];
+  const int ka = r.k < 0 ? -r.k : r.k;
+  const int nl = c.l1 - c.l0 + 1;
+  const bool neg = c.l1 < 0;
+  const int m0 = neg ? -c.l1 : c.l0;
+  const double zsg = neg ? -1.0 : 1.0;
+  double ss[kEwChunk], sc[kEwChunk];
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    ss += __shfl_xor(ss, o);
-    sc += __shfl_xor(sc, o);
+  for (int j = 0; j < kEwChunk; j++) ss[j] = sc[j] = 0.0;
+  for (int i = threadIdx.x; i < d.n; i += kEwBlock) {
+    const double xs = ew_tab(d, 0, r.h, 0, i), xc = ew_tab(d, 0, r.h, 1, i);
+    const double ys = (r.k < 0 ? -1 : 1) * ew_tab(d, 1, ka, 0, i);
+    const double yc = ew_tab(d, 1, ka, 1, i);
+    const double xys = xs * yc + xc * ys, xyc = xc * yc - xs * ys;
+    const double zs1 = ew_tab(d, 2, 1, 0, i), zc1 = ew_tab(d, 2, 1, 1, i);
+    double zs = ew_tab(d, 2, m0, 0, i), zc = ew_tab(d, 2, m0, 1, i);
+#pragma unroll
+    for (int jj = 0; jj < kEwChunk; jj++) {
+      if (jj < nl) {
+        if (jj > 0) {
+          const int m = m0 + jj;
+          if (m == 1) {
+            zs = zs1;
+            zc = zc1;
+          } else {
+            const double s2 = zs1 * zc + zc1 * zs;
+            const double c2 = zc1 * zc - zs1 * zs;
+            zs = s2;
+            zc = c2;
+          }
+        }
+        const double zsl = zsg * zs;
+        ss[jj] += xys * zc + xyc * zsl;
+        sc[jj] += xyc * zc - xys * zsl;
+      }
+    }
   }
-  if (lane == 0) d.sk[kk] = make_double2(ss, sc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < kEwChunk; j++) {
+    const double a = ew_wave_sum(ss[j]), b = ew_wave_sum(sc[j]);
+    if (lane == 0) {
+      red[wave][2 * j] = a;
+      red[wave][2 * j + 1] = b;
+    }
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nl) {
+    const int jj = threadIdx.x;
+    double a = 0, b = 0;
+#pragma unroll
+    for (int w = 0; w < kEwBlock / 64; w++) {
+      a += red[w][2 * jj];
+      b += red[w][2 * jj + 1];
+    }
+    const int l = neg ? c.l1 - jj : c.l0 + jj;
+    d.sk[r.start + (l - r.l_lo)] = make_double2(a, b);
+  }
 }
 
 // Force on atom i: every k-vector, then the correction of its excluded
@@ -113,15 +169,41 @@ __global__ __launch_bounds__(kEwBlock) void k_ew_force(EwDev d,
   if (i >= d.n) return;
   double fx = 0, fy = 0, fz = 0;
   const double a0 = 8.0 * M_PI * d.Vr;
-  for (int kk = 0; kk < d.nk; kk++) {
-    const double4 k = d.kv[kk];
-    const double2 s = d.sk[kk];
-    double sA, cA;
-    ew_phase(d, d.hkl[kk], i, sA, cA);
-    const double w = a0 * k.w * (sA * s.y - cA * s.x);
-    fx += k.x * w;
-    fy += k.y * w;
-    fz += k.z * w;
+  const double zs1 = ew_tab(d, 2, 1, 0, i), zc1 = ew_tab(d, 2, 1, 1, i);
+  for (int ri = 0; ri < d.n_runs; ri++) {
+    const EwRun r = d.runs[ri];
+    const int ka = r.k < 0 ? -r.k : r.k;
+    const double xs = ew_tab(d, 0, r.h, 0, i), xc = ew_tab(d, 0, r.h, 1, i);
+    const double ys = (r.k < 0 ? -1 : 1) * ew_tab(d, 1, ka, 0, i);
+    const double yc = ew_tab(d, 1, ka, 1, i);
+    const double xys = xs * yc + xc * ys, xyc = xc * yc - xs * ys;
+    const int mmax = max(-r.l_lo, r.l_hi);
+    double zs = 0.0, zc = 1.0;  // exp(i m z) for m = 0
+    for (int m = 0; m <= mmax; m++) {
+      if (m == 1) {
+        zs = zs1;
+        zc = zc1;
+      } else if (m > 1) {  // the step k_ew_tables takes
+        const double s2 = zs1 * zc + zc1 * zs;
+        const double c2 = zc1 * zc - zs1 * zs;
+        zs = s2;
+        zc = c2;
+      }
+#pragma unroll
+      for (int sg = 0; sg < 2; sg++) {
+        const int l = sg == 0 ? m : -m;
+        if ((sg == 1 && m == 0) || l < r.l_lo || l > r.l_hi) continue;
+        const double zsl = (l < 0 ? -1 : 1) * zs;
+        const double sA = xys * zc + xyc * zsl, cA = xyc * zc - xys * zsl;
+        const int kk = r.start + (l - r.l_lo);
+        const double4 k = d.kv[kk];
+        const double2 s = d.sk[kk];
+        const double w = a0 * k.w * (sA * s.y - cA * s.x);
+        fx += k.x * w;
+        fy += k.y * w;
+        fz += k.z * w;
+      }
+    }
   }
   for (int e = d.xptr[i]; e < d.xptr[i + 1]; e++) {
     const int row = d.xent[e] >> 1, role = d.xent[e] & 1;
