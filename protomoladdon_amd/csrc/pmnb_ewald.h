@@ -31,6 +31,7 @@ namespace pmnb {
 
 constexpr int kEwBlock = 256;
 constexpr int kEwChunk = 16;
+constexpr int kEwLanes = kEwBlock / 64;  // waves sharing an atom in k_ew_force
 
 struct EwRun {
   int h, k, l_lo, l_hi, start;  // l in [l_lo, l_hi] at list index start + l - l_lo
@@ -162,15 +163,21 @@ __global__ __launch_bounds__(kEwBlock) void k_ew_sk(EwDev d) {
 
 // Force on atom i: every k-vector, then the correction of its excluded
 // pairs (the pair force of row (a1, a2) acts as -f on a1 and +f on a2).
+// A block holds 64 atoms; its wave w takes runs w, w + 4, ... (every lane
+// of a wave on the same run, so the k-vector loads are wave-uniform), and
+// the four partial forces are added in wave order through LDS.
 __global__ __launch_bounds__(kEwBlock) void k_ew_force(EwDev d,
                                                        const double *x,
                                                        double *f) {
-  const int i = blockIdx.x * kEwBlock + threadIdx.x;
-  if (i >= d.n) return;
+  __shared__ double part_f[kEwLanes][3][64];
+  const int lane = threadIdx.x & 63, part = threadIdx.x >> 6;
+  const int i0 = blockIdx.x * 64 + lane;
+  const bool act = i0 < d.n;
+  const int i = act ? i0 : d.n - 1;  // inactive lanes read a real atom
   double fx = 0, fy = 0, fz = 0;
   const double a0 = 8.0 * M_PI * d.Vr;
   const double zs1 = ew_tab(d, 2, 1, 0, i), zc1 = ew_tab(d, 2, 1, 1, i);
-  for (int ri = 0; ri < d.n_runs; ri++) {
+  for (int ri = part; ri < d.n_runs; ri += kEwLanes) {
     const EwRun r = d.runs[ri];
     const int ka = r.k < 0 ? -r.k : r.k;
     const double xs = ew_tab(d, 0, r.h, 0, i), xc = ew_tab(d, 0, r.h, 1, i);
@@ -204,6 +211,18 @@ __global__ __launch_bounds__(kEwBlock) void k_ew_force(EwDev d,
         fz += k.z * w;
       }
     }
+  }
+  part_f[part][0][lane] = fx;
+  part_f[part][1][lane] = fy;
+  part_f[part][2][lane] = fz;
+  __syncthreads();
+  if (!act || part != 0) return;
+  fx = fy = fz = 0;
+#pragma unroll
+  for (int w = 0; w < kEwLanes; w++) {
+    fx += part_f[w][0][lane];
+    fy += part_f[w][1][lane];
+    fz += part_f[w][2][lane];
   }
   for (int e = d.xptr[i]; e < d.xptr[i + 1]; e++) {
     const int row = d.xent[e] >> 1, role = d.xent[e] & 1;
