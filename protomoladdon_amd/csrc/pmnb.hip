@@ -2418,7 +2418,7 @@ int pmnb_ewald_evaluate_device(pmnb_ewald *ew, const double *d_positions,
   hipLaunchKernelGGL(k_ew_tables, dim3(nb), dim3(kEwBlock), 0, s, d, d_positions);
   if (ew->n_chunks > 0)
     hipLaunchKernelGGL(k_ew_sk, dim3(ew->n_chunks), dim3(kEwBlock), 0, s, d);
-  hipLaunchKernelGGL(k_ew_force, dim3(blocks_for(ew->n, 64)), dim3(kEwBlock), 0,
+  hipLaunchKernelGGL(k_ew_force, dim3(blocks_for(ew->n, 128)), dim3(kEwBlock), 0,
                      s, d, d_positions, d_forces);
   hipLaunchKernelGGL(k_ew_finish, dim3(1), dim3(kEwBlock), 0, s, d, d_positions,
                      d_out, (int)((flags & PMNB_FLAG_VIRIAL) != 0),

@@ -17,8 +17,9 @@
 // k_ew_sk (one block per chunk of at most kEwChunk l values of one sign of
 // a run: the structure factors, threads over atoms, the x-y product formed
 // once per atom and chunk and exp(i |l| z) stepped from one table value,
-// fixed-order reductions); k_ew_force (one thread per atom:
-// per run the x-y product once, then exp(i l z) for l = 0, 1, ... by the
+// fixed-order reductions); k_ew_force (two atoms per lane, four waves on
+// different runs: per run the x-y product once, then exp(i l z) for
+// l = 0, 1, ... by the
 // same addition-theorem step that built the tables, so the values are the
 // tables' without reading them, and the terms of +l and -l; then the
 // correction of the atom's excluded pairs, each pair evaluated by both of
@@ -163,66 +164,94 @@ __global__ __launch_bounds__(kEwBlock) void k_ew_sk(EwDev d) {
 
 // Force on atom i: every k-vector, then the correction of its excluded
 // pairs (the pair force of row (a1, a2) acts as -f on a1 and +f on a2).
-// A block holds 64 atoms; its wave w takes runs w, w + 4, ... (every lane
-// of a wave on the same run, so the k-vector loads are wave-uniform), and
-// the four partial forces are added in wave order through LDS.
+// A block holds 2 x 64 atoms (each lane two, which share the k-vector
+// loads); its wave w takes runs w, w + 4, ... (every lane of a wave on the
+// same run, so those loads are wave-uniform), and the four partial forces
+// of an atom are added in wave order through LDS, by wave 0 for the first
+// 64 atoms and by wave 1 for the second.
 __global__ __launch_bounds__(kEwBlock) void k_ew_force(EwDev d,
                                                        const double *x,
                                                        double *f) {
-  __shared__ double part_f[kEwLanes][3][64];
+  __shared__ double part_f[kEwLanes][2][3][64];
   const int lane = threadIdx.x & 63, part = threadIdx.x >> 6;
-  const int i0 = blockIdx.x * 64 + lane;
-  const bool act = i0 < d.n;
-  const int i = act ? i0 : d.n - 1;  // inactive lanes read a real atom
-  double fx = 0, fy = 0, fz = 0;
+  int at[2];
+  double fx[2] = {0, 0}, fy[2] = {0, 0}, fz[2] = {0, 0};
+  double zs1[2], zc1[2];
+#pragma unroll
+  for (int u = 0; u < 2; u++) {
+    const int i0 = blockIdx.x * 128 + u * 64 + lane;
+    at[u] = i0 < d.n ? i0 : d.n - 1;  // past the end: a real atom, not written
+    zs1[u] = ew_tab(d, 2, 1, 0, at[u]);
+    zc1[u] = ew_tab(d, 2, 1, 1, at[u]);
+  }
   const double a0 = 8.0 * M_PI * d.Vr;
-  const double zs1 = ew_tab(d, 2, 1, 0, i), zc1 = ew_tab(d, 2, 1, 1, i);
   for (int ri = part; ri < d.n_runs; ri += kEwLanes) {
     const EwRun r = d.runs[ri];
     const int ka = r.k < 0 ? -r.k : r.k;
-    const double xs = ew_tab(d, 0, r.h, 0, i), xc = ew_tab(d, 0, r.h, 1, i);
-    const double ys = (r.k < 0 ? -1 : 1) * ew_tab(d, 1, ka, 0, i);
-    const double yc = ew_tab(d, 1, ka, 1, i);
-    const double xys = xs * yc + xc * ys, xyc = xc * yc - xs * ys;
+    double xys[2], xyc[2], zs[2], zc[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int i = at[u];
+      const double xs = ew_tab(d, 0, r.h, 0, i), xc = ew_tab(d, 0, r.h, 1, i);
+      const double ys = (r.k < 0 ? -1 : 1) * ew_tab(d, 1, ka, 0, i);
+      const double yc = ew_tab(d, 1, ka, 1, i);
+      xys[u] = xs * yc + xc * ys;
+      xyc[u] = xc * yc - xs * ys;
+      zs[u] = 0.0;  // exp(i m z) for m = 0
+      zc[u] = 1.0;
+    }
     const int mmax = max(-r.l_lo, r.l_hi);
-    double zs = 0.0, zc = 1.0;  // exp(i m z) for m = 0
     for (int m = 0; m <= mmax; m++) {
-      if (m == 1) {
-        zs = zs1;
-        zc = zc1;
-      } else if (m > 1) {  // the step k_ew_tables takes
-        const double s2 = zs1 * zc + zc1 * zs;
-        const double c2 = zc1 * zc - zs1 * zs;
-        zs = s2;
-        zc = c2;
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        if (m == 1) {
+          zs[u] = zs1[u];
+          zc[u] = zc1[u];
+        } else if (m > 1) {  // the step k_ew_tables takes
+          const double s2 = zs1[u] * zc[u] + zc1[u] * zs[u];
+          const double c2 = zc1[u] * zc[u] - zs1[u] * zs[u];
+          zs[u] = s2;
+          zc[u] = c2;
+        }
       }
 #pragma unroll
       for (int sg = 0; sg < 2; sg++) {
         const int l = sg == 0 ? m : -m;
         if ((sg == 1 && m == 0) || l < r.l_lo || l > r.l_hi) continue;
-        const double zsl = (l < 0 ? -1 : 1) * zs;
-        const double sA = xys * zc + xyc * zsl, cA = xyc * zc - xys * zsl;
         const int kk = r.start + (l - r.l_lo);
         const double4 k = d.kv[kk];
         const double2 s = d.sk[kk];
-        const double w = a0 * k.w * (sA * s.y - cA * s.x);
-        fx += k.x * w;
-        fy += k.y * w;
-        fz += k.z * w;
+        const double ak = a0 * k.w;
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          const double zsl = (l < 0 ? -1 : 1) * zs[u];
+          const double sA = xys[u] * zc[u] + xyc[u] * zsl;
+          const double cA = xyc[u] * zc[u] - xys[u] * zsl;
+          const double w = ak * (sA * s.y - cA * s.x);
+          fx[u] += k.x * w;
+          fy[u] += k.y * w;
+          fz[u] += k.z * w;
+        }
       }
     }
   }
-  part_f[part][0][lane] = fx;
-  part_f[part][1][lane] = fy;
-  part_f[part][2][lane] = fz;
+#pragma unroll
+  for (int u = 0; u < 2; u++) {
+    part_f[part][u][0][lane] = fx[u];
+    part_f[part][u][1][lane] = fy[u];
+    part_f[part][u][2][lane] = fz[u];
+  }
   __syncthreads();
-  if (!act || part != 0) return;
-  fx = fy = fz = 0;
+  if (part > 1) return;
+  const int i0 = blockIdx.x * 128 + part * 64 + lane;
+  if (i0 >= d.n) return;
+  const int i = i0;
+  double gx = 0, gy = 0, gz = 0;
 #pragma unroll
   for (int w = 0; w < kEwLanes; w++) {
-    fx += part_f[w][0][lane];
-    fy += part_f[w][1][lane];
-    fz += part_f[w][2][lane];
+    gx += part_f[w][part][0][lane];
+    gy += part_f[w][part][1][lane];
+    gz += part_f[w][part][2][lane];
   }
   for (int e = d.xptr[i]; e < d.xptr[i + 1]; e++) {
     const int row = d.xent[e] >> 1, role = d.xent[e] & 1;
@@ -237,18 +266,18 @@ __global__ __launch_bounds__(kEwBlock) void k_ew_force(EwDev d,
     const double e1 = erf(d.alpha * r) * rr;
     const double g = qq * (d.two_alpha_pi * exp(-d.alpha2 * r2) - e1) * rr * rr;
     if (role == 0) {
-      fx -= rx * g;
-      fy -= ry * g;
-      fz -= rz * g;
+      gx -= rx * g;
+      gy -= ry * g;
+      gz -= rz * g;
     } else {
-      fx += rx * g;
-      fy += ry * g;
-      fz += rz * g;
+      gx += rx * g;
+      gy += ry * g;
+      gz += rz * g;
     }
   }
-  f[3 * i] += fx;
-  f[3 * i + 1] += fy;
-  f[3 * i + 2] += fz;
+  f[3 * i] += gx;
+  f[3 * i + 1] += gy;
+  f[3 * i + 2] += gz;
 }
 
 // out[1] += reciprocal + correction + constant energies; with the virial
