@@ -9,7 +9,12 @@ self and charged-system energies) by what the mathematics fixes:
   coulombScalingFactor) exclusions and a net charge: the correction, the
   1 - csf factor and the charged-system term each cancel an alpha
   dependence, so a dropped or mis-signed term breaks it;
-* forces are minus the finite-difference gradient of the k-space energy.
+* forces are minus the finite-difference gradient of the k-space energy;
+* the atomic virial: its diagonal is minus the derivative of the k-space
+  energy under a stretch of one box axis (atoms carried along) in an
+  orthorhombic box, and the full tensor of real (exact image sum) plus
+  k-space virial is alpha-independent, which pins the off-diagonal terms
+  and the charged-system diagonal.
 """
 import math
 
@@ -29,7 +34,15 @@ MADELUNG_NACL = 1.747564594633182
 
 def make_system(pos, q, L, bonds=(), angles=(), dihedrals=(), mol=None,
                 csf=1.0):
+    """L is the edge of a cubic box or the three edges of an orthorhombic
+    one (origin at the box centre)."""
     n = len(pos)
+    if np.ndim(L) == 0:
+        pbc = PeriodicBoundaryConditions.cubic(L)
+    else:
+        L = np.asarray(L, float)
+        pbc = PeriodicBoundaryConditions((L[0], 0, 0), (0, L[1], 0),
+                                         (0, 0, L[2]), 0.5 * L)
     excl = build_exclusion_table(n, "scaled1-4",
                                  np.asarray(bonds, int).reshape(-1, 2),
                                  np.asarray(angles, int).reshape(-1, 3),
@@ -37,27 +50,31 @@ def make_system(pos, q, L, bonds=(), angles=(), dihedrals=(), mol=None,
     topo = Topology(np.zeros(n, np.int32), q,
                     np.arange(n, dtype=np.int32) if mol is None else mol,
                     lorentz_berthelot_table([1.0], [0.0]), excl,
-                    PeriodicBoundaryConditions.cubic(L), CubicCellManager(2.0),
+                    pbc, CubicCellManager(2.0),
                     coulomb_scaling_factor=csf, masses=np.ones(n))
     w = Workload("ewald", topo, np.asarray(pos, float), 4.0, (3.0, 4.0),
                  "coulomb", (3.0, 4.0))
     return w, O.System.from_workload(w)
 
 
-def real_space_exact(w, alpha, rc):
+def real_space_exact(w, alpha, rc, virial=False):
     """Sum over images of q_i q_j erfc(alpha r)/r (i != j or a non-zero
     image); at the minimum image FULL pairs are left out and MODIFIED pairs
-    scaled by coulombScalingFactor, as the cutoff force treats them."""
+    scaled by coulombScalingFactor, as the cutoff force treats them.  With
+    virial=True also returns the atomic virial sum of F_j (x) d over the
+    same terms, row-major (d = x_j + image - x_i, F_j the force on j)."""
     t = w.topology
-    pos, qs, L = w.positions, t.scaled_charges, t.boundary_conditions.e1[0]
+    pos, qs = w.positions, t.scaled_charges
+    L = t.boundary_conditions.lengths()
     cls = {}
     for a, b, c in t.exclusions.table():
         cls[(min(a, b), max(a, b))] = c
-    nimg = int(math.ceil(rc / L)) + 1
+    nimg = int(math.ceil(rc / L.min())) + 1
     sh = np.array([[i, j, k] for i in range(-nimg, nimg + 1)
                    for j in range(-nimg, nimg + 1)
                    for k in range(-nimg, nimg + 1)], float) * L
     e = 0.0
+    vir = np.zeros((3, 3))
     n = len(pos)
     for i in range(n):
         for j in range(i, n):
@@ -71,9 +88,14 @@ def real_space_exact(w, alpha, rc):
             if c is not None:
                 at_min = np.all(np.abs(sh) < 1e-9, axis=1)
                 fac[at_min] = 0.0 if c == 2 else t.coulomb_scaling_factor
-            v = np.sum(fac[keep] * erfc(alpha * r[keep]) / r[keep])
-            e += (0.5 if i == j else 1.0) * qs[i] * qs[j] * v
-    return e
+            rk, dk, fk = r[keep], d[keep], fac[keep]
+            w_ij = (0.5 if i == j else 1.0) * qs[i] * qs[j]
+            e += w_ij * np.sum(fk * erfc(alpha * rk) / rk)
+            if virial:
+                g = fk * (erfc(alpha * rk) / rk + 2 * alpha / math.sqrt(math.pi)
+                          * np.exp(-(alpha * rk) ** 2)) / (rk * rk)
+                vir += w_ij * np.einsum("n,na,nb->ab", g, dk, dk)
+    return (e, vir.reshape(9)) if virial else e
 
 
 def rock_salt(ncell, a):
@@ -151,3 +173,37 @@ def test_kspace_forces_are_energy_gradient():
                 ep.append(O.ewald_kspace(sp, 0.7, 1e-12)[1])
             fd = -(ep[0] - ep[1]) / (2 * h)
             assert abs(fd - f[i, k]) <= 1e-6 * max(1.0, abs(f[i, k]))
+
+
+def test_virial_diagonal_is_minus_the_axis_strain_derivative():
+    pos, q, _, bonds, angles, dih, mol = chains(seed=11)
+    L = np.array([13.0, 14.5, 12.2])
+    pos = np.mod(pos, L)
+    args = (q, None, bonds, angles, dih, mol, 0.5)
+
+    def energy(axis, eps):
+        s_ = np.ones(3)
+        s_[axis] += eps
+        _, sp = make_system(pos * s_, args[0], L * s_, *args[2:])
+        return O.ewald_kspace(sp, 0.7, 1e-14)[1]
+
+    _, s = make_system(pos, args[0], L, *args[2:])
+    _, e0, vir, _ = O.ewald_kspace(s, 0.7, 1e-14)
+    assert np.abs(vir - vir.reshape(3, 3).T.reshape(9)).max() < 1e-12
+    h = 1e-6
+    for axis in range(3):
+        de = (energy(axis, h) - energy(axis, -h)) / (2 * h)
+        assert abs(-de - vir[4 * axis]) <= 1e-6 * max(1.0, abs(vir[4 * axis]))
+
+
+def test_total_virial_is_alpha_independent():
+    pos, q, L, bonds, angles, dih, mol = chains(seed=7)
+    w, s = make_system(pos, q, L, bonds, angles, dih, mol, csf=0.5)
+    totals = []
+    for alpha in (0.55, 0.8):
+        _, _, vir_k, info = O.ewald_kspace(s, alpha, 1e-14)
+        _, vir_r = real_space_exact(w, alpha, info[1], virial=True)
+        totals.append(vir_r + vir_k)
+    off = [1, 2, 5]
+    assert np.abs(totals[0][off]).min() > 1e-2 * np.abs(totals[0]).max()
+    assert np.abs(totals[0] - totals[1]).max() < 1e-8 * np.abs(totals[0]).max()
