@@ -62,6 +62,19 @@ __device__ __forceinline__ double &ew_tab(const EwDev &d, int axis, int m,
   return d.tab[((size_t)(axis * d.H + m) * 2 + sc) * d.n + i];
 }
 
+// (s, c) <- (s, c) * exp(i z) for (s1, c1) = (sin z, cos z): the
+// addition-theorem step of the tables, with fused multiply-adds.  k_ew_sk and
+// k_ew_force take the same step, so their values of exp(i m z) are the
+// tables' bit for bit; one step from (0, 1) gives (s1, c1) exactly
+// (fma(s1, 1, c1 * 0) = s1), so m = 1 needs no case of its own.
+__device__ __forceinline__ void ew_step(double s1, double c1, double &s,
+                                        double &c) {
+  const double s2 = fma(s1, c, c1 * s);
+  const double c2 = fma(c1, c, -(s1 * s));
+  s = s2;
+  c = c2;
+}
+
 __global__ __launch_bounds__(kEwBlock) void k_ew_tables(EwDev d,
                                                         const double *x) {
   const int i = blockIdx.x * kEwBlock + threadIdx.x;
@@ -79,12 +92,9 @@ __global__ __launch_bounds__(kEwBlock) void k_ew_tables(EwDev d,
     ew_tab(d, a, 1, 0, i) = s1;
     ew_tab(d, a, 1, 1, i) = c1;
     for (int m = 2; m < d.H; m++) {
-      const double s2 = sn * c1 + cs * s1;
-      const double c2 = cs * c1 - sn * s1;
-      ew_tab(d, a, m, 0, i) = s2;
-      ew_tab(d, a, m, 1, i) = c2;
-      s1 = s2;
-      c1 = c2;
+      ew_step(sn, cs, s1, c1);
+      ew_tab(d, a, m, 0, i) = s1;
+      ew_tab(d, a, m, 1, i) = c1;
     }
   }
 }
@@ -115,27 +125,17 @@ __global__ __launch_bounds__(kEwBlock) void k_ew_sk(EwDev d) {
     const double xs = ew_tab(d, 0, r.h, 0, i), xc = ew_tab(d, 0, r.h, 1, i);
     const double ys = (r.k < 0 ? -1 : 1) * ew_tab(d, 1, ka, 0, i);
     const double yc = ew_tab(d, 1, ka, 1, i);
-    const double xys = xs * yc + xc * ys, xyc = xc * yc - xs * ys;
+    const double xys = fma(xs, yc, xc * ys), xyc = fma(xc, yc, -(xs * ys));
+    // the sign of l folded into the factors of sin(|l| z)
+    const double pxyc = zsg * xyc, nxys = -zsg * xys;
     const double zs1 = ew_tab(d, 2, 1, 0, i), zc1 = ew_tab(d, 2, 1, 1, i);
     double zs = ew_tab(d, 2, m0, 0, i), zc = ew_tab(d, 2, m0, 1, i);
 #pragma unroll
     for (int jj = 0; jj < kEwChunk; jj++) {
       if (jj < nl) {
-        if (jj > 0) {
-          const int m = m0 + jj;
-          if (m == 1) {
-            zs = zs1;
-            zc = zc1;
-          } else {
-            const double s2 = zs1 * zc + zc1 * zs;
-            const double c2 = zc1 * zc - zs1 * zs;
-            zs = s2;
-            zc = c2;
-          }
-        }
-        const double zsl = zsg * zs;
-        ss[jj] += xys * zc + xyc * zsl;
-        sc[jj] += xyc * zc - xys * zsl;
+        if (jj > 0) ew_step(zs1, zc1, zs, zc);
+        ss[jj] = fma(xys, zc, fma(pxyc, zs, ss[jj]));
+        sc[jj] = fma(xyc, zc, fma(nxys, zs, sc[jj]));
       }
     }
   }
@@ -173,7 +173,10 @@ __global__ __launch_bounds__(kEwBlock) void k_ew_force(EwDev d,
                                                        const double *x,
                                                        double *f) {
   __shared__ double part_f[kEwLanes][2][3][64];
-  const int lane = threadIdx.x & 63, part = threadIdx.x >> 6;
+  // part is the wave's index: uniform, so the run and k-vector data below
+  // come through scalar loads
+  const int lane = threadIdx.x & 63,
+            part = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   int at[2];
   double fx[2] = {0, 0}, fy[2] = {0, 0}, fz[2] = {0, 0};
   double zs1[2], zc1[2];
@@ -195,24 +198,16 @@ __global__ __launch_bounds__(kEwBlock) void k_ew_force(EwDev d,
       const double xs = ew_tab(d, 0, r.h, 0, i), xc = ew_tab(d, 0, r.h, 1, i);
       const double ys = (r.k < 0 ? -1 : 1) * ew_tab(d, 1, ka, 0, i);
       const double yc = ew_tab(d, 1, ka, 1, i);
-      xys[u] = xs * yc + xc * ys;
-      xyc[u] = xc * yc - xs * ys;
+      xys[u] = fma(xs, yc, xc * ys);
+      xyc[u] = fma(xc, yc, -(xs * ys));
       zs[u] = 0.0;  // exp(i m z) for m = 0
       zc[u] = 1.0;
     }
     const int mmax = max(-r.l_lo, r.l_hi);
     for (int m = 0; m <= mmax; m++) {
+      if (m > 0) {  // the step k_ew_tables takes
 #pragma unroll
-      for (int u = 0; u < 2; u++) {
-        if (m == 1) {
-          zs[u] = zs1[u];
-          zc[u] = zc1[u];
-        } else if (m > 1) {  // the step k_ew_tables takes
-          const double s2 = zs1[u] * zc[u] + zc1[u] * zs[u];
-          const double c2 = zc1[u] * zc[u] - zs1[u] * zs[u];
-          zs[u] = s2;
-          zc[u] = c2;
-        }
+        for (int u = 0; u < 2; u++) ew_step(zs1[u], zc1[u], zs[u], zc[u]);
       }
 #pragma unroll
       for (int sg = 0; sg < 2; sg++) {
@@ -222,15 +217,16 @@ __global__ __launch_bounds__(kEwBlock) void k_ew_force(EwDev d,
         const double4 k = d.kv[kk];
         const double2 s = d.sk[kk];
         const double ak = a0 * k.w;
+        const double ps = ak * s.y, pc = ak * s.x;  // w = sA ps - cA pc
 #pragma unroll
         for (int u = 0; u < 2; u++) {
-          const double zsl = (l < 0 ? -1 : 1) * zs[u];
-          const double sA = xys[u] * zc[u] + xyc[u] * zsl;
-          const double cA = xyc[u] * zc[u] - xys[u] * zsl;
-          const double w = ak * (sA * s.y - cA * s.x);
-          fx[u] += k.x * w;
-          fy[u] += k.y * w;
-          fz[u] += k.z * w;
+          const double zsl = sg == 0 ? zs[u] : -zs[u];  // sin(l z), l = +-m
+          const double sA = fma(xyc[u], zsl, xys[u] * zc[u]);
+          const double cA = fma(-xys[u], zsl, xyc[u] * zc[u]);
+          const double w = fma(sA, ps, -(cA * pc));
+          fx[u] = fma(k.x, w, fx[u]);
+          fy[u] = fma(k.y, w, fy[u]);
+          fz[u] = fma(k.z, w, fz[u]);
         }
       }
     }
