@@ -2310,13 +2310,14 @@ int pmnb_ewald_create(pmnb_ctx *ctx, double alpha, double accuracy,
     else
       runs.back().l_hi = hkl[i].z;
   }
-  // chunks of one sign of l each (|l| monotonic inside a chunk)
+  // chunks of |l| values, both signs of l in one chunk: |l| runs over
+  // [m_lo, m_hi], the |l| of the run's l interval
   for (int r = 0; r < (int)runs.size(); r++) {
-    const int parts[2][2] = {{runs[r].l_lo, std::min(runs[r].l_hi, -1)},
-                             {std::max(runs[r].l_lo, 0), runs[r].l_hi}};
-    for (const auto &pt : parts)
-      for (int l0 = pt[0]; l0 <= pt[1]; l0 += kEwChunk)
-        chunks.push_back(EwChunk{r, l0, std::min(l0 + kEwChunk - 1, pt[1])});
+    const int lo = runs[r].l_lo, hi = runs[r].l_hi;
+    const int m_hi = std::max(std::abs(lo), std::abs(hi));
+    const int m_lo = (lo <= 0 && hi >= 0) ? 0 : std::min(std::abs(lo), std::abs(hi));
+    for (int m0 = m_lo; m0 <= m_hi; m0 += kEwChunk)
+      chunks.push_back(EwChunk{r, m0, std::min(m0 + kEwChunk - 1, m_hi)});
   }
   ew->n_runs = (int)runs.size();
   ew->n_chunks = (int)chunks.size();

@@ -14,10 +14,10 @@
 // Kernels: k_ew_tables (one thread per atom: the sin/cos tables of the
 // three axes, charge folded into x as the reference does, stored
 // [axis][order][sin|cos][atom] so a wave reads consecutive atoms);
-// k_ew_sk (one block per chunk of at most kEwChunk l values of one sign of
-// a run: the structure factors, threads over atoms, the x-y product formed
-// once per atom and chunk and exp(i |l| z) stepped from one table value,
-// fixed-order reductions); k_ew_force (two atoms per lane, four waves on
+// k_ew_sk (one block per chunk of at most kEwChunk values of |l| of a run,
+// both signs: the structure factors, threads over atoms, the x-y product
+// formed once per atom and chunk and exp(i |l| z) stepped from one table
+// value, serving +l and -l; fixed-order reductions); k_ew_force (two atoms per lane, four waves on
 // different runs: per run the x-y product once, then exp(i l z) for
 // l = 0, 1, ... by the
 // same addition-theorem step that built the tables, so the values are the
@@ -31,14 +31,14 @@
 namespace pmnb {
 
 constexpr int kEwBlock = 256;
-constexpr int kEwChunk = 16;
+constexpr int kEwChunk = 8;   // |l| values per structure-factor block
 constexpr int kEwLanes = kEwBlock / 64;  // waves sharing an atom in k_ew_force
 
 struct EwRun {
   int h, k, l_lo, l_hi, start;  // l in [l_lo, l_hi] at list index start + l - l_lo
 };
 struct EwChunk {
-  int run, l0, l1;              // l in [l0, l1] of run
+  int run, m0, m1;              // l = +-m, m in [m0, m1], of run
 };
 
 struct EwDev {
@@ -105,59 +105,63 @@ __device__ __forceinline__ double ew_wave_sum(double v) {
   return v;
 }
 
-// Chunks never contain both signs of l, so |l| is monotonic in a chunk:
-// slot jj of the accumulators holds l = l0 + jj (l0 >= 0) or l = l1 - jj
-// (l1 < 0), and |l| grows with jj, so exp(i |l| z) is stepped from the
-// table value of the chunk's smallest |l| by the table's own step.
+// A chunk is an interval [m0, m1] of |l| (at most kEwChunk values) of one
+// run, taken with both signs: slot jj of the accumulators holds l = +m and
+// l = -m for m = m0 + jj, which share exp(i m z) (stepped from the table
+// value at m0 by the table's own step) and the x-y product; sin(-m z) =
+// -sin(m z).  A sign whose l is not in the run is computed and not stored.
 __global__ __launch_bounds__(kEwBlock) void k_ew_sk(EwDev d) {
-  __shared__ double red[kEwBlock / 64][2 * kEwChunk];
+  __shared__ double red[kEwBlock / 64][4 * kEwChunk];
   const EwChunk c = d.chunks[blockIdx.x];
   const EwRun r = d.runs[c.run];
   const int ka = r.k < 0 ? -r.k : r.k;
-  const int nl = c.l1 - c.l0 + 1;
-  const bool neg = c.l1 < 0;
-  const int m0 = neg ? -c.l1 : c.l0;
-  const double zsg = neg ? -1.0 : 1.0;
-  double ss[kEwChunk], sc[kEwChunk];
+  const int nm = c.m1 - c.m0 + 1;
+  const int m0 = c.m0;
+  double sp[kEwChunk], cp[kEwChunk], sn[kEwChunk], cn[kEwChunk];
 #pragma unroll
-  for (int j = 0; j < kEwChunk; j++) ss[j] = sc[j] = 0.0;
+  for (int j = 0; j < kEwChunk; j++) sp[j] = cp[j] = sn[j] = cn[j] = 0.0;
   for (int i = threadIdx.x; i < d.n; i += kEwBlock) {
     const double xs = ew_tab(d, 0, r.h, 0, i), xc = ew_tab(d, 0, r.h, 1, i);
     const double ys = (r.k < 0 ? -1 : 1) * ew_tab(d, 1, ka, 0, i);
     const double yc = ew_tab(d, 1, ka, 1, i);
     const double xys = fma(xs, yc, xc * ys), xyc = fma(xc, yc, -(xs * ys));
-    // the sign of l folded into the factors of sin(|l| z)
-    const double pxyc = zsg * xyc, nxys = -zsg * xys;
     const double zs1 = ew_tab(d, 2, 1, 0, i), zc1 = ew_tab(d, 2, 1, 1, i);
     double zs = ew_tab(d, 2, m0, 0, i), zc = ew_tab(d, 2, m0, 1, i);
 #pragma unroll
     for (int jj = 0; jj < kEwChunk; jj++) {
-      if (jj < nl) {
+      if (jj < nm) {
         if (jj > 0) ew_step(zs1, zc1, zs, zc);
-        ss[jj] = fma(xys, zc, fma(pxyc, zs, ss[jj]));
-        sc[jj] = fma(xyc, zc, fma(nxys, zs, sc[jj]));
+        // sin/cos of (x + y +- m z)
+        sp[jj] = fma(xyc, zs, fma(xys, zc, sp[jj]));
+        cp[jj] = fma(-xys, zs, fma(xyc, zc, cp[jj]));
+        sn[jj] = fma(-xyc, zs, fma(xys, zc, sn[jj]));
+        cn[jj] = fma(xys, zs, fma(xyc, zc, cn[jj]));
       }
     }
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int j = 0; j < kEwChunk; j++) {
-    const double a = ew_wave_sum(ss[j]), b = ew_wave_sum(sc[j]);
+    const double a = ew_wave_sum(sp[j]), b = ew_wave_sum(cp[j]);
+    const double e = ew_wave_sum(sn[j]), g = ew_wave_sum(cn[j]);
     if (lane == 0) {
-      red[wave][2 * j] = a;
-      red[wave][2 * j + 1] = b;
+      red[wave][4 * j] = a;
+      red[wave][4 * j + 1] = b;
+      red[wave][4 * j + 2] = e;
+      red[wave][4 * j + 3] = g;
     }
   }
   __syncthreads();
-  if ((int)threadIdx.x < nl) {
-    const int jj = threadIdx.x;
+  if ((int)threadIdx.x < 2 * nm) {
+    const int jj = threadIdx.x >> 1, neg = threadIdx.x & 1;
+    const int l = neg ? -(m0 + jj) : m0 + jj;
+    if ((neg && l == 0) || l < r.l_lo || l > r.l_hi) return;
     double a = 0, b = 0;
 #pragma unroll
     for (int w = 0; w < kEwBlock / 64; w++) {
-      a += red[w][2 * jj];
-      b += red[w][2 * jj + 1];
+      a += red[w][4 * jj + 2 * neg];
+      b += red[w][4 * jj + 2 * neg + 1];
     }
-    const int l = neg ? c.l1 - jj : c.l0 + jj;
     d.sk[r.start + (l - r.l_lo)] = make_double2(a, b);
   }
 }
