@@ -416,14 +416,17 @@ def leapfrog_run(system, state, n_steps, masses, constraints, rest, timestep,
 
 
 # ------------------------------------------------------------------ Ewald
-def ewald_kspace(system, alpha, accuracy, virial=True):
+def ewald_kspace(system, alpha, accuracy, virial=True, extended=False):
     """Reciprocal + correction + self + charged-system terms of
     NonbondedFullEwaldSystemForce: returns (forces, electrostatic energy,
-    virial[9], info) with info = (alpha, rc, kc, number of k-vectors)."""
+    virial[9], info) with info = (alpha, rc, kc, number of k-vectors).
+    extended=True sums the structure factors and energies in long double
+    (same terms, without the double running sum's rounding error)."""
     lib = load()
     if not hasattr(lib, "_ew_ready"):
         lib.orc_ewald_eval.argtypes = [ctypes.POINTER(OrcSystem), ctypes.c_double,
-                                  ctypes.c_double] + [ctypes.c_void_p] * 4
+                                  ctypes.c_double] + [ctypes.c_void_p] * 4 + [
+                                      ctypes.c_int]
         lib._ew_ready = True
     st = system.struct()
     f = np.zeros((system.n, 3))
@@ -432,6 +435,7 @@ def ewald_kspace(system, alpha, accuracy, virial=True):
     info = np.zeros(4)
     if lib.orc_ewald_eval(ctypes.byref(st), float(alpha), float(accuracy),
                      f.ctypes.data, e.ctypes.data,
-                     vir.ctypes.data if virial else None, info.ctypes.data):
+                     vir.ctypes.data if virial else None, info.ctypes.data,
+                     int(bool(extended))):
         raise RuntimeError("oracle Ewald evaluation failed")
     return f, e[1], vir, info

@@ -17,11 +17,12 @@
 // k_ew_sk (one block per chunk of at most kEwChunk values of |l| of a run,
 // both signs: the structure factors, threads over atoms, the x-y product
 // formed once per atom and chunk and exp(i |l| z) stepped from one table
-// value, serving +l and -l; fixed-order reductions); k_ew_force (two atoms per lane, four waves on
-// different runs: per run the x-y product once, then exp(i l z) for
-// l = 0, 1, ... by the
-// same addition-theorem step that built the tables, so the values are the
-// tables' without reading them, and the terms of +l and -l; then the
+// value, serving +l and -l; fixed-order reductions); k_ew_force (two atoms
+// per lane, the block's waves on different runs: per run the x-y product
+// once, then exp(i l z) for l = 0, 1, ... by the same addition-theorem step
+// that built the tables, so the values are the tables' without reading
+// them, and the terms of +l and -l, with per-k records (k-vector and force
+// prefactors) written by k_ew_sk; then the
 // correction of the atom's excluded pairs, each pair evaluated by both of
 // its atoms so no atomics are needed); k_ew_finish (one block: energy and
 // virial sums in a fixed order).  Work is N x K (atoms x k-vectors), as in
@@ -32,13 +33,20 @@ namespace pmnb {
 
 constexpr int kEwBlock = 256;
 constexpr int kEwChunk = 8;   // |l| values per structure-factor block
-constexpr int kEwLanes = kEwBlock / 64;  // waves sharing an atom in k_ew_force
+// waves sharing an atom in k_ew_force: 4, or 16 when the system gives
+// fewer blocks than the chip has CUs
+constexpr int kEwLanesLarge = 4, kEwLanesSmall = 16;
 
 struct EwRun {
   int h, k, l_lo, l_hi, start;  // l in [l_lo, l_hi] at list index start + l - l_lo
 };
 struct EwChunk {
   int run, m0, m1;              // l = +-m, m in [m0, m1], of run
+};
+// What the force needs of one k-vector: the force on atom i is
+// k * (sA_i ps - cA_i pc), ps = a0 b S_cos, pc = a0 b S_sin, a0 = 8 pi / V
+struct EwKF {
+  double kx, ky, kz, ps, pc;
 };
 
 struct EwDev {
@@ -51,6 +59,7 @@ struct EwDev {
   const double *q;            // scaled charges
   double *tab;                // 3 * H * 2 * n
   double2 *sk;                // per k: (sum sin, sum cos)
+  EwKF *kf;                   // per k: k-vector and force prefactors
   const int *xptr, *xent;     // per atom: exclusion rows (row << 1 | role)
   const int *xrows;           // 3 per row: a1, a2, class
   int n_excl;
@@ -162,20 +171,27 @@ __global__ __launch_bounds__(kEwBlock) void k_ew_sk(EwDev d) {
       a += red[w][4 * jj + 2 * neg];
       b += red[w][4 * jj + 2 * neg + 1];
     }
-    d.sk[r.start + (l - r.l_lo)] = make_double2(a, b);
+    const int kk = r.start + (l - r.l_lo);
+    d.sk[kk] = make_double2(a, b);
+    const double4 k = d.kv[kk];
+    const double ak = 8.0 * M_PI * d.Vr * k.w;
+    d.kf[kk] = EwKF{k.x, k.y, k.z, ak * b, ak * a};
   }
 }
 
 // Force on atom i: every k-vector, then the correction of its excluded
 // pairs (the pair force of row (a1, a2) acts as -f on a1 and +f on a2).
 // A block holds 2 x 64 atoms (each lane two, which share the k-vector
-// loads); its wave w takes runs w, w + 4, ... (every lane of a wave on the
-// same run, so those loads are wave-uniform), and the four partial forces
-// of an atom are added in wave order through LDS, by wave 0 for the first
-// 64 atoms and by wave 1 for the second.
-__global__ __launch_bounds__(kEwBlock) void k_ew_force(EwDev d,
-                                                       const double *x,
-                                                       double *f) {
+// loads); its wave w takes runs w, w + kEwLanes, ... (every lane of a wave
+// on the same run, so those loads are wave-uniform), and the kEwLanes
+// partial forces of an atom are added in wave order through LDS, by wave 0
+// for the first 64 atoms and by wave 1 for the second.  Four waves per block
+// on a 96k-atom system (750 blocks) measured faster than five or six; a
+// small system, with fewer blocks than CUs, takes sixteen.
+template <int kEwLanes>
+__global__ __launch_bounds__(64 * kEwLanes) void k_ew_force(EwDev d,
+                                                            const double *x,
+                                                            double *f) {
   __shared__ double part_f[kEwLanes][2][3][64];
   // part is the wave's index: uniform, so the run and k-vector data below
   // come through scalar loads
@@ -191,7 +207,6 @@ __global__ __launch_bounds__(kEwBlock) void k_ew_force(EwDev d,
     zs1[u] = ew_tab(d, 2, 1, 0, at[u]);
     zc1[u] = ew_tab(d, 2, 1, 1, at[u]);
   }
-  const double a0 = 8.0 * M_PI * d.Vr;
   for (int ri = part; ri < d.n_runs; ri += kEwLanes) {
     const EwRun r = d.runs[ri];
     const int ka = r.k < 0 ? -r.k : r.k;
@@ -207,6 +222,7 @@ __global__ __launch_bounds__(kEwBlock) void k_ew_force(EwDev d,
       zs[u] = 0.0;  // exp(i m z) for m = 0
       zc[u] = 1.0;
     }
+    const EwKF *kr = d.kf + (r.start - r.l_lo);  // kr[l], l in [l_lo, l_hi]
     const int mmax = max(-r.l_lo, r.l_hi);
     for (int m = 0; m <= mmax; m++) {
       if (m > 0) {  // the step k_ew_tables takes
@@ -217,20 +233,16 @@ __global__ __launch_bounds__(kEwBlock) void k_ew_force(EwDev d,
       for (int sg = 0; sg < 2; sg++) {
         const int l = sg == 0 ? m : -m;
         if ((sg == 1 && m == 0) || l < r.l_lo || l > r.l_hi) continue;
-        const int kk = r.start + (l - r.l_lo);
-        const double4 k = d.kv[kk];
-        const double2 s = d.sk[kk];
-        const double ak = a0 * k.w;
-        const double ps = ak * s.y, pc = ak * s.x;  // w = sA ps - cA pc
+        const EwKF k = kr[l];
 #pragma unroll
         for (int u = 0; u < 2; u++) {
           const double zsl = sg == 0 ? zs[u] : -zs[u];  // sin(l z), l = +-m
           const double sA = fma(xyc[u], zsl, xys[u] * zc[u]);
           const double cA = fma(-xys[u], zsl, xyc[u] * zc[u]);
-          const double w = fma(sA, ps, -(cA * pc));
-          fx[u] = fma(k.x, w, fx[u]);
-          fy[u] = fma(k.y, w, fy[u]);
-          fz[u] = fma(k.z, w, fz[u]);
+          const double w = fma(sA, k.ps, -(cA * k.pc));
+          fx[u] = fma(k.kx, w, fx[u]);
+          fy[u] = fma(k.ky, w, fy[u]);
+          fz[u] = fma(k.kz, w, fz[u]);
         }
       }
     }
