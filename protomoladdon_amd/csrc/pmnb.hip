@@ -2232,7 +2232,7 @@ struct pmnb_ewald {
   double alpha = 0, alpha2 = 0, rc = 0, kc = 0, fac = 0, V = 0, Vr = 0;
   double csf = 1.0, e_self = 0, e_charged = 0;
   int n_runs = 0, n_chunks = 0;
-  DevBuf kv, q, tab, sk, kf, xptr, xent, xrows, runs, chunks;
+  DevBuf kv, q, tab, sk, kf, part, xptr, xent, xrows, runs, chunks;
 };
 
 extern "C" {
@@ -2353,6 +2353,7 @@ int pmnb_ewald_create(pmnb_ctx *ctx, double alpha, double accuracy,
     e = ew->tab.ensure((size_t)3 * ew->H * 2 * std::max(n_atoms, 1) * sizeof(double));
   if (e == hipSuccess) e = ew->sk.ensure((size_t)std::max(ew->nk, 1) * sizeof(double2));
   if (e == hipSuccess) e = ew->kf.ensure((size_t)std::max(ew->nk, 1) * sizeof(EwKF));
+  if (e == hipSuccess) e = ew->part.ensure((size_t)kEwFinBlocks * kEwFinSums * sizeof(double));
   if (e == hipSuccess) e = up(ew->xptr, cnt.data(), (n_atoms + 1) * sizeof(int));
   if (e == hipSuccess) e = up(ew->xent, ent.data(), 2 * (size_t)n_excl * sizeof(int));
   if (e == hipSuccess) e = up(ew->xrows, excl, 3 * (size_t)n_excl * sizeof(int));
@@ -2407,6 +2408,7 @@ int pmnb_ewald_evaluate_device(pmnb_ewald *ew, const double *d_positions,
   d.tab = ew->tab.as<double>();
   d.sk = ew->sk.as<double2>();
   d.kf = ew->kf.as<EwKF>();
+  d.part = ew->part.as<double>();
   d.xptr = ew->xptr.as<int>();
   d.xent = ew->xent.as<int>();
   d.xrows = ew->xrows.as<int>();
@@ -2428,8 +2430,10 @@ int pmnb_ewald_evaluate_device(pmnb_ewald *ew, const double *d_positions,
   else
     hipLaunchKernelGGL(k_ew_force<kEwLanesSmall>, dim3(nbf), dim3(64 * kEwLanesSmall),
                        0, s, d, d_positions, d_forces);
-  hipLaunchKernelGGL(k_ew_finish, dim3(1), dim3(kEwBlock), 0, s, d, d_positions,
-                     d_out, (int)((flags & PMNB_FLAG_VIRIAL) != 0),
+  hipLaunchKernelGGL(k_ew_partials, dim3(kEwFinBlocks), dim3(kEwBlock), 0, s, d,
+                     d_positions);
+  hipLaunchKernelGGL(k_ew_finish, dim3(1), dim3(64), 0, s, d, d_out,
+                     (int)((flags & PMNB_FLAG_VIRIAL) != 0),
                      ew->e_self + ew->e_charged, ew->e_charged);
   HIPCHK(hipGetLastError());
   return PMNB_OK;
@@ -2439,7 +2443,7 @@ void pmnb_ewald_destroy(pmnb_ewald *ew) {
   if (!ew) return;
   (void)hipSetDevice(ew->device);
   (void)hipDeviceSynchronize();
-  DevBuf *bufs[] = {&ew->kv, &ew->q, &ew->tab, &ew->sk, &ew->kf,
+  DevBuf *bufs[] = {&ew->kv, &ew->q, &ew->tab, &ew->sk, &ew->kf, &ew->part,
                     &ew->xptr, &ew->xent, &ew->xrows, &ew->runs, &ew->chunks};
   for (DevBuf *b : bufs) b->release();
   delete ew;

@@ -24,9 +24,10 @@
 // them, and the terms of +l and -l, with per-k records (k-vector and force
 // prefactors) written by k_ew_sk; then the
 // correction of the atom's excluded pairs, each pair evaluated by both of
-// its atoms so no atomics are needed); k_ew_finish (one block: energy and
-// virial sums in a fixed order).  Work is N x K (atoms x k-vectors), as in
-// the reference.
+// its atoms so no atomics are needed); k_ew_partials and k_ew_finish
+// (energy and virial sums: fixed slices over 128 blocks, then one wave adds
+// the blocks' sums in block order).  Work is N x K (atoms x k-vectors), as
+// in the reference.
 #pragma once
 
 namespace pmnb {
@@ -60,6 +61,7 @@ struct EwDev {
   double *tab;                // 3 * H * 2 * n
   double2 *sk;                // per k: (sum sin, sum cos)
   EwKF *kf;                   // per k: k-vector and force prefactors
+  double *part;               // k_ew_partials' sums, 14 per block
   const int *xptr, *xent;     // per atom: exclusion rows (row << 1 | role)
   const int *xrows;           // 3 per row: a1, a2, class
   int n_excl;
@@ -292,20 +294,36 @@ __global__ __launch_bounds__(64 * kEwLanes) void k_ew_force(EwDev d,
   f[3 * i + 2] += gz;
 }
 
-// out[1] += reciprocal + correction + constant energies; with the virial
-// flag out[2..10] += reciprocal and correction virial + charged-system
-// diagonal.  consts: (self + charged energy, charged energy).
-__global__ __launch_bounds__(kEwBlock) void k_ew_finish(EwDev d,
-                                                        const double *x,
-                                                        double *out,
-                                                        int virial,
-                                                        double e_const,
-                                                        double e_charged) {
-  __shared__ double red[kEwBlock];
+// Energy and virial: k_ew_partials spreads the k-vectors and the excluded
+// pairs over kEwFinBlocks blocks (contiguous slices), each writing its 14
+// sums (reciprocal energy, six reciprocal virial sums, correction energy,
+// six correction virial sums) after a fixed-order tree; k_ew_finish adds
+// the blocks' sums in block order and scales.  Both orders are fixed, so
+// results repeat bit for bit.
+constexpr int kEwFinBlocks = 128, kEwFinSums = 14;
+
+__device__ __forceinline__ double ew_block_sum(double v, double *red) {
   const int t = threadIdx.x;
-  double acc[8];  // energy, xx, xy, xz, yy, yz, zz, intra energy
+  red[t] = v;
+  __syncthreads();
+  for (int o = kEwBlock / 2; o > 0; o >>= 1) {
+    if (t < o) red[t] += red[t + o];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(kEwBlock) void k_ew_partials(EwDev d,
+                                                          const double *x) {
+  __shared__ double red[kEwBlock];
+  const int t = threadIdx.x, b = blockIdx.x;
+  double acc[8];  // energy, xx, xy, xz, yy, yz, zz, correction energy
   for (int c = 0; c < 8; c++) acc[c] = 0;
-  for (int kk = t; kk < d.nk; kk += kEwBlock) {
+  const int k0 = (int)((long long)d.nk * b / kEwFinBlocks),
+            k1 = (int)((long long)d.nk * (b + 1) / kEwFinBlocks);
+  for (int kk = k0 + t; kk < k1; kk += kEwBlock) {
     const double4 k = d.kv[kk];
     const double2 s = d.sk[kk];
     const double k2 = k.x * k.x + k.y * k.y + k.z * k.z;
@@ -319,10 +337,11 @@ __global__ __launch_bounds__(kEwBlock) void k_ew_finish(EwDev d,
     acc[5] -= e * c * k.y * k.z;
     acc[6] += e * (1.0 - c * k.z * k.z);
   }
-  // correction: energy (acc[7]) and virial of each excluded pair, scaled
-  // into the same accumulators after the reciprocal factor is applied
+  // correction: energy (acc[7]) and virial of each excluded pair
   double cv[6] = {0, 0, 0, 0, 0, 0};
-  for (int row = t; row < d.n_excl; row += kEwBlock) {
+  const int r0 = (int)((long long)d.n_excl * b / kEwFinBlocks),
+            r1 = (int)((long long)d.n_excl * (b + 1) / kEwFinBlocks);
+  for (int row = r0 + t; row < r1; row += kEwBlock) {
     const int a1 = d.xrows[3 * row], a2 = d.xrows[3 * row + 1],
               cls = d.xrows[3 * row + 2];
     double rx, ry, rz, r2;
@@ -342,27 +361,32 @@ __global__ __launch_bounds__(kEwBlock) void k_ew_finish(EwDev d,
     cv[4] += f1 * rz;
     cv[5] += f2 * rz;
   }
-  double tot[14];
+  double *out = d.part + (size_t)b * kEwFinSums;
   for (int c = 0; c < 8; c++) {
-    red[t] = acc[c];
-    __syncthreads();
-    for (int o = kEwBlock / 2; o > 0; o >>= 1) {
-      if (t < o) red[t] += red[t + o];
-      __syncthreads();
-    }
-    tot[c] = red[0];
-    __syncthreads();
+    const double v = ew_block_sum(acc[c], red);
+    if (t == 0) out[c] = v;
   }
   for (int c = 0; c < 6; c++) {
-    red[t] = cv[c];
-    __syncthreads();
-    for (int o = kEwBlock / 2; o > 0; o >>= 1) {
-      if (t < o) red[t] += red[t + o];
-      __syncthreads();
-    }
-    tot[8 + c] = red[0];
-    __syncthreads();
+    const double v = ew_block_sum(cv[c], red);
+    if (t == 0) out[8 + c] = v;
   }
+}
+
+// One wave: lane c < 14 adds sum c of every block, in block order; then
+// out[1] += reciprocal + correction + constant energies, and with the
+// virial flag out[2..10] += reciprocal and correction virial +
+// charged-system diagonal (e_const = self + charged energy).
+__global__ __launch_bounds__(64) void k_ew_finish(EwDev d, double *out,
+                                                  int virial, double e_const,
+                                                  double e_charged) {
+  __shared__ double tot[kEwFinSums];
+  const int t = threadIdx.x;
+  if (t < kEwFinSums) {
+    double v = 0.0;
+    for (int b = 0; b < kEwFinBlocks; b++) v += d.part[(size_t)b * kEwFinSums + t];
+    tot[t] = v;
+  }
+  __syncthreads();
   if (t == 0) {
     const double c = 4.0 * M_PI * d.Vr;
     out[1] += c * tot[0] + tot[7] + e_const;
