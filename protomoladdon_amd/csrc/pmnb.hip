@@ -2423,13 +2423,13 @@ int pmnb_ewald_evaluate_device(pmnb_ewald *ew, const double *d_positions,
   hipLaunchKernelGGL(k_ew_tables, dim3(nb), dim3(kEwBlock), 0, s, d, d_positions);
   if (ew->n_chunks > 0)
     hipLaunchKernelGGL(k_ew_sk, dim3(ew->n_chunks), dim3(kEwBlock), 0, s, d);
-  const int nbf = blocks_for(ew->n, 128);
+  const int nbf = blocks_for(ew->n, 64 * kEwAtomsLarge);
   if (nbf >= 256)
-    hipLaunchKernelGGL(k_ew_force<kEwLanesLarge>, dim3(nbf), dim3(64 * kEwLanesLarge),
-                       0, s, d, d_positions, d_forces);
+    hipLaunchKernelGGL((k_ew_force<kEwLanesLarge, kEwAtomsLarge>), dim3(nbf),
+                       dim3(64 * kEwLanesLarge), 0, s, d, d_positions, d_forces);
   else
-    hipLaunchKernelGGL(k_ew_force<kEwLanesSmall>, dim3(nbf), dim3(64 * kEwLanesSmall),
-                       0, s, d, d_positions, d_forces);
+    hipLaunchKernelGGL((k_ew_force<kEwLanesSmall, 2>), dim3(blocks_for(ew->n, 128)),
+                       dim3(64 * kEwLanesSmall), 0, s, d, d_positions, d_forces);
   hipLaunchKernelGGL(k_ew_partials, dim3(kEwFinBlocks), dim3(kEwBlock), 0, s, d,
                      d_positions);
   hipLaunchKernelGGL(k_ew_finish, dim3(1), dim3(64), 0, s, d, d_out,

@@ -37,6 +37,7 @@ constexpr int kEwChunk = 8;   // |l| values per structure-factor block
 // waves sharing an atom in k_ew_force: 4, or 16 when the system gives
 // fewer blocks than the chip has CUs
 constexpr int kEwLanesLarge = 4, kEwLanesSmall = 16;
+constexpr int kEwAtomsLarge = 2;  // atoms per lane of the large launch
 
 struct EwRun {
   int h, k, l_lo, l_hi, start;  // l in [l_lo, l_hi] at list index start + l - l_lo
@@ -183,38 +184,41 @@ __global__ __launch_bounds__(kEwBlock) void k_ew_sk(EwDev d) {
 
 // Force on atom i: every k-vector, then the correction of its excluded
 // pairs (the pair force of row (a1, a2) acts as -f on a1 and +f on a2).
-// A block holds 2 x 64 atoms (each lane two, which share the k-vector
-// loads); its wave w takes runs w, w + kEwLanes, ... (every lane of a wave
-// on the same run, so those loads are wave-uniform), and the kEwLanes
-// partial forces of an atom are added in wave order through LDS, by wave 0
-// for the first 64 atoms and by wave 1 for the second.  Four waves per block
-// on a 96k-atom system (750 blocks) measured faster than five or six; a
-// small system, with fewer blocks than CUs, takes sixteen.
-template <int kEwLanes>
+// A block holds kEwAtoms x 64 atoms (each lane kEwAtoms, which share the
+// k-vector loads); its wave w takes runs w, w + kEwLanes, ... (every lane of
+// a wave on the same run, so those loads are wave-uniform), and the kEwLanes
+// partial forces of an atom are added in wave order through LDS, by wave u
+// for the atoms of slot u (kEwLanes >= kEwAtoms).  Four waves per block
+// and two atoms per lane on a 96k-atom system (750 blocks) measured faster
+// than five or six waves, and than one atom per lane (more waves, but one
+// dependent exp(i m z) chain per lane) or three; a small system, with fewer
+// blocks than CUs, takes sixteen waves.
+template <int kEwLanes, int kEwAtoms>
 __global__ __launch_bounds__(64 * kEwLanes) void k_ew_force(EwDev d,
                                                             const double *x,
                                                             double *f) {
-  __shared__ double part_f[kEwLanes][2][3][64];
+  __shared__ double part_f[kEwLanes][kEwAtoms][3][64];
   // part is the wave's index: uniform, so the run and k-vector data below
   // come through scalar loads
   const int lane = threadIdx.x & 63,
             part = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  int at[2];
-  double fx[2] = {0, 0}, fy[2] = {0, 0}, fz[2] = {0, 0};
-  double zs1[2], zc1[2];
+  int at[kEwAtoms];
+  double fx[kEwAtoms], fy[kEwAtoms], fz[kEwAtoms];
+  double zs1[kEwAtoms], zc1[kEwAtoms];
 #pragma unroll
-  for (int u = 0; u < 2; u++) {
-    const int i0 = blockIdx.x * 128 + u * 64 + lane;
+  for (int u = 0; u < kEwAtoms; u++) {
+    const int i0 = blockIdx.x * (64 * kEwAtoms) + u * 64 + lane;
     at[u] = i0 < d.n ? i0 : d.n - 1;  // past the end: a real atom, not written
+    fx[u] = fy[u] = fz[u] = 0.0;
     zs1[u] = ew_tab(d, 2, 1, 0, at[u]);
     zc1[u] = ew_tab(d, 2, 1, 1, at[u]);
   }
   for (int ri = part; ri < d.n_runs; ri += kEwLanes) {
     const EwRun r = d.runs[ri];
     const int ka = r.k < 0 ? -r.k : r.k;
-    double xys[2], xyc[2], zs[2], zc[2];
+    double xys[kEwAtoms], xyc[kEwAtoms], zs[kEwAtoms], zc[kEwAtoms];
 #pragma unroll
-    for (int u = 0; u < 2; u++) {
+    for (int u = 0; u < kEwAtoms; u++) {
       const int i = at[u];
       const double xs = ew_tab(d, 0, r.h, 0, i), xc = ew_tab(d, 0, r.h, 1, i);
       const double ys = (r.k < 0 ? -1 : 1) * ew_tab(d, 1, ka, 0, i);
@@ -229,7 +233,7 @@ __global__ __launch_bounds__(64 * kEwLanes) void k_ew_force(EwDev d,
     for (int m = 0; m <= mmax; m++) {
       if (m > 0) {  // the step k_ew_tables takes
 #pragma unroll
-        for (int u = 0; u < 2; u++) ew_step(zs1[u], zc1[u], zs[u], zc[u]);
+        for (int u = 0; u < kEwAtoms; u++) ew_step(zs1[u], zc1[u], zs[u], zc[u]);
       }
 #pragma unroll
       for (int sg = 0; sg < 2; sg++) {
@@ -237,7 +241,7 @@ __global__ __launch_bounds__(64 * kEwLanes) void k_ew_force(EwDev d,
         if ((sg == 1 && m == 0) || l < r.l_lo || l > r.l_hi) continue;
         const EwKF k = kr[l];
 #pragma unroll
-        for (int u = 0; u < 2; u++) {
+        for (int u = 0; u < kEwAtoms; u++) {
           const double zsl = sg == 0 ? zs[u] : -zs[u];  // sin(l z), l = +-m
           const double sA = fma(xyc[u], zsl, xys[u] * zc[u]);
           const double cA = fma(-xys[u], zsl, xyc[u] * zc[u]);
@@ -250,14 +254,14 @@ __global__ __launch_bounds__(64 * kEwLanes) void k_ew_force(EwDev d,
     }
   }
 #pragma unroll
-  for (int u = 0; u < 2; u++) {
+  for (int u = 0; u < kEwAtoms; u++) {
     part_f[part][u][0][lane] = fx[u];
     part_f[part][u][1][lane] = fy[u];
     part_f[part][u][2][lane] = fz[u];
   }
   __syncthreads();
-  if (part > 1) return;
-  const int i0 = blockIdx.x * 128 + part * 64 + lane;
+  if (part >= kEwAtoms) return;
+  const int i0 = blockIdx.x * (64 * kEwAtoms) + part * 64 + lane;
   if (i0 >= d.n) return;
   const int i = i0;
   double gx = 0, gy = 0, gz = 0;
