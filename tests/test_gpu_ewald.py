@@ -80,3 +80,24 @@ def test_kspace_matches_oracle(case):
     scale = abs(er) + np.sum(q * q) * dinfo[0] / np.sqrt(np.pi)
     assert abs(od[1] - er) <= 1e-13 * scale
     assert np.abs(od[2:11] - vr).max() <= 1e-10 * np.abs(vr).max()
+
+
+def test_kspace_matches_oracle_orthorhombic_box():
+    """Three different box edges (13.0 x 14.5 x 12.2 A): per-axis k-vector
+    bounds, tables and minimum images; scaled-1-4 chains with a net charge
+    (the charged-system energy and virial diagonal)."""
+    from tests.test_oracle_ewald import chains, make_system
+    pos, q, _, bonds, angles, dih, mol = chains(seed=11)
+    L = np.array([13.0, 14.5, 12.2])
+    w, s = make_system(np.mod(pos, L), q, L, bonds, angles, dih, mol, csf=0.5)
+    alpha, acc = 0.7, 1e-8
+    print("orthorhombic", w.n_atoms, flush=True)
+    fr, er, vr, info = O.ewald_kspace(s, alpha, acc)
+    fd, od, dinfo = run_device(w, alpha, acc)
+    print("alpha %.6f rc %.4f kc %.4f nk %d" % dinfo, flush=True)
+    assert dinfo[3] == int(info[3])
+    assert np.abs(fd - fr).max() <= 1e-10 * np.abs(fr).max()
+    qs = w.topology.scaled_charges
+    scale = abs(er) + np.sum(qs * qs) * alpha / np.sqrt(np.pi)
+    assert abs(od[1] - er) <= 1e-13 * scale
+    assert np.abs(od[2:11] - vr).max() <= 1e-10 * np.abs(vr).max()
