@@ -1,16 +1,20 @@
 """Benchmark of the short-range nonbonded force path on one MI355X.
 
 Workload: 32,000 rigid-geometry waters (96,000 atoms) in a periodic 98.6 A
-box, LJ (O-O) + Ewald real-space erfc(0.32 r)/r, C2 switching 7.5 -> 9 A,
-cell list rebuilt from the positions on every step.  One step = one full
-force evaluation (cell hashing, counting sort, pair kernel, reductions) on
-positions already resident in HBM.  ns/day = steps/s * 1 fs * 86400 / 1e6;
-pair evals/s = P_alg * steps/s with P_alg the in-cutoff pair count.
+box, LJ (O-O) + Ewald real-space erfc(0.32 r)/r, C2 switching 7.5 -> 9 A.
+One step = one whole 1 fs leapfrog step on the device: half kicks, drift,
+SHAKE and RATTLE, the cell list rebuilt from the positions and the force
+evaluated (pair kernel and reductions), with positions, velocities and
+forces resident in HBM.  ns/day = steps/s * 1 fs * 86400 / 1e6 of that MD
+step is `value`.  pair evals/s = P_alg * force evaluations/s, P_alg the
+in-cutoff pair count, comes from force-only evaluations at fixed positions.
 
 Prints one JSON line.  The mixed-precision path gives `value`; the FP64 GPU
-path runs the same workload beside it.  The CPU baseline is the FP64 C
-restatement of the reference path (oracle/pmnb_oracle.c), one host thread,
-on a bounded sample of steps.
+path runs the same workload beside it.  Extra keys hold the batched-replica
+and force-matching workloads in compact form.  The CPU baseline is the FP64
+C restatement of the reference path (oracle/pmnb_oracle.c), one host thread
+and all host threads, on a bounded sample of steps; the same leg checks the
+GPU forces against that FP64 oracle (RMS relative error per workload).
 """
 import argparse
 import json
@@ -133,28 +137,77 @@ def time_ewald_kspace(w, alpha, accuracy, reps, torch):
             "atom_kvector_terms_per_s": w.n_atoms * nk / (ms * 1e-3)}
 
 
-def cpu_baseline(w, budget_s):
+def host_cpu():
+    """(model name, logical CPUs visible to this process)."""
+    import platform
+    import subprocess
+    model = platform.processor()
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def rms_rel(f, ref):
+    return float(np.sqrt(np.mean(np.sum((f - ref) ** 2, 1)) /
+                         np.mean(np.sum(ref ** 2, 1))))
+
+
+def cpu_baseline(w, budget_s, checks, batch=None):
+    """The FP64 C restatement timed on the host (one thread, then all
+    threads) on a bounded sample of force evaluations of `w`; then, as the
+    checker, the RMS relative force error of each GPU result in `checks`
+    ({name: (workload, {precision: forces})}) and of each replica of
+    `batch` ([(workload, mixed forces)]) against the FP64 oracle."""
     from oracle import oracle as O
     O.build()
     s = O.System.from_workload(w, flags=1)
     times = []
-    npairs = 0
+    ref = None
     t_start = time.perf_counter()
     while True:
         t0 = time.perf_counter()
         r = O.evaluate(s)
         times.append(time.perf_counter() - t0)
-        npairs = r.n_pairs
+        ref = r if ref is None else ref
         if time.perf_counter() - t_start > budget_s or len(times) >= 20:
             break
-    t = float(np.median(times))
-    return {"value": ns_per_day(1.0 / t), "unit": "ns/day", "cores": 1,
+    t1 = float(np.median(times))
+    model, ncpu = host_cpu()
+    threads = min(ncpu, 16)
+    tt = []
+    for _ in range(max(3, min(10, len(times)))):
+        t0 = time.perf_counter()
+        _, _, _, n_thr, used = O.evaluate_threads(s, threads)
+        tt.append(time.perf_counter() - t0)
+    tall = float(np.median(tt))
+    base = {"value": ns_per_day(1.0 / t1), "unit": "ns/day", "cores": 1,
             "kind": "port",
-            "pair_evals_per_s": npairs / t,
-            "sample": "%d full force evaluations of the 96k-atom workload "
-                      "(median %.3f s each), FP64 C restatement of "
-                      "NonbondedCutoffSystemForce, cellsize rc/2, 1 thread, "
-                      "gcc -O2" % (len(times), t)}
+            "pair_evals_per_s": ref.n_pairs / t1,
+            "sample": "%d force evaluations of the 96k-atom workload (median "
+                      "%.3f s each; one force evaluation per 1 fs step), FP64 "
+                      "C restatement of NonbondedCutoffSystemForce, cellsize "
+                      "rc/2, 1 thread, gcc -O2" % (len(times), t1),
+            "all_threads": {"value": ns_per_day(1.0 / tall), "unit": "ns/day",
+                            "cores": used, "s_per_eval": tall,
+                            "pair_evals_per_s": n_thr / tall,
+                            "sample": "%d evaluations, OpenMP over cells"
+                                      % len(tt)},
+            "host_cpu": model, "nproc": ncpu}
+    acc = {}
+    for name, (wk, results) in checks.items():
+        r = ref if wk is w else O.evaluate(O.System.from_workload(wk, flags=1))
+        acc[name] = {p: rms_rel(f, r.forces) for p, f in results.items()}
+    if batch:
+        errs = [rms_rel(f, O.evaluate(O.System.from_workload(
+            wk, flags=1)).forces) for wk, f in batch]
+        acc["batch512"] = {"mixed_max": max(errs),
+                           "mixed_mean": float(np.mean(errs))}
+    return base, acc
 
 
 def bench_batch(args, torch):
@@ -211,6 +264,7 @@ def bench_batch(args, torch):
     for f in singles:
         f.close()
     bf.close()
+    args._batch_check = (ws, off, fb)
     return {
         "metric": "batched replicas: ns/day per replica and aggregate pair "
                   "evals/s, 512 waters boxes LJ+Ewald-real, one graph replay "
@@ -337,12 +391,15 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=0,
                     help="steps of the FP64 path (default: steps/4)")
     ap.add_argument("--cpu-budget", type=float, default=20.0,
-                    help="seconds of CPU baseline sampling (0 = skip)")
+                    help="seconds of CPU baseline sampling (0 = skip the "
+                         "CPU leg and the oracle accuracy checks)")
     ap.add_argument("--workload", default="water96k",
                     choices=["water96k", "batch512", "forcematch250k"],
-                    help="water96k: the headline line; batch512 and "
-                         "forcematch250k: the batched-replica and "
-                         "differentiation workloads")
+                    help="water96k: the headline line (with compact batch512 "
+                         "and forcematch250k summaries); batch512 and "
+                         "forcematch250k alone: their own full lines")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="leave out the batch512 / forcematch250k summaries")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB "
                          "(3 times): the reference kernel that scales the "
@@ -377,10 +434,10 @@ def main():
     w = systems.config1()
     flags = 1  # atomic virial on, as in the per-pair FLOP convention
 
-    print("mixed precision ...", file=sys.stderr, flush=True)
+    print("force, mixed ...", file=sys.stderr, flush=True)
     mx = time_gpu(w, "mixed", args.steps, args.warmup, flags, torch)
     fp64_steps = args.fp64_steps or max(10, args.steps // 4)
-    print("fp64 ...", file=sys.stderr, flush=True)
+    print("force, fp64 ...", file=sys.stderr, flush=True)
     f64 = time_gpu(w, "fp64", fp64_steps, max(2, args.warmup // 4), flags,
                    torch)
 
@@ -408,68 +465,110 @@ def main():
     host_ms = 1e3 * (time.perf_counter() - t0) / nh
     fh.close()
 
+    # forces of the other single-system workloads for the accuracy check
+    checks = {"water96k": (w, {"mixed": mx["forces"], "fp64": f64["forces"]})}
+    if args.cpu_budget > 0:
+        for name, wk in (("water3k", systems.config0()),
+                         ("chains250k", systems.config2())):
+            print("forces of %s ..." % name, file=sys.stderr, flush=True)
+            checks[name] = (wk, {p: time_gpu(wk, p, 1, 1, flags, torch)[
+                "forces"] for p in ("mixed", "fp64")})
+
+    extras = {}
+    if not args.no_extras:
+        ea = argparse.Namespace(steps=max(20, args.steps // 4),
+                                warmup=max(3, args.warmup // 4))
+        print("batch512 ...", file=sys.stderr, flush=True)
+        b = bench_batch(ea, torch)
+        extras["batch512"] = {k: b[k] for k in (
+            "value", "unit", "ms_per_step", "aggregate_ns_per_day",
+            "pair_evals_per_s", "sequential_ms_per_step",
+            "speedup_vs_sequential", "max_abs_force_diff_vs_sequential",
+            "autotune", "config")}
+        print("forcematch250k ...", file=sys.stderr, flush=True)
+        fm = bench_forcematch(ea, torch)
+        extras["forcematch250k"] = {k: fm[k] for k in (
+            "value", "unit", "ms_per_step", "fp64", "mixed",
+            "speedup_vs_fp64", "rms_rel_grad_err_mixed_vs_fp64",
+            "fd_check_64_params_worst_rel_err", "config")}
+        if args.cpu_budget > 0:
+            ws, off, fb = ea._batch_check
+            checks["batch512"] = [(wk, fb[off[r]:off[r + 1]])
+                                  for r, wk in enumerate(ws)]
+
     sps = 1e3 / mx["ms_per_step"]
     sps64 = 1e3 / f64["ms_per_step"]
     P = mx["pairs"]
-    ref = f64["forces"]
-    rms = float(np.sqrt(np.mean(np.sum((mx["forces"] - ref) ** 2, 1)) /
-                        np.mean(np.sum(ref ** 2, 1))))
     achieved = P * FLOP_PER_PAIR / (mx["kernel_ms"] * 1e-3) / 1e12
     achieved64 = P * FLOP_PER_PAIR / (f64["kernel_ms"] * 1e-3) / 1e12
 
     # HBM bytes per launch of the pair kernel, from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE passes (scaled by a copy kernel of known size)
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "k_tile_water96k_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "k_half_water96k_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
 
-    cpu = None
+    cpu, accuracy = None, None
     if args.cpu_budget > 0:
-        print("cpu baseline ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(w, args.cpu_budget)
+        print("cpu baseline and oracle checks ...", file=sys.stderr,
+              flush=True)
+        batch = checks.pop("batch512", None)
+        cpu, accuracy = cpu_baseline(w, args.cpu_budget, checks, batch)
 
     line = {
         "metric": METRIC,
-        "value": ns_per_day(sps),
+        "value": md_mx["value"],
         "unit": "ns/day",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": mx["ms_per_step"],
+        "ms_per_step": md_mx["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "water96k_ewald: 32,000 waters (96,000 atoms), 98.6 A "
-                        "periodic box, LJ O-O + Ewald real space alpha 0.32, "
-                        "C2 7.5->9 A, cell list rebuilt every step, virial on",
+            "workload": "water96k_ewald MD: 32,000 waters (96,000 atoms), "
+                        "98.6 A periodic box, LJ O-O + Ewald real space "
+                        "alpha 0.32, C2 7.5->9 A; one step = leapfrog 1 fs "
+                        "+ SHAKE/RATTLE + cell-list rebuild + force, virial "
+                        "on",
             "n_atoms": w.n_atoms,
             "pairs_per_step": P,
         },
         "pair_evals_per_s": P * sps,
-        "fp64_gpu": {"value": ns_per_day(sps64), "unit": "ns/day",
-                     "ms_per_step": f64["ms_per_step"],
+        "force_only": {"value": ns_per_day(sps), "unit": "ns/day",
+                       "ms_per_step": mx["ms_per_step"],
+                       "pair_kernel_ms": mx["kernel_ms"],
+                       "cell_build_ms": mx["build_ms"]},
+        "fp64_gpu": {"value": md_64["value"], "unit": "ns/day",
+                     "ms_per_step": md_64["ms_per_step"],
+                     "force_only_ns_per_day": ns_per_day(sps64),
+                     "force_only_ms_per_step": f64["ms_per_step"],
                      "pair_evals_per_s": f64["pairs"] * sps64,
                      "pair_kernel_ms": f64["kernel_ms"],
                      "cell_build_ms": f64["build_ms"],
                      "pair_kernel_tflops": achieved64,
                      "pairs_per_step": f64["pairs"]},
-        "speedup_vs_fp64": sps / sps64,
+        "speedup_vs_fp64": {"md": md_mx["value"] / md_64["value"],
+                            "force_only": sps / sps64,
+                            "pair_kernel": f64["kernel_ms"] / mx["kernel_ms"]},
         "md": {"integrator": "Leapfrog 1 fs + SHAKE/RATTLE (1e-5, 30), "
                              "whole step on the device, virial on",
                "mixed": md_mx, "fp64": md_64},
         "ewald_kspace": ewald,
         "host_api_ms_per_step": host_ms,
-        "rms_rel_force_err_vs_fp64": rms,
+        "accuracy_rms_rel_force_err_vs_fp64_oracle": accuracy,
         "roofline": {"bound": "valu", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "traffic_unit": "bytes per launch (HBM, rocprofv3)",
-                     "kernel": "k_tile<mixed, Ewald-real, C2>",
+                     "peak_without_packed_fp32": PEAK_FP64_TFLOPS,
+                     "frac_without_packed_fp32": achieved / PEAK_FP64_TFLOPS,
+                     "kernel": "k_half<mixed, Ewald-real, C2>",
                      "kernel_ms": mx["kernel_ms"],
                      "cell_build_ms": mx["build_ms"],
                      "achieved_incl_cell_build": P * FLOP_PER_PAIR / (
@@ -477,6 +576,7 @@ def main():
                      "flop_per_pair": FLOP_PER_PAIR},
         "cpu_baseline": cpu,
     }
+    line.update(extras)
     print(json.dumps(line), flush=True)
 
 

@@ -66,6 +66,8 @@ extern "C" {
 #define PMNB_EXCL_FULL 2
 
 typedef struct pmnb_ctx pmnb_ctx;
+typedef struct pmnb_ewald pmnb_ewald; /* Ewald k-space terms (below) */
+typedef struct pmnb_pme pmnb_pme;     /* PME k-space terms (below) */
 
 /* Force parameters: what NonbondedCutoffForce::doMake
    (force/nonbonded/NonbondedCutoffForce.h:71-75) and OneAtomPairTwo::make
@@ -144,6 +146,17 @@ int pmnb_evaluate_device(pmnb_ctx *ctx, const double *d_positions,
 int pmnb_pairset(pmnb_ctx *ctx, const double *positions, int32_t *pairs,
                  int64_t capacity, int64_t *n_pairs);
 
+/* The pairs the force's own pair kernel evaluates at these positions: one
+   evaluation (its forces and energies discarded) with the kernel's pair
+   record on, which appends every pair the kernel accepts at the point
+   OneAtomPairTwo::doOneAtomPair would compute it (force/OneAtomPairTwo.h:
+   57-80: minimum image, d^2 <= rc^2, not FULL excluded).  Same output
+   format as pmnb_pairset.  pmnb_pairset decides the pairs with a separate
+   FP64 kernel; this one reports what the production kernels (either
+   precision, either path) decide.  No counterpart in the reference. */
+int pmnb_force_pairs(pmnb_ctx *ctx, const double *positions, int32_t *pairs,
+                     int64_t capacity, int64_t *n_pairs);
+
 /* HIP-event timing of the pair stage on the stream it is launched on.
    enable = 1 starts (and clears) timing; enable = 0 stops it.  Either call
    first waits for the recorded launches and returns the summed durations
@@ -153,6 +166,20 @@ int pmnb_pairset(pmnb_ctx *ctx, const double *positions, int32_t *pairs,
    benchmark's per-kernel roofline figure. */
 int pmnb_kernel_timing(pmnb_ctx *ctx, int32_t enable, double *ms,
                        int64_t *launches);
+
+/* Pair path a context's evaluations take (known after pmnb_set_box):
+   PMNB_PATH_TILE: the algorithm cutoff reaches every pair within the
+     switching cutoff, so the pair set does not depend on the cell grid,
+     and at most 8 LJ types: the path's own column grid and tile kernel;
+   PMNB_PATH_LIST: the reference's cell grid and delta list
+     (topology/CellListEnumeratorPeriodicBoundaries.h:46-83) drive a
+     neighbour list per evaluation;
+   PMNB_PATH_SCAN: the same cells scanned directly (more than 512 cell
+     offsets). */
+#define PMNB_PATH_TILE 0
+#define PMNB_PATH_LIST 1
+#define PMNB_PATH_SCAN 2
+int pmnb_pair_path(const pmnb_ctx *ctx, int32_t *path);
 
 /* Number of in-cutoff pairs counted by the last evaluate or pairset call
    (P_alg); waits for the device. */
@@ -246,8 +273,9 @@ int pmnb_batch_run_graph(pmnb_batch *b, const double *d_positions,
                          double *d_forces, double *d_out, uint32_t flags,
                          int32_t n_steps, void *stream);
 
-/* Autotuner: times every pair-kernel shape (waves per block, buffered
-   candidates, force slots) and persistent grid size on these positions
+/* Autotuner: times every pair-kernel shape (buffered candidates per wave
+   with the waves per SIMD its LDS allows: 192 / 3, 128 / 4, 256 / 2 for
+   mixed) and persistent grid size on these positions
    and keeps the fastest for later evaluations; returns the shape index,
    the grid cap in blocks (0 = one block per work group of items) and its
    time per evaluation in ms (any pointer may be NULL).  The outputs are
@@ -256,8 +284,14 @@ int pmnb_batch_autotune(pmnb_batch *b, const double *d_positions,
                         double *d_forces, double *d_out, uint32_t flags,
                         int32_t *shape, int32_t *grid_blocks, double *ms);
 
-/* Sets the pair-kernel shape (0 .. 5; FP64: 0 .. 1) and grid cap by hand. */
+/* Sets the pair-kernel shape (mixed 0 .. 2, FP64 0) and grid cap by hand. */
 int pmnb_batch_set_shape(pmnb_batch *b, int32_t shape, int32_t grid_blocks);
+
+/* pmnb_force_pairs for the whole batch at device positions d_positions:
+   atom indices are the batch's (replicas in order of addition). */
+int pmnb_batch_force_pairs(pmnb_batch *b, const double *d_positions,
+                           int32_t *pairs, int64_t capacity,
+                           int64_t *n_pairs);
 
 /* In-cutoff pairs of all replicas in the last evaluation; waits for the
    device. */
@@ -334,6 +368,13 @@ int pmnb_lf_set_bonded(pmnb_leapfrog *lf, int32_t n_bonds,
                        const double *angle_theta0, const double *ub_k,
                        const double *ub_r0);
 
+/* Adds k-space terms created on the integrator's context to its force
+   group, evaluated after the cutoff force in every calculateForces: a PME
+   (pmnb_pme_create) and/or a direct Ewald sum (pmnb_ewald_create); either
+   may be NULL, and NULL for both removes them.  They must outlive the
+   integrator's runs. */
+int pmnb_lf_set_kspace(pmnb_leapfrog *lf, pmnb_pme *pme, pmnb_ewald *ewald);
+
 /* StandardIntegrator::calculateForces of the integrator's force group:
    clears d_forces and d_out (20 or 22 doubles, as pmnb_lf_run_device),
    then evaluates the cutoff force and the bonded terms at d_positions.
@@ -367,10 +408,16 @@ void pmnb_lf_destroy(pmnb_leapfrog *lf);
    force's Ewald real-space functor it completes the Ewald sum.
    alpha <= 0 selects the reference's default alpha (:603-604); kc =
    2 alpha sqrt(-ln accuracy).  The box is the context's (orthogonal).
-   scaled_charge, coulomb_scaling_factor and the exclusion rows are those
-   given to pmnb_set_topology.  The work is atoms x k-vectors, as in the
-   reference. */
-typedef struct pmnb_ewald pmnb_ewald;
+   scaled_charge and coulomb_scaling_factor are those given to
+   pmnb_set_topology.  The correction walks the rows of
+   ExclusionTable::getTable() (topology/ExclusionTable.cpp:103-110): of the
+   rows given, those of an atom with itself and of class PMNB_EXCL_NONE
+   are dropped and a pair given again keeps the class of its first row, so
+   the rows given to pmnb_set_topology may be passed as they are.  The
+   k-vectors, charges and rows are fixed at creation: after pmnb_set_box
+   with another box or any pmnb_set_topology call, evaluate returns
+   PMNB_ERR_STATE until the terms are created again.  The work is atoms x
+   k-vectors, as in the reference. */
 
 int pmnb_ewald_create(pmnb_ctx *ctx, double alpha, double accuracy,
                       int32_t n_atoms, const double *scaled_charge,
@@ -390,6 +437,40 @@ int pmnb_ewald_evaluate_device(pmnb_ewald *ew, const double *d_positions,
                                uint32_t flags, void *stream);
 
 void pmnb_ewald_destroy(pmnb_ewald *ew);
+
+/* ---------------------------------------------------------------- PME
+   Smooth particle-mesh Ewald: the reciprocal term of
+   NonbondedPMEwaldSystemForce<PeriodicBoundaryConditions, ..., real =
+   false, reciprocal = true, correction = true, BSpline>
+   (mdl/interface/force/nonbonded/NonbondedPMEwaldSystemForce.h:815-862,
+   Grid.h, BSpline.cpp) with its correction, point self and charged-system
+   terms (:878-912, :593-612), all added to the electrostatic energy; with
+   the cutoff force's Ewald real-space functor it is the PME sum.
+   Parameters as the force line's (:1000-1011): grid nx ny nz, B-spline
+   order (even, 2..8 here, each size >= order), cutoff rc (sets the default
+   alpha), accuracy, alpha (<= 0: the reference's default, erfc(alpha rc) /
+   rc = accuracy by bisection).  The charge grid is summed as 64-bit
+   fixed-point integers (2^-32), so results repeat bit for bit.  The
+   correction rows follow pmnb_ewald_create's rule.  Fixed at creation like
+   pmnb_ewald (PMNB_ERR_STATE after a box or topology change). */
+
+int pmnb_pme_create(pmnb_ctx *ctx, int32_t nx, int32_t ny, int32_t nz,
+                    int32_t order, double cutoff, double accuracy,
+                    double alpha, int32_t n_atoms,
+                    const double *scaled_charge,
+                    double coulomb_scaling_factor, int32_t n_excl,
+                    const int32_t *excl, pmnb_pme **out);
+
+/* alpha and kc = 2 alpha sqrt(-ln accuracy) (pointers may be NULL). */
+int pmnb_pme_info(const pmnb_pme *pme, double *alpha, double *kc);
+
+/* Enqueued on `stream` (NULL = the context's stream): forces +=,
+   d_out[1] +=, d_out[2..10] += atomic virial with PMNB_FLAG_VIRIAL. */
+int pmnb_pme_evaluate_device(pmnb_pme *pme, const double *d_positions,
+                             double *d_forces, double *d_out, uint32_t flags,
+                             void *stream);
+
+void pmnb_pme_destroy(pmnb_pme *pme);
 
 #ifdef __cplusplus
 }

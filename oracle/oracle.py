@@ -88,6 +88,9 @@ class System:
         self.lj = np.ascontiguousarray(t.lennard_jones_parameters.table,
                                        dtype=np.float64)
         self.excl = np.ascontiguousarray(t.exclusions.table(), dtype=np.int32)
+        # ExclusionTable::getTable() rows, which the Ewald correction walks
+        self.excl_table = np.ascontiguousarray(t.exclusions.get_table(),
+                                               dtype=np.int32)
         if com is None and (flags & 2):
             com = t.molecule_centers(self.pos)
         self.com = (np.ascontiguousarray(com, dtype=np.float64)
@@ -180,6 +183,28 @@ def _run(fn, system, record_pairs, pair_cap=None):
 def evaluate(system, record_pairs=False):
     """Cell-list evaluation in the reference's enumeration order."""
     return _run(load().orc_evaluate, system, record_pairs)
+
+
+def evaluate_threads(system, nthreads=0):
+    """orc_evaluate over host threads (0 = all): (forces, energies[2],
+    virial[9], pairs, threads used).  CPU-baseline timing only."""
+    lib = load()
+    if not hasattr(lib, "_thr_ready"):
+        lib.orc_evaluate_threads.argtypes = [ctypes.POINTER(OrcSystem)] + [
+            ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+        lib.orc_evaluate_threads.restype = ctypes.c_int
+        lib._thr_ready = True
+    st = system.struct()
+    f = np.zeros((system.n, 3))
+    e = np.zeros(2)
+    vir = np.zeros(9)
+    n = ctypes.c_int64()
+    used = lib.orc_evaluate_threads(ctypes.byref(st), f.ctypes.data,
+                                    e.ctypes.data, vir.ctypes.data,
+                                    int(nthreads), ctypes.byref(n))
+    if used < 0:
+        raise RuntimeError("threaded oracle evaluation unavailable")
+    return f, e, vir, n.value, used
 
 
 def evaluate_bruteforce(system, record_pairs=False):
@@ -429,6 +454,10 @@ def ewald_kspace(system, alpha, accuracy, virial=True, extended=False):
                                       ctypes.c_int]
         lib._ew_ready = True
     st = system.struct()
+    # the correction term iterates ExclusionTable::getTable()
+    # (NonbondedFullEwaldSystemForce.h:1105-1129)
+    st.n_excl = len(system.excl_table)
+    st.excl = system.excl_table.ctypes.data if len(system.excl_table) else None
     f = np.zeros((system.n, 3))
     e = np.zeros(2)
     vir = np.zeros(9)

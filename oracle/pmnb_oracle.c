@@ -34,6 +34,9 @@
  * the reference's own x86-64 build, so sums follow the same rounding.
  */
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -569,6 +572,99 @@ int orc_evaluate(const orc_system *s, double *forces, double *energies2,
   free(head); free(next); free(cell_lin); free(delta);
   eval_finish(&ev, energies2, vir9, molvir9, npairs);
   return 0;
+}
+
+/*
+ * orc_evaluate spread over host threads, for the CPU baseline's all-core
+ * figure (the reference itself has no threading): the same cell lists,
+ * delta list and per-pair function; thread t walks the cells c with
+ * c % nthreads == t (c and its delta partners, as the serial walk does)
+ * into a private force array and energy/virial sums, then the arrays are
+ * added in thread order.  Pair records are not kept.  Returns the number
+ * of threads used, -1 on allocation failure.
+ */
+int orc_evaluate_threads(const orc_system *s, double *forces,
+                         double *energies2, double *vir9, int nthreads,
+                         int64_t *npairs) {
+#ifdef _OPENMP
+  orc_eval ev0;
+  orc_grid g;
+  int N = s->n_atoms, ncell, i, c, nd, *delta, *head, *next, T;
+  double *fbuf, *sums;
+  int64_t np = 0;
+  if (eval_init(&ev0, s, forces, NULL, 0)) return -1;
+  orc_grid_init(&g, &ev0.pbc, s->cell_size);
+  ncell = g.n[0] * g.n[1] * g.n[2];
+  T = nthreads > 0 ? nthreads : omp_get_max_threads();
+  head = (int *)malloc(sizeof(int) * ncell);
+  next = (int *)malloc(sizeof(int) * (N ? N : 1));
+  delta = (int *)malloc(sizeof(int) * 3 * (2 * g.n[0] + 1) * (2 * g.n[1] + 1) *
+                        (2 * g.n[2] + 1));
+  fbuf = (double *)calloc((size_t)T * 3 * (N ? N : 1), sizeof(double));
+  sums = (double *)calloc((size_t)T * 11, sizeof(double));
+  if (!head || !next || !delta || !fbuf || !sums) return -1;
+  for (c = 0; c < ncell; c++) head[c] = -1;
+  for (i = N - 1; i >= 0; i--) {
+    int c3[3], lin;
+    orc_cell_of(&ev0.pbc, &g, s->pos + 3 * i, c3);
+    lin = (c3[0] * g.n[1] + c3[1]) * g.n[2] + c3[2];
+    next[i] = head[lin];
+    head[lin] = i;
+  }
+  nd = orc_delta_list(&g, s->alg_cutoff, delta,
+                      (2 * g.n[0] + 1) * (2 * g.n[1] + 1) * (2 * g.n[2] + 1));
+#pragma omp parallel num_threads(T) reduction(+ : np)
+  {
+    const int t = omp_get_thread_num();
+    orc_eval ev = ev0;
+    int cc, k;
+    ev.f = fbuf + (size_t)t * 3 * (N ? N : 1);
+    ev.e_lj = ev.e_el = 0.0;
+    for (k = 0; k < 9; k++) ev.vir[k] = ev.molvir[k] = 0.0;
+    ev.npairs = 0;
+    for (cc = t; cc < ncell; cc += T) {
+      int cx, cy, cz, d, a, b;
+      if (head[cc] < 0) continue;
+      cx = cc / (g.n[1] * g.n[2]);
+      cy = (cc / g.n[2]) % g.n[1];
+      cz = cc % g.n[2];
+      for (a = head[cc]; a != -1; a = next[a])
+        for (b = next[a]; b != -1; b = next[b]) one_pair(&ev, a, b);
+      for (d = 0; d < nd; d++) {
+        int px = (cx + delta[3 * d] + g.maxN[0]) % g.n[0];
+        int py = (cy + delta[3 * d + 1] + g.maxN[1]) % g.n[1];
+        int pz = (cz + delta[3 * d + 2] + g.maxN[2]) % g.n[2];
+        int cj = (px * g.n[1] + py) * g.n[2] + pz;
+        if (head[cj] < 0 || cj < cc) continue;
+        for (a = head[cc]; a != -1; a = next[a])
+          for (b = head[cj]; b != -1; b = next[b]) one_pair(&ev, a, b);
+      }
+    }
+    sums[11 * t] = ev.e_lj;
+    sums[11 * t + 1] = ev.e_el;
+    for (k = 0; k < 9; k++) sums[11 * t + 2 + k] = ev.vir[k];
+    np += ev.npairs;
+  }
+  for (c = 0; c < T; c++) {
+    const double *fb = fbuf + (size_t)c * 3 * (N ? N : 1);
+    for (i = 0; i < 3 * N; i++) forces[i] += fb[i];
+    if (energies2) {
+      energies2[0] += sums[11 * c];
+      energies2[1] += sums[11 * c + 1];
+    }
+    if (vir9)
+      for (i = 0; i < 9; i++) vir9[i] += sums[11 * c + 2 + i];
+  }
+  if (npairs) *npairs = np;
+  free(head); free(next); free(delta); free(fbuf); free(sums);
+  free(ev0.ex.key);
+  free(ev0.ex.cls);
+  return T;
+#else
+  (void)s; (void)forces; (void)energies2; (void)vir9; (void)nthreads;
+  (void)npairs;
+  return -1;
+#endif
 }
 
 /*

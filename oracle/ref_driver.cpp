@@ -25,6 +25,9 @@
 //   double com[nMol*3]
 // kind: 0 LJ(C2)+Coulomb(C2), 1 LJ(C2)+EwaldReal(C2), 2 LJ(C2)+Coulomb(C1)
 // flags: bit0 virial, bit1 molecular virial
+// With a third argument R (timing mode) the force runs R times without the
+// pair recorder and the wall time of each evaluation is printed ("ms t");
+// the output file then holds the last evaluation's results and no pairs.
 // Output file: double f[3N], double e[2] (LJ, Coulomb), double vir[9],
 //   double molvir[9], int64 nPairs, int32 pairs[nPairs*3] (i, j, class)
 #include <protomol/topology/Topology.h>
@@ -41,6 +44,7 @@
 #include <protomol/type/ScalarStructure.h>
 #include <protomol/type/Vector3DBlock.h>
 
+#include <ctime>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -107,10 +111,52 @@ static void runForce(F &force, PbcTopology *topo, Vector3DBlock &pos,
   force.evaluate(topo, &pos, &forces, &en);
 }
 
+static void die(const char *m);
+
+// One evaluation of the force `kind` selects; returns its wall time in ms.
+template <class C>
+static double dispatch(int kind, double algCutoff,
+                       const C2SwitchingFunction &swLJ, const double *par,
+                       PbcTopology *topo, Vector3DBlock &pos,
+                       Vector3DBlock &forces, ScalarStructure &en) {
+  timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  if (kind == 0) {
+    typedef OneAtomPairTwo<PeriodicBoundaryConditions, C2SwitchingFunction,
+      LennardJonesForce, C2SwitchingFunction, CoulombForce, C> P;
+    NonbondedCutoffSystemForce<CubicCellManager, P>
+      f(algCutoff, P(LennardJonesForce(), swLJ, CoulombForce(),
+                     C2SwitchingFunction(par[14], par[15])));
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    runForce(f, topo, pos, forces, en);
+  } else if (kind == 1) {
+    typedef OneAtomPairTwo<PeriodicBoundaryConditions, C2SwitchingFunction,
+      LennardJonesForce, C2SwitchingFunction, EwaldRealForce, C> P;
+    NonbondedCutoffSystemForce<CubicCellManager, P>
+      f(algCutoff, P(LennardJonesForce(), swLJ, EwaldRealForce(),
+                     C2SwitchingFunction(par[14], par[15])));
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    runForce(f, topo, pos, forces, en);
+  } else if (kind == 2) {
+    typedef OneAtomPairTwo<PeriodicBoundaryConditions, C2SwitchingFunction,
+      LennardJonesForce, C1SwitchingFunction, CoulombForce, C> P;
+    NonbondedCutoffSystemForce<CubicCellManager, P>
+      f(algCutoff, P(LennardJonesForce(), swLJ, CoulombForce(),
+                     C1SwitchingFunction(par[15])));
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    runForce(f, topo, pos, forces, en);
+  } else {
+    die("unknown kind");
+  }
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  return 1e3 * (t1.tv_sec - t0.tv_sec) + 1e-6 * (t1.tv_nsec - t0.tv_nsec);
+}
+
 static void die(const char *m) {fprintf(stderr, "ref_driver: %s\n", m); exit(2);}
 
 int main(int argc, char **argv) {
-  if (argc != 3) die("usage: ref_driver in.bin out.bin");
+  if (argc != 3 && argc != 4)
+    die("usage: ref_driver in.bin out.bin [timed repeats]");
   FILE *fi = fopen(argv[1], "rb");
   if (!fi) die("cannot open input");
   int32_t hdr[8];
@@ -175,29 +221,19 @@ int main(int argc, char **argv) {
 
   C2SwitchingFunction swLJ(par[12], par[13]);
   const double algCutoff = par[16];
-  if (kind == 0) {
-    typedef OneAtomPairTwo<PeriodicBoundaryConditions, C2SwitchingFunction,
-      LennardJonesForce, C2SwitchingFunction, CoulombForce, PairRecorder> P;
-    NonbondedCutoffSystemForce<CubicCellManager, P>
-      f(algCutoff, P(LennardJonesForce(), swLJ, CoulombForce(),
-                     C2SwitchingFunction(par[14], par[15])));
-    runForce(f, topo, pos, forces, en);
-  } else if (kind == 1) {
-    typedef OneAtomPairTwo<PeriodicBoundaryConditions, C2SwitchingFunction,
-      LennardJonesForce, C2SwitchingFunction, EwaldRealForce, PairRecorder> P;
-    NonbondedCutoffSystemForce<CubicCellManager, P>
-      f(algCutoff, P(LennardJonesForce(), swLJ, EwaldRealForce(),
-                     C2SwitchingFunction(par[14], par[15])));
-    runForce(f, topo, pos, forces, en);
-  } else if (kind == 2) {
-    typedef OneAtomPairTwo<PeriodicBoundaryConditions, C2SwitchingFunction,
-      LennardJonesForce, C1SwitchingFunction, CoulombForce, PairRecorder> P;
-    NonbondedCutoffSystemForce<CubicCellManager, P>
-      f(algCutoff, P(LennardJonesForce(), swLJ, CoulombForce(),
-                     C1SwitchingFunction(par[15])));
-    runForce(f, topo, pos, forces, en);
+  const int repeats = argc == 4 ? atoi(argv[3]) : 0;
+  std::vector<double> ms;
+  if (repeats > 0) {
+    // timing: the same force without the pair recorder, evaluated
+    // `repeats` times (cell lists rebuilt each time, as every step does)
+    for (int r = 0; r < repeats; r++) {
+      for (int i = 0; i < N; i++) forces[i] = Vector3D(0.0, 0.0, 0.0);
+      en.clear();
+      ms.push_back(dispatch<NoConstraint>(kind, algCutoff, swLJ, par, topo,
+                                          pos, forces, en));
+    }
   } else {
-    die("unknown kind");
+    dispatch<PairRecorder>(kind, algCutoff, swLJ, par, topo, pos, forces, en);
   }
 
   FILE *fo = fopen(argv[2], "wb");
@@ -221,6 +257,7 @@ int main(int argc, char **argv) {
   fwrite(&np, 8, 1, fo);
   if (np) fwrite(&g_pairs[0], 4, g_pairs.size(), fo);
   fclose(fo);
+  for (size_t k = 0; k < ms.size(); k++) printf("ms %.6f\n", ms[k]);
   delete topo;
   return 0;
 }

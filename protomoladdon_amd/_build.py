@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libpmnb.so")
 SOURCES = ["pmnb.hip"]
 HEADERS = ["pmnb_common.h", "pmnb_cells.h", "pmnb_pair.h", "pmnb_list.h",
            "pmnb_tile.h", "pmnb_queue.h", "pmnb_adjoint.h",
-           "pmnb_tilemask.h", "pmnb_leapfrog.h", "pmnb_ewald.h"]
+           "pmnb_tilemask.h", "pmnb_leapfrog.h", "pmnb_ewald.h", "pmnb_pme.h"]
 
 # -ffp-contract=off: FP64 expressions round exactly as the reference's x86-64
 # build (no FMA); FP32 kernels ask for fused multiply-adds explicitly.
@@ -22,6 +22,8 @@ HIPCC_FLAGS = [
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
+# hipFFT for the PME transforms (a library FFT, as rocBLAS is for GEMMs)
+LINK_FLAGS = ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lhipfft"]
 
 
 def hipcc():
@@ -32,7 +34,7 @@ def hipcc():
 def build(verbose=False):
     """Compiles libpmnb.so (always; callers decide when to rebuild)."""
     cmd = [hipcc()] + HIPCC_FLAGS + ["-o", LIB_PATH + ".tmp"] + [
-        os.path.join(CSRC, s) for s in SOURCES]
+        os.path.join(CSRC, s) for s in SOURCES] + LINK_FLAGS
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -29,18 +29,21 @@ PMNB_FLAG_MOLVIRIAL = 2
 # Every function declared in include/pmnb.h.
 EXPORTED = (
     "pmnb_create", "pmnb_set_topology", "pmnb_set_box", "pmnb_evaluate",
-    "pmnb_evaluate_device", "pmnb_pairset", "pmnb_kernel_timing",
+    "pmnb_evaluate_device", "pmnb_pairset", "pmnb_force_pairs",
+    "pmnb_kernel_timing", "pmnb_pair_path",
     "pmnb_last_pair_count",
     "pmnb_last_error", "pmnb_destroy", "pmnb_version",
     "pmnb_batch_create", "pmnb_batch_add", "pmnb_batch_size",
     "pmnb_batch_evaluate_device", "pmnb_batch_run_graph",
-    "pmnb_batch_autotune", "pmnb_batch_set_shape", "pmnb_batch_pair_count",
+    "pmnb_batch_autotune", "pmnb_batch_set_shape", "pmnb_batch_force_pairs",
+    "pmnb_batch_pair_count",
     "pmnb_batch_last_error", "pmnb_batch_destroy",
     "pmnb_set_parameters_device", "pmnb_adjoint_device",
     "pmnb_lf_create", "pmnb_lf_run_device", "pmnb_lf_set_bonded",
     "pmnb_lf_forces_device", "pmnb_lf_iterations", "pmnb_lf_destroy",
     "pmnb_ewald_create", "pmnb_ewald_info", "pmnb_ewald_evaluate_device",
-    "pmnb_ewald_destroy",
+    "pmnb_ewald_destroy", "pmnb_pme_create", "pmnb_pme_info",
+    "pmnb_pme_evaluate_device", "pmnb_pme_destroy", "pmnb_lf_set_kspace",
 )
 
 
@@ -111,6 +114,13 @@ def load():
     lib.pmnb_evaluate_device.restype = ctypes.c_int
     lib.pmnb_pairset.argtypes = [_p, _p, _p, _i64, ctypes.POINTER(_i64)]
     lib.pmnb_pairset.restype = ctypes.c_int
+    lib.pmnb_force_pairs.argtypes = [_p, _p, _p, _i64, ctypes.POINTER(_i64)]
+    lib.pmnb_force_pairs.restype = ctypes.c_int
+    lib.pmnb_batch_force_pairs.argtypes = [_p, _p, _p, _i64,
+                                           ctypes.POINTER(_i64)]
+    lib.pmnb_batch_force_pairs.restype = ctypes.c_int
+    lib.pmnb_pair_path.argtypes = [_p, ctypes.POINTER(_i32)]
+    lib.pmnb_pair_path.restype = ctypes.c_int
     lib.pmnb_kernel_timing.argtypes = [_p, _i32, _p, ctypes.POINTER(_i64)]
     lib.pmnb_kernel_timing.restype = ctypes.c_int
     lib.pmnb_last_pair_count.argtypes = [_p, ctypes.POINTER(_i64)]
@@ -174,6 +184,18 @@ def load():
     lib.pmnb_ewald_evaluate_device.restype = ctypes.c_int
     lib.pmnb_ewald_destroy.argtypes = [_p]
     lib.pmnb_ewald_destroy.restype = None
+    lib.pmnb_pme_create.argtypes = [_p, _i32, _i32, _i32, _i32, _dbl, _dbl,
+                                    _dbl, _i32, _p, _dbl, _i32, _p,
+                                    ctypes.POINTER(_p)]
+    lib.pmnb_pme_create.restype = ctypes.c_int
+    lib.pmnb_pme_info.argtypes = [_p, _p, _p]
+    lib.pmnb_pme_info.restype = ctypes.c_int
+    lib.pmnb_pme_evaluate_device.argtypes = [_p, _p, _p, _p, _u32, _p]
+    lib.pmnb_pme_evaluate_device.restype = ctypes.c_int
+    lib.pmnb_pme_destroy.argtypes = [_p]
+    lib.pmnb_pme_destroy.restype = None
+    lib.pmnb_lf_set_kspace.argtypes = [_p, _p, _p]
+    lib.pmnb_lf_set_kspace.restype = ctypes.c_int
     lib.pmnb_lf_destroy.argtypes = [_p]
     lib.pmnb_lf_destroy.restype = None
     _LIB = lib

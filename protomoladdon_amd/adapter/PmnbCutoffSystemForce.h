@@ -6,11 +6,12 @@
  *
  * Drop-in for NonbondedCutoffSystemForce<CubicCellManager,
  *   OneAtomPairTwo<PeriodicBoundaryConditions, C2SwitchingFunction,
- *                  LennardJonesForce, TSwitchElec, CoulombForce>>
- * (force/nonbonded/NonbondedCutoffSystemForce.h:11-78): same keyword, id,
- * parameters and make() arguments, so the same force line in a .conf file
- * resolves to it once it is registered in place of the CPU exemplar
- * (module/NonbondedCutoffForceModule.cpp:75-80).
+ *                  LennardJonesForce, TSwitchElec, TElec>>
+ * (force/nonbonded/NonbondedCutoffSystemForce.h:11-78) with TElec =
+ * CoulombForce (force/CoulombForce.h:13-60) or PmnbEwaldRealForce (below):
+ * same keyword, id, parameters and make() arguments, so the same force line
+ * in a .conf file resolves to it once it is registered in place of the CPU
+ * exemplar (module/NonbondedCutoffForceModule.cpp:74-80).
  */
 #ifndef PMNB_CUTOFF_SYSTEM_FORCE_H
 #define PMNB_CUTOFF_SYSTEM_FORCE_H
@@ -26,12 +27,71 @@
 #include <protomol/topology/CubicCellManager.h>
 #include <protomol/type/ScalarStructure.h>
 #include <protomol/base/Report.h>
+#include <protomol/base/MathUtilities.h>
+#include <protomol/config/Parameter.h>
 
 #include <vector>
 
 #include "pmnb.h"
 
 namespace ProtoMol {
+
+// The Ewald real-space term qq erfc(alpha r) / r as a cutoff functor with
+// the Abramowitz-Stegun erfc of NonbondedFullEwaldSystemForce.h:810-821
+// (the reference computes this term only inside its full Ewald force).  It
+// has CoulombForce's interface (force/CoulombForce.h:13-60) plus one
+// parameter, -alpha, so OneAtomPairTwo reads it from the force line; the
+// device computes the term, operator() is the same arithmetic on the host.
+class PmnbEwaldRealForce {
+public:
+  enum {DIST_R2 = 1};
+  enum {CUTOFF = 0};
+  PmnbEwaldRealForce() : myAlpha(-1.0) {}
+  explicit PmnbEwaldRealForce(Real alpha) : myAlpha(alpha) {}
+
+  void operator()(Real &energy, Real &force, Real distSquared,
+                  Real /*rDistSquared*/, const Vector3D &,
+                  const GenericTopology *topo, int atom1, int atom2,
+                  ExclusionClass excl) const {
+    Real qq = topo->atoms[atom1].scaledCharge * topo->atoms[atom2].scaledCharge;
+    if (excl == EXCLUSION_MODIFIED) qq *= topo->coulombScalingFactor;
+    const Real r = sqrt(distSquared), rr = 1.0 / r, ar = myAlpha * r;
+    const Real e = qq * exp(-ar * ar);
+    energy = poly5(ar) * e * rr;
+    force = (energy + 2.0 * myAlpha / sqrt(M_PI) * e) * rr * rr;
+  }
+  static void accumulateEnergy(ScalarStructure *energies, Real energy) {
+    (*energies)[ScalarStructure::COULOMB] += energy;
+  }
+  static Real getEnergy(const ScalarStructure *energies) {
+    return (*energies)[ScalarStructure::COULOMB];
+  }
+  static std::string getId() {return "EwaldReal";}
+  static unsigned int getParameterSize() {return 1;}
+  void getParameters(std::vector<Parameter> &parameters) const {
+    parameters.push_back(Parameter("-alpha",
+                                   Value(myAlpha, ConstraintValueType::Positive()),
+                                   Text("Ewald splitting parameter (1/A)")));
+  }
+  static PmnbEwaldRealForce make(const std::vector<Value> &values) {
+    Real a = -1.0;
+    if (!values.empty()) values[0].get(a);
+    return PmnbEwaldRealForce(a);
+  }
+  Real alpha() const {return myAlpha;}
+
+private:
+  Real myAlpha;
+};
+
+template<class TElec>
+struct PmnbElecKind;
+template<> struct PmnbElecKind<CoulombForce> {
+  enum {VALUE = PMNB_ELEC_COULOMB};
+};
+template<> struct PmnbElecKind<PmnbEwaldRealForce> {
+  enum {VALUE = PMNB_ELEC_EWALD_REAL};
+};
 
 template<class TSwitchElec>
 struct PmnbSwitchKind;
@@ -42,18 +102,21 @@ template<> struct PmnbSwitchKind<C1SwitchingFunction> {
   enum {VALUE = PMNB_SWITCH_C1};
 };
 
-template<class TSwitchElec>
+template<class TSwitchElec, class TElec = CoulombForce>
 class PmnbCutoffSystemForce : public SystemForce {
   typedef OneAtomPairTwo<PeriodicBoundaryConditions, C2SwitchingFunction,
-                         LennardJonesForce, TSwitchElec, CoulombForce> Pair;
+                         LennardJonesForce, TSwitchElec, TElec> Pair;
   typedef Topology<PeriodicBoundaryConditions, CubicCellManager> RealTopology;
 
 public:
   PmnbCutoffSystemForce() : myCutoff(0.0), myCtx(0), myTopo(0), myN(-1) {}
 
-  // values: the OneAtomPairTwo parameters (C2 switchon, C2 cutoff, then the
-  // electrostatic switch's) and the algorithm -cutoff last, the order of
-  // NonbondedCutoffForce::doMake (NonbondedCutoffForce.h:71-75).
+  // values: the OneAtomPairTwo parameters in the order of
+  // OneAtomPairTwo::make (force/OneAtomPairTwo.h:141-154): LJ (none), its
+  // C2 switchon and cutoff, the electrostatic functor's (-alpha for
+  // PmnbEwaldRealForce), the electrostatic switch's; the algorithm -cutoff
+  // comes last, as NonbondedCutoffForce::doMake reads it
+  // (NonbondedCutoffForce.h:71-75).
   PmnbCutoffSystemForce(Real cutoff, const std::vector<Value> &pairValues,
                         int precision = PMNB_PREC_MIXED, int device = 0)
     : myCutoff(cutoff), myPairValues(pairValues), myPair(Pair::make(pairValues)),
@@ -128,25 +191,31 @@ private:
   void setup(const RealTopology *t) {
     if (!myCtx) {
       pmnb_params p;
-      Real v[4];
-      for (int k = 0; k < 4 && k < (int)myPairValues.size(); k++)
+      std::vector<Real> v(myPairValues.size(), 0.0);
+      for (unsigned int k = 0; k < myPairValues.size(); k++)
         myPairValues[k].get(v[k]);
-      p.elec = PMNB_ELEC_COULOMB;
+      // [switchon, cutoff] of LJ's C2, the electrostatic functor's values,
+      // then its switch's: [switchon, cutoff] (C2) or [cutoff] (C1)
+      const unsigned int e0 = 2, s0 = e0 + TElec::getParameterSize();
+      if (v.size() < s0 + TSwitchElec::getParameterSize())
+        Report::report << Report::error << "[PmnbCutoffSystemForce] "
+                       << "missing force parameters." << Report::endr;
+      p.elec = PmnbElecKind<TElec>::VALUE;
       p.elec_switch = PmnbSwitchKind<TSwitchElec>::VALUE;
       p.precision = myPrecision;
       p.device = myDevice;
       p.lj_switchon = v[0];
       p.lj_cutoff = v[1];
       if (p.elec_switch == PMNB_SWITCH_C2) {
-        p.elec_switchon = v[2];
-        p.elec_cutoff = v[3];
+        p.elec_switchon = v[s0];
+        p.elec_cutoff = v[s0 + 1];
       } else {
         p.elec_switchon = 0.0;
-        p.elec_cutoff = v[2];
+        p.elec_cutoff = v[s0];
       }
       p.algorithm_cutoff = myCutoff;
       p.cell_size = t->cellManager.getCellSize();
-      p.ewald_alpha = 0.0;
+      p.ewald_alpha = p.elec == PMNB_ELEC_EWALD_REAL ? v[e0] : 0.0;
       check(pmnb_create(&p, &myCtx));
     }
     const int n = (int)t->atoms.size();

@@ -124,6 +124,20 @@ class BatchedCutoffForce:
         _lib.check_batch(self._lib, self._b, self._lib.pmnb_batch_set_shape(
             self._b, int(shape), int(grid_blocks)))
 
+    def force_pairs(self, d_positions):
+        """The pairs the batch's pair kernel evaluates at the device
+        positions (one evaluation with its pair record on): sorted (P, 3)
+        int32 rows (i, j, class), i < j, indices into the batch's atoms."""
+        lib = self._lib
+        n = ctypes.c_int64()
+        _lib.check_batch(lib, self._b, lib.pmnb_batch_force_pairs(
+            self._b, d_positions.data_ptr(), None, 0, ctypes.byref(n)))
+        out = np.zeros((max(n.value, 1), 3), dtype=np.int32)
+        _lib.check_batch(lib, self._b, lib.pmnb_batch_force_pairs(
+            self._b, d_positions.data_ptr(), out.ctypes.data, n.value,
+            ctypes.byref(n)))
+        return out[:n.value]
+
     def pair_count(self):
         n = ctypes.c_int64()
         _lib.check_batch(self._lib, self._b, self._lib.pmnb_batch_pair_count(

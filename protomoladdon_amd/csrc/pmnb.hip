@@ -6,6 +6,7 @@
 // list from the current positions, enumerate cell pairs, evaluate pairs,
 // accumulate forces, energies and virials.  All of it runs on the device.
 #include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
 
 #include <algorithm>
 #include <cmath>
@@ -26,6 +27,7 @@
 #include "pmnb_tilemask.h"
 #include "pmnb_leapfrog.h"
 #include "pmnb_ewald.h"
+#include "pmnb_pme.h"
 
 using namespace pmnb;
 
@@ -96,10 +98,21 @@ struct pmnb_ctx {
   DevBuf d_rep_of;
   int rep_uploaded = 0;           // reps and col_rep are on the device
   DevBuf d_partial_lj;            // adjoint: LJ-parameter partials per item
+  // tile path: per sorted slot the 64-bit fixed-point force (3 words mixed,
+  // 6 FP64), all zero between evaluations (the finish kernel clears it)
+  DevBuf d_ffix;
+  const void *ffix_zeroed = nullptr;  // allocation last cleared
+  // pair record of the force kernels (parity tests): on for one evaluation
+  PairLog plog{nullptr, 0, nullptr};
+  DevBuf d_plog, d_plog_count;
+  // pinned host staging of pmnb_evaluate: positions, forces, out + count
+  double *h_stage = nullptr;
+  size_t h_stage_bytes = 0;
   int shape = 0;                  // tile-path kernel shape (mixed)
   int grid_cap = 0;               // persistent pair grid: max blocks (0: none)
   DevBuf d_items, d_nitems, d_reps, d_col_rep, d_rep_first;
   double box_key[12] = {0};       // e1, e2, e3, origin of the current box
+  unsigned long long topo_gen = 0;  // pmnb_set_topology calls so far
   int cap = 0;                    // list slots per atom
   // pair-kernel timing
   bool timing = false;
@@ -485,7 +498,21 @@ int ensure_eval_buffers(pmnb_ctx *ctx) {
   HIPCHK(ctx->d_sq.ensure(N * sizeof(double)));
   HIPCHK(ctx->d_smol.ensure(N * sizeof(int)));
   HIPCHK(ctx->d_scell.ensure(N * sizeof(int)));
-  if (ctx->use_tile) HIPCHK(ctx->d_srec.ensure(N * sizeof(int4)));
+  if (ctx->use_tile) {
+    HIPCHK(ctx->d_srec.ensure(N * sizeof(int4)));
+    const int W = ctx->prm.precision == PMNB_PREC_FP64 ? FixWords<true>::W
+                                                        : FixWords<false>::W;
+    HIPCHK(ctx->d_ffix.ensure(N * W * sizeof(long long)));
+    if (ctx->ffix_zeroed != ctx->d_ffix.p) {
+      const long long nw = (long long)(ctx->d_ffix.bytes / sizeof(long long));
+      hipLaunchKernelGGL(k_zero_i64, dim3((unsigned)((nw + kBlock - 1) / kBlock)),
+                         dim3(kBlock), 0, ctx->stream, ctx->d_ffix.as<long long>(),
+                         nw);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      ctx->ffix_zeroed = ctx->d_ffix.p;
+    }
+  }
   const int ncol_t = ctx->use_tile ? ctx->ncol_total : 0;
   const int tile_items = n / kTileAtoms + ncol_t + 1;  // partial row per item
   HIPCHK(ctx->d_partial.ensure((size_t)std::max(blocks_for(n), tile_items) *
@@ -534,91 +561,75 @@ int ensure_eval_buffers(pmnb_ctx *ctx) {
   return PMNB_OK;
 }
 
-// Tile-path pair kernels the autotuner chooses from.  Shape 0 is k_tm
-// (per-lane masks, register sums); the others are k_tq (pair queue) with
-// {waves per block, buffered records, force slots per atom, LDS atomics};
-// FP64 has shape 0 and k_tq as shape 1.
-constexpr int kMixedShapes = 6;
-using MixedShape0 = TqShape<1, 128, 16, false>;
-using MixedShape1 = TqShape<2, 128, 16, false>;
-using MixedShape2 = TqShape<4, 128, 16, false>;
-using MixedShape3 = TqShape<2, 128, 32, false>;
-using MixedShape4 = TqShape<2, 256, 32, false>;
-using F64Shape = TqShape<1, 128, 32, true>;
+// Pair-kernel shapes of the tile path: {buffered records per wave, waves
+// per SIMD the register allocation must allow}.  LDS per record: 56 bytes
+// (mixed), 112 (FP64), four waves per block.  A single system uses shape
+// 0; the batch autotuner chooses among them.
+constexpr int kMixedShapes = 3, kF64Shapes = 1;
 
-struct TqArgs {
+struct HalfArgs {
   int max_items;   // upper bound of the work items
-  int grid_cap;    // persistent grid: at most this many blocks (0: none)
+  int grid_cap;    // persistent grid (batch): at most this many blocks
   hipStream_t s;
   const ForceParams *P;
   const Sorted *S;
   const TileRep *reps;
   const int4 *items;
   const int *nitems;
-  double *forces, *partial;
+  long long *ffix;
+  double *partial;
   unsigned long long *npairs;
   int pstride;     // partial row stride (items)
-  bool single;     // one replica without wrapped x/y: k_tile for shape 0
+  bool single;     // one replica without wrapped x/y
   TileGrid G;      // its column grid
+  PairLog log;
 };
 
-template <bool F64, int EL, int SW, class SHP>
-int launch_tq_shape(const TqArgs &a) {
-  int nb = blocks_for(a.max_items, SHP::NW);
+template <bool F64, int EL, int SW, int BUF, int OCC>
+void launch_half_shape(const HalfArgs &a) {
+  if (a.single) {
+    // a multiple of 8 blocks (FP64 maps blocks to XCD-contiguous items)
+    const int nb = 8 * blocks_for(std::max(1, blocks_for(a.max_items,
+                                                         kTmWaves)), 8);
+    hipLaunchKernelGGL((k_half<F64, EL, SW, BUF, OCC, false>), dim3(nb),
+                       dim3(kBlock), 0, a.s, *a.P, *a.S, a.reps, a.G, a.items,
+                       a.nitems, a.ffix, a.partial, a.pstride, a.npairs,
+                       a.log);
+    return;
+  }
+  int nb = blocks_for(a.max_items, kTmWaves);
   if (a.grid_cap > 0) nb = std::min(nb, a.grid_cap);
   nb = std::max(nb, 1);
-  hipLaunchKernelGGL((k_tq<F64, EL, SW, SHP>), dim3(nb), dim3(64 * SHP::NW), 0,
-                     a.s, *a.P, *a.S, a.reps, a.items, a.nitems, a.forces,
-                     a.partial, a.pstride, a.npairs);
-  return nb;
+  hipLaunchKernelGGL((k_half<F64, EL, SW, BUF, OCC, true>), dim3(nb),
+                     dim3(kBlock), 0, a.s, *a.P, *a.S, a.reps, a.G, a.items,
+                     a.nitems, a.ffix, a.partial, a.pstride, a.npairs, a.log);
 }
 
 template <bool F64, int EL, int SW>
-int launch_tq(int shape, const TqArgs &a) {
-  if (shape == 0 && a.single) {
-    // a multiple of 8 blocks (k_tile maps blocks to XCD-contiguous items)
-    const int nb = 8 * blocks_for(std::max(1, blocks_for(a.max_items,
-                                                         kTmWaves)), 8);
-    hipLaunchKernelGGL((k_tile<F64, EL, SW, F64 ? kTmBuf64 : kTmBuf32>), dim3(nb),
-                       dim3(kBlock), 0, a.s, *a.P, *a.S, a.G, a.items,
-                       a.nitems, a.forces, a.partial, a.pstride, a.npairs);
-    return nb;
-  }
-  if (shape == 0) {
-    int nb = blocks_for(a.max_items, kTmWaves);
-    if (a.grid_cap > 0) nb = std::min(nb, a.grid_cap);
-    nb = std::max(nb, 1);
-    hipLaunchKernelGGL((k_tm<F64, EL, SW, F64 ? kTmBuf64 : kTmBuf32>), dim3(nb),
-                       dim3(kBlock), 0, a.s, *a.P, *a.S, a.reps, a.items,
-                       a.nitems, a.forces, a.partial, a.pstride, a.npairs);
-    return nb;
-  }
+void launch_half(int shape, const HalfArgs &a) {
   if constexpr (F64) {
-    return launch_tq_shape<true, EL, SW, F64Shape>(a);
+    launch_half_shape<true, EL, SW, 128, 2>(a);
   } else {
-    switch (shape) {
-      case 1: return launch_tq_shape<false, EL, SW, MixedShape0>(a);
-      case 2: return launch_tq_shape<false, EL, SW, MixedShape1>(a);
-      case 3: return launch_tq_shape<false, EL, SW, MixedShape2>(a);
-      case 4: return launch_tq_shape<false, EL, SW, MixedShape3>(a);
-      default: return launch_tq_shape<false, EL, SW, MixedShape4>(a);
-    }
+    if (a.single || shape == 0) launch_half_shape<false, EL, SW, 192, 3>(a);
+    else if (shape == 1) launch_half_shape<false, EL, SW, 128, 4>(a);
+    else launch_half_shape<false, EL, SW, 256, 2>(a);
   }
 }
 
 // Pair kernel of the tile path for the force's precision and functors.
-void launch_pair_tq(bool mixed, int el, int sw, int shape, const TqArgs &a) {
+void launch_pair_half(bool mixed, int el, int sw, int shape,
+                      const HalfArgs &a) {
   if (mixed) {
-    if (el == PMNB_ELEC_NONE) launch_tq<false, 0, 0>(shape, a);
-    else if (el == PMNB_ELEC_COULOMB) launch_tq<false, 1, 0>(shape, a);
-    else launch_tq<false, 2, 0>(shape, a);
+    if (el == PMNB_ELEC_NONE) launch_half<false, 0, 0>(shape, a);
+    else if (el == PMNB_ELEC_COULOMB) launch_half<false, 1, 0>(shape, a);
+    else launch_half<false, 2, 0>(shape, a);
   } else {
-    if (el == PMNB_ELEC_NONE) launch_tq<true, 0, 0>(shape, a);
+    if (el == PMNB_ELEC_NONE) launch_half<true, 0, 0>(shape, a);
     else if (el == PMNB_ELEC_COULOMB && sw == PMNB_SWITCH_C2)
-      launch_tq<true, 1, 0>(shape, a);
-    else if (el == PMNB_ELEC_COULOMB) launch_tq<true, 1, 1>(shape, a);
-    else if (sw == PMNB_SWITCH_C2) launch_tq<true, 2, 0>(shape, a);
-    else launch_tq<true, 2, 1>(shape, a);
+      launch_half<true, 1, 0>(shape, a);
+    else if (el == PMNB_ELEC_COULOMB) launch_half<true, 1, 1>(shape, a);
+    else if (sw == PMNB_SWITCH_C2) launch_half<true, 2, 0>(shape, a);
+    else launch_half<true, 2, 1>(shape, a);
   }
 }
 
@@ -824,17 +835,27 @@ int run_device(pmnb_ctx *ctx, const double *d_pos, double *d_forces,
     int4 *items = ctx->d_items.as<int4>();
     int *nitems = ctx->d_nitems.as<int>();
     int *rep_first = ctx->d_rep_first.as<int>();
+    long long *ffix = ctx->d_ffix.as<long long>();
     if (ev_built) HIPCHK(hipEventRecord(ev_built, s));
     if (ctx->n > 0) {
       const bool single = !ctx->batch && nrep == 1 &&
                           !ctx->reps[0].wrap[0] && !ctx->reps[0].wrap[1];
-      TqArgs a{max_items, ctx->grid_cap, s, &P, &S, reps, items,
-               nitems, d_forces, part, np, max_items, single, ctx->tile};
-      launch_pair_tq(mixed, el, sw, ctx->shape, a);
+      HalfArgs a{max_items, ctx->grid_cap, s, &P, &S, reps, items, nitems,
+                 ffix, part, np, max_items, single, ctx->tile, ctx->plog};
+      launch_pair_half(mixed, el, sw, ctx->shape, a);
     }
     if (ev_stop) HIPCHK(hipEventRecord(ev_stop, s));
-    hipLaunchKernelGGL(k_finish_rep, dim3(nrep, ACC_N), dim3(1024), 0, s, part,
-                       max_items, rep_first, d_out);
+    // fixed-point forces into the caller's array, per-replica sums
+    const int ab = blocks_for(ctx->n, 1024);
+    const dim3 fg((unsigned)(ab + nrep * ACC_N));
+    if (mixed)
+      hipLaunchKernelGGL(k_half_finish<false>, fg, dim3(1024), 0, s, ffix,
+                         S.atom, ctx->n, d_forces, ab, part, max_items,
+                         rep_first, d_out);
+    else
+      hipLaunchKernelGGL(k_half_finish<true>, fg, dim3(1024), 0, s, ffix,
+                         S.atom, ctx->n, d_forces, ab, part, max_items,
+                         rep_first, d_out);
     HIPCHK(hipGetLastError());
     return PMNB_OK;
   }
@@ -844,10 +865,10 @@ int run_device(pmnb_ctx *ctx, const double *d_pos, double *d_forces,
   if (list) {
     if (ctx->per_pair_wrap)
       hipLaunchKernelGGL(k_build_list<true>, dim3(nb), dim3(kBlock), 0, s, P,
-                         S, lst, cnt, cap, np);
+                         S, lst, cnt, cap, np, ctx->plog);
     else
       hipLaunchKernelGGL(k_build_list<false>, dim3(nb), dim3(kBlock), 0, s, P,
-                         S, lst, cnt, cap, np);
+                         S, lst, cnt, cap, np, ctx->plog);
   }
   if (ev_built) HIPCHK(hipEventRecord(ev_built, s));
   if (list && mixed) {
@@ -883,29 +904,29 @@ int run_device(pmnb_ctx *ctx, const double *d_pos, double *d_forces,
   if (mixed) {
     if (el == PMNB_ELEC_NONE)
       hipLaunchKernelGGL(k_pair_mixed<0>, dim3(nb), dim3(kBlock), 0, s, P, S,
-                         d_forces, pd, np, only, cap);
+                         d_forces, pd, np, only, cap, ctx->plog);
     else if (el == PMNB_ELEC_COULOMB)
       hipLaunchKernelGGL(k_pair_mixed<1>, dim3(nb), dim3(kBlock), 0, s, P, S,
-                         d_forces, pd, np, only, cap);
+                         d_forces, pd, np, only, cap, ctx->plog);
     else
       hipLaunchKernelGGL(k_pair_mixed<2>, dim3(nb), dim3(kBlock), 0, s, P, S,
-                         d_forces, pd, np, only, cap);
+                         d_forces, pd, np, only, cap, ctx->plog);
   } else {
     if (el == PMNB_ELEC_NONE)
       hipLaunchKernelGGL((k_pair_fp64<0, 0>), dim3(nb), dim3(kBlock), 0, s, P,
-                         S, d_forces, pd, np, only, cap);
+                         S, d_forces, pd, np, only, cap, ctx->plog);
     else if (el == PMNB_ELEC_COULOMB && sw == PMNB_SWITCH_C2)
       hipLaunchKernelGGL((k_pair_fp64<1, 0>), dim3(nb), dim3(kBlock), 0, s, P,
-                         S, d_forces, pd, np, only, cap);
+                         S, d_forces, pd, np, only, cap, ctx->plog);
     else if (el == PMNB_ELEC_COULOMB)
       hipLaunchKernelGGL((k_pair_fp64<1, 1>), dim3(nb), dim3(kBlock), 0, s, P,
-                         S, d_forces, pd, np, only, cap);
+                         S, d_forces, pd, np, only, cap, ctx->plog);
     else if (sw == PMNB_SWITCH_C2)
       hipLaunchKernelGGL((k_pair_fp64<2, 0>), dim3(nb), dim3(kBlock), 0, s, P,
-                         S, d_forces, pd, np, only, cap);
+                         S, d_forces, pd, np, only, cap, ctx->plog);
     else
       hipLaunchKernelGGL((k_pair_fp64<2, 1>), dim3(nb), dim3(kBlock), 0, s, P,
-                         S, d_forces, pd, np, only, cap);
+                         S, d_forces, pd, np, only, cap, ctx->plog);
   }
   if (ev_stop) HIPCHK(hipEventRecord(ev_stop, s));
   hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, s, part, nb,
@@ -1075,6 +1096,7 @@ int pmnb_set_topology(pmnb_ctx *ctx, int32_t n_atoms, const int32_t *type,
   ctx->csf = coulomb_scaling_factor;
   ctx->max_excl_delta = maxd;
   ctx->have_topo = true;
+  ctx->topo_gen++;
   // the choice of pair kernel depends on the topology: rebuild the grid
   if (ctx->have_box) {
     const double *v = ctx->box_key;
@@ -1129,13 +1151,28 @@ int pmnb_evaluate(pmnb_ctx *ctx, const double *positions, double *forces,
     return fail(ctx, PMNB_ERR_ARG, "null molecular virial or centres");
   HIPCHK(hipSetDevice(ctx->prm.device));
   const size_t N = (size_t)std::max(ctx->n, 1);
+  const size_t n3 = (size_t)ctx->n * 3;
   hipStream_t s = ctx->stream;
   HIPCHK(ctx->d_pos.ensure(N * 3 * sizeof(double)));
   HIPCHK(ctx->d_force.ensure(N * 3 * sizeof(double)));
   HIPCHK(ctx->d_out.ensure(20 * sizeof(double)));
-  if (ctx->n > 0)
-    HIPCHK(hipMemcpyAsync(ctx->d_pos.p, positions, ctx->n * 3 * sizeof(double),
+  // pinned staging (allocated once per size): DMA both ways at full PCIe
+  // rate instead of the runtime's pageable bounce buffers
+  const size_t want = (2 * N * 3 + 24) * sizeof(double);
+  if (ctx->h_stage_bytes < want) {
+    if (ctx->h_stage) HIPCHK(hipHostFree(ctx->h_stage));
+    ctx->h_stage = nullptr;
+    ctx->h_stage_bytes = 0;
+    HIPCHK(hipHostMalloc((void **)&ctx->h_stage, want, hipHostMallocDefault));
+    ctx->h_stage_bytes = want;
+  }
+  double *h_pos = ctx->h_stage, *h_frc = ctx->h_stage + N * 3;
+  double *h_out = ctx->h_stage + 2 * N * 3;
+  if (n3) {
+    std::memcpy(h_pos, positions, n3 * sizeof(double));
+    HIPCHK(hipMemcpyAsync(ctx->d_pos.p, h_pos, n3 * sizeof(double),
                           hipMemcpyHostToDevice, s));
+  }
   launch_fill_i32(ctx->d_force.as<int>(), (int)(N * 6), 0, s);
   launch_fill_i32(ctx->d_out.as<int>(), 40, 0, s);
   const double *d_com = nullptr;
@@ -1150,24 +1187,23 @@ int pmnb_evaluate(pmnb_ctx *ctx, const double *positions, double *forces,
   rc = run_device(ctx, ctx->d_pos.as<double>(), ctx->d_force.as<double>(),
                   ctx->d_out.as<double>(), d_com, flags, s);
   if (rc) return rc;
-  std::vector<double> f(N * 3);
-  double out[20];
-  unsigned long long np = 0;
-  if (ctx->n > 0)
-    HIPCHK(hipMemcpyAsync(f.data(), ctx->d_force.p, ctx->n * 3 * sizeof(double),
+  if (n3)
+    HIPCHK(hipMemcpyAsync(h_frc, ctx->d_force.p, n3 * sizeof(double),
                           hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(out, ctx->d_out.p, sizeof out, hipMemcpyDeviceToHost,
-                        s));
-  HIPCHK(hipMemcpyAsync(&np, ctx->d_npairs.p, sizeof np, hipMemcpyDeviceToHost,
-                        s));
+  HIPCHK(hipMemcpyAsync(h_out, ctx->d_out.p, 20 * sizeof(double),
+                        hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(h_out + 20, ctx->d_npairs.p, sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  for (size_t i = 0; i < (size_t)ctx->n * 3; i++) forces[i] += f[i];
-  energies[0] += out[0];
-  energies[1] += out[1];
+  for (size_t i = 0; i < n3; i++) forces[i] += h_frc[i];
+  energies[0] += h_out[0];
+  energies[1] += h_out[1];
   if (flags & (PMNB_FLAG_VIRIAL | PMNB_FLAG_MOLVIRIAL))
-    for (int k = 0; k < 9; k++) virial[k] += out[2 + k];
+    for (int k = 0; k < 9; k++) virial[k] += h_out[2 + k];
   if (flags & PMNB_FLAG_MOLVIRIAL)
-    for (int k = 0; k < 9; k++) molvirial[k] += out[11 + k];
+    for (int k = 0; k < 9; k++) molvirial[k] += h_out[11 + k];
+  unsigned long long np = 0;
+  std::memcpy(&np, h_out + 20, sizeof np);
   ctx->last_pairs = (long long)np;
   return PMNB_OK;
 }
@@ -1230,6 +1266,75 @@ int pmnb_pairset(pmnb_ctx *ctx, const double *positions, int32_t *pairs,
   return PMNB_OK;
 }
 
+namespace {
+
+// One evaluation at device positions d_pos with the force kernels' pair
+// record on; the forces and energies go to scratch arrays.  Returns the
+// recorded (i, j, class) triples sorted by (i, j) in pairs (when capacity
+// allows) and their number.
+int record_force_pairs(pmnb_ctx *ctx, const double *d_pos, int32_t *pairs,
+                       int64_t capacity, int64_t *n_pairs) {
+  hipStream_t s = ctx->stream;
+  const size_t N = (size_t)std::max(ctx->n, 1);
+  const int nout = 20 * std::max(1, (int)ctx->reps.size());
+  HIPCHK(ctx->d_force.ensure(N * 3 * sizeof(double)));
+  HIPCHK(ctx->d_out.ensure((size_t)nout * sizeof(double)));
+  const long long cap = pairs ? capacity : 0;
+  HIPCHK(ctx->d_plog.ensure((size_t)std::max<long long>(cap, 1) * 3 *
+                            sizeof(int)));
+  HIPCHK(ctx->d_plog_count.ensure(sizeof(unsigned long long)));
+  launch_fill_i32(ctx->d_force.as<int>(), (int)(N * 6), 0, s);
+  launch_fill_i32(ctx->d_out.as<int>(), 2 * nout, 0, s);
+  launch_fill_i32(ctx->d_plog_count.as<int>(), 2, 0, s);
+  ctx->plog = PairLog{ctx->d_plog.as<int>(), cap,
+                      ctx->d_plog_count.as<unsigned long long>()};
+  const int rc = run_device(ctx, d_pos, ctx->d_force.as<double>(),
+                            ctx->d_out.as<double>(), nullptr, 0, s);
+  ctx->plog = PairLog{nullptr, 0, nullptr};
+  if (rc) return rc;
+  unsigned long long np = 0;
+  HIPCHK(hipMemcpyAsync(&np, ctx->d_plog_count.p, sizeof np,
+                        hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  *n_pairs = (int64_t)np;
+  if (!pairs) return PMNB_OK;
+  if ((long long)np > cap)
+    return fail(ctx, PMNB_ERR_CAPACITY, "pair set has %llu pairs, capacity %lld",
+                np, cap);
+  if (np > 0)
+    HIPCHK(hipMemcpy(pairs, ctx->d_plog.p, (size_t)np * 3 * sizeof(int),
+                     hipMemcpyDeviceToHost));
+  std::vector<size_t> ord((size_t)np);
+  for (size_t k = 0; k < ord.size(); k++) ord[k] = k;
+  std::sort(ord.begin(), ord.end(), [&](size_t x, size_t y) {
+    if (pairs[3 * x] != pairs[3 * y]) return pairs[3 * x] < pairs[3 * y];
+    return pairs[3 * x + 1] < pairs[3 * y + 1];
+  });
+  std::vector<int32_t> tmp((size_t)np * 3);
+  for (size_t k = 0; k < ord.size(); k++)
+    for (int c = 0; c < 3; c++) tmp[3 * k + c] = pairs[3 * ord[k] + c];
+  std::copy(tmp.begin(), tmp.end(), pairs);
+  return PMNB_OK;
+}
+
+}  // namespace
+
+int pmnb_force_pairs(pmnb_ctx *ctx, const double *positions, int32_t *pairs,
+                     int64_t capacity, int64_t *n_pairs) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (!n_pairs || (ctx->n > 0 && !positions) || capacity < 0)
+    return fail(ctx, PMNB_ERR_ARG, "bad force-pairs arguments");
+  HIPCHK(hipSetDevice(ctx->prm.device));
+  const size_t N = (size_t)std::max(ctx->n, 1);
+  HIPCHK(ctx->d_pos.ensure(N * 3 * sizeof(double)));
+  if (ctx->n > 0)
+    HIPCHK(hipMemcpyAsync(ctx->d_pos.p, positions, ctx->n * 3 * sizeof(double),
+                          hipMemcpyHostToDevice, ctx->stream));
+  return record_force_pairs(ctx, ctx->d_pos.as<double>(), pairs, capacity,
+                            n_pairs);
+}
+
 int pmnb_kernel_timing(pmnb_ctx *ctx, int32_t enable, double *ms,
                        int64_t *launches) {
   if (!ctx) return PMNB_ERR_ARG;
@@ -1251,6 +1356,15 @@ int pmnb_kernel_timing(pmnb_ctx *ctx, int32_t enable, double *ms,
   if (launches) *launches = (int64_t)(used / 3);
   ctx->ev_used = 0;
   ctx->timing = enable != 0;
+  return PMNB_OK;
+}
+
+int pmnb_pair_path(const pmnb_ctx *ctx, int32_t *path) {
+  if (!ctx || !path) return PMNB_ERR_ARG;
+  if (!ctx->have_box || !ctx->have_topo) return PMNB_ERR_STATE;
+  const bool list = ctx->use_list && ctx->n < (1 << kEntryAtomBits);
+  *path = ctx->use_tile ? PMNB_PATH_TILE : (list ? PMNB_PATH_LIST
+                                                 : PMNB_PATH_SCAN);
   return PMNB_OK;
 }
 
@@ -1287,9 +1401,11 @@ void pmnb_destroy(pmnb_ctx *ctx) {
                     &ctx->d_partial2, &ctx->d_shell_off, &ctx->d_box,
                     &ctx->d_items,
                     &ctx->d_nitems, &ctx->d_reps, &ctx->d_col_rep,
-                    &ctx->d_rep_first, &ctx->d_rep_of, &ctx->d_partial_lj};
+                    &ctx->d_rep_first, &ctx->d_rep_of, &ctx->d_partial_lj,
+                    &ctx->d_ffix, &ctx->d_plog, &ctx->d_plog_count};
   for (DevBuf *b : bufs) b->release();
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+  if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1598,7 +1714,9 @@ int pmnb_batch_run_graph(pmnb_batch *b, const double *d_positions,
 
 int pmnb_batch_set_shape(pmnb_batch *b, int32_t shape, int32_t grid_blocks) {
   if (!b) return PMNB_ERR_ARG;
-  if (shape < 0 || shape >= kMixedShapes || grid_blocks < 0)
+  const int nshape = b->ctx->prm.precision == PMNB_PREC_MIXED ? kMixedShapes
+                                                              : kF64Shapes;
+  if (shape < 0 || shape >= nshape || grid_blocks < 0)
     return fail(b->ctx, PMNB_ERR_ARG, "shape %d / grid %d out of range", shape,
                 grid_blocks);
   b->ctx->shape = shape;
@@ -1618,7 +1736,7 @@ int pmnb_batch_autotune(pmnb_batch *b, const double *d_positions,
   HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
                                ctx->prm.device));
   const bool mixed = ctx->prm.precision == PMNB_PREC_MIXED;
-  const int nshape = mixed ? kMixedShapes : 2;
+  const int nshape = mixed ? kMixedShapes : kF64Shapes;
   const int caps[4] = {0, 2 * ncu, 4 * ncu, 8 * ncu};
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
@@ -1664,6 +1782,19 @@ int pmnb_batch_autotune(pmnb_batch *b, const double *d_positions,
   return PMNB_OK;
 }
 
+int pmnb_batch_force_pairs(pmnb_batch *b, const double *d_positions,
+                           int32_t *pairs, int64_t capacity,
+                           int64_t *n_pairs) {
+  if (!b) return PMNB_ERR_ARG;
+  pmnb_ctx *ctx = b->ctx;
+  if (!n_pairs || !d_positions || capacity < 0)
+    return fail(ctx, PMNB_ERR_ARG, "bad force-pairs arguments");
+  int rc = batch_finalize(b);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(ctx->prm.device));
+  return record_force_pairs(ctx, d_positions, pairs, capacity, n_pairs);
+}
+
 int pmnb_batch_pair_count(pmnb_batch *b, int64_t *n_pairs) {
   if (!b) return PMNB_ERR_ARG;
   return pmnb_last_pair_count(b->ctx, n_pairs);
@@ -1700,6 +1831,9 @@ struct pmnb_leapfrog {
   // bonded terms evaluated with the force (pmnb_lf_set_bonded)
   int n_bonds = 0, n_angles = 0;
   DevBuf bptr, bent, aptr, aent, bonds, bk, br0, angles, ak, at0, ubk, ubr0;
+  // k-space terms evaluated with the force (pmnb_lf_set_kspace)
+  pmnb_pme *pme = nullptr;
+  pmnb_ewald *ewald = nullptr;
 };
 
 namespace {
@@ -1855,6 +1989,16 @@ int lf_calc_forces(pmnb_leapfrog *lf, const double *d_pos, double *d_force,
   pmnb_ctx *ctx = lf->ctx;
   int rc = run_device(ctx, d_pos, d_force, d_out, nullptr, flags, s);
   if (rc) return rc;
+  if (lf->pme) {
+    rc = pmnb_pme_evaluate_device(lf->pme, d_pos, d_force, d_out,
+                                  flags & PMNB_FLAG_VIRIAL, s);
+    if (rc) return rc;
+  }
+  if (lf->ewald) {
+    rc = pmnb_ewald_evaluate_device(lf->ewald, d_pos, d_force, d_out,
+                                    flags & PMNB_FLAG_VIRIAL, s);
+    if (rc) return rc;
+  }
   if (lf->n_bonds + lf->n_angles == 0) return PMNB_OK;
   const LfBonded B = lf_bonded(lf);
   hipLaunchKernelGGL(k_lf_bonded, dim3(blocks_for(lf->n, kLfBlock)),
@@ -2226,6 +2370,62 @@ int pmnb_lf_forces_device(pmnb_leapfrog *lf, const double *d_positions,
 }  // extern "C"
 
 // ------------------------------------------------------------------ Ewald
+namespace {
+
+// The rows ExclusionTable::add keeps in getTable()
+// (topology/ExclusionTable.cpp:103-110) from rows given in add order: no
+// row of an atom with itself, no EXCLUSION_NONE row, one row per pair with
+// the class of its first add, in the order given.  The correction terms of
+// the Ewald sums walk these.  Rows are (a1 < a2, class).
+void correction_rows(int n_atoms, int n_excl, const int32_t *excl,
+                     std::vector<int32_t> &rows) {
+  rows.clear();
+  rows.reserve(3 * (size_t)n_excl);
+  std::vector<long long> keys((size_t)n_excl, -1);
+  for (int i = 0; i < n_excl; i++) {
+    int a = excl[3 * i], c = excl[3 * i + 1];
+    if (a == c || excl[3 * i + 2] == PMNB_EXCL_NONE) continue;
+    if (a > c) std::swap(a, c);
+    keys[i] = (long long)a * n_atoms + c;
+  }
+  std::vector<long long> sorted_keys;
+  for (long long k : keys)
+    if (k >= 0) sorted_keys.push_back(k);
+  std::sort(sorted_keys.begin(), sorted_keys.end());
+  std::vector<char> taken(sorted_keys.size(), 0);
+  for (int i = 0; i < n_excl; i++) {
+    if (keys[i] < 0) continue;
+    const size_t pos = std::lower_bound(sorted_keys.begin(), sorted_keys.end(),
+                                        keys[i]) -
+                       sorted_keys.begin();
+    if (taken[pos]) continue;  // a later add of a pair already held
+    taken[pos] = 1;
+    rows.push_back((int32_t)(keys[i] / n_atoms));
+    rows.push_back((int32_t)(keys[i] % n_atoms));
+    rows.push_back(excl[3 * i + 2]);
+  }
+}
+
+// Per atom, its correction rows in row order (CSR: cnt[n + 1], entries
+// row << 1 | role, role 0 for a1 and 1 for a2).
+void correction_csr(int n_atoms, int n_excl, const int32_t *excl,
+                    std::vector<int> &cnt, std::vector<int> &ent) {
+  cnt.assign(n_atoms + 1, 0);
+  for (int i = 0; i < n_excl; i++) {
+    cnt[excl[3 * i] + 1]++;
+    cnt[excl[3 * i + 1] + 1]++;
+  }
+  for (int i = 0; i < n_atoms; i++) cnt[i + 1] += cnt[i];
+  ent.assign(std::max(2 * n_excl, 1), 0);
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  for (int i = 0; i < n_excl; i++) {
+    ent[fill[excl[3 * i]]++] = i << 1;
+    ent[fill[excl[3 * i + 1]]++] = (i << 1) | 1;
+  }
+}
+
+}  // namespace
+
 struct pmnb_ewald {
   pmnb_ctx *ctx = nullptr;
   int device = 0, n = 0, nk = 0, H = 0, n_excl = 0;
@@ -2233,6 +2433,9 @@ struct pmnb_ewald {
   double csf = 1.0, e_self = 0, e_charged = 0;
   int n_runs = 0, n_chunks = 0;
   DevBuf kv, q, tab, sk, kf, part, xptr, xent, xrows, runs, chunks;
+  // the box and topology the k-vectors, charges and rows were built for
+  double box_key[12] = {0};
+  unsigned long long topo_gen = 0;
 };
 
 extern "C" {
@@ -2254,16 +2457,26 @@ int pmnb_ewald_create(pmnb_ctx *ctx, double alpha, double accuracy,
     return fail(ctx, PMNB_ERR_ARG, "accuracy must lie in (0, 1)");
   if (n_excl < 0 || (n_excl > 0 && !excl))
     return fail(ctx, PMNB_ERR_ARG, "bad exclusion table");
-  for (int i = 0; i < n_excl; i++)
+  for (int i = 0; i < n_excl; i++) {
     for (int k = 0; k < 2; k++)
       if (excl[3 * i + k] < 0 || excl[3 * i + k] >= n_atoms)
         return fail(ctx, PMNB_ERR_ARG, "exclusion row %d out of range", i);
+    if (excl[3 * i + 2] < PMNB_EXCL_NONE || excl[3 * i + 2] > PMNB_EXCL_FULL)
+      return fail(ctx, PMNB_ERR_ARG, "exclusion row %d has class %d", i,
+                  excl[3 * i + 2]);
+  }
+  std::vector<int32_t> rows;
+  correction_rows(n_atoms, n_excl, excl, rows);
+  n_excl = (int32_t)(rows.size() / 3);
+  excl = rows.empty() ? nullptr : rows.data();
   const Box &b = ctx->box;
   pmnb_ewald *ew = new pmnb_ewald;
   ew->ctx = ctx;
   ew->device = ctx->prm.device;
   ew->n = n_atoms;
   ew->n_excl = n_excl;
+  std::memcpy(ew->box_key, ctx->box_key, sizeof ew->box_key);
+  ew->topo_gen = ctx->topo_gen;
   ew->csf = coulomb_scaling_factor;
   // NonbondedFullEwaldSystemForce::initialize (:597-700), orthogonal box
   const double L[3] = {b.e[0], b.e[1], b.e[2]};
@@ -2329,17 +2542,8 @@ int pmnb_ewald_create(pmnb_ctx *ctx, double alpha, double accuracy,
   if (std::fabs(qs * 0.00268283) > 1.0e-5)
     ew->e_charged = -M_PI / (2.0 * ew->V * ew->alpha2) * qs * qs;
   // per-atom exclusion rows in row order
-  std::vector<int> cnt(n_atoms + 1, 0);
-  for (int i = 0; i < n_excl; i++) {
-    cnt[excl[3 * i] + 1]++;
-    cnt[excl[3 * i + 1] + 1]++;
-  }
-  for (int i = 0; i < n_atoms; i++) cnt[i + 1] += cnt[i];
-  std::vector<int> ent(std::max(2 * n_excl, 1)), fill(cnt.begin(), cnt.end() - 1);
-  for (int i = 0; i < n_excl; i++) {
-    ent[fill[excl[3 * i]]++] = i << 1;
-    ent[fill[excl[3 * i + 1]]++] = (i << 1) | 1;
-  }
+  std::vector<int> cnt, ent;
+  correction_csr(n_atoms, n_excl, excl, cnt, ent);
   auto up = [&](DevBuf &d, const void *src, size_t bytes) -> hipError_t {
     hipError_t e = d.ensure(std::max<size_t>(bytes, 16));
     if (e == hipSuccess && bytes && src)
@@ -2384,8 +2588,11 @@ int pmnb_ewald_evaluate_device(pmnb_ewald *ew, const double *d_positions,
   pmnb_ctx *ctx = ew->ctx;
   int rc = check_ready(ctx);
   if (rc) return rc;
-  if (ctx->n != ew->n)
-    return fail(ctx, PMNB_ERR_STATE, "topology changed since pmnb_ewald_create");
+  if (ctx->n != ew->n || ctx->topo_gen != ew->topo_gen ||
+      std::memcmp(ctx->box_key, ew->box_key, sizeof ew->box_key) != 0)
+    return fail(ctx, PMNB_ERR_STATE,
+                "the box or topology changed since pmnb_ewald_create; "
+                "create the k-space terms again");
   if (flags & PMNB_FLAG_MOLVIRIAL)
     return fail(ctx, PMNB_ERR_UNSUPPORTED,
                 "the molecular virial of the k-space term is not built");
@@ -2447,6 +2654,371 @@ void pmnb_ewald_destroy(pmnb_ewald *ew) {
                     &ew->xptr, &ew->xent, &ew->xrows, &ew->runs, &ew->chunks};
   for (DevBuf *b : bufs) b->release();
   delete ew;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------- PME
+struct pmnb_pme {
+  pmnb_ctx *ctx = nullptr;
+  int device = 0, n = 0, n_excl = 0, order = 0;
+  int nx = 0, ny = 0, nz = 0;
+  double alpha = 0, kc = 0, V = 0;
+  double csf = 1.0, e_self = 0, e_charged = 0;
+  DevBuf q, qfix, qreal, qk, mod, ex, part, ew_part, xptr, xent, xrows;
+  int n_part = 0;
+  hipfftHandle fwd = 0, inv = 0;
+  bool have_fwd = false, have_inv = false;
+  double box_key[12] = {0};
+  unsigned long long topo_gen = 0;
+};
+
+namespace {
+
+// Host restatement of BSpline::set at w (BSpline.cpp:58-95), for dftmod.
+void host_bspline(int order, double w, std::vector<double> &th) {
+  th.assign(order, 0.0);
+  if (order == 1) {
+    th[0] = 1;
+    return;
+  }
+  if (order == 2) {
+    th[1] = w;
+    th[0] = 1.0 - w;
+    return;
+  }
+  th[order - 1] = 0.0;
+  th[1] = w;
+  th[0] = 1.0 - w;
+  for (int k = 2; k < order - 1; k++) {
+    const double div = 1.0 / k;
+    th[k] = div * w * th[k - 1];
+    for (int j = 1; j < k; j++)
+      th[k - j] = div * ((w + j) * th[k - j - 1] + (k + 1 - j - w) * th[k - j]);
+    th[0] = div * (1 - w) * th[0];
+  }
+  const double div = 1.0 / (order - 1.0);
+  th[order - 1] = div * w * th[order - 2];
+  for (int j = 1; j < order - 1; j++)
+    th[order - 1 - j] =
+        div * ((w + j) * th[order - j - 2] + (order - j - w) * th[order - 1 - j]);
+  th[0] = div * (1 - w) * th[0];
+}
+
+// Grid::dftmod (Grid.h:590-602): B(m) = 1 / |sum_j M(j) e^{2 pi i m j / n}|^2.
+void host_dftmod(int order, int n, std::vector<double> &out) {
+  std::vector<double> th;
+  host_bspline(order, 0.0, th);
+  out.assign(n, 0.0);
+  for (int i = 0; i < n; i++) {
+    double sc = 0.0, ss = 0.0;
+    for (int j = 0; j < order; j++) {
+      const double x = M_PI * 2.0 * i * j / (double)n;
+      sc += th[j] * std::cos(x);
+      ss += th[j] * std::sin(x);
+    }
+    out[i] = 1.0 / (sc * sc + ss * ss);
+  }
+}
+
+PmeDev pme_dev(pmnb_pme *pm) {
+  PmeDev d{};
+  d.n = pm->n;
+  d.nx = pm->nx;
+  d.ny = pm->ny;
+  d.nz = pm->nz;
+  d.nzh = pm->nz / 2 + 1;
+  const Box &b = pm->ctx->box;
+  const int nn[3] = {pm->nx, pm->ny, pm->nz};
+  for (int a = 0; a < 3; a++) {
+    // Grid::initialize (Grid.h:507-528): h^-1 = n / L, offsets
+    d.hr[a] = nn[a] / b.e[a];
+    d.off[a] = -(pm->order - 1) / 2 + nn[a];
+    d.Lr[a] = 1.0 / b.e[a];
+  }
+  d.piVr = 1.0 / (M_PI * pm->V);
+  d.fac = M_PI * M_PI / (pm->alpha * pm->alpha);
+  d.q = pm->q.as<double>();
+  d.qfix = pm->qfix.as<long long>();
+  d.qreal = pm->qreal.as<double>();
+  d.qk = pm->qk.as<double2>();
+  const double *mod = pm->mod.as<double>(), *ex = pm->ex.as<double>();
+  d.mod[0] = mod;
+  d.mod[1] = mod + pm->nx;
+  d.mod[2] = mod + pm->nx + pm->ny;
+  d.ex[0] = ex;
+  d.ex[1] = ex + pm->nx;
+  d.ex[2] = ex + pm->nx + pm->ny;
+  d.part = pm->part.as<double>();
+  d.n_part = pm->n_part;
+  return d;
+}
+
+EwDev pme_corr_dev(pmnb_pme *pm) {
+  EwDev e{};
+  e.box = pm->ctx->box;
+  e.n = pm->n;
+  e.nk = 0;  // correction terms only
+  e.q = pm->q.as<double>();
+  e.part = pm->ew_part.as<double>();
+  e.xptr = pm->xptr.as<int>();
+  e.xent = pm->xent.as<int>();
+  e.xrows = pm->xrows.as<int>();
+  e.n_excl = pm->n_excl;
+  e.alpha = pm->alpha;
+  e.alpha2 = pm->alpha * pm->alpha;
+  e.two_alpha_pi = 2.0 * pm->alpha / std::sqrt(M_PI);
+  e.csf = pm->csf;
+  e.Vr = 1.0 / pm->V;
+  e.fac = 1.0 / (4.0 * pm->alpha * pm->alpha);
+  return e;
+}
+
+template <int P>
+void pme_spread(pmnb_pme *pm, const PmeDev &d, const double *x,
+                hipStream_t s) {
+  hipLaunchKernelGGL(k_pme_spread<P>, dim3(blocks_for(pm->n, kPmeBlock)),
+                     dim3(kPmeBlock), 0, s, d, x);
+  const long long total = (long long)pm->nx * pm->ny * pm->nz;
+  hipLaunchKernelGGL(k_pme_to_real,
+                     dim3((unsigned)((total + kPmeBlock - 1) / kPmeBlock)),
+                     dim3(kPmeBlock), 0, s, d, total);
+}
+
+template <int P>
+void pme_launch_gather(pmnb_pme *pm, const PmeDev &d, const EwDev &e,
+                       const double *x, double *f, hipStream_t s) {
+  hipLaunchKernelGGL(k_pme_gather<P>, dim3(blocks_for(pm->n, kPmeBlock)),
+                     dim3(kPmeBlock), 0, s, d, e, x, f);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pmnb_pme_create(pmnb_ctx *ctx, int32_t nx, int32_t ny, int32_t nz,
+                    int32_t order, double cutoff, double accuracy,
+                    double alpha, int32_t n_atoms,
+                    const double *scaled_charge,
+                    double coulomb_scaling_factor, int32_t n_excl,
+                    const int32_t *excl, pmnb_pme **out) {
+  if (!ctx || !out) return PMNB_ERR_ARG;
+  *out = nullptr;
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  // NonbondedPMEwaldSystemForce::doMake (NonbondedPMEwaldSystemForce.h
+  // :1029-1058): even order >= 2, every grid size >= order, cutoff and
+  // accuracy positive
+  if (order < 2 || (order % 2) != 0 || nx < order || ny < order ||
+      nz < order || !(cutoff > 0.0) || !(accuracy > 0.0))
+    return fail(ctx, PMNB_ERR_ARG,
+                "PME needs an even order >= 2, grid sizes >= order, cutoff "
+                "and accuracy > 0");
+  if (order > 8)
+    return fail(ctx, PMNB_ERR_UNSUPPORTED, "B-spline orders 2 to 8 are built");
+  if ((long long)nx * ny * nz > (1ll << 28))
+    return fail(ctx, PMNB_ERR_UNSUPPORTED, "PME grid too large");
+  if (n_atoms != ctx->n)
+    return fail(ctx, PMNB_ERR_ARG, "n_atoms %d differs from the topology's %d",
+                n_atoms, ctx->n);
+  if (n_atoms > 0 && !scaled_charge)
+    return fail(ctx, PMNB_ERR_ARG, "null charges");
+  if (n_excl < 0 || (n_excl > 0 && !excl))
+    return fail(ctx, PMNB_ERR_ARG, "bad exclusion table");
+  for (int i = 0; i < n_excl; i++) {
+    for (int k = 0; k < 2; k++)
+      if (excl[3 * i + k] < 0 || excl[3 * i + k] >= n_atoms)
+        return fail(ctx, PMNB_ERR_ARG, "exclusion row %d out of range", i);
+    if (excl[3 * i + 2] < PMNB_EXCL_NONE || excl[3 * i + 2] > PMNB_EXCL_FULL)
+      return fail(ctx, PMNB_ERR_ARG, "exclusion row %d has class %d", i,
+                  excl[3 * i + 2]);
+  }
+  std::vector<int32_t> rows;
+  correction_rows(n_atoms, n_excl, excl, rows);
+  n_excl = (int32_t)(rows.size() / 3);
+  const Box &b = ctx->box;
+  pmnb_pme *pm = new pmnb_pme;
+  pm->ctx = ctx;
+  pm->device = ctx->prm.device;
+  pm->n = n_atoms;
+  pm->n_excl = n_excl;
+  pm->order = order;
+  pm->nx = nx;
+  pm->ny = ny;
+  pm->nz = nz;
+  pm->csf = coulomb_scaling_factor;
+  std::memcpy(pm->box_key, ctx->box_key, sizeof pm->box_key);
+  pm->topo_gen = ctx->topo_gen;
+  // NonbondedPMEwaldSystemForce::initialize (:525-612), orthogonal box
+  pm->V = std::fabs((b.e[0] * b.e[1]) * b.e[2]);
+  if (alpha > 0.0) {
+    pm->alpha = alpha;
+  } else {
+    // erfc(alpha rc) / rc = accuracy by doubling, then 100 bisection steps
+    double a = 1.0;
+    while (std::erfc(a * cutoff) / cutoff >= accuracy) a *= 2.0;
+    double lo = 0.0, hi = a;
+    for (int i = 0; i < 100; i++) {
+      a = 0.5 * (lo + hi);
+      if (std::erfc(a * cutoff) / cutoff >= accuracy) lo = a;
+      else hi = a;
+    }
+    pm->alpha = a;
+  }
+  pm->kc = 2.0 * pm->alpha * std::sqrt(-std::log(accuracy));
+  double q2 = 0.0, qs = 0.0;
+  for (int i = 0; i < n_atoms; i++) q2 += scaled_charge[i] * scaled_charge[i];
+  for (int i = 0; i < n_atoms; i++) qs += scaled_charge[i];
+  pm->e_self = -q2 * pm->alpha / std::sqrt(M_PI);
+  pm->e_charged = 0.0;
+  if (std::fabs(qs * 0.00268283) > 1.0e-5)
+    pm->e_charged = -M_PI / (2.0 * pm->V * pm->alpha * pm->alpha) * qs * qs;
+  // per-axis tables: B(m) and exp(-pi^2 m^2 / alpha^2), m = i0 / L
+  const double fac = M_PI * M_PI / (pm->alpha * pm->alpha);
+  std::vector<double> mod, ex, tmp;
+  const int nn[3] = {nx, ny, nz};
+  for (int a = 0; a < 3; a++) {
+    host_dftmod(order, nn[a], tmp);
+    mod.insert(mod.end(), tmp.begin(), tmp.end());
+    const double Lr = 1.0 / b.e[a];
+    for (int i = 0; i < nn[a]; i++) {
+      const int i0 = i <= nn[a] / 2 ? i : i - nn[a];
+      const double m = i0 * Lr;
+      ex.push_back(std::exp(-fac * (m * m)));
+    }
+  }
+  std::vector<int> cnt, ent;
+  correction_csr(n_atoms, n_excl, rows.empty() ? nullptr : rows.data(), cnt,
+                 ent);
+  const long long total = (long long)nx * ny * nz;
+  const long long half = (long long)nx * ny * (nz / 2 + 1);
+  pm->n_part = (int)((half + kPmeBlock - 1) / kPmeBlock);
+  auto up = [&](DevBuf &d, const void *src, size_t bytes) -> hipError_t {
+    hipError_t e = d.ensure(std::max<size_t>(bytes, 16));
+    if (e == hipSuccess && bytes && src)
+      e = hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice);
+    return e;
+  };
+  hipError_t e = hipSetDevice(pm->device);
+  if (e == hipSuccess) e = up(pm->q, scaled_charge, (size_t)n_atoms * sizeof(double));
+  if (e == hipSuccess) e = pm->qfix.ensure((size_t)total * sizeof(long long));
+  if (e == hipSuccess) e = hipMemset(pm->qfix.p, 0, (size_t)total * sizeof(long long));
+  if (e == hipSuccess) e = pm->qreal.ensure((size_t)total * sizeof(double));
+  if (e == hipSuccess) e = pm->qk.ensure((size_t)half * sizeof(double2));
+  if (e == hipSuccess) e = up(pm->mod, mod.data(), mod.size() * sizeof(double));
+  if (e == hipSuccess) e = up(pm->ex, ex.data(), ex.size() * sizeof(double));
+  if (e == hipSuccess)
+    e = pm->part.ensure((size_t)pm->n_part * kPmeSums * sizeof(double));
+  if (e == hipSuccess)
+    e = pm->ew_part.ensure((size_t)kEwFinBlocks * kEwFinSums * sizeof(double));
+  if (e == hipSuccess) e = up(pm->xptr, cnt.data(), (n_atoms + 1) * sizeof(int));
+  if (e == hipSuccess) e = up(pm->xent, ent.data(), 2 * (size_t)n_excl * sizeof(int));
+  if (e == hipSuccess) e = up(pm->xrows, rows.data(), rows.size() * sizeof(int));
+  if (e != hipSuccess) {
+    pmnb_pme_destroy(pm);
+    return fail(ctx, PMNB_ERR_HIP, "PME set-up failed: %s", hipGetErrorString(e));
+  }
+  if (hipfftPlan3d(&pm->fwd, nx, ny, nz, HIPFFT_D2Z) != HIPFFT_SUCCESS) {
+    pmnb_pme_destroy(pm);
+    return fail(ctx, PMNB_ERR_HIP, "hipFFT D2Z plan %dx%dx%d failed", nx, ny, nz);
+  }
+  pm->have_fwd = true;
+  if (hipfftPlan3d(&pm->inv, nx, ny, nz, HIPFFT_Z2D) != HIPFFT_SUCCESS) {
+    pmnb_pme_destroy(pm);
+    return fail(ctx, PMNB_ERR_HIP, "hipFFT Z2D plan %dx%dx%d failed", nx, ny, nz);
+  }
+  pm->have_inv = true;
+  *out = pm;
+  return PMNB_OK;
+}
+
+int pmnb_pme_info(const pmnb_pme *pm, double *alpha, double *kc) {
+  if (!pm) return PMNB_ERR_ARG;
+  if (alpha) *alpha = pm->alpha;
+  if (kc) *kc = pm->kc;
+  return PMNB_OK;
+}
+
+int pmnb_pme_evaluate_device(pmnb_pme *pm, const double *d_positions,
+                             double *d_forces, double *d_out, uint32_t flags,
+                             void *stream) {
+  if (!pm) return PMNB_ERR_ARG;
+  pmnb_ctx *ctx = pm->ctx;
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (ctx->n != pm->n || ctx->topo_gen != pm->topo_gen ||
+      std::memcmp(ctx->box_key, pm->box_key, sizeof pm->box_key) != 0)
+    return fail(ctx, PMNB_ERR_STATE,
+                "the box or topology changed since pmnb_pme_create; create "
+                "the PME terms again");
+  if (flags & PMNB_FLAG_MOLVIRIAL)
+    return fail(ctx, PMNB_ERR_UNSUPPORTED,
+                "the molecular virial of the PME terms is not built");
+  if (pm->n > 0 && (!d_positions || !d_forces || !d_out))
+    return fail(ctx, PMNB_ERR_ARG, "null device pointer");
+  HIPCHK(hipSetDevice(pm->device));
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  if (pm->n == 0) return PMNB_OK;
+  const PmeDev d = pme_dev(pm);
+  const EwDev e = pme_corr_dev(pm);
+  const int virial = (flags & PMNB_FLAG_VIRIAL) != 0;
+  switch (pm->order) {
+    case 2: pme_spread<2>(pm, d, d_positions, s); break;
+    case 4: pme_spread<4>(pm, d, d_positions, s); break;
+    case 6: pme_spread<6>(pm, d, d_positions, s); break;
+    default: pme_spread<8>(pm, d, d_positions, s); break;
+  }
+  HIPCHK(hipGetLastError());
+  if (hipfftSetStream(pm->fwd, s) != HIPFFT_SUCCESS ||
+      hipfftExecD2Z(pm->fwd, d.qreal, (hipfftDoubleComplex *)d.qk) !=
+          HIPFFT_SUCCESS)
+    return fail(ctx, PMNB_ERR_HIP, "hipFFT D2Z failed");
+  hipLaunchKernelGGL(k_pme_scalar, dim3(pm->n_part), dim3(kPmeBlock), 0, s, d,
+                     virial);
+  HIPCHK(hipGetLastError());
+  if (hipfftSetStream(pm->inv, s) != HIPFFT_SUCCESS ||
+      hipfftExecZ2D(pm->inv, (hipfftDoubleComplex *)d.qk, d.qreal) !=
+          HIPFFT_SUCCESS)
+    return fail(ctx, PMNB_ERR_HIP, "hipFFT Z2D failed");
+  switch (pm->order) {
+    case 2: pme_launch_gather<2>(pm, d, e, d_positions, d_forces, s); break;
+    case 4: pme_launch_gather<4>(pm, d, e, d_positions, d_forces, s); break;
+    case 6: pme_launch_gather<6>(pm, d, e, d_positions, d_forces, s); break;
+    default: pme_launch_gather<8>(pm, d, e, d_positions, d_forces, s); break;
+  }
+  hipLaunchKernelGGL(k_pme_finish, dim3(1), dim3(64), 0, s, d, d_out, virial);
+  hipLaunchKernelGGL(k_ew_partials, dim3(kEwFinBlocks), dim3(kEwBlock), 0, s,
+                     e, d_positions);
+  hipLaunchKernelGGL(k_ew_finish, dim3(1), dim3(64), 0, s, e, d_out, virial,
+                     pm->e_self + pm->e_charged, pm->e_charged);
+  HIPCHK(hipGetLastError());
+  return PMNB_OK;
+}
+
+void pmnb_pme_destroy(pmnb_pme *pm) {
+  if (!pm) return;
+  (void)hipSetDevice(pm->device);
+  (void)hipDeviceSynchronize();
+  if (pm->have_fwd) (void)hipfftDestroy(pm->fwd);
+  if (pm->have_inv) (void)hipfftDestroy(pm->inv);
+  DevBuf *bufs[] = {&pm->q, &pm->qfix, &pm->qreal, &pm->qk, &pm->mod, &pm->ex,
+                    &pm->part, &pm->ew_part, &pm->xptr, &pm->xent, &pm->xrows};
+  for (DevBuf *b : bufs) b->release();
+  delete pm;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int pmnb_lf_set_kspace(pmnb_leapfrog *lf, pmnb_pme *pme, pmnb_ewald *ewald) {
+  if (!lf) return PMNB_ERR_ARG;
+  if ((pme && pme->ctx != lf->ctx) || (ewald && ewald->ctx != lf->ctx))
+    return fail(lf->ctx, PMNB_ERR_ARG,
+                "the k-space terms belong to another context");
+  lf->pme = pme;
+  lf->ewald = ewald;
+  return PMNB_OK;
 }
 
 }  // extern "C"

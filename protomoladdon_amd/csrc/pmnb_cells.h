@@ -370,6 +370,12 @@ k_sort_gather_r(const RepT *__restrict__ reps, const int *__restrict__ rep_of,
   }
 }
 
+// Sets n 64-bit integers to zero.
+__global__ void k_zero_i64(long long *p, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 // Sets n doubles to zero (a kernel node, not a memset, inside graphs).
 __global__ void k_zero_f64(double *p, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -210,4 +210,23 @@ __device__ __forceinline__ int excl_class(const unsigned long long *full,
   return 0;
 }
 
+// Record of the pairs a force kernel evaluates (parity tests): when out is
+// not null every evaluated pair appends (i, j, class) with i < j original
+// indices; *count counts all of them, out holds the first cap.
+struct PairLog {
+  int *out;
+  long long cap;
+  unsigned long long *count;
+};
+
+__device__ __forceinline__ void log_pair(const PairLog &lg, int a, int b,
+                                         int cls) {
+  const unsigned long long k = atomicAdd(lg.count, 1ull);
+  if ((long long)k < lg.cap) {
+    lg.out[3 * k] = a < b ? a : b;
+    lg.out[3 * k + 1] = a < b ? b : a;
+    lg.out[3 * k + 2] = cls;
+  }
+}
+
 }  // namespace pmnb

@@ -87,6 +87,37 @@ __device__ __forceinline__ void ew_step(double s1, double c1, double &s,
   c = c2;
 }
 
+// Intramolecular correction force on atom i (:1105-1129): every excluded
+// pair of i, in row order, each pair evaluated by both of its atoms (no
+// atomics).
+__device__ __forceinline__ void ew_correction_force(const EwDev &d,
+                                                    const double *x, int i,
+                                                    double &gx, double &gy,
+                                                    double &gz) {
+  for (int e = d.xptr[i]; e < d.xptr[i + 1]; e++) {
+    const int row = d.xent[e] >> 1, role = d.xent[e] & 1;
+    const int a1 = d.xrows[3 * row], a2 = d.xrows[3 * row + 1],
+              cls = d.xrows[3 * row + 2];
+    double rx, ry, rz, r2;
+    min_diff_exact(d.box, x[3 * a1], x[3 * a1 + 1], x[3 * a1 + 2], x[3 * a2],
+                   x[3 * a2 + 1], x[3 * a2 + 2], rx, ry, rz, r2);
+    double qq = d.q[a1] * d.q[a2];
+    if (cls == 1) qq *= 1 - d.csf;  // EXCLUSION_MODIFIED
+    const double r = sqrt(r2), rr = 1 / r;
+    const double e1 = erf(d.alpha * r) * rr;
+    const double g = qq * (d.two_alpha_pi * exp(-d.alpha2 * r2) - e1) * rr * rr;
+    if (role == 0) {
+      gx -= rx * g;
+      gy -= ry * g;
+      gz -= rz * g;
+    } else {
+      gx += rx * g;
+      gy += ry * g;
+      gz += rz * g;
+    }
+  }
+}
+
 __global__ __launch_bounds__(kEwBlock) void k_ew_tables(EwDev d,
                                                         const double *x) {
   const int i = blockIdx.x * kEwBlock + threadIdx.x;
@@ -271,28 +302,7 @@ __global__ __launch_bounds__(64 * kEwLanes) void k_ew_force(EwDev d,
     gy += part_f[w][part][1][lane];
     gz += part_f[w][part][2][lane];
   }
-  for (int e = d.xptr[i]; e < d.xptr[i + 1]; e++) {
-    const int row = d.xent[e] >> 1, role = d.xent[e] & 1;
-    const int a1 = d.xrows[3 * row], a2 = d.xrows[3 * row + 1],
-              cls = d.xrows[3 * row + 2];
-    double rx, ry, rz, r2;
-    min_diff_exact(d.box, x[3 * a1], x[3 * a1 + 1], x[3 * a1 + 2], x[3 * a2],
-                   x[3 * a2 + 1], x[3 * a2 + 2], rx, ry, rz, r2);
-    double qq = d.q[a1] * d.q[a2];
-    if (cls == 1) qq *= 1 - d.csf;  // EXCLUSION_MODIFIED
-    const double r = sqrt(r2), rr = 1 / r;
-    const double e1 = erf(d.alpha * r) * rr;
-    const double g = qq * (d.two_alpha_pi * exp(-d.alpha2 * r2) - e1) * rr * rr;
-    if (role == 0) {
-      gx -= rx * g;
-      gy -= ry * g;
-      gz -= rz * g;
-    } else {
-      gx += rx * g;
-      gy += ry * g;
-      gz += rz * g;
-    }
-  }
+  ew_correction_force(d, x, i, gx, gy, gz);
   f[3 * i] += gx;
   f[3 * i + 1] += gy;
   f[3 * i + 2] += gz;

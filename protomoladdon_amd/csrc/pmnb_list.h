@@ -33,7 +33,7 @@ template <bool WRAP>
 __global__ void __launch_bounds__(kBlock)
 k_build_list(ForceParams P, Sorted S, unsigned *__restrict__ list,
              int *__restrict__ count, int cap,
-             unsigned long long *__restrict__ npairs) {
+             unsigned long long *__restrict__ npairs, PairLog plog) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = P.n_atoms;
   unsigned cnt = 0;
@@ -88,12 +88,17 @@ k_build_list(ForceParams P, Sorted S, unsigned *__restrict__ list,
                          S.z[t], ex, ey, ez, e2d);
           if (e2d > P.cut2) continue;
         }
-        if (S.mol[t] == mi &&
-            excl_class(S.xfull, S.xmod, P.max_excl_delta, a, S.atom[t]) == 2)
-          continue;
+        const int cls = S.mol[t] == mi ? excl_class(S.xfull, S.xmod,
+                                                    P.max_excl_delta, a,
+                                                    S.atom[t])
+                                       : 0;
+        if (cls == 2) continue;
         if (m < cap) list[(size_t)m * n + s] = (unsigned)t | ((unsigned)d << kEntryAtomBits);
         m++;
-        if (s < t) cnt++;
+        if (s < t) {
+          cnt++;
+          if (plog.out) log_pair(plog, a, S.atom[t], cls);
+        }
       }
     }
     count[s] = m;

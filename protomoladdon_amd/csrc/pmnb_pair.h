@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(kBlock)
 k_pair_fp64(ForceParams P, Sorted S, double *__restrict__ forces,
             double *__restrict__ partial,
             unsigned long long *__restrict__ npairs,
-            const int *__restrict__ only, int cap) {
+            const int *__restrict__ only, int cap, PairLog plog) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[ACC_N];
 #pragma unroll
@@ -189,7 +189,10 @@ k_pair_fp64(ForceParams P, Sorted S, double *__restrict__ forces,
         } else if (P.flags & 1u) {
           add_virial(acc, ACC_VXX, gx, gy, gz, dx, dy, dz, 0.5);
         }
-        if (a < b) cnt++;
+        if (a < b) {
+          cnt++;
+          if (plog.out && !only) log_pair(plog, a, b, excl);
+        }
       }
     }
     forces[3 * a] += fx;
@@ -282,7 +285,7 @@ __global__ void __launch_bounds__(kBlock)
 k_pair_mixed(ForceParams P, Sorted S, double *__restrict__ forces,
              double *__restrict__ partial,
              unsigned long long *__restrict__ npairs,
-             const int *__restrict__ only, int cap) {
+             const int *__restrict__ only, int cap, PairLog plog) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[ACC_N];
 #pragma unroll
@@ -360,7 +363,10 @@ k_pair_mixed(ForceParams P, Sorted S, double *__restrict__ forces,
             add_virial(acc, ACC_MXX, gx, gy, gz, cdx, cdy, cdz, 0.5);
           }
         }
-        if (a < b) cnt++;
+        if (a < b) {
+          cnt++;
+          if (plog.out && !only) log_pair(plog, a, b, excl);
+        }
       }
       // Flush this cell's FP32 partials: forces to 64-bit fixed point,
       // energies and virial to FP64.

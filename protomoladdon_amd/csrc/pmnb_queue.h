@@ -155,15 +155,15 @@ k_scan_items(const int *__restrict__ count, int *__restrict__ start,
               rep_first);
 }
 
-// Per-replica sums of the item partials (accumulator k of item b at
-// partial[k * pstride + b]), one block per (replica, accumulator) with a
-// fixed reduction tree; out[20 r + ..]:
+// Per-replica sum of one accumulator of the item partials (accumulator k
+// of item b at partial[k * pstride + b]) by one block of 1024 threads with
+// a fixed reduction tree, added into out[20 r + ..]:
 // [0] LJ, [1] electrostatic, [2..10] virial 3x3, [11..19] molecular virial.
-__global__ void __launch_bounds__(1024)
-k_finish_rep(const double *__restrict__ partial, int pstride,
-             const int *__restrict__ rep_first, double *__restrict__ out) {
+__device__ __forceinline__ void finish_rep_block(
+    const double *__restrict__ partial, int pstride,
+    const int *__restrict__ rep_first, double *__restrict__ out, int r,
+    int k) {
   __shared__ double red[16];
-  const int r = blockIdx.x, k = blockIdx.y;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i0 = rep_first[r], i1 = rep_first[r + 1];
   const double *pk = partial + (size_t)k * pstride;
@@ -200,6 +200,13 @@ k_finish_rep(const double *__restrict__ partial, int pstride,
       default: o[base + 8] += v; break;
     }
   }
+}
+
+// One block per (replica, accumulator).
+__global__ void __launch_bounds__(1024)
+k_finish_rep(const double *__restrict__ partial, int pstride,
+             const int *__restrict__ rep_first, double *__restrict__ out) {
+  finish_rep_block(partial, pstride, rep_first, out, blockIdx.x, blockIdx.y);
 }
 
 // Tile-path kernel shape: waves per block, buffered records per wave, force

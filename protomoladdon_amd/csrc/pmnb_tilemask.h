@@ -1,25 +1,38 @@
-// Tile kernel with per-lane pair masks (k_tm), for one system or a batch of
-// replicas (TileRep, pmnb_queue.h).
+// Half-shell tile kernel with per-lane pair masks (k_half), for one system
+// or a batch of replicas (TileRep, pmnb_queue.h).
 //
 // One wave handles 16 consecutive sorted atoms of one column, 4 lanes per
-// atom (lane = atom slot + 16 * split).  Candidates of the 3 x 3 neighbour
-// columns stream into an LDS buffer one z level at a time (stream_levels),
-// after a test of their distance to the wave's bounding box.  When the
-// buffer is nearly full each lane tests every 4th buffered candidate against
-// its own atom into a pass mask, then evaluates the set bits one by one,
-// reading each candidate from LDS one pair ahead of its arithmetic.  The LJ
-// table and the exclusion masks of the wave's own atoms are held in LDS.
-// Forces stay in registers: FP32 partials per buffer, added into 64-bit
-// fixed point (mixed) or FP64 sums, and the 4 lanes of an atom combine by
-// shuffles at the end.  Pairs within `band` of the cutoff are re-decided
-// with the reference's FP64 minimum image.  Every pair is visited from both
-// atoms; energies and virials take half.  Each item writes one partial row
-// of energies and virials.
+// atom (lane = atom slot + 16 * split).  Every unordered pair is evaluated
+// once (Newton's third law, as NonbondedCutoffForce::doEvaluate and
+// OneAtomPairTwo::doOneAtomPair do: force/nonbonded/NonbondedCutoffForce.h
+// :48-52, force/OneAtomPairTwo.h:106-111): a wave takes its own column's
+// atoms of higher sorted index and four of the eight neighbour columns,
+// the image offsets (0, +1), (+1, -1), (+1, 0), (+1, +1).  Of a pair's two
+// views, i -> j at image offset d and j -> i at -d, exactly one is taken:
+// for d != 0 one of d, -d is in that half, for d = 0 the lower sorted index
+// takes it.  With two columns on an axis the offsets -1 and +1 reach the
+// same column at images one box (>= 2 cutoffs) apart, so at most one image
+// of a pair is within the cutoff and it is taken once.
 //
-// k_tm walks the work items of any number of replicas with a grid stride;
-// k_tile is its single-system form.  Against k_tq this trades lane idling (a
-// lane with few pairs waits for the lane of its wave with most) for no LDS
-// traffic per pair and higher occupancy; the autotuner picks between them.
+// Candidates of those columns stream into an LDS buffer one z level at a
+// time (stream_levels) after a test of their distance to the wave's
+// bounding box.  When the buffer is nearly full each lane tests every 4th
+// buffered candidate against its own atom into a pass mask, then evaluates
+// the set bits one by one, reading each candidate from LDS one pair ahead
+// of its arithmetic.  The LJ table and the exclusion masks of the wave's own
+// atoms are held in LDS.
+//
+// Forces: the atom's own force is summed in registers (FP32 partials per
+// buffer added in 64-bit fixed point, or FP64 sums); the partner's force is
+// added into the candidate's LDS slot as 64-bit fixed-point integers, and
+// after each buffer the slots go to the per-atom fixed-point array in HBM
+// with integer atomics.  Integer sums do not depend on the order of the
+// adds, so the result is bitwise reproducible.  Mixed: x, y, z at 2^-32.
+// FP64: each component as (hi, lo) with value hi 2^-24 + lo 2^-56, exact
+// to 2^-57 per term.  k_half_finish converts the array into the caller's
+// forces (+=) and clears it for the next evaluation.  Pairs within `band`
+// of the cutoff are re-decided with the reference's FP64 minimum image.
+// Each item writes one partial row of energies and virials.
 #pragma once
 
 #include <type_traits>
@@ -30,29 +43,43 @@ namespace pmnb {
 
 constexpr int kTmSplit = 4;                 // lanes per atom
 constexpr int kTmWaves = kBlock / 64;       // waves per block
-// Waves per SIMD the register allocation must allow (FP32 / FP64 kernels).
-#ifndef PMNB_TM_OCC32
-#define PMNB_TM_OCC32 3
-#endif
-#ifndef PMNB_TM_OCC64
-#define PMNB_TM_OCC64 3
-#endif
-// Candidate records buffered per wave (FP32 / FP64 kernels): a larger
-// buffer evens out the lanes' pair counts per buffer; 3 blocks per CU must
-// still fit in LDS (32 / 64 bytes per record, 4 waves per block).
-#ifndef PMNB_TM_BUF32
-#define PMNB_TM_BUF32 256
-#endif
-#ifndef PMNB_TM_BUF64
-#define PMNB_TM_BUF64 192
-#endif
-constexpr int kTmBuf32 = PMNB_TM_BUF32, kTmBuf64 = PMNB_TM_BUF64;
+
+// Fixed-point words per atom of the force array: mixed x, y, z; FP64
+// (hi, lo) of x, y, z.
+template <bool F64>
+struct FixWords {
+  static constexpr int W = F64 ? 6 : 3;
+};
+
+// FP64 value as hi 2^-24 + lo 2^-56 with lo in [0, 2^32]: hi and frac are
+// exact, lo rounds once (|error| <= 2^-57).  Values are clamped to
+// +-2^37 so the integers cannot overflow.
+__device__ __forceinline__ void fix64_split(double v, long long &hi,
+                                            long long &lo) {
+  v = fmin(fmax(v, -137438953472.0), 137438953472.0);
+  const double s = v * 16777216.0;
+  const double h = floor(s);
+  hi = (long long)h;
+  lo = (long long)__builtin_rint((s - h) * 4294967296.0);
+}
+
+__device__ __forceinline__ double fix64_value(long long hi, long long lo) {
+  return (double)hi * (1.0 / 16777216.0) +
+         (double)lo * (1.0 / 72057594037927936.0);
+}
+
+__device__ __forceinline__ void lds_add_i64(long long *p, long long v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void global_add_i64(long long *p, long long v) {
+  atomicAdd(reinterpret_cast<unsigned long long *>(p), (unsigned long long)v);
+}
 
 // Per-lane pass masks over the buffered records split + 4 j, j < jmax, in
 // NW 32-bit words: word w holds j = 32 w .. 32 w + cw - 1 (cw = min(32,
 // jmax - 32 w)) with record j at bit cw - 1 - (j - 32 w), so the test
-// shifts its bits in.  Records n .. 4 jmax - 1 hold a far sentinel; the
-// atom itself passes and is skipped by the pair code.
+// shifts its bits in.  Records n .. 4 jmax - 1 hold a far sentinel.
 // (Both walk the words by compile-time recursion, so the masks stay in
 // registers.)
 template <int W, int NW, class Test>
@@ -136,14 +163,15 @@ __device__ __forceinline__ void tm_test(const float4 *__restrict__ rec, int n,
   mask_test<0, NW>((n + kTmSplit - 1) / kTmSplit, m, test);
 }
 
-// Streams the candidate records of a wave's neighbour columns one z level at
-// a time, the wave's own levels first and then alternately one below and one
+// Streams the candidate records of a wave's half shell one z level at a
+// time, the wave's own levels first and then alternately one below and one
 // above.  A buffer then holds candidates from every direction around the
-// wave's atoms, so the atoms' pair counts per buffer are alike and the lanes
-// of the mask loop wait less for one another (a buffer filled column by
-// column holds the candidates of one side).  Lane l < 9 keeps neighbour
-// column (l / 3 - 1, l % 3 - 1); a level's cells of all passing columns are
-// one virtual run, walked 64 records per call of
+// wave's atoms, so the atoms' pair counts per buffer are alike and the
+// lanes of the mask loop wait less for one another.  Lane l in 4 .. 8 keeps
+// neighbour column (l / 3 - 1, l % 3 - 1): (0, 0), (0, +1), (+1, -1),
+// (+1, 0), (+1, +1).  In its own column (lane 4) only records of sorted
+// index above a0 (the wave's first atom) are taken.  A level's cells of all
+// passing columns are one virtual run, walked 64 records per call of
 //   emit(ok, t, ox, oy, z0): sorted record t (valid if ok) with column image
 //   offsets ox, oy and level base z0, all relative to the wave's home cell.
 // wx / wy: single column on that axis (taken once, per-pair image); wz: the
@@ -151,10 +179,11 @@ __device__ __forceinline__ void tm_test(const float4 *__restrict__ rec, int n,
 template <class Emit>
 __device__ __forceinline__ void stream_levels(
     const Sorted &S, const TileGrid &G, int coff, int cx, int cy, bool wx,
-    bool wy, bool wz, int zlo, int zhi, int zA, int zB, float bx0, float bx1,
-    float by0, float by1, float bz0, float bz1, float hi, int lane,
+    bool wy, bool wz, int zlo, int zhi, int zA, int zB, int a0, float bx0,
+    float bx1, float by0, float by1, float bz0, float bz1, float hi, int lane,
     Emit &&emit) {
   const int ox = lane / 3 - 1, oy = lane % 3 - 1;
+  const bool own = lane == 4;
   const float gx = wx ? 0.f
                       : fmaxf(0.f, fmaxf((float)ox * G.csx - bx1,
                                          bx0 - (float)(ox + 1) * G.csx));
@@ -162,8 +191,8 @@ __device__ __forceinline__ void stream_levels(
                       : fmaxf(0.f, fmaxf((float)oy * G.csy - by1,
                                          by0 - (float)(oy + 1) * G.csy));
   const float gxy2 = fmaf(gx, gx, gy * gy);
-  const bool cok = lane < 9 && !(wx && ox != 0) && !(wy && oy != 0) &&
-                   gxy2 <= hi;
+  const bool cok = lane >= 4 && lane < 9 && !(wx && ox != 0) &&
+                   !(wy && oy != 0) && gxy2 <= hi;
   int tx = cx + ox, ty = cy + oy;
   tx = tx < 0 ? tx + G.nx : (tx >= G.nx ? tx - G.nx : tx);
   ty = ty < 0 ? ty + G.ny : (ty >= G.ny ? ty - G.ny : ty);
@@ -186,6 +215,11 @@ __device__ __forceinline__ void stream_levels(
     const bool lok = cok && fmaf(gz, gz, gxy2) <= hi;
     cnt = lok ? S.ccount[tbase + w] : 0;
     start = lok ? S.cstart[tbase + w] : 0;
+    if (own) {  // records above a0 only
+      const int lo = max(start, a0 + 1);
+      cnt = max(0, start + cnt - lo);
+      start = lo;
+    }
   };
   // level q = 0 (zlo) always lies in zA .. zB; the next level's cells are
   // read before the current level is walked
@@ -200,7 +234,7 @@ __device__ __forceinline__ void stream_levels(
     int cnt_n = 0, start_n = 0;
     if (qn < nlev) level_cell(zn, cnt_n, start_n);
     // inclusive prefix of the counts over lanes 0 .. 8 (row shifts within
-    // the first 16 lanes)
+    // the first 16 lanes; lanes 0 .. 3 count nothing)
     int inc = cnt;
     inc += __builtin_amdgcn_update_dpp(0, inc, 0x111, 0xf, 0xf, false);
     inc += __builtin_amdgcn_update_dpp(0, inc, 0x112, 0xf, 0xf, false);
@@ -210,14 +244,14 @@ __device__ __forceinline__ void stream_levels(
     if (total > 0) {
       const float z0 = (float)(zc - zlo) * G.csz;
       const int rel = start - (inc - cnt);  // record = run index + rel
-      int ends[8];
+      int ends[4];
 #pragma unroll
-      for (int j = 0; j < 8; j++) ends[j] = __builtin_amdgcn_readlane(inc, j);
+      for (int j = 0; j < 4; j++) ends[j] = __builtin_amdgcn_readlane(inc, 4 + j);
       for (int base = 0; base < total; base += 64) {
         const int idx = base + lane;
-        int k = 0;
+        int k = 4;
 #pragma unroll
-        for (int j = 0; j < 8; j++) k += idx >= ends[j] ? 1 : 0;
+        for (int j = 0; j < 4; j++) k += idx >= ends[j] ? 1 : 0;
         const int t = idx + __shfl(rel, k, 64);
         emit(idx < total, t, (float)(k / 3 - 1) * G.csx,
              (float)(k % 3 - 1) * G.csy, z0);
@@ -231,55 +265,85 @@ __device__ __forceinline__ void stream_levels(
   }
 }
 
-template <bool F64, int ELEC, int ELSW, int BUF>
+// The pair kernel.  REP = false: one system, one item per wave, the column
+// grid and box as arguments, x and y with at least two columns; REP = true:
+// a batch of replicas (per-item TileRep, per-pair minimum image on axes
+// with a single column), a persistent item loop with a grid stride.  OCC:
+// waves per SIMD the register allocation must allow.
+template <bool F64, int ELEC, int ELSW, int BUF, int OCC, bool REP>
 __global__ void __launch_bounds__(kBlock)
-__attribute__((amdgpu_waves_per_eu(F64 ? PMNB_TM_OCC64 : PMNB_TM_OCC32)))
-k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
-     const int4 *__restrict__ items, const int *__restrict__ nitems,
-     double *__restrict__ forces, double *__restrict__ partial, int pstride,
-     unsigned long long *__restrict__ npairs) {
+__attribute__((amdgpu_waves_per_eu(OCC)))
+k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
+       TileGrid G1, const int4 *__restrict__ items,
+       const int *__restrict__ nitems, long long *__restrict__ ffix,
+       double *__restrict__ partial, int pstride,
+       unsigned long long *__restrict__ npairs, PairLog plog) {
   constexpr int kNW = (BUF / kTmSplit + 31) / 32;  // mask words
+  constexpr int W = FixWords<F64>::W;
   __shared__ float4 recf[kTmWaves][BUF];
   __shared__ int4 reci[kTmWaves][BUF];   // sorted index, type, mol, atom
   __shared__ double4 recd[F64 ? kTmWaves : 1][F64 ? BUF : 1];
+  // fixed-point force on each buffered record from this wave's atoms
+  __shared__ long long pfix[kTmWaves][W][BUF];
   // exclusion masks of the wave's own atoms (excl_own layout)
   __shared__ unsigned long long xm[kTmWaves][kTileAtoms][4];
   __shared__ float4 ljs[F64 ? 1 : kQLj];  // FP32 LJ table (T * T <= kQLj)
+  // values equal on all lanes of the wave are marked uniform (scalar
+  // registers) where that measured faster
+  auto U = [](auto v) { return (F64 || REP) ? uni(v) : v; };
+  const int wave = uni((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+  for (int k = lane; k < W * BUF; k += 64) (&pfix[wave][0][0])[k] = 0;
   if (!F64) {
     for (int k = threadIdx.x; k < P.n_types * P.n_types; k += kBlock)
       ljs[k] = S.ljf[k];
-    __syncthreads();
   }
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
+  __syncthreads();
   const F32Params &F = P.f;
   const float hi = F.cut2 + F.band, lo = F.cut2 - F.band;
   const bool vir = (P.flags & 3u) != 0;
   const int n_items = *nitems;
+  // FP64 single system: blocks go to the 8 XCDs in turn; block b takes the
+  // items of block position (b % 8) * per + b / 8 (the grid is a multiple
+  // of 8), so each XCD works through one contiguous range of items and the
+  // candidates neighbouring items share stay in that XCD's L2.
+  int item_idx;
+  if (REP || !F64) {
+    item_idx = blockIdx.x * kTmWaves + wave;
+  } else {
+    const int per = gridDim.x / 8;
+    item_idx = ((int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8)) *
+                   kTmWaves + wave;
+  }
   unsigned cnt = 0;
-  for (int item_idx = blockIdx.x * kTmWaves + wave; item_idx < n_items;
-       item_idx += gridDim.x * kTmWaves) {
+  for (; item_idx < n_items; item_idx += gridDim.x * kTmWaves) {
     double acc[ACC_N];
 #pragma unroll
     for (int k = 0; k < ACC_N; k++) acc[k] = 0.0;
     const int4 item = items[item_idx];
-    const int a0 = uni(item.x), a1 = uni(item.y), col = uni(item.z);
-    const TileRep &R = reps[uni(item.w)];
-    const TileGrid G = R.tg;
-    const int coff = R.cell_off;
-    const bool wx = R.wrap[0] != 0, wy = R.wrap[1] != 0;
+    const int a0 = U(item.x), a1 = U(item.y), col = U(item.z);
+    const TileRep *Rp = REP ? &reps[uni(item.w)] : nullptr;
+    const TileGrid G = REP ? Rp->tg : G1;
+    const int coff = REP ? Rp->cell_off : 0;
+    const bool wx = REP ? Rp->wrap[0] != 0 : false;
+    const bool wy = REP ? Rp->wrap[1] != 0 : false;
+    const Box *bptr = REP ? &Rp->box : S.box;
     float L[3], iL[3];
-    for (int k = 0; k < 3; k++) {
-      L[k] = R.L[k];
-      iL[k] = R.iL[k];
+    if (REP) {
+      for (int k = 0; k < 3; k++) {
+        L[k] = Rp->L[k];
+        iL[k] = Rp->iL[k];
+      }
+    } else {
+      L[0] = F.Lx; L[1] = F.Ly; L[2] = F.Lz;
+      iL[0] = F.iLx; iL[1] = F.iLy; iL[2] = F.iLz;
     }
     const int cx = col / G.ny, cy = col % G.ny;
     const int slot = lane & (kTileAtoms - 1), split = lane >> 4;
     const int i = a0 + slot;
     const bool active = i < a1;
     const int ii = active ? i : a0;
-    const int zlo = uni((S.cell[a0] - coff) % G.nz);
-    const int zhi = uni((S.cell[a1 - 1] - coff) % G.nz);
+    const int zlo = U((S.cell[a0] - coff) % G.nz);
+    const int zhi = U((S.cell[a1 - 1] - coff) % G.nz);
     const float4 li = S.loc[ii];
     const float xi = li.x, yi = li.y;
     const float zi = (float)((S.cell[ii] - coff) % G.nz - zlo) * G.csz + li.z;
@@ -294,12 +358,12 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
     const int ljbase = ti * P.n_types;  // row of this atom's type in ljs
     const double *ljrow64 = S.lj + 4 * (ti * P.n_types);
     // bounding box of the wave's atoms in home-relative coordinates
-    const float bx0 = (wave_min(active ? xi : 1e30f)),
-                bx1 = (wave_max(active ? xi : -1e30f));
-    const float by0 = (wave_min(active ? yi : 1e30f)),
-                by1 = (wave_max(active ? yi : -1e30f));
-    const float bz0 = (wave_min(active ? zi : 1e30f)),
-                bz1 = (wave_max(active ? zi : -1e30f));
+    const float bx0 = U(wave_min(active ? xi : 1e30f)),
+                bx1 = U(wave_max(active ? xi : -1e30f));
+    const float by0 = U(wave_min(active ? yi : 1e30f)),
+                by1 = U(wave_max(active ? yi : -1e30f));
+    const float bz0 = U(wave_min(active ? zi : 1e30f)),
+                bz1 = U(wave_max(active ? zi : -1e30f));
     int zA = zlo - G.rz, zB = zhi + G.rz;
     const bool wz = zB - zA + 1 > G.nz;  // all z cells: per-pair image
     if (wz) zB = zA + G.nz - 1;
@@ -318,7 +382,9 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
                     auto WM) __attribute__((always_inline)) {
       constexpr int M = decltype(WM)::value;
       const int t = q4.x;
-      if (t == i) return;
+      // the atom itself and the wave's own atoms below it (records of the
+      // own column start above a0; other columns hold no atom of this wave)
+      if ((unsigned)(t - a0) <= (unsigned)slot) return;
       float dx = r.x - xi, dy = r.y - yi, dz = r.z - zi;
       if (M == 2 || (M == 3 && wx)) dx = fmaf(-L[0], rintf(dx * iL[0]), dx);
       if (M == 2 || (M == 3 && wy)) dy = fmaf(-L[1], rintf(dy * iL[1]), dy);
@@ -328,19 +394,19 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
       double ex = 0.0, ey = 0.0, ez = 0.0, e2d = 0.0;
       if (F64) {
         const double4 rd = recd[wave][k];
-        min_diff_exact(R.box, xd, yd, zd, rd.x, rd.y, rd.z, ex, ey, ez, e2d);
+        min_diff_exact(*bptr, xd, yd, zd, rd.x, rd.y, rd.z, ex, ey, ez, e2d);
         if (e2d > P.cut2) return;
       } else if (r2 >= lo) {
         // near the cutoff: decide with the reference's FP64 minimum image
-        min_diff_exact(rare_box(&R.box), xd, yd, zd, S.x[t], S.y[t], S.z[t],
+        min_diff_exact(rare_box(bptr), xd, yd, zd, S.x[t], S.y[t], S.z[t],
                        ex, ey, ez, e2d);
         if (e2d > P.cut2) return;
       }
       const bool same = q4.z == mi;
-      const int excl =
-          same ? excl_own(xm[wave][slot], q4.w - ai) : 0;
+      const int excl = same ? excl_own(xm[wave][slot], q4.w - ai) : 0;
       if (excl == 2) return;
-      if (i < t) cnt++;
+      cnt++;
+      if (plog.out) log_pair(plog, ai, q4.w, excl);
       const bool molv = (P.flags & 2u) && !same;
       if (F64) {
         const double *lp = ljrow64 + 4 * q4.y;
@@ -348,19 +414,32 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
         pair_f64<ELEC, ELSW>(P, e2d, excl != 1 ? lp[0] : lp[2],
                              excl != 1 ? lp[1] : lp[3], qi * recd[wave][k].w,
                              excl, ff, e1, e2);
-        acc[ACC_ELJ] += 0.5 * e1;
-        acc[ACC_EEL] += 0.5 * e2;
+        acc[ACC_ELJ] += e1;
+        acc[ACC_EEL] += e2;
         const double gx = ex * ff, gy = ey * ff, gz = ez * ff;
         fx64 -= gx;
         fy64 -= gy;
         fz64 -= gz;
+        long long h, l;
+        fix64_split(gx, h, l);
+        lds_add_i64(&pfix[wave][0][k], h);
+        lds_add_i64(&pfix[wave][1][k], l);
+        fix64_split(gy, h, l);
+        lds_add_i64(&pfix[wave][2][k], h);
+        lds_add_i64(&pfix[wave][3][k], l);
+        fix64_split(gz, h, l);
+        lds_add_i64(&pfix[wave][4][k], h);
+        lds_add_i64(&pfix[wave][5][k], l);
         if (molv || (P.flags & 1u))
-          add_virial(acc, ACC_VXX, gx, gy, gz, ex, ey, ez, 0.5);
+          add_virial(acc, ACC_VXX, gx, gy, gz, ex, ey, ez, 1.0);
         if (molv) {
+          const Box &bm = rare_box(bptr);
           double cdx, cdy, cdz, cd2;
-          min_diff_exact(rare_box(&R.box), S.com[3 * mi], S.com[3 * mi + 1], S.com[3 * mi + 2], S.com[3 * q4.z], S.com[3 * q4.z + 1], S.com[3 * q4.z + 2],
-                         cdx, cdy, cdz, cd2);
-          add_virial(acc, ACC_MXX, gx, gy, gz, cdx, cdy, cdz, 0.5);
+          min_diff_exact(bm, S.com[3 * mi], S.com[3 * mi + 1],
+                         S.com[3 * mi + 2], S.com[3 * q4.z],
+                         S.com[3 * q4.z + 1], S.com[3 * q4.z + 2], cdx, cdy,
+                         cdz, cd2);
+          add_virial(acc, ACC_MXX, gx, gy, gz, cdx, cdy, cdz, 1.0);
         }
       } else {
         const float4 lp = ljs[ljbase + q4.y];
@@ -373,6 +452,9 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
         px -= gx;
         py -= gy;
         pz -= gz;
+        lds_add_i64(&pfix[wave][0][k], to_fixed_f32(gx));
+        lds_add_i64(&pfix[wave][1][k], to_fixed_f32(gy));
+        lds_add_i64(&pfix[wave][2][k], to_fixed_f32(gz));
         if (vir) {
           if (molv || (P.flags & 1u)) {
             v0 = fmaf(gx, dx, v0);
@@ -383,20 +465,23 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
             v5 = fmaf(gz, dz, v5);
           }
           if (molv) {
+            const Box &bm = rare_box(bptr);
             double cdx, cdy, cdz, cd2;
-            min_diff_exact(rare_box(&R.box), S.com[3 * mi], S.com[3 * mi + 1], S.com[3 * mi + 2], S.com[3 * q4.z], S.com[3 * q4.z + 1], S.com[3 * q4.z + 2],
-                           cdx, cdy, cdz, cd2);
-            add_virial(acc, ACC_MXX, gx, gy, gz, cdx, cdy, cdz, 0.5);
+            min_diff_exact(bm, S.com[3 * mi], S.com[3 * mi + 1],
+                           S.com[3 * mi + 2], S.com[3 * q4.z],
+                           S.com[3 * q4.z + 1], S.com[3 * q4.z + 2], cdx, cdy,
+                           cdz, cd2);
+            add_virial(acc, ACC_MXX, gx, gy, gz, cdx, cdy, cdz, 1.0);
           }
         }
       }
     };
 
     const int wmode = (wx ? 1 : 0) | (wy ? 2 : 0) | (wz ? 4 : 0);
-    // Tests the buffered candidates against this lane's atom (the atom
-    // itself has r2 == 0 and is left out), then evaluates the pairs that
-    // pass; the next pair's records are read from LDS before the current
-    // pair's arithmetic.
+    // Tests the buffered candidates against this lane's atom, evaluates the
+    // pairs that pass (the next pair's records are read from LDS before the
+    // current pair's arithmetic), then moves the candidates' fixed-point
+    // forces to the force array and clears their slots.
     auto process = [&](int n) __attribute__((always_inline)) {
       // records n .. n + 3 (the last group of 4) are made far away
       if (lane < kTmSplit && n + lane < BUF)
@@ -472,21 +557,37 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
       else
         run(std::integral_constant<int, 2>{});
       if (!F64) {
-        Fx += to_fixed((double)px);
-        Fy += to_fixed((double)py);
-        Fz += to_fixed((double)pz);
-        acc[ACC_ELJ] += 0.5 * (double)pelj;
-        acc[ACC_EEL] += 0.5 * (double)peel;
+        Fx += to_fixed_f32(px);
+        Fy += to_fixed_f32(py);
+        Fz += to_fixed_f32(pz);
+        acc[ACC_ELJ] += (double)pelj;
+        acc[ACC_EEL] += (double)peel;
         if (vir) {
-          acc[ACC_VXX] += 0.5 * (double)v0;
-          acc[ACC_VXY] += 0.5 * (double)v1;
-          acc[ACC_VXZ] += 0.5 * (double)v2;
-          acc[ACC_VYY] += 0.5 * (double)v3;
-          acc[ACC_VYZ] += 0.5 * (double)v4;
-          acc[ACC_VZZ] += 0.5 * (double)v5;
+          acc[ACC_VXX] += (double)v0;
+          acc[ACC_VXY] += (double)v1;
+          acc[ACC_VXZ] += (double)v2;
+          acc[ACC_VYY] += (double)v3;
+          acc[ACC_VYZ] += (double)v4;
+          acc[ACC_VZZ] += (double)v5;
         }
       }
-      wave_sync_lds();  // buffer is refilled next
+      wave_sync_lds();  // every lane's slot adds are in
+      for (int k = lane; k < n; k += 64) {
+        long long v[W];
+        long long any = 0;
+#pragma unroll
+        for (int c = 0; c < W; c++) {
+          v[c] = pfix[wave][c][k];
+          any |= v[c];
+          pfix[wave][c][k] = 0;
+        }
+        if (any) {
+          long long *dst = ffix + (size_t)W * reci[wave][k].x;
+#pragma unroll
+          for (int c = 0; c < W; c++) global_add_i64(dst + c, v[c]);
+        }
+      }
+      wave_sync_lds();  // buffer and slots are refilled next
     };
 
     auto emit = [&](bool ok, int t, float oxc, float oyc,
@@ -519,10 +620,11 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
         nbuf = 0;
       }
     };
-    stream_levels(S, G, coff, cx, cy, wx, wy, wz, zlo, zhi, zA, zB, bx0, bx1,
-                  by0, by1, bz0, bz1, hi, lane, emit);
+    stream_levels(S, G, coff, cx, cy, wx, wy, wz, zlo, zhi, zA, zB, a0, bx0,
+                  bx1, by0, by1, bz0, bz1, hi, lane, emit);
     if (nbuf > 0) process(nbuf);
-    // the 4 lanes of an atom add their partial forces
+    // the 4 lanes of an atom add their partial forces, then the atom's
+    // force joins the partner forces other waves add to it
     if (F64) {
       fx64 += __shfl_down(fx64, 32, 64);
       fy64 += __shfl_down(fy64, 32, 64);
@@ -531,9 +633,17 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
       fy64 += __shfl_down(fy64, 16, 64);
       fz64 += __shfl_down(fz64, 16, 64);
       if (active && split == 0) {
-        forces[3 * ai] += fx64;
-        forces[3 * ai + 1] += fy64;
-        forces[3 * ai + 2] += fz64;
+        long long *dst = ffix + (size_t)W * ii;
+        long long h, l;
+        fix64_split(fx64, h, l);
+        global_add_i64(dst, h);
+        global_add_i64(dst + 1, l);
+        fix64_split(fy64, h, l);
+        global_add_i64(dst + 2, h);
+        global_add_i64(dst + 3, l);
+        fix64_split(fz64, h, l);
+        global_add_i64(dst + 4, h);
+        global_add_i64(dst + 5, l);
       }
     } else {
       Fx += __shfl_down(Fx, 32, 64);
@@ -543,9 +653,10 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
       Fy += __shfl_down(Fy, 16, 64);
       Fz += __shfl_down(Fz, 16, 64);
       if (active && split == 0) {
-        forces[3 * ai] += (double)Fx * kFixedInv;
-        forces[3 * ai + 1] += (double)Fy * kFixedInv;
-        forces[3 * ai + 2] += (double)Fz * kFixedInv;
+        long long *dst = ffix + (size_t)W * ii;
+        global_add_i64(dst, Fx);
+        global_add_i64(dst + 1, Fy);
+        global_add_i64(dst + 2, Fz);
       }
     }
     // this item's energies and virials
@@ -555,362 +666,44 @@ k_tm(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
       for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
       if (lane == 0) partial[(size_t)k * pstride + item_idx] = v;
     }
+    if (!REP) break;  // one item per wave
   }
   unsigned long long tot = block_count<kTmWaves>(cnt);
   if (threadIdx.x == 0 && tot) atomicAdd(npairs, tot);
 }
 
-// Tests the buffered records against one atom (mask_test layout).  WRAPZ:
-// minimum image along z.
-template <bool WRAPZ, int NW>
-__device__ __forceinline__ void tile_test(const float4 *__restrict__ rec,
-                                          int n, int split, float xi,
-                                          float yi, float zi, float hi,
-                                          float Lz, float iLz,
-                                          unsigned (&m)[NW]) {
-  auto test = [&](int j) -> unsigned {
-    const float4 r = rec[split + kTmSplit * j];
-    const float dx = r.x - xi, dy = r.y - yi;
-    float dz = r.z - zi;
-    if (WRAPZ) dz = fmaf(-Lz, rintf(dz * iLz), dz);
-    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    return (unsigned)(r2 <= hi);
-  };
-  mask_test<0, NW>((n + kTmSplit - 1) / kTmSplit, m, test);
-}
-
-// Single-system form of k_tm: one wave per item, the column grid as a
-// kernel argument, minimum image along z only when the cutoff reaches every
-// z cell (x and y need at least two columns).  Same per-lane work order and
-// per-item partial rows as k_tm, so both give the same bits; without a
-// replica table it needs fewer scalar registers, which measures faster for
-// one large system.
-template <bool F64, int ELEC, int ELSW, int BUF>
-__global__ void __launch_bounds__(kBlock)
-__attribute__((amdgpu_waves_per_eu(F64 ? PMNB_TM_OCC64 : PMNB_TM_OCC32)))
-k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
-       const int *__restrict__ nitems, double *__restrict__ forces,
-       double *__restrict__ partial, int pstride,
-       unsigned long long *__restrict__ npairs) {
-  constexpr int kNW = (BUF / kTmSplit + 31) / 32;  // mask words
-  __shared__ float4 recf[kTmWaves][BUF];
-  __shared__ int4 reci[kTmWaves][BUF];
-  __shared__ double4 recd[F64 ? kTmWaves : 1][F64 ? BUF : 1];
-  // exclusion masks of the wave's own atoms (excl_own layout)
-  __shared__ unsigned long long xm[kTmWaves][kTileAtoms][4];
-  __shared__ float4 ljs[F64 ? 1 : kQLj];  // FP32 LJ table (T * T <= kQLj)
-  if (!F64) {
-    for (int k = threadIdx.x; k < P.n_types * P.n_types; k += kBlock)
-      ljs[k] = S.ljf[k];
-    __syncthreads();
-  }
-  // FP64: values equal on all lanes of the wave are marked uniform (uni), so
-  // the work item, its z range and box live in scalar registers (in the
-  // FP32 kernel the extra scalar registers spill and cost more than they
-  // save)
-  auto U = [](auto v) { return F64 ? uni(v) : v; };
-  const int wave = U((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
-  // FP64: blocks go to the 8 XCDs in turn; block b takes the items of block
-  // position (b % 8) * per + b / 8 (the grid is a multiple of 8), so each
-  // XCD works through one contiguous range of items and the candidates that
-  // neighbouring items share stay in that XCD's L2 (measured faster for the
-  // FP64 kernel, slightly slower for the FP32 one).
-  const int per = gridDim.x / 8;
-  const int blk =
-      F64 ? (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8) : blockIdx.x;
-  const int item_idx = blk * kTmWaves + wave;
-  double acc[ACC_N];
-#pragma unroll
-  for (int k = 0; k < ACC_N; k++) acc[k] = 0.0;
-  unsigned cnt = 0;
-  if (item_idx < *nitems) {
-    const int4 item = items[item_idx];
-    const int a0 = U(item.x), a1 = U(item.y), col = U(item.z);
-    const int cx = col / G.ny, cy = col % G.ny;
-    const int slot = lane & (kTileAtoms - 1), split = lane >> 4;
-    const int i = a0 + slot;
-    const bool active = i < a1;
-    const int ii = active ? i : a0;
-    const F32Params &F = P.f;
-    const float hi = F.cut2 + F.band, lo = F.cut2 - F.band;
-    const int zlo = U(S.cell[a0] % G.nz), zhi = U(S.cell[a1 - 1] % G.nz);
-    const float4 li = S.loc[ii];
-    const int zci = S.cell[ii] % G.nz;
-    const float xi = li.x, yi = li.y;
-    const float zi = (float)(zci - zlo) * G.csz + li.z;
-    const int ti = S.type[ii], mi = S.mol[ii], ai = S.atom[ii];
-    if (split == 0) {
-#pragma unroll
-      for (int k = 0; k < 4; k++) xm[wave][slot][k] = S.xown[4 * ai + k];
-    }
-    wave_sync_lds();
-    const double qi = S.q[ii];
-    const double xd = S.x[ii], yd = S.y[ii], zd = S.z[ii];
-    const int ljbase = ti * P.n_types;  // row of this atom's type in ljs
-    const double *ljrow64 = S.lj + 4 * (ti * P.n_types);
-    // bounding box of the wave's atoms in home-relative coordinates
-    const float bx0 = U(wave_min(active ? xi : 1e30f)),
-                bx1 = U(wave_max(active ? xi : -1e30f));
-    const float by0 = U(wave_min(active ? yi : 1e30f)),
-                by1 = U(wave_max(active ? yi : -1e30f));
-    const float bz0 = U(wave_min(active ? zi : 1e30f)),
-                bz1 = U(wave_max(active ? zi : -1e30f));
-    int zA = zlo - G.rz, zB = zhi + G.rz;
-    const bool clampz = zB - zA + 1 > G.nz;
-    if (clampz) zB = zA + G.nz - 1;
-    const float Lz = F.Lz, iLz = F.iLz;
-    long long Fx = 0, Fy = 0, Fz = 0;
-    double fx64 = 0.0, fy64 = 0.0, fz64 = 0.0;
-    const bool vir = (P.flags & 3u) != 0;
-    int nbuf = 0;
-
-    // FP32 partials of one buffer (flushed by process)
-    float px = 0.f, py = 0.f, pz = 0.f, pelj = 0.f, peel = 0.f;
-    float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
-
-    // One pair: candidate record k of this wave's buffer.
-    // WZ: minimum image along z, 0 no, 1 yes, 2 as the runtime flag (a
-    // compile-time tag, so the common case carries no select).
-    auto eval = [&](const float4 &r, const int4 &q4, int k,
-                    auto WZ) __attribute__((always_inline)) {
-      constexpr int M = decltype(WZ)::value;
-      const int t = q4.x;
-      if (t == i) return;
-      float dx = r.x - xi, dy = r.y - yi, dz = r.z - zi;
-      if (M == 1 || (M == 2 && clampz)) dz = fmaf(-Lz, rintf(dz * iLz), dz);
-      const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-      double ex = 0.0, ey = 0.0, ez = 0.0, e2d = 0.0;
-      if (F64) {
-        const double4 rd = recd[wave][k];
-        min_diff_exact(*S.box, xd, yd, zd, rd.x, rd.y, rd.z, ex, ey, ez, e2d);
-        if (e2d > P.cut2) return;
-      } else if (r2 >= lo) {
-        // near the cutoff: decide with the reference's FP64 minimum image
-        min_diff_exact(*S.box, xd, yd, zd, S.x[t], S.y[t], S.z[t], ex, ey, ez,
-                       e2d);
-        if (e2d > P.cut2) return;
-      }
-      const bool same = q4.z == mi;
-      const int excl =
-          same ? excl_own(xm[wave][slot], q4.w - ai) : 0;
-      if (excl == 2) return;
-      if (i < t) cnt++;
-      const bool molv = (P.flags & 2u) && !same;
-      if (F64) {
-        const double *lp = ljrow64 + 4 * q4.y;
-        double ff, e1, e2;
-        pair_f64<ELEC, ELSW>(P, e2d, excl != 1 ? lp[0] : lp[2],
-                             excl != 1 ? lp[1] : lp[3], qi * recd[wave][k].w,
-                             excl, ff, e1, e2);
-        acc[ACC_ELJ] += 0.5 * e1;
-        acc[ACC_EEL] += 0.5 * e2;
-        const double gx = ex * ff, gy = ey * ff, gz = ez * ff;
-        fx64 -= gx;
-        fy64 -= gy;
-        fz64 -= gz;
-        if (molv || (P.flags & 1u))
-          add_virial(acc, ACC_VXX, gx, gy, gz, ex, ey, ez, 0.5);
-        if (molv) {
-          double cdx, cdy, cdz, cd2;
-          min_diff_exact(*S.box, S.com[3 * mi], S.com[3 * mi + 1],
-                         S.com[3 * mi + 2], S.com[3 * q4.z],
-                         S.com[3 * q4.z + 1], S.com[3 * q4.z + 2], cdx, cdy,
-                         cdz, cd2);
-          add_virial(acc, ACC_MXX, gx, gy, gz, cdx, cdy, cdz, 0.5);
-        }
-      } else {
-        const float4 lp = ljs[ljbase + q4.y];
-        float ff, e1, e2;
-        pair_f32<ELEC>(F, r2, excl != 1 ? lp.x : lp.z, excl != 1 ? lp.y : lp.w,
-                       li.w * r.w, excl, ff, e1, e2);
-        pelj += e1;
-        peel += e2;
-        const float gx = dx * ff, gy = dy * ff, gz = dz * ff;
-        px -= gx;
-        py -= gy;
-        pz -= gz;
-        if (vir) {
-          if (molv || (P.flags & 1u)) {
-            v0 = fmaf(gx, dx, v0);
-            v1 = fmaf(gx, dy, v1);
-            v2 = fmaf(gx, dz, v2);
-            v3 = fmaf(gy, dy, v3);
-            v4 = fmaf(gy, dz, v4);
-            v5 = fmaf(gz, dz, v5);
-          }
-          if (molv) {
-            double cdx, cdy, cdz, cd2;
-            min_diff_exact(*S.box, S.com[3 * mi], S.com[3 * mi + 1],
-                           S.com[3 * mi + 2], S.com[3 * q4.z],
-                           S.com[3 * q4.z + 1], S.com[3 * q4.z + 2], cdx, cdy,
-                           cdz, cd2);
-            add_virial(acc, ACC_MXX, gx, gy, gz, cdx, cdy, cdz, 0.5);
-          }
-        }
-      }
-    };
-
-    // Tests the buffered candidates against this lane's atom (the atom
-    // itself has r2 == 0 and is left out), then evaluates the pairs that
-    // pass; the next pair's records are read from LDS before the current
-    // pair's arithmetic.
-    auto process = [&](int n) __attribute__((always_inline)) {
-      // records n .. n + 3 (the last group of 4) are made far away
-      if (lane < kTmSplit && n + lane < BUF)
-        recf[wave][n + lane] = make_float4(1e18f, 1e18f, 1e18f, 0.f);
-      wave_sync_lds();
-      unsigned m[kNW];
-      if (clampz)
-        tile_test<true>(recf[wave], n, split, xi, yi, zi, hi, Lz, iLz, m);
-      else
-        tile_test<false>(recf[wave], n, split, xi, yi, zi, hi, Lz, iLz, m);
-      if (!active) {
-#pragma unroll
-        for (int w = 0; w < kNW; w++) m[w] = 0u;
-      }
-      const int jmax = (n + kTmSplit - 1) / kTmSplit;
-      px = py = pz = pelj = peel = 0.f;
-      v0 = v1 = v2 = v3 = v4 = v5 = 0.f;
-      auto run = [&](auto WZ) __attribute__((always_inline)) {
-        if (!mask_any(m)) return;
-        if constexpr (F64) {
-          int k = mask_pop<0, kNW, false>(m, split, jmax);
-          float4 r = recf[wave][k];
-          int4 q4 = reci[wave][k];
-          while (true) {
-            const bool more = mask_any(m);
-            int kn = k;
-            if (more) kn = mask_pop<0, kNW, false>(m, split, jmax);
-            const float4 rn = recf[wave][kn];
-            const int4 q4n = reci[wave][kn];
-            eval(r, q4, k, WZ);
-            if (!more) break;
-            r = rn;
-            q4 = q4n;
-            k = kn;
-          }
-        } else {
-          // two record sets in turn (no register moves); the next pop is
-          // taken unconditionally and its index replaced by 0 when the
-          // mask was empty, so every LDS read stays in the buffer
-          int k0 = mask_pop<0, kNW, true>(m, split, jmax);
-          float4 r0 = recf[wave][k0];
-          int4 q0 = reci[wave][k0];
-          while (true) {
-            const bool more1 = mask_any(m);
-            int k1 = mask_pop<0, kNW, true>(m, split, jmax);
-            k1 = more1 ? k1 : 0;
-            const float4 r1 = recf[wave][k1];
-            const int4 q1 = reci[wave][k1];
-            eval(r0, q0, k0, WZ);
-            if (!more1) break;
-            const bool more0 = mask_any(m);
-            k0 = mask_pop<0, kNW, true>(m, split, jmax);
-            k0 = more0 ? k0 : 0;
-            r0 = recf[wave][k0];
-            q0 = reci[wave][k0];
-            eval(r1, q1, k1, WZ);
-            if (!more0) break;
-          }
-        }
-      };
-      // FP64 keeps one loop with a runtime flag (fewer spilled registers)
-      if (F64)
-        run(std::integral_constant<int, 2>{});
-      else if (clampz)
-        run(std::integral_constant<int, 1>{});
-      else
-        run(std::integral_constant<int, 0>{});
-      if (!F64) {
-        Fx += to_fixed((double)px);
-        Fy += to_fixed((double)py);
-        Fz += to_fixed((double)pz);
-        acc[ACC_ELJ] += 0.5 * (double)pelj;
-        acc[ACC_EEL] += 0.5 * (double)peel;
-        if (vir) {
-          acc[ACC_VXX] += 0.5 * (double)v0;
-          acc[ACC_VXY] += 0.5 * (double)v1;
-          acc[ACC_VXZ] += 0.5 * (double)v2;
-          acc[ACC_VYY] += 0.5 * (double)v3;
-          acc[ACC_VYZ] += 0.5 * (double)v4;
-          acc[ACC_VZZ] += 0.5 * (double)v5;
-        }
-      }
-      wave_sync_lds();  // buffer is refilled next
-    };
-
-    auto emit = [&](bool ok, int t, float oxc, float oyc,
-                    float z0) __attribute__((always_inline)) {
-      float4 rf = make_float4(0.f, 0.f, 0.f, 0.f);
-      int4 ri = make_int4(0, 0, 0, 0);
-      if (ok) {
-        const float4 lt = S.loc[t];
-        ri = S.rec[t];
-        rf.x = oxc + lt.x;
-        rf.y = oyc + lt.y;
-        rf.z = z0 + lt.z;
-        rf.w = lt.w;
-        float gzv = fmaxf(0.f, fmaxf(bz0 - rf.z, rf.z - bz1));
-        if (clampz) gzv = 0.f;
-        const float gxv = fmaxf(0.f, fmaxf(bx0 - rf.x, rf.x - bx1));
-        const float gyv = fmaxf(0.f, fmaxf(by0 - rf.y, rf.y - by1));
-        ok = fmaf(gxv, gxv, fmaf(gyv, gyv, gzv * gzv)) <= hi;
-      }
-      const unsigned long long ball = __ballot(ok);
-      if (ok) {
-        const int pos = nbuf + lane_prefix(ball);
-        recf[wave][pos] = rf;
-        reci[wave][pos] = ri;
-        if (F64) recd[wave][pos] = make_double4(S.x[t], S.y[t], S.z[t],
-                                                S.q[t]);
-      }
-      nbuf += __popcll(ball);
-      if (nbuf > BUF - 64) {
-        process(nbuf);
-        nbuf = 0;
-      }
-    };
-    stream_levels(S, G, 0, cx, cy, false, false, clampz, zlo, zhi, zA, zB,
-                  bx0, bx1, by0, by1, bz0, bz1, hi, lane, emit);
-    if (nbuf > 0) process(nbuf);
-    // the 4 lanes of an atom add their partial forces
+// After the pair kernel: blocks below atom_blocks convert the fixed-point
+// force of each sorted slot into the caller's forces (+= at the original
+// index) and clear it; the others sum the item partials of one (replica,
+// accumulator) each (finish_rep_block).
+template <bool F64>
+__global__ void __launch_bounds__(1024)
+k_half_finish(long long *__restrict__ ffix, const int *__restrict__ atom,
+              int n, double *__restrict__ forces, int atom_blocks,
+              const double *__restrict__ partial, int pstride,
+              const int *__restrict__ rep_first, double *__restrict__ out) {
+  constexpr int W = FixWords<F64>::W;
+  if ((int)blockIdx.x < atom_blocks) {
+    const int s = blockIdx.x * 1024 + threadIdx.x;
+    if (s >= n) return;
+    long long *src = ffix + (size_t)W * s;
+    const int a = atom[s];
     if (F64) {
-      fx64 += __shfl_down(fx64, 32, 64);
-      fy64 += __shfl_down(fy64, 32, 64);
-      fz64 += __shfl_down(fz64, 32, 64);
-      fx64 += __shfl_down(fx64, 16, 64);
-      fy64 += __shfl_down(fy64, 16, 64);
-      fz64 += __shfl_down(fz64, 16, 64);
-      if (active && split == 0) {
-        forces[3 * ai] += fx64;
-        forces[3 * ai + 1] += fy64;
-        forces[3 * ai + 2] += fz64;
+      for (int c = 0; c < 3; c++) {
+        forces[3 * a + c] += fix64_value(src[2 * c], src[2 * c + 1]);
+        src[2 * c] = 0;
+        src[2 * c + 1] = 0;
       }
     } else {
-      Fx += __shfl_down(Fx, 32, 64);
-      Fy += __shfl_down(Fy, 32, 64);
-      Fz += __shfl_down(Fz, 32, 64);
-      Fx += __shfl_down(Fx, 16, 64);
-      Fy += __shfl_down(Fy, 16, 64);
-      Fz += __shfl_down(Fz, 16, 64);
-      if (active && split == 0) {
-        forces[3 * ai] += (double)Fx * kFixedInv;
-        forces[3 * ai + 1] += (double)Fy * kFixedInv;
-        forces[3 * ai + 2] += (double)Fz * kFixedInv;
+      for (int c = 0; c < 3; c++) {
+        forces[3 * a + c] += (double)src[c] * kFixedInv;
+        src[c] = 0;
       }
     }
+    return;
   }
-  // this item's energies and virials (one row per item, as k_tm)
-  if (item_idx < *nitems) {
-#pragma unroll
-    for (int k = 0; k < ACC_N; k++) {
-      double v = acc[k];
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-      if (lane == 0) partial[(size_t)k * pstride + item_idx] = v;
-    }
-  }
-  unsigned long long tot = block_count(cnt);
-  if (threadIdx.x == 0 && tot) atomicAdd(npairs, tot);
+  const int b = blockIdx.x - atom_blocks;
+  finish_rep_block(partial, pstride, rep_first, out, b / ACC_N, b % ACC_N);
 }
 
 }  // namespace pmnb

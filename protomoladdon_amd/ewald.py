@@ -41,7 +41,8 @@ class NonbondedFullEwaldKSpace:
         self.close()
         lib = self.force._lib
         q = np.ascontiguousarray(topology.scaled_charges, dtype=np.float64)
-        ex = np.ascontiguousarray(topology.exclusions.table(), dtype=np.int32)
+        ex = np.ascontiguousarray(topology.exclusions.get_table(),
+                                dtype=np.int32)
         ew = ctypes.c_void_p()
         _lib.check(lib, self.force._ctx, lib.pmnb_ewald_create(
             self.force._ctx, self.alpha, self.accuracy, topology.n_atoms,

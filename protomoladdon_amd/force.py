@@ -281,6 +281,31 @@ class NonbondedCutoffSystemForce:
             ctypes.byref(n)))
         return out[:n.value]
 
+    def force_pairs(self, topology, positions):
+        """The pairs the force's own pair kernel evaluates at `positions`
+        (one evaluation with its pair record on): sorted (P, 3) int32 rows
+        (i, j, exclusion class), i < j."""
+        self._ensure(topology)
+        pos = np.ascontiguousarray(positions, dtype=np.float64)
+        lib = self._lib
+        n = ctypes.c_int64()
+        _lib.check(lib, self._ctx, lib.pmnb_force_pairs(
+            self._ctx, pos.ctypes.data, None, 0, ctypes.byref(n)))
+        out = np.zeros((max(n.value, 1), 3), dtype=np.int32)
+        _lib.check(lib, self._ctx, lib.pmnb_force_pairs(
+            self._ctx, pos.ctypes.data, out.ctypes.data, n.value,
+            ctypes.byref(n)))
+        return out[:n.value]
+
+    def pair_path(self, topology):
+        """'tile', 'list' or 'scan': the pair kernels this force's
+        evaluations take for the topology (pmnb_pair_path)."""
+        self._ensure(topology)
+        p = ctypes.c_int32()
+        _lib.check(self._lib, self._ctx,
+                   self._lib.pmnb_pair_path(self._ctx, ctypes.byref(p)))
+        return ("tile", "list", "scan")[p.value]
+
     def kernel_timing(self, enable):
         """Starts (True) or stops (False) HIP-event timing of the pair stage;
         returns (build_ms, pair_ms, evaluations) recorded since timing was

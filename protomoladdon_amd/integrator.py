@@ -105,11 +105,14 @@ class LeapfrogIntegrator:
     bonded: dict with bonds (B, 2), bond_k, bond_r0, angles (A, 3), angle_k,
     angle_theta0 (radians) and optionally ub_k, ub_r0; or None.  With
     bonded terms the energy block has 22 slots ([20] bond, [21] angle).
+    kspace: k-space terms on the same force (a NonbondedPMEwaldKSpace or a
+    NonbondedFullEwaldKSpace) evaluated in every force evaluation, or None.
     """
 
     keyword = "Leapfrog"
 
-    def __init__(self, timestep, force, shake=None, rattle=None, bonded=None):
+    def __init__(self, timestep, force, shake=None, rattle=None, bonded=None,
+                 kspace=None):
         if not timestep > 0:
             raise ValueError("Leapfrog: timestep > 0")
         self.timestep = float(timestep)
@@ -117,6 +120,9 @@ class LeapfrogIntegrator:
         self.shake = shake
         self.rattle = rattle
         self.bonded = bonded
+        self.kspace = kspace
+        if kspace is not None and kspace.force is not force:
+            raise ValueError("the k-space terms must belong to the same force")
         self._lf = None
         self._key = None
 
@@ -160,6 +166,14 @@ class LeapfrogIntegrator:
         self._lf = lf
         self._key = key
         self._constraints = constraints
+        if self.kspace is not None:
+            from .pme import NonbondedPMEwaldKSpace
+            self.kspace._ensure(topology)
+            if isinstance(self.kspace, NonbondedPMEwaldKSpace):
+                pm, ew = self.kspace._pme, None
+            else:
+                pm, ew = None, self.kspace._ew
+            _lib.check(lib, ctx, lib.pmnb_lf_set_kspace(lf, pm, ew))
         if self.bonded is not None:
             b = self.bonded
             arr = {k: np.ascontiguousarray(b[k], dtype=np.int32 if k in (

@@ -353,9 +353,10 @@ def chain_topology(n_chains, chain_len, L, seed, cell_size, sigma=CHAIN_SIGMA,
 
 
 def chain_workload(name, n_chains, chain_len, L, seed, rc=10.0, rs=8.5,
-                   alpha=0.34, cell_size=None):
+                   alpha=0.34, cell_size=None, sigma=CHAIN_SIGMA,
+                   eps=CHAIN_EPS):
     cell_size = rc / 2 if cell_size is None else cell_size
-    topo = chain_topology(n_chains, chain_len, L, seed, cell_size)
+    topo = chain_topology(n_chains, chain_len, L, seed, cell_size, sigma, eps)
     pos = chain_positions(n_chains, chain_len, L, seed)
     return Workload(name, topo, pos, rc, (rs, rc), "ewald", (rs, rc), alpha,
                     meta={"n_chains": n_chains, "chain_len": chain_len,
@@ -380,6 +381,30 @@ def small_water(seed=1234, n_mol=300, elec="ewald", rc=9.0, rs=7.5,
                 alpha=0.32):
     L = (n_mol / WATER_DENSITY) ** (1.0 / 3.0)
     return water_workload("water_small", n_mol, L, seed, rc, rs, elec, alpha)
+
+
+def with_algorithm(w, cutoff, cell_size, name=None):
+    """The workload with another NonbondedCutoff -cutoff and cell size.
+    With the cutoff below the switching cutoff the pair set is the one the
+    reference's cell grid and delta list reach
+    (topology/CellListEnumeratorPeriodicBoundaries.h:46-83,114-134)."""
+    w.cutoff = float(cutoff)
+    w.topology.cell_manager = CubicCellManager(cell_size)
+    w.topology.touch()
+    if name:
+        w.name = name
+    return w
+
+
+def many_type_chains(seed=1234, n_chains=60, n_types=12, chain_len=20):
+    """small_chains with n_types LJ types (sigma 3.0 .. 4.0 A, epsilon
+    0.05 .. 0.2 kcal/mol, Lorentz-Berthelot): more than the tile path
+    holds, so the force takes the general cell-list path."""
+    L = 138.0 * (n_chains / 12500.0) ** (1.0 / 3.0)
+    sigma = tuple(np.linspace(3.0, 4.0, n_types))
+    eps = tuple(np.linspace(0.05, 0.2, n_types))
+    return chain_workload("chains_%dtypes" % n_types, n_chains, chain_len, L,
+                          seed, sigma=sigma, eps=eps)
 
 
 CONFIGS = {0: config0, 1: config1, 2: config2}

@@ -110,12 +110,29 @@ class ExclusionTable:
         return self._lookup.get((int(a1), int(a2)), EXCLUSION_NONE)
 
     def table(self):
-        """(n, 3) int32 rows (a1, a2, class) in insertion order."""
+        """(n, 3) int32 rows (a1, a2, class) of every add, in insertion
+        order (the lookup resolves a repeated pair to its last class)."""
         if not self._chunks:
             return np.zeros((0, 3), dtype=np.int32)
         if len(self._chunks) > 1:
             self._chunks = [np.concatenate(self._chunks)]
         return self._chunks[0]
+
+    def get_table(self):
+        """ExclusionTable::getTable() (topology/ExclusionTable.h:87): the
+        rows ExclusionTable::add stores (ExclusionTable.cpp:103-110), one
+        per pair with the class of its FIRST add (its set compares atoms
+        only), no rows of one atom with itself and no EXCLUSION_NONE rows,
+        in insertion order.  The Ewald correction term iterates this view
+        (force/nonbonded/NonbondedFullEwaldSystemForce.h:1105-1129)."""
+        rows = self.table()
+        keep = (rows[:, 0] != rows[:, 1]) & (rows[:, 2] != EXCLUSION_NONE)
+        rows = rows[keep]
+        if not len(rows):
+            return np.zeros((0, 3), dtype=np.int32)
+        key = rows[:, 0].astype(np.int64) * max(self.n_atoms, 1) + rows[:, 1]
+        _, first = np.unique(key, return_index=True)
+        return np.ascontiguousarray(rows[np.sort(first)], dtype=np.int32)
 
     @classmethod
     def from_array(cls, n_atoms, rows):
@@ -199,9 +216,11 @@ def charmm_table(rmin_half, epsilon, rmin14_half=None, epsilon14=None):
 
 def build_exclusion_table(n_atoms, exclude, bonds=(), angles=(),
                           dihedrals=()):
-    """buildExclusionTable (BuildTopology.cpp:1098-1155): bonds FULL, then
-    angle ends FULL (unless 1-2), then dihedral ends FULL for 1-4 or MODIFIED
-    for scaled1-4 (unless 1-2 or 1-3), in that add order."""
+    """buildExclusionTable (BuildTopology.cpp:1098-1155): bond ends FULL,
+    then (unless 1-2) angle ends FULL, then (unless 1-2 or 1-3 exclusion)
+    dihedral ends FULL for 1-4 or MODIFIED for scaled1-4, in that add
+    order and unconditionally: in a ring a pair may be added more than
+    once, the lookup then gives its last class and getTable() its first."""
     t = ExclusionTable(n_atoms)
     if exclude == EXCLUDE_NONE:
         return t

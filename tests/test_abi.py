@@ -55,7 +55,11 @@ def test_protomol_adapter_compiles_against_the_reference_headers():
             "template class ProtoMol::PmnbCutoffSystemForce<"
             "ProtoMol::C2SwitchingFunction>;\n"
             "template class ProtoMol::PmnbCutoffSystemForce<"
-            "ProtoMol::C1SwitchingFunction>;\n")
+            "ProtoMol::C1SwitchingFunction>;\n"
+            "template class ProtoMol::PmnbCutoffSystemForce<"
+            "ProtoMol::C2SwitchingFunction, ProtoMol::PmnbEwaldRealForce>;\n"
+            "template class ProtoMol::PmnbCutoffSystemForce<"
+            "ProtoMol::C1SwitchingFunction, ProtoMol::PmnbEwaldRealForce>;\n")
     r = subprocess.run(["g++", "-std=c++0x", "-fpermissive", "-w",
                         "-fsyntax-only", "-I" + src,
                         "-I" + os.path.join(ROOT, "include"), "-I" + ROOT,

@@ -101,3 +101,58 @@ def test_kspace_matches_oracle_orthorhombic_box():
     scale = abs(er) + np.sum(qs * qs) * alpha / np.sqrt(np.pi)
     assert abs(od[1] - er) <= 1e-13 * scale
     assert np.abs(od[2:11] - vr).max() <= 1e-10 * np.abs(vr).max()
+
+
+@pytest.mark.parametrize("rows", ["get_table", "as_added"])
+def test_kspace_matches_oracle_rings(rows):
+    """A 5-ring and a 6-ring, whose exclusion adds repeat pairs: the
+    correction walks ExclusionTable::getTable() (one row per pair, first
+    class).  "as_added" hands the C ABI every row as added (duplicates, and
+    the 5-ring's 1-3 pairs with both classes): pmnb_ewald_create keeps the
+    first row of each pair itself."""
+    from tests.test_oracle_ewald import make_system, rings
+    pos, q, L, bonds, angles, dih, mol = rings()
+    w, s = make_system(pos, q, L, bonds, angles, dih, mol, csf=0.5)
+    if rows == "as_added":
+        ex = w.topology.exclusions
+        ex.get_table = ex.table
+    alpha, acc = 0.6, 1e-8
+    fr, er, vr, info = O.ewald_kspace(s, alpha, acc)
+    fd, od, dinfo = run_device(w, alpha, acc)
+    assert dinfo[3] == int(info[3])
+    assert np.abs(fd - fr).max() <= 1e-10 * np.abs(fr).max()
+    qs = w.topology.scaled_charges
+    scale = abs(er) + np.sum(qs * qs) * alpha / np.sqrt(np.pi)
+    assert abs(od[1] - er) <= 1e-13 * scale
+    assert np.abs(od[2:11] - vr).max() <= 1e-10 * np.abs(vr).max()
+
+
+def test_kspace_state_follows_box_and_topology():
+    """The k-space terms are built for one box and topology: after either
+    changes, evaluate fails with PMNB_ERR_STATE until they are rebuilt
+    (the Python wrapper rebuilds them by itself)."""
+    import ctypes
+    import torch
+    from protomoladdon_amd import _lib
+    w = systems.small_water(n_mol=100, elec="coulomb")
+    force = w.make_force("fp64")
+    ks = NonbondedFullEwaldKSpace(force, 0.35, 1e-6)
+    dev = torch.device("cuda:0")
+    x = torch.tensor(w.positions, dtype=torch.float64, device=dev)
+    f = torch.zeros_like(x)
+    out = torch.zeros(20, dtype=torch.float64, device=dev)
+    ks.evaluate_device(w.topology, x, f, out)
+    lib, ctx = force._lib, force._ctx
+    L = float(w.topology.boundary_conditions.lengths()[0]) * 1.01
+    e1 = (ctypes.c_double * 3)(L, 0, 0)
+    e2 = (ctypes.c_double * 3)(0, L, 0)
+    e3 = (ctypes.c_double * 3)(0, 0, L)
+    o = (ctypes.c_double * 3)(L / 2, L / 2, L / 2)
+    assert lib.pmnb_set_box(ctx, e1, e2, e3, o) == 0
+    rc = lib.pmnb_ewald_evaluate_device(ks._ew, x.data_ptr(), f.data_ptr(),
+                                        out.data_ptr(), 0, None)
+    assert rc == 4  # PMNB_ERR_STATE
+    assert b"changed" in lib.pmnb_last_error(ctx)
+    torch.cuda.synchronize()
+    ks.close()
+    force.close()

@@ -76,9 +76,21 @@ def test_oracle_bitwise_equals_reference_golden(name, make):
 @pytest.mark.skipif(not O.ref_available(),
                     reason="reference build (oracle/_ref) not present")
 @pytest.mark.parametrize("case", ["water_coulomb", "water_ewald", "chains",
-                                  "small_box", "far_images", "origin"])
+                                  "small_box", "far_images", "origin",
+                                  "water_acut", "replica_acut", "water_scan",
+                                  "chains_12types"])
 def test_oracle_bitwise_equals_reference_build(case):
-    if case == "water_coulomb":
+    # the last four: algorithm cutoff below the switching cutoff (the pair
+    # set the reference's cell grid and delta list reach) and 12 LJ types
+    if case == "water_acut":
+        w = systems.with_algorithm(systems.small_water(n_mol=400), 6.5, 3.0)
+    elif case == "replica_acut":
+        w = systems.with_algorithm(systems.config3_replica(0, 167), 6.5, 2.0)
+    elif case == "water_scan":
+        w = systems.with_algorithm(systems.small_water(n_mol=300), 8.9, 1.5)
+    elif case == "chains_12types":
+        w = systems.many_type_chains()
+    elif case == "water_coulomb":
         w = systems.small_water(n_mol=250, elec="coulomb", rc=8.0, rs=6.5)
     elif case == "water_ewald":
         w = systems.small_water(n_mol=250)
@@ -291,3 +303,19 @@ def test_modified_pairs_are_scaled_by_half():
     e_none = energy_with(EXCLUSION_NONE)
     np.testing.assert_allclose(e_mod - e_full, 0.5 * (e_none - e_full),
                                rtol=1e-9, atol=1e-9)
+
+
+def test_threaded_restatement_matches_serial():
+    """The all-core CPU baseline (orc_evaluate_threads) computes what
+    orc_evaluate does: the same pair count, forces and energies up to the
+    order of the sums."""
+    w = systems.small_chains(n_chains=80)
+    s = O.System.from_workload(w, flags=1)
+    r = O.evaluate(s)
+    f, e, vir, n, used = O.evaluate_threads(s, 4)
+    assert used == 4 and n == r.n_pairs
+    scale = np.abs(r.forces).max()
+    assert np.abs(f - r.forces).max() <= 1e-12 * scale
+    np.testing.assert_allclose(e, r.energies, rtol=1e-12)
+    np.testing.assert_allclose(vir, r.virial, rtol=1e-11,
+                               atol=1e-12 * np.abs(r.virial).max())

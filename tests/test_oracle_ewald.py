@@ -225,3 +225,70 @@ def test_total_virial_is_alpha_independent():
     off = [1, 2, 5]
     assert np.abs(totals[0][off]).min() > 1e-2 * np.abs(totals[0]).max()
     assert np.abs(totals[0] - totals[1]).max() < 1e-8 * np.abs(totals[0]).max()
+
+
+def rings(sizes=(5, 6), L=14.0, seed=17, bond=1.5):
+    """One ring molecule of each size in `sizes` (regular polygons of side
+    `bond`, random orientation and place), with the bonds, angles and
+    dihedrals around each ring; random charges with a net total."""
+    rng = np.random.default_rng(seed)
+    pos, bonds, angles, dih, mol = [], [], [], [], []
+    for m, k in enumerate(sizes):
+        R = bond / (2 * math.sin(math.pi / k))
+        u, v = np.linalg.qr(rng.normal(size=(3, 3)))[0][:, :2].T
+        c = rng.uniform(0.3 * L, 0.7 * L, 3)
+        b = len(pos)
+        for j in range(k):
+            t = 2 * math.pi * j / k
+            pos.append(c + R * (math.cos(t) * u + math.sin(t) * v))
+            mol.append(m)
+        ring = [b + (j % k) for j in range(k + 3)]
+        bonds += [(ring[j], ring[j + 1]) for j in range(k)]
+        angles += [(ring[j], ring[j + 1], ring[j + 2]) for j in range(k)]
+        dih += [(ring[j], ring[j + 1], ring[j + 2], ring[j + 3])
+                for j in range(k)]
+    q = rng.normal(0, 0.4, len(pos))
+    q[0] += 0.25
+    return (np.array(pos), q, L, bonds, angles, dih, np.array(mol, np.int32))
+
+
+def real_space_exact_classes(w, alpha, rc, classes):
+    """real_space_exact with the pair classes given by `classes` rows."""
+    from protomoladdon_amd.topology import ExclusionTable
+    t = w.topology
+    saved = t.exclusions
+    t.exclusions = ExclusionTable.from_array(t.n_atoms, classes)
+    try:
+        return real_space_exact(w, alpha, rc)
+    finally:
+        t.exclusions = saved
+
+
+def test_ring_exclusions_follow_get_table():
+    """In rings the builder adds pairs more than once (ExclusionTable::add,
+    ExclusionTable.cpp:103-110): a 6-ring has 18 rows of 15 pairs (each 1-4
+    pair from both sides), in a 5-ring every 1-4 end is also a 1-3 pair
+    (FULL first, MODIFIED last).  getTable() keeps one row per pair with
+    its first class, and the correction term walks it once per pair."""
+    pos, q, L, bonds, angles, dih, mol = rings()
+    w, s = make_system(pos, q, L, bonds, angles, dih, mol, csf=0.5)
+    ex = w.topology.exclusions
+    rows, view = ex.table(), ex.get_table()
+    assert len(rows) == 15 + 18 and len(view) == 10 + 15
+    assert len({(a, b) for a, b, _ in view.tolist()}) == len(view)
+    ring5 = [tuple(r) for r in view.tolist() if r[0] < 5]
+    assert all(c == 2 for _, _, c in ring5)            # first add: FULL
+    assert all(ex.check(a, b) == 2 for a, b, _ in ring5 if abs(a - b) in (1, 4))
+    assert all(ex.check(a, b) == 1 for a, b, _ in ring5 if abs(a - b) in (2, 3))
+    # real space with getTable's classes + k-space is alpha-independent;
+    # correcting a pair twice, or with its last class, breaks it
+    totals = []
+    for alpha in (0.55, 0.8):
+        _, e_k, _, info = O.ewald_kspace(s, alpha, 1e-14)
+        totals.append(real_space_exact_classes(w, alpha, info[1], view) + e_k)
+    assert abs(totals[0] - totals[1]) < 1e-8 * abs(totals[0])
+    # the rows as added (duplicates included) give another correction
+    s_dup = O.System.from_workload(w)
+    s_dup.excl_table = np.ascontiguousarray(rows)
+    e_dup = O.ewald_kspace(s_dup, 0.55, 1e-14)[1]
+    assert abs(e_dup - O.ewald_kspace(s, 0.55, 1e-14)[1]) > 1e-3
