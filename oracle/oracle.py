@@ -336,7 +336,25 @@ class OrcLf(ctypes.Structure):
         ("bond_r0", ctypes.c_void_p), ("angles", ctypes.c_void_p),
         ("angle_k", ctypes.c_void_p), ("angle_t0", ctypes.c_void_p),
         ("ub_k", ctypes.c_void_p), ("ub_r0", ctypes.c_void_p),
+        ("extra", ctypes.c_void_p), ("extra_user", ctypes.c_void_p),
     ]
+
+
+EXTRA_FN = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double),
+                            ctypes.POINTER(ctypes.c_double),
+                            ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
+
+
+def _extra_callback(n, extra):
+    """A C callback for orc_lf.extra from extra(x (n, 3)) -> (forces (n,
+    3), electrostatic energy), added to the force group's sums."""
+    def cb(px, pf, pe, _user):
+        x = np.ctypeslib.as_array(px, shape=(n, 3))
+        f = np.ctypeslib.as_array(pf, shape=(n, 3))
+        fe, ee = extra(x.copy())
+        f += fe
+        pe[0] += ee
+    return EXTRA_FN(cb)
 
 
 class MDState:
@@ -354,7 +372,7 @@ class MDState:
 
 
 def _lf_struct(masses, constraints, rest, timestep, shake, rattle, bonded,
-               keep):
+               keep, extra=None):
     cons = np.ascontiguousarray(constraints, dtype=np.int32).reshape(-1, 2)
     rest = np.ascontiguousarray(rest, dtype=np.float64)
     mass = np.ascontiguousarray(masses, dtype=np.float64)
@@ -382,14 +400,18 @@ def _lf_struct(masses, constraints, rest, timestep, shake, rattle, bonded,
         lf.angle_k, lf.angle_t0 = arr("angle_k"), arr("angle_theta0")
         if "ub_k" in bonded:
             lf.ub_k, lf.ub_r0 = arr("ub_k"), arr("ub_r0")
+    if extra is not None:
+        cb = _extra_callback(len(mass), extra)
+        keep.append(cb)
+        lf.extra = ctypes.cast(cb, ctypes.c_void_p).value
     return lf
 
 
-def lf_forces(system, positions, masses, bonded=None):
+def lf_forces(system, positions, masses, bonded=None, extra=None):
     """calculateForces of the leapfrog's force group (cleared forces and
-    energies, then the bonded terms and the cutoff force): returns an
-    MDState-ready (forces, energies (LJ, electrostatic), bonded (bond,
-    angle))."""
+    energies, then the bonded terms, the cutoff force and `extra`, a
+    callable x -> (forces, electrostatic energy)): returns an MDState-ready
+    (forces, energies (LJ, electrostatic), bonded (bond, angle))."""
     lib = load()
     if not hasattr(lib, "_lff_ready"):
         lib.orc_lf_forces.argtypes = [ctypes.POINTER(OrcSystem),
@@ -397,7 +419,7 @@ def lf_forces(system, positions, masses, bonded=None):
         lib._lff_ready = True
     keep = []
     lf = _lf_struct(masses, np.zeros((0, 2)), np.zeros(0), 1.0, None, None,
-                    bonded, keep)
+                    bonded, keep, extra)
     x = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3).copy()
     f = np.zeros_like(x)
     e, b = np.zeros(2), np.zeros(2)
@@ -409,12 +431,15 @@ def lf_forces(system, positions, masses, bonded=None):
 
 
 def leapfrog_run(system, state, n_steps, masses, constraints, rest, timestep,
-                 shake=None, rattle=None, bonded=None, max_calls=1 << 16):
+                 shake=None, rattle=None, bonded=None, max_calls=1 << 16,
+                 extra=None):
     """LeapfrogIntegrator::run(n_steps) on `state` (updated in place) with
     the cutoff force of `system` and the optional bonded terms as forces.
     shake / rattle are (epsilon, max_iter) or None; bonded is a dict with
     bonds, bond_k, bond_r0, angles, angle_k, angle_theta0 (radians) and
-    optionally ub_k, ub_r0.  Returns the sweep count of every SHAKE and
+    optionally ub_k, ub_r0; extra (optional) is a further force of the
+    group, a callable x -> (forces, electrostatic energy) such as the
+    k-space terms.  Returns the sweep count of every SHAKE and
     RATTLE call in call order."""
     lib = load()
     if not hasattr(lib, "_lf_ready"):
@@ -427,7 +452,7 @@ def leapfrog_run(system, state, n_steps, masses, constraints, rest, timestep,
     st = system.struct()
     keep = []
     lf = _lf_struct(masses, constraints, rest, timestep, shake, rattle,
-                    bonded, keep)
+                    bonded, keep, extra)
     iters = np.zeros(max_calls, dtype=np.int32)
     n = lib.orc_leapfrog_run(ctypes.byref(st), ctypes.byref(lf),
                              state.x.ctypes.data, state.v.ctypes.data,

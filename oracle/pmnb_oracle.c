@@ -758,6 +758,10 @@ typedef struct {
   const double *bond_k, *bond_r0;
   const int32_t *angles;      /* 3 per angle */
   const double *angle_k, *angle_t0, *ub_k, *ub_r0;  /* t0 in radians */
+  /* optional further force of the force group (k-space terms in the
+     tests), called after the cutoff force: forces += , *e_el += */
+  void (*extra)(const double *x, double *f, double *e_el, void *user);
+  void *extra_user;
 } orc_lf;
 
 typedef struct {
@@ -959,7 +963,11 @@ static int md_forces(orc_md *m) {
   m->bonded2[0] = m->bonded2[1] = 0;
   md_bonded(m);
   sys.pos = m->x;
-  return orc_evaluate(&sys, m->f, m->energies2, NULL, NULL, NULL, 0, NULL);
+  if (orc_evaluate(&sys, m->f, m->energies2, NULL, NULL, NULL, 0, NULL))
+    return -1;
+  if (m->lf->extra)
+    m->lf->extra(m->x, m->f, m->energies2 + 1, m->lf->extra_user);
+  return 0;
 }
 
 /*

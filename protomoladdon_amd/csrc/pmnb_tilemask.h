@@ -483,9 +483,12 @@ k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
     // current pair's arithmetic), then moves the candidates' fixed-point
     // forces to the force array and clears their slots.
     auto process = [&](int n) __attribute__((always_inline)) {
-      // records n .. n + 3 (the last group of 4) are made far away
-      if (lane < kTmSplit && n + lane < BUF)
-        recf[wave][n + lane] = make_float4(1e18f, 1e18f, 1e18f, 0.f);
+      // records n .. n + 3 (the last group of 4) get NaN coordinates, which
+      // fail the distance test under any minimum image
+      if (lane < kTmSplit && n + lane < BUF) {
+        const float nan = __builtin_nanf("");
+        recf[wave][n + lane] = make_float4(nan, nan, nan, 0.f);
+      }
       wave_sync_lds();
       unsigned m[kNW];
       const float4 *rb = recf[wave];

@@ -102,6 +102,37 @@ def test_free_flight():
     assert not st.f.any()
 
 
+def test_extra_force_of_the_group_is_kicked():
+    """A further force of the force group (the hook the k-space terms use):
+    a constant force F gives the constant-acceleration path x_n = x_0 +
+    n h v_0 + n^2 h^2 F / (2 m), v_n = v_0 + n h F / m, and its energy joins
+    the electrostatic slot of every evaluation."""
+    pos = np.array([[5.0, 5.0, 5.0], [35.0, 35.0, 35.0]])
+    m = np.array([12.0, 16.0])
+    _, s = toy_system(pos, m)
+    v0 = np.array([[0.1, -0.2, 0.3], [-0.05, 0.02, 0.07]])
+    F = np.array([[0.3, -0.1, 0.2], [-0.4, 0.25, 0.05]])
+    calls = []
+
+    def extra(x):
+        calls.append(x.copy())
+        return F, 1.25
+    f0, e0, _ = O.lf_forces(s, pos, m, extra=extra)
+    np.testing.assert_array_equal(f0, F)
+    assert e0[1] == 1.25
+    st = O.MDState(pos, v0, f0, energies=e0)
+    n = 12
+    O.leapfrog_run(s, st, n, m, np.zeros((0, 2)), np.zeros(0), 1.0,
+                   extra=extra)
+    assert len(calls) == 1 + n
+    np.testing.assert_allclose(st.v, v0 + n * H * F / m[:, None], rtol=1e-13)
+    np.testing.assert_allclose(
+        st.x, pos + n * H * v0 + (n * H) ** 2 * F / (2 * m[:, None]),
+        rtol=1e-13)
+    np.testing.assert_array_equal(calls[-1], st.x)
+    assert st.energies[1] == 1.25
+
+
 def test_shake_velocity_and_momentum():
     d0 = 1.2
     pos = np.array([[10.0, 10.0, 10.0], [10.0 + d0, 10.0, 10.0]])
