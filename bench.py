@@ -33,6 +33,7 @@ FLOP_PER_PAIR = 90        # LJ + Ewald real (A&S) + shared C2 + atomic virial
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
 PEAK_FP64_TFLOPS = 78.6   # MI355X published FP64 vector peak
 DT_FS = 1.0
+GRAPH_CHUNK = 20          # MD steps per captured run (replayed)
 
 
 def ns_per_day(steps_per_s):
@@ -101,13 +102,28 @@ def time_md(w, precision, steps, warmup, flags, torch):
     its, n_calls, unconverged = lf.record_iterations(0)
     # call order: SHAKE, then (RATTLE, SHAKE) per later step, then RATTLE
     shake_its, rattle_its = its[0::2], its[1::2]
+    # the same steps as replays of one captured run(chunk) (pmnb_lf_run_graph)
+    chunk = GRAPH_CHUNK if steps >= GRAPH_CHUNK else steps
+    reps = max(1, steps // chunk)
+    lf.run_graph(t, x, v, f, out, chunk, 1, cons, flags=flags, stream=sp)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lf.run_graph(t, x, v, f, out, chunk, reps, cons, flags=flags, stream=sp)
+    torch.cuda.synchronize()
+    dtg = time.perf_counter() - t0
+    _, _, unconverged_g = lf.record_iterations(0)
     lf.close()
     force.close()
-    return {"value": ns_per_day(steps / dt), "unit": "ns/day",
-            "ms_per_step": 1e3 * dt / steps, "steps": steps,
+    ms_eager = 1e3 * dt / steps
+    ms_graph = 1e3 * dtg / (chunk * reps)
+    return {"value": ns_per_day(1e3 / ms_graph), "unit": "ns/day",
+            "ms_per_step": ms_graph, "steps": chunk * reps,
+            "graph": "%d replays of one captured run(%d)" % (reps, chunk),
+            "eager_ms_per_step": ms_eager,
+            "eager_ns_per_day": ns_per_day(1e3 / ms_eager),
             "mean_shake_sweeps": float(shake_its.mean()),
             "mean_rattle_sweeps": float(rattle_its.mean()),
-            "unconverged_calls": int(unconverged)}
+            "unconverged_calls": int(unconverged + unconverged_g)}
 
 
 def time_ewald_kspace(w, alpha, accuracy, reps, torch):
@@ -263,6 +279,7 @@ def bench_batch(args, torch):
     diff = float(np.abs(frs.cpu().numpy() - fb).max())
     for f in singles:
         f.close()
+    md = bench_batch_md(args, torch, ws, bf, pos)
     bf.close()
     args._batch_check = (ws, off, fb)
     return {
@@ -285,7 +302,51 @@ def bench_batch(args, torch):
         "speedup_vs_sequential": dseq / dt,
         "max_abs_force_diff_vs_sequential": diff,
         "autotune": {"shape": shape, "grid_blocks": grid, "ms": tune_ms},
+        "md": md,
     }
+
+
+def bench_batch_md(args, torch, ws, bf, pos0):
+    """Leapfrog 1 fs + SHAKE/RATTLE of every replica (pmnb_batch_lf_*): one
+    captured run(n) of all replicas, launched once."""
+    from protomoladdon_amd.batch import BatchedLeapfrog, OUT_SLOTS
+    from protomoladdon_amd.integrator import topology_constraints
+    R = len(ws)
+    masses = [w.topology.masses for w in ws]
+    cons = [topology_constraints(w.topology) for w in ws]
+    rng = np.random.default_rng(4321)
+    v0 = np.concatenate([rng.standard_normal((w.n_atoms, 3)) * np.sqrt(
+        0.001987191 * 300.0 / w.topology.masses)[:, None] for w in ws])
+    lf = BatchedLeapfrog(DT_FS, bf, masses, cons, shake=(1e-5, 30),
+                         rattle=(1e-5, 30))
+    x = pos0.clone()
+    v = torch.from_numpy(v0).to(x.device)
+    f = torch.zeros_like(x)
+    out = torch.zeros(R * OUT_SLOTS, dtype=torch.float64, device=x.device)
+    lf.initialize(x, f, out, 1)
+    lf.run(x, v, f, out, 2, 1)
+    torch.cuda.synchronize()
+    n = args.md_steps
+    print("batched dynamics: capture of run(%d) ..." % n, file=sys.stderr,
+          flush=True)
+    lf.record_iterations(1 << 20)
+    t0 = time.perf_counter()
+    lf.run_graph(x, v, f, out, n, 1, 1)   # capture + instantiate + run
+    torch.cuda.synchronize()
+    t_first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    lf.run_graph(x, v, f, out, n, 1, 1)   # one replay: n steps
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / n
+    its, n_calls, unconverged = lf.record_iterations(0)
+    lf.close()
+    return {"steps_per_launch": n, "ms_per_step": 1e3 * dt,
+            "ns_per_day_per_replica": ns_per_day(1.0 / dt),
+            "aggregate_ns_per_day": R * ns_per_day(1.0 / dt),
+            "first_call_s_incl_capture": t_first,
+            "unconverged_calls": int(unconverged),
+            "integrator": "Leapfrog 1 fs + SHAKE/RATTLE (1e-5, 30) per "
+                          "replica, one graph launch of %d steps" % n}
 
 
 def bench_forcematch(args, torch):
@@ -398,6 +459,9 @@ def main():
                     help="water96k: the headline line (with compact batch512 "
                          "and forcematch250k summaries); batch512 and "
                          "forcematch250k alone: their own full lines")
+    ap.add_argument("--batch-md-steps", type=int, default=1000,
+                    help="leapfrog steps of the 512 replicas in one graph "
+                         "launch")
     ap.add_argument("--no-extras", action="store_true",
                     help="leave out the batch512 / forcematch250k summaries")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -424,6 +488,7 @@ def main():
         del src, dst
 
     if args.workload == "batch512":
+        args.md_steps = args.batch_md_steps
         print(json.dumps(bench_batch(args, torch)), flush=True)
         return
     if args.workload == "forcematch250k":
@@ -477,14 +542,15 @@ def main():
     extras = {}
     if not args.no_extras:
         ea = argparse.Namespace(steps=max(20, args.steps // 4),
-                                warmup=max(3, args.warmup // 4))
+                                warmup=max(3, args.warmup // 4),
+                                md_steps=args.batch_md_steps)
         print("batch512 ...", file=sys.stderr, flush=True)
         b = bench_batch(ea, torch)
         extras["batch512"] = {k: b[k] for k in (
             "value", "unit", "ms_per_step", "aggregate_ns_per_day",
             "pair_evals_per_s", "sequential_ms_per_step",
             "speedup_vs_sequential", "max_abs_force_diff_vs_sequential",
-            "autotune", "config")}
+            "autotune", "config", "md")}
         print("forcematch250k ...", file=sys.stderr, flush=True)
         fm = bench_forcematch(ea, torch)
         extras["forcematch250k"] = {k: fm[k] for k in (

@@ -383,11 +383,46 @@ int pmnb_lf_forces_device(pmnb_leapfrog *lf, const double *d_positions,
                           double *d_forces, double *d_out, uint32_t flags,
                           void *stream);
 
+/* n_replays replays of one captured HIP graph of
+   pmnb_lf_run_device(n_steps) on the same arrays: the result is that of
+   n_replays consecutive pmnb_lf_run_device(n_steps) calls, bit for bit
+   (LeapfrogIntegrator::run called n_replays times, as the reference's
+   driver does between outputs).  The graph is captured on the first call
+   and again whenever the arrays, flags, n_steps, the topology, the box or
+   the kernel shape change; every allocation and upload happens before the
+   capture.  Enqueued on `stream` without host synchronisation.  Not with
+   k-space terms (PMNB_ERR_UNSUPPORTED) or kernel timing on
+   (PMNB_ERR_STATE).  No counterpart in the reference: a launch-overhead
+   form of run for callers that step many times. */
+int pmnb_lf_run_graph(pmnb_leapfrog *lf, double *d_positions,
+                      double *d_velocities, double *d_forces, double *d_out,
+                      int32_t n_steps, int32_t n_replays, uint32_t flags,
+                      void *stream);
+
+/* The integrator over a batch of replicas (pmnb_batch): each replica runs
+   LeapfrogIntegrator::run with its own SHAKE / RATTLE sweep counts (its
+   mean error over its own constraints), so every replica's trajectory is,
+   bit for bit, the one pmnb_lf_create + pmnb_lf_run_device give for that
+   replica alone with the same force.  Arrays are the batch's (atoms of
+   the replicas in order of addition; constraint atoms are batch indices
+   and may not join two replicas); d_out holds 20 doubles per replica.
+   pmnb_lf_run_device, pmnb_lf_run_graph, pmnb_lf_forces_device,
+   pmnb_lf_iterations (n_replicas counts per call, [call * n_replicas +
+   replica]) and pmnb_lf_destroy apply; bonded and k-space terms are not
+   taken.  Adding replicas to the batch afterwards invalidates it.  No
+   counterpart in the reference (one integrator per process there). */
+int pmnb_batch_lf_create(pmnb_batch *b, const pmnb_lf_params *params,
+                         int32_t n_atoms, const double *mass,
+                         int32_t n_constraints,
+                         const int32_t *constraint_atoms,
+                         const double *rest_length, pmnb_leapfrog **out);
+
 /* Starts (capacity > 0) or stops (0) recording the sweep count of every
    SHAKE and RATTLE call, and returns what was recorded since the last
    start: up to `capacity_out` counts into iters (may be NULL), the number
    of calls and the number that reached max_iter + 1 sweeps (the
-   reference's "not converged" warning).  Waits for the device. */
+   reference's "not converged" warning).  Waits for the device.  A batch's
+   integrator records one count per replica and call. */
 int pmnb_lf_iterations(pmnb_leapfrog *lf, int64_t capacity, int32_t *iters,
                        int64_t capacity_out, int64_t *n_calls,
                        int64_t *n_unconverged);

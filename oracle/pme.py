@@ -11,17 +11,23 @@ leg, as the checker.  The product never imports it.
 
 Conventions followed (reference lines in Grid.h unless said otherwise):
 * grid coordinates x = position * n / L wrapped into [0, n) by adding or
-  subtracting n (anterpolateCharge, :124-131); base point int(x) and the
-  stencil of `order` points starting (order - 1) / 2 below it (:133-138,
-  offsets :510-512);
+  subtracting n (anterpolateCharge, :107-131); base point int(x) and the
+  stencil of `order` points starting (order - 1) / 2 below it, the
+  quotient truncated as C's integer division does (:509-511: 0 for order
+  2, 1 for order 4);
 * the B-spline weights and derivatives of BSpline::set (BSpline.cpp:58-95);
 * unnormalised transforms (FFTW plans without scaling); the energy is half
   the sum over m != 0 of theta(m) |S(m)|^2 with theta(m) =
   B(m) exp(-pi^2 m^2 / alpha^2) / (pi V m^2) and m = (i0/Lx, j0/Ly, k0/Lz)
-  (scalarSum, :214-304), B(m) = 1 / |sum_j M(j) e^{2 pi i m j / n}|^2
+  (scalarSum, :237-341), B(m) = 1 / |sum_j M(j) e^{2 pi i m j / n}|^2
   (dftmod, :590-602);
 * the virial slots XX..ZZ of 0.5 sum e (delta - v m m^T), v = 2 (1/m^2 +
-  pi^2/alpha^2) (:272-300);
+  pi^2/alpha^2), summed as scalarSum sums them (:283-304): over the half
+  spectrum k = 0 .. nz/2 with e doubled for 0 < k < nz/2 (and k = nz/2 when
+  nz is odd), each term taking the signed frequencies of its own point.
+  On the planes i = nx/2 and j = ny/2 of an even grid the doubled point's
+  conjugate has the same i0 (or j0) but the opposite k0, so these
+  off-diagonal terms differ from a sum over the full grid;
 * the force on atom i, -q_i grad of the convolved grid times n / L per axis
   (interpolateForce, :171-199).
 """
@@ -129,7 +135,7 @@ def pme_reciprocal(pos, q, box, grid, order, alpha, virial=True):
     L = np.asarray(box, dtype=np.float64)
     n = np.asarray(grid, dtype=np.int64)
     hr = n / L
-    offset = -(order - 1) // 2
+    offset = -((order - 1) // 2)   # C: -(order - 1) / 2, truncated
     idx, th, dth = [], [], []
     for a in range(3):
         x = _grid_coord(pos[:, a], n[a], hr[a])
@@ -162,15 +168,22 @@ def pme_reciprocal(pos, q, box, grid, order, alpha, virial=True):
     theta = B * np.exp(-fac * m2) / (math.pi * V * m2)
     theta[0, 0, 0] = 0.0
     e_m = theta * (S.real ** 2 + S.imag ** 2)
-    energy = 0.5 * e_m.sum()
+    # scalarSum's half spectrum: k = 0 .. nz/2, e doubled where the
+    # conjugate point lies outside it
+    kh = int(n[2]) // 2 + 1
+    k_idx = np.arange(kh)
+    dbl = np.where((k_idx > 0) & ((k_idx != int(n[2]) // 2) | (int(n[2]) & 1)),
+                   2.0, 1.0)
+    e_h = e_m[:, :, :kh] * dbl[None, None, :]
+    energy = 0.5 * e_h.sum()
     vir = np.zeros(9)
     if virial:
-        v = 2.0 * (1.0 / m2 + fac)
-        comps = [MX, MY, MZ]
+        v = 2.0 * (1.0 / m2[:, :, :kh] + fac)
+        comps = [MX[:, :, :kh], MY[:, :, :kh], MZ[:, :, :kh]]
         for a in range(3):
             for b in range(3):
                 d = 1.0 if a == b else 0.0
-                vir[3 * a + b] = 0.5 * np.sum(e_m * (d - v * comps[a] *
+                vir[3 * a + b] = 0.5 * np.sum(e_h * (d - v * comps[a] *
                                                      comps[b]))
     conv = np.fft.ifftn(theta * S).real * Q.size
     # force interpolation

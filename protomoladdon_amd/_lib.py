@@ -41,6 +41,7 @@ EXPORTED = (
     "pmnb_set_parameters_device", "pmnb_adjoint_device",
     "pmnb_lf_create", "pmnb_lf_run_device", "pmnb_lf_set_bonded",
     "pmnb_lf_forces_device", "pmnb_lf_iterations", "pmnb_lf_destroy",
+    "pmnb_lf_run_graph", "pmnb_batch_lf_create",
     "pmnb_ewald_create", "pmnb_ewald_info", "pmnb_ewald_evaluate_device",
     "pmnb_ewald_destroy", "pmnb_pme_create", "pmnb_pme_info",
     "pmnb_pme_evaluate_device", "pmnb_pme_destroy", "pmnb_lf_set_kspace",
@@ -166,6 +167,11 @@ def load():
     lib.pmnb_lf_create.restype = ctypes.c_int
     lib.pmnb_lf_run_device.argtypes = [_p, _p, _p, _p, _p, _i32, _u32, _p]
     lib.pmnb_lf_run_device.restype = ctypes.c_int
+    lib.pmnb_lf_run_graph.argtypes = [_p, _p, _p, _p, _p, _i32, _i32, _u32, _p]
+    lib.pmnb_lf_run_graph.restype = ctypes.c_int
+    lib.pmnb_batch_lf_create.argtypes = [_p, ctypes.POINTER(PmnbLfParams), _i32,
+                                         _p, _i32, _p, _p, ctypes.POINTER(_p)]
+    lib.pmnb_batch_lf_create.restype = ctypes.c_int
     lib.pmnb_lf_iterations.argtypes = [_p, _i64, _p, _i64,
                                        ctypes.POINTER(_i64),
                                        ctypes.POINTER(_i64)]

@@ -154,3 +154,94 @@ class BatchedCutoffForce:
             self.close()
         except Exception:
             pass
+
+
+class BatchedLeapfrog:
+    """LeapfrogIntegrator with SHAKE / RATTLE over every replica of a
+    BatchedCutoffForce (``pmnb_batch_lf_*``): each replica keeps its own
+    sweep counts, so its trajectory is the one a LeapfrogIntegrator gives
+    for that replica alone with the same force, bit for bit.  Positions,
+    velocities and forces are the batch's concatenated device arrays;
+    ``d_out`` holds OUT_SLOTS doubles per replica.
+
+    constraints: per replica (pairs (M, 2) with replica-local indices, rest
+    lengths (M,)), e.g. ``integrator.topology_constraints`` of each
+    replica's topology; masses: per replica arrays.
+    """
+
+    def __init__(self, timestep, batch, masses, constraints, shake=None,
+                 rattle=None):
+        from .integrator import LeapfrogIntegrator
+        if len(masses) != batch.n_replicas or len(constraints) != batch.n_replicas:
+            raise ValueError("one mass array and one constraint list per replica")
+        self.batch = batch
+        self._lib = batch._lib
+        proto = LeapfrogIntegrator(timestep, None, shake=shake, rattle=rattle)
+        prm = proto._params()
+        mass = np.ascontiguousarray(np.concatenate(masses), dtype=np.float64)
+        pairs, rest = [], []
+        for r, (pr, rl) in enumerate(constraints):
+            pairs.append(np.asarray(pr, dtype=np.int64).reshape(-1, 2) +
+                         batch.offsets[r])
+            rest.append(np.asarray(rl, dtype=np.float64))
+        pairs = np.ascontiguousarray(np.concatenate(pairs), dtype=np.int32)
+        rest = np.ascontiguousarray(np.concatenate(rest), dtype=np.float64)
+        if len(mass) != batch.n_atoms:
+            raise ValueError("masses do not cover the batch's atoms")
+        lf = ctypes.c_void_p()
+        _lib.check_batch(self._lib, batch._b, self._lib.pmnb_batch_lf_create(
+            batch._b, ctypes.byref(prm), batch.n_atoms, mass.ctypes.data,
+            len(pairs), pairs.ctypes.data if len(pairs) else None,
+            rest.ctypes.data if len(pairs) else None, ctypes.byref(lf)))
+        self._lf = lf
+
+    def _check(self, code):
+        _lib.check_batch(self._lib, self.batch._b, code)
+
+    def initialize(self, d_positions, d_forces, d_out, flags=0, stream=None):
+        """Forces and energies at the start positions (cleared first)."""
+        self._check(self._lib.pmnb_lf_forces_device(
+            self._lf, d_positions.data_ptr(), d_forces.data_ptr(),
+            d_out.data_ptr(), int(flags), self.batch._stream(stream)))
+
+    def run(self, d_positions, d_velocities, d_forces, d_out, n_steps,
+            flags=0, stream=None):
+        """LeapfrogIntegrator::run(n_steps) of every replica, eagerly."""
+        self._check(self._lib.pmnb_lf_run_device(
+            self._lf, d_positions.data_ptr(), d_velocities.data_ptr(),
+            d_forces.data_ptr(), d_out.data_ptr(), int(n_steps), int(flags),
+            self.batch._stream(stream)))
+
+    def run_graph(self, d_positions, d_velocities, d_forces, d_out, n_steps,
+                  n_replays=1, flags=0, stream=None):
+        """n_replays calls of run(n_steps) as replays of one captured HIP
+        graph."""
+        self._check(self._lib.pmnb_lf_run_graph(
+            self._lf, d_positions.data_ptr(), d_velocities.data_ptr(),
+            d_forces.data_ptr(), d_out.data_ptr(), int(n_steps),
+            int(n_replays), int(flags), self.batch._stream(stream)))
+
+    def record_iterations(self, capacity):
+        """Starts (capacity > 0) or stops (0) recording sweep counts;
+        returns (counts (calls, replicas), n_calls, n_unconverged)."""
+        R = self.batch.n_replicas
+        cap_out = 1 << 22
+        out = np.zeros(cap_out, dtype=np.int32)
+        n_calls = ctypes.c_int64()
+        n_unc = ctypes.c_int64()
+        self._check(self._lib.pmnb_lf_iterations(
+            self._lf, int(capacity), out.ctypes.data, cap_out,
+            ctypes.byref(n_calls), ctypes.byref(n_unc)))
+        n = min(n_calls.value * R, cap_out) // R
+        return out[:n * R].reshape(n, R).copy(), n_calls.value, n_unc.value
+
+    def close(self):
+        if self._lf is not None:
+            self._lib.pmnb_lf_destroy(self._lf)
+            self._lf = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

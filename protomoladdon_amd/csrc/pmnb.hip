@@ -695,84 +695,133 @@ void launch_adj_t(int T, int max_items, hipStream_t s, const ForceParams &P,
   }
 }
 
+// The per-evaluation device arrays of run_device: the context's own
+// (ctx_bufs, allocated by ensure_eval_buffers) or, for
+// pmnb_evaluate_device_noalloc, carved from the static scratch array.
+struct EvalBufs {
+  int *cell_of, *count, *start, *rank, *sorted;
+  double *sx, *sy, *sz, *sq;
+  float4 *loc;
+  int *stype, *smol, *scell;
+  int4 *srec;
+  long long *ffix;
+  double *partial;
+  unsigned long long *npairs;
+  int4 *items;
+  int *nitems, *rep_first;
+  const TileRep *reps;
+  const int *col_rep;
+  const Box *box;
+};
+
+EvalBufs ctx_bufs(const pmnb_ctx *ctx) {
+  EvalBufs B{};
+  B.cell_of = ctx->d_cell_of.as<int>();
+  B.count = ctx->d_count.as<int>();
+  B.start = ctx->d_start.as<int>();
+  B.rank = ctx->d_fill.as<int>();
+  B.sorted = ctx->d_sorted.as<int>();
+  B.sx = ctx->d_sx.as<double>();
+  B.sy = ctx->d_sy.as<double>();
+  B.sz = ctx->d_sz.as<double>();
+  B.sq = ctx->d_sq.as<double>();
+  B.loc = ctx->d_loc.as<float4>();
+  B.stype = ctx->d_stype.as<int>();
+  B.smol = ctx->d_smol.as<int>();
+  B.scell = ctx->d_scell.as<int>();
+  B.srec = ctx->d_srec.as<int4>();
+  B.ffix = ctx->d_ffix.as<long long>();
+  B.partial = ctx->d_partial.as<double>();
+  B.npairs = ctx->d_npairs.as<unsigned long long>();
+  B.items = ctx->d_items.as<int4>();
+  B.nitems = ctx->d_nitems.as<int>();
+  B.rep_first = ctx->d_rep_first.as<int>();
+  B.reps = ctx->d_reps.as<TileRep>();
+  B.col_rep = ctx->d_col_rep.as<int>();
+  B.box = ctx->d_box.as<Box>();
+  return B;
+}
+
 // On the tile path every replica's cells are numbered from its offset
 // (ctx->grid.ncell is then the total).
-int build_cells(pmnb_ctx *ctx, const double *d_pos, hipStream_t s) {
+int build_cells(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
+                hipStream_t s) {
   const int n = ctx->n, nc = ctx->grid.ncell;
-  const TileRep *reps = ctx->d_reps.as<TileRep>();
+  const TileRep *reps = B.reps;
   const int *rep_of = ctx->batch ? ctx->d_rep_of.as<int>() : nullptr;
-  int *rank = ctx->d_fill.as<int>();
+  int *rank = B.rank;
   hipLaunchKernelGGL(k_clear, dim3(blocks_for(nc)), dim3(kBlock), 0, s,
-                     ctx->d_count.as<int>(), nc,
-                     ctx->d_npairs.as<unsigned long long>());
+                     B.count, nc,
+                     B.npairs);
   if (n > 0 && ctx->use_tile) {
     hipLaunchKernelGGL(k_assign_r<TileRep>, dim3(blocks_for(n)), dim3(kBlock),
-                       0, s, reps, rep_of, d_pos, n, ctx->d_cell_of.as<int>(),
-                       ctx->d_count.as<int>(), rank);
+                       0, s, reps, rep_of, d_pos, n, B.cell_of,
+                       B.count, rank);
   } else if (n > 0) {
     hipLaunchKernelGGL(k_assign, dim3(blocks_for(n)), dim3(kBlock), 0, s,
-                       ctx->box, ctx->grid, d_pos, n, ctx->d_cell_of.as<int>(),
-                       ctx->d_count.as<int>(), rank);
+                       ctx->box, ctx->grid, d_pos, n, B.cell_of,
+                       B.count, rank);
   }
   if (ctx->use_tile) {
     // starts and the work items of the tile path in one block
     hipLaunchKernelGGL(k_scan_items, dim3(1), dim3(1024), 0, s,
-                       ctx->d_count.as<int>(), ctx->d_start.as<int>(), nc,
-                       reps, ctx->d_col_rep.as<int>(), ctx->ncol_total,
-                       (int)ctx->reps.size(), ctx->d_items.as<int4>(),
-                       ctx->d_nitems.as<int>(), ctx->d_rep_first.as<int>());
+                       B.count, B.start, nc,
+                       reps, B.col_rep, ctx->ncol_total,
+                       (int)ctx->reps.size(), B.items,
+                       B.nitems, B.rep_first);
   } else {
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s,
-                       ctx->d_count.as<int>(), ctx->d_start.as<int>(), nc);
+                       B.count, B.start, nc);
   }
   if (n > 0) {
     hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, s,
-                       ctx->d_cell_of.as<int>(), n, ctx->d_start.as<int>(),
-                       rank, ctx->d_sorted.as<int>());
+                       B.cell_of, n, B.start,
+                       rank, B.sorted);
   }
   if (n > 0 && ctx->use_tile) {
     hipLaunchKernelGGL(k_sort_gather_r<TileRep>,
                        dim3(blocks_for(nc, kBlock / 64)), dim3(kBlock), 0, s,
-                       reps, rep_of, d_pos, ctx->d_start.as<int>(),
-                       ctx->d_count.as<int>(), nc, ctx->d_type.as<int>(),
+                       reps, rep_of, d_pos, B.start,
+                       B.count, nc, ctx->d_type.as<int>(),
                        ctx->d_q.as<double>(), ctx->d_mol.as<int>(),
-                       ctx->d_sorted.as<int>(), ctx->d_sx.as<double>(),
-                       ctx->d_sy.as<double>(), ctx->d_sz.as<double>(),
-                       ctx->d_loc.as<float4>(), ctx->d_stype.as<int>(),
-                       ctx->d_sq.as<double>(), ctx->d_smol.as<int>(),
-                       ctx->d_scell.as<int>(), ctx->d_srec.as<int4>());
+                       B.sorted, B.sx,
+                       B.sy, B.sz,
+                       B.loc, B.stype,
+                       B.sq, B.smol,
+                       B.scell, B.srec);
   } else if (n > 0) {
     hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(nc, kBlock / 64)),
                        dim3(kBlock), 0, s,
-                       ctx->d_start.as<int>(), ctx->d_count.as<int>(), nc,
-                       ctx->d_sorted.as<int>());
+                       B.start, B.count, nc,
+                       B.sorted);
     hipLaunchKernelGGL(k_gather, dim3(blocks_for(n)), dim3(kBlock), 0, s,
-                       ctx->box, ctx->grid, d_pos, ctx->d_sorted.as<int>(),
-                       ctx->d_cell_of.as<int>(), ctx->d_type.as<int>(),
+                       ctx->box, ctx->grid, d_pos, B.sorted,
+                       B.cell_of, ctx->d_type.as<int>(),
                        ctx->d_q.as<double>(), ctx->d_mol.as<int>(), n,
-                       ctx->d_sx.as<double>(), ctx->d_sy.as<double>(),
-                       ctx->d_sz.as<double>(), ctx->d_loc.as<float4>(),
-                       ctx->d_stype.as<int>(), ctx->d_sq.as<double>(),
-                       ctx->d_smol.as<int>(), ctx->d_scell.as<int>());
+                       B.sx, B.sy,
+                       B.sz, B.loc,
+                       B.stype, B.sq,
+                       B.smol, B.scell);
   }
   HIPCHK(hipGetLastError());
   return PMNB_OK;
 }
 
-Sorted make_sorted(const pmnb_ctx *ctx, const double *d_com) {
+Sorted make_sorted(const pmnb_ctx *ctx, const EvalBufs &B,
+                   const double *d_com) {
   Sorted S{};
-  S.x = ctx->d_sx.as<double>();
-  S.y = ctx->d_sy.as<double>();
-  S.z = ctx->d_sz.as<double>();
-  S.loc = ctx->d_loc.as<float4>();
-  S.atom = ctx->d_sorted.as<int>();
-  S.type = ctx->d_stype.as<int>();
-  S.q = ctx->d_sq.as<double>();
-  S.mol = ctx->d_smol.as<int>();
-  S.cell = ctx->d_scell.as<int>();
-  S.rec = ctx->d_srec.as<int4>();
-  S.cstart = ctx->d_start.as<int>();
-  S.ccount = ctx->d_count.as<int>();
+  S.x = B.sx;
+  S.y = B.sy;
+  S.z = B.sz;
+  S.loc = B.loc;
+  S.atom = B.sorted;
+  S.type = B.stype;
+  S.q = B.sq;
+  S.mol = B.smol;
+  S.cell = B.scell;
+  S.rec = B.srec;
+  S.cstart = B.start;
+  S.ccount = B.count;
   S.delta = ctx->d_shell.as<int4>();
   S.xfull = ctx->d_xfull.as<unsigned long long>();
   S.xmod = ctx->d_xmod.as<unsigned long long>();
@@ -780,7 +829,7 @@ Sorted make_sorted(const pmnb_ctx *ctx, const double *d_com) {
   S.lj = ctx->d_lj.as<double>();
   S.ljf = ctx->d_ljf.as<float4>();
   S.com = d_com;
-  S.box = ctx->d_box.as<Box>();
+  S.box = B.box;
   return S;
 }
 
@@ -793,11 +842,10 @@ int check_ready(pmnb_ctx *ctx) {
   return PMNB_OK;
 }
 
-int run_device(pmnb_ctx *ctx, const double *d_pos, double *d_forces,
-               double *d_out, const double *d_com, uint32_t flags,
-               hipStream_t s) {
-  int rc = ensure_eval_buffers(ctx);
-  if (rc) return rc;
+int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
+                    double *d_forces, double *d_out, const double *d_com,
+                    uint32_t flags, hipStream_t s) {
+  int rc = PMNB_OK;
   if ((flags & PMNB_FLAG_MOLVIRIAL) && !d_com)
     return fail(ctx, PMNB_ERR_ARG, "molecular virial needs molecule centres");
   // timing: [start, built) = cell build (+ work items or neighbour lists),
@@ -814,14 +862,14 @@ int run_device(pmnb_ctx *ctx, const double *d_pos, double *d_forces,
     ev_stop = ctx->ev_pool[ctx->ev_used + 2];
     ctx->ev_used += 3;
   }
-  rc = build_cells(ctx, d_pos, s);
+  rc = build_cells(ctx, B, d_pos, s);
   if (rc) return rc;
   const ForceParams P = make_params(ctx, flags);
-  const Sorted S = make_sorted(ctx, d_com);
+  const Sorted S = make_sorted(ctx, B, d_com);
   const int nb = std::max(1, blocks_for(ctx->n));
-  double *part = ctx->d_partial.as<double>();
+  double *part = B.partial;
   double *part2 = ctx->d_partial2.as<double>();
-  unsigned long long *np = ctx->d_npairs.as<unsigned long long>();
+  unsigned long long *np = B.npairs;
   const bool mixed = ctx->prm.precision == PMNB_PREC_MIXED;
   const int el = ctx->prm.elec, sw = ctx->prm.elec_switch;
   const bool list = ctx->cap > 0;
@@ -831,11 +879,11 @@ int run_device(pmnb_ctx *ctx, const double *d_pos, double *d_forces,
   if (ctx->use_tile) {
     const int ncol = ctx->ncol_total, nrep = (int)ctx->reps.size();
     const int max_items = ctx->n / kTileAtoms + ncol + 1;
-    const TileRep *reps = ctx->d_reps.as<TileRep>();
-    int4 *items = ctx->d_items.as<int4>();
-    int *nitems = ctx->d_nitems.as<int>();
-    int *rep_first = ctx->d_rep_first.as<int>();
-    long long *ffix = ctx->d_ffix.as<long long>();
+    const TileRep *reps = B.reps;
+    int4 *items = B.items;
+    int *nitems = B.nitems;
+    int *rep_first = B.rep_first;
+    long long *ffix = B.ffix;
     if (ev_built) HIPCHK(hipEventRecord(ev_built, s));
     if (ctx->n > 0) {
       const bool single = !ctx->batch && nrep == 1 &&
@@ -933,6 +981,15 @@ int run_device(pmnb_ctx *ctx, const double *d_pos, double *d_forces,
                      list ? part2 : part, list ? nb : 0, d_out);
   HIPCHK(hipGetLastError());
   return PMNB_OK;
+}
+
+int run_device(pmnb_ctx *ctx, const double *d_pos, double *d_forces,
+               double *d_out, const double *d_com, uint32_t flags,
+               hipStream_t s) {
+  const int rc = ensure_eval_buffers(ctx);
+  if (rc) return rc;
+  return run_device_bufs(ctx, ctx_bufs(ctx), d_pos, d_forces, d_out, d_com,
+                         flags, s);
 }
 
 }  // namespace
@@ -1223,13 +1280,13 @@ int pmnb_pairset(pmnb_ctx *ctx, const double *positions, int32_t *pairs,
                           hipMemcpyHostToDevice, s));
   rc = ensure_eval_buffers(ctx);
   if (rc) return rc;
-  rc = build_cells(ctx, ctx->d_pos.as<double>(), s);
+  rc = build_cells(ctx, ctx_bufs(ctx), ctx->d_pos.as<double>(), s);
   if (rc) return rc;
   const long long cap = pairs ? capacity : 0;
   HIPCHK(ctx->d_pairs.ensure((size_t)std::max<long long>(cap, 1) * 3 *
                              sizeof(int)));
   const ForceParams P = make_params(ctx, 0);
-  const Sorted S = make_sorted(ctx, nullptr);
+  const Sorted S = make_sorted(ctx, ctx_bufs(ctx), nullptr);
   if (ctx->n > 0)
     hipLaunchKernelGGL(k_pairset, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, s,
                        P, S, ctx->d_pairs.as<int>(), cap,
@@ -1459,10 +1516,10 @@ int pmnb_adjoint_device(pmnb_ctx *ctx, const double *d_positions,
   const int max_items = ctx->n / kTileAtoms + ncol + 1;
   HIPCHK(ctx->d_partial_lj.ensure((size_t)max_items * 10 * 4 *
                                   sizeof(double)));
-  rc = build_cells(ctx, d_positions, s);
+  rc = build_cells(ctx, ctx_bufs(ctx), d_positions, s);
   if (rc) return rc;
   const ForceParams P = make_params(ctx, 0);
-  const Sorted S = make_sorted(ctx, nullptr);
+  const Sorted S = make_sorted(ctx, ctx_bufs(ctx), nullptr);
   const TileRep *reps = ctx->d_reps.as<TileRep>();
   int4 *items = ctx->d_items.as<int4>();
   int *nitems = ctx->d_nitems.as<int>();
@@ -1834,6 +1891,20 @@ struct pmnb_leapfrog {
   // k-space terms evaluated with the force (pmnb_lf_set_kspace)
   pmnb_pme *pme = nullptr;
   pmnb_ewald *ewald = nullptr;
+  // replicas (one for a single system): constraint-group segments, the
+  // replica of each group block, energy-block doubles cleared per force
+  int n_reps = 1, n_out = 20;
+  bool batch = false;
+  DevBuf reps, blk_rep;
+  // captured run (pmnb_lf_run_graph) and what it was captured for
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  const void *gkey[4] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t gflags = 0;
+  int gsteps = 0, gshape = 0, gcap = 0;
+  unsigned long long gtopo = 0;
+  double gbox[12] = {0};
 };
 
 namespace {
@@ -1860,7 +1931,10 @@ LfDev lf_dev(pmnb_leapfrog *lf, double *x, double *v, double *f,
   d.n_free = lf->nf;
   d.ngb = lf->ngb;
   d.n_cons = lf->n_cons;
-  d.n_out = (lf->n_bonds + lf->n_angles) > 0 ? 22 : 20;
+  d.n_out = lf->n_out + ((lf->n_bonds + lf->n_angles) > 0 ? 2 : 0);
+  d.reps = lf->reps.as<LfRep>();
+  d.blk_rep = lf->blk_rep.as<int>();
+  d.n_reps = lf->n_reps;
   // LeapfrogIntegrator / StandardIntegrator use timestep * INV_TIMEFACTOR,
   // the modifiers timestep / TIMEFACTOR.
   d.h = lf->p.timestep * kInvTimeFactor;
@@ -2012,13 +2086,13 @@ int lf_calc_forces(pmnb_leapfrog *lf, const double *d_pos, double *d_force,
   return PMNB_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int pmnb_lf_create(pmnb_ctx *ctx, const pmnb_lf_params *params,
+// The integrator over ctx's atoms, split into replicas at the atom indices
+// rep_first (R + 1 ascending entries, rep_first[0] = 0, rep_first[R] =
+// n_atoms); a constraint must join two atoms of one replica.
+int lf_create_impl(pmnb_ctx *ctx, const pmnb_lf_params *params,
                    int32_t n_atoms, const double *mass, int32_t n_constraints,
                    const int32_t *constraint_atoms, const double *rest_length,
+                   const std::vector<int> &rep_first, bool batch,
                    pmnb_leapfrog **out) {
   if (!ctx || !params || !out) return PMNB_ERR_ARG;
   *out = nullptr;
@@ -2051,6 +2125,18 @@ int pmnb_lf_create(pmnb_ctx *ctx, const pmnb_lf_params *params,
     if (!(rest_length[c] > 0))
       return fail(ctx, PMNB_ERR_ARG, "constraint %d has rest length <= 0", c);
     parent[root(a)] = root(b);
+  }
+  const int R = (int)rep_first.size() - 1;
+  auto rep_of = [&](int a) {
+    return (int)(std::upper_bound(rep_first.begin(), rep_first.end(), a) -
+                 rep_first.begin()) - 1;
+  };
+  std::vector<int> rep_cons(std::max(R, 1), 0);
+  for (int c = 0; c < n_constraints; c++) {
+    const int ra = rep_of(constraint_atoms[2 * c]);
+    if (ra != rep_of(constraint_atoms[2 * c + 1]))
+      return fail(ctx, PMNB_ERR_ARG, "constraint %d joins two replicas", c);
+    rep_cons[ra]++;
   }
   std::vector<char> constrained(n_atoms, 0);
   for (int c = 0; c < 2 * n_constraints; c++) constrained[constraint_atoms[c]] = 1;
@@ -2118,6 +2204,25 @@ int pmnb_lf_create(pmnb_ctx *ctx, const pmnb_lf_params *params,
     if (!constrained[i]) h_free.push_back(i);
   std::vector<double> h_rmass(n_atoms);
   for (int i = 0; i < n_atoms; i++) h_rmass[i] = 1 / mass[i];
+  // replica segments: groups are ordered by their lowest atom and replicas
+  // are contiguous atom ranges, so each replica's groups are contiguous
+  std::vector<LfRep> h_reps(std::max(R, 1), LfRep{0, 0, 0, 0, 0});
+  {
+    int g = 0, blk = 0;
+    for (int r = 0; r < R; r++) {
+      LfRep &q = h_reps[r];
+      q.goff = g;
+      while (g < ng && rep_of(gat[g][0]) == r) g++;
+      q.ng = g - q.goff;
+      q.gb0 = blk;
+      q.gnb = blocks_for(q.ng, kLfBlock);
+      q.ncons = rep_cons[r];
+      blk += q.gnb;
+    }
+  }
+  std::vector<int> h_blk_rep;
+  for (int r = 0; r < R; r++)
+    for (int k = 0; k < h_reps[r].gnb; k++) h_blk_rep.push_back(r);
 
   pmnb_leapfrog *lf = new pmnb_leapfrog;
   lf->ctx = ctx;
@@ -2130,8 +2235,11 @@ int pmnb_lf_create(pmnb_ctx *ctx, const pmnb_lf_params *params,
   lf->n = n_atoms;
   lf->ng = ng;
   lf->nf = (int)h_free.size();
-  lf->ngb = blocks_for(ng, kLfBlock);
+  lf->ngb = (int)h_blk_rep.size();
   lf->nfb = blocks_for(lf->nf, kLfBlock);
+  lf->n_reps = std::max(R, 1);
+  lf->n_out = batch ? 20 * R : 20;
+  lf->batch = batch;
   lf->n_cons = n_constraints;
   lf->shape = shape;
   auto up = [&](DevBuf &b, const void *src, size_t bytes) -> hipError_t {
@@ -2162,8 +2270,13 @@ int pmnb_lf_create(pmnb_ctx *ctx, const pmnb_lf_params *params,
                              sizeof(double));
   if (e == hipSuccess) e = lf->nonconv.ensure(4 * sizeof(int));
   if (e == hipSuccess) e = hipMemset(lf->nonconv.p, 0, 4 * sizeof(int));
-  if (e == hipSuccess) e = lf->hint.ensure(8 * sizeof(int));
-  if (e == hipSuccess) e = hipMemset(lf->hint.p, 0xff, 8 * sizeof(int));  // -1
+  if (e == hipSuccess)
+    e = up(lf->reps, h_reps.data(), h_reps.size() * sizeof(LfRep));
+  if (e == hipSuccess)
+    e = up(lf->blk_rep, h_blk_rep.data(), h_blk_rep.size() * sizeof(int));
+  const size_t hint_bytes = (size_t)8 * lf->n_reps * sizeof(int);
+  if (e == hipSuccess) e = lf->hint.ensure(hint_bytes);
+  if (e == hipSuccess) e = hipMemset(lf->hint.p, 0xff, hint_bytes);  // -1
   if (e != hipSuccess) {
     pmnb_lf_destroy(lf);
     return fail(ctx, PMNB_ERR_HIP, "leapfrog set-up failed: %s",
@@ -2171,6 +2284,29 @@ int pmnb_lf_create(pmnb_ctx *ctx, const pmnb_lf_params *params,
   }
   *out = lf;
   return PMNB_OK;
+}
+
+void lf_drop_graph(pmnb_leapfrog *lf) {
+  if (lf->gexec) (void)hipGraphExecDestroy(lf->gexec);
+  if (lf->graph) (void)hipGraphDestroy(lf->graph);
+  lf->gexec = nullptr;
+  lf->graph = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pmnb_lf_create(pmnb_ctx *ctx, const pmnb_lf_params *params,
+                   int32_t n_atoms, const double *mass, int32_t n_constraints,
+                   const int32_t *constraint_atoms, const double *rest_length,
+                   pmnb_leapfrog **out) {
+  if (ctx && ctx->batch)
+    return fail(ctx, PMNB_ERR_ARG, "a batch's integrator is made by "
+                                   "pmnb_batch_lf_create");
+  return lf_create_impl(ctx, params, n_atoms, mass, n_constraints,
+                        constraint_atoms, rest_length,
+                        std::vector<int>{0, n_atoms}, false, out);
 }
 
 int pmnb_lf_run_device(pmnb_leapfrog *lf, double *d_positions,
@@ -2252,13 +2388,16 @@ int pmnb_lf_iterations(pmnb_leapfrog *lf, int64_t capacity, int32_t *iters,
   HIPCHK(hipStreamSynchronize(ctx->stream));
   HIPCHK(hipDeviceSynchronize());
   if (n_calls) *n_calls = lf->calls;
-  const long long have = std::min<long long>(lf->calls, lf->iters_cap);
+  const long long have = std::min<long long>(lf->calls * lf->n_reps,
+                                             lf->iters_cap);
   const long long n_out = std::min<long long>(have, std::max<int64_t>(capacity_out, 0));
   if (iters && n_out > 0)
     HIPCHK(hipMemcpy(iters, lf->iters.p, n_out * sizeof(int), hipMemcpyDeviceToHost));
   int nc[4] = {0, 0, 0, 0};
   HIPCHK(hipMemcpy(nc, lf->nonconv.p, sizeof nc, hipMemcpyDeviceToHost));
   if (n_unconverged) *n_unconverged = (int64_t)nc[kStShake] + nc[kStRattle];
+  // a captured run holds the record's address and capacity
+  lf_drop_graph(lf);
   if (capacity > 0) HIPCHK(lf->iters.ensure((size_t)capacity * sizeof(int)));
   lf->iters_cap = capacity > 0 ? capacity : 0;
   lf->calls = 0;
@@ -2271,13 +2410,15 @@ void pmnb_lf_destroy(pmnb_leapfrog *lf) {
   // the context may already be destroyed: only the device is used here
   (void)hipSetDevice(lf->device);
   (void)hipDeviceSynchronize();
+  lf_drop_graph(lf);
+  if (lf->cap_stream) (void)hipStreamDestroy(lf->cap_stream);
   DevBuf *bufs[] = {&lf->mass, &lf->rmass, &lf->g_atoms, &lf->g_cons,
                     &lf->g_rest, &lf->g_rinv, &lf->free_atoms, &lf->last,
                     &lf->xs, &lf->vs, &lf->partial_s, &lf->partial_r,
                     &lf->iters, &lf->nonconv, &lf->hint, &lf->bptr,
                     &lf->bent, &lf->aptr, &lf->aent, &lf->bonds, &lf->bk,
                     &lf->br0, &lf->angles, &lf->ak, &lf->at0, &lf->ubk,
-                    &lf->ubr0};
+                    &lf->ubr0, &lf->reps, &lf->blk_rep};
   for (DevBuf *b : bufs) b->release();
   delete lf;
 }
@@ -2291,6 +2432,10 @@ int pmnb_lf_set_bonded(pmnb_leapfrog *lf, int32_t n_bonds,
   if (!lf) return PMNB_ERR_ARG;
   pmnb_ctx *ctx = lf->ctx;
   const int n = lf->n;
+  if (lf->batch && (n_bonds > 0 || n_angles > 0))
+    return fail(ctx, PMNB_ERR_UNSUPPORTED,
+                "a batch's integrator takes no bonded terms");
+  lf_drop_graph(lf);
   if (n_bonds < 0 || n_angles < 0 ||
       (n_bonds > 0 && (!bond_atoms || !bond_k || !bond_r0)) ||
       (n_angles > 0 && (!angle_atoms || !angle_k || !angle_theta0)))
@@ -2363,8 +2508,102 @@ int pmnb_lf_forces_device(pmnb_leapfrog *lf, const double *d_positions,
   if (lf->n == 0) return PMNB_OK;
   launch_fill_i32(reinterpret_cast<int *>(d_forces), lf->n * 6, 0, s);
   launch_fill_i32(reinterpret_cast<int *>(d_out),
-                  (lf->n_bonds + lf->n_angles) > 0 ? 44 : 40, 0, s);
+                  2 * (lf->n_out + ((lf->n_bonds + lf->n_angles) > 0 ? 2 : 0)),
+                  0, s);
   return lf_calc_forces(lf, d_positions, d_forces, d_out, flags, true, s);
+}
+
+int pmnb_lf_run_graph(pmnb_leapfrog *lf, double *d_positions,
+                      double *d_velocities, double *d_forces, double *d_out,
+                      int32_t n_steps, int32_t n_replays, uint32_t flags,
+                      void *stream) {
+  if (!lf || n_steps < 0 || n_replays < 0) return PMNB_ERR_ARG;
+  pmnb_ctx *ctx = lf->ctx;
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (ctx->n != lf->n)
+    return fail(ctx, PMNB_ERR_STATE, "topology changed since the integrator "
+                                     "was created");
+  if (n_steps == 0 || n_replays == 0 || lf->n == 0) return PMNB_OK;
+  if (!d_positions || !d_velocities || !d_forces || !d_out)
+    return fail(ctx, PMNB_ERR_ARG, "null device pointer");
+  if (flags & PMNB_FLAG_MOLVIRIAL)
+    return fail(ctx, PMNB_ERR_UNSUPPORTED,
+                "the molecular virial needs molecule centres per step");
+  if (lf->pme || lf->ewald)
+    return fail(ctx, PMNB_ERR_UNSUPPORTED,
+                "k-space terms are not captured; use pmnb_lf_run_device");
+  if (ctx->timing)
+    return fail(ctx, PMNB_ERR_STATE, "kernel timing is on: no capture");
+  HIPCHK(hipSetDevice(ctx->prm.device));
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  const bool same =
+      lf->gexec && lf->gkey[0] == d_positions && lf->gkey[1] == d_velocities &&
+      lf->gkey[2] == d_forces && lf->gkey[3] == d_out && lf->gflags == flags &&
+      lf->gsteps == n_steps && lf->gtopo == ctx->topo_gen &&
+      std::equal(lf->gbox, lf->gbox + 12, ctx->box_key) &&
+      lf->gshape == ctx->shape && lf->gcap == ctx->grid_cap;
+  if (!same) {
+    lf_drop_graph(lf);
+    // every allocation, upload and first-use clear of the run happens
+    // before the capture, and SHAKE's previous positions are set here
+    HIPCHK(hipStreamSynchronize(s));
+    rc = ensure_eval_buffers(ctx);
+    if (rc) return rc;
+    if (!lf->last_ready) {
+      HIPCHK(hipMemcpyAsync(lf->last.p, d_positions,
+                            (size_t)lf->n * 3 * sizeof(double),
+                            hipMemcpyDeviceToDevice, s));
+      lf->last_ready = true;
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (!lf->cap_stream)
+      HIPCHK(hipStreamCreateWithFlags(&lf->cap_stream, hipStreamNonBlocking));
+    hipStream_t cs = lf->cap_stream;
+    HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    rc = pmnb_lf_run_device(lf, d_positions, d_velocities, d_forces, d_out,
+                            n_steps, flags, cs);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(cs, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    HIPCHK(e);
+    lf->graph = g;
+    HIPCHK(hipGraphInstantiate(&lf->gexec, g, nullptr, nullptr, 0));
+    lf->gkey[0] = d_positions;
+    lf->gkey[1] = d_velocities;
+    lf->gkey[2] = d_forces;
+    lf->gkey[3] = d_out;
+    lf->gflags = flags;
+    lf->gsteps = n_steps;
+    lf->gtopo = ctx->topo_gen;
+    std::copy(ctx->box_key, ctx->box_key + 12, lf->gbox);
+    lf->gshape = ctx->shape;
+    lf->gcap = ctx->grid_cap;
+  }
+  for (int k = 0; k < n_replays; k++) HIPCHK(hipGraphLaunch(lf->gexec, s));
+  return PMNB_OK;
+}
+
+int pmnb_batch_lf_create(pmnb_batch *b, const pmnb_lf_params *params,
+                         int32_t n_atoms, const double *mass,
+                         int32_t n_constraints,
+                         const int32_t *constraint_atoms,
+                         const double *rest_length, pmnb_leapfrog **out) {
+  if (!b || !out) return PMNB_ERR_ARG;
+  *out = nullptr;
+  int rc = batch_finalize(b);
+  if (rc) return rc;
+  std::vector<int> first{0};
+  const std::vector<int> &ro = b->rep_of;
+  for (size_t i = 1; i < ro.size(); i++)
+    if (ro[i] != ro[i - 1]) first.push_back((int)i);
+  first.push_back((int)ro.size());
+  return lf_create_impl(b->ctx, params, n_atoms, mass, n_constraints,
+                        constraint_atoms, rest_length, first, true, out);
 }
 
 }  // extern "C"
@@ -3016,6 +3255,7 @@ int pmnb_lf_set_kspace(pmnb_leapfrog *lf, pmnb_pme *pme, pmnb_ewald *ewald) {
   if ((pme && pme->ctx != lf->ctx) || (ewald && ewald->ctx != lf->ctx))
     return fail(lf->ctx, PMNB_ERR_ARG,
                 "the k-space terms belong to another context");
+  lf_drop_graph(lf);
   lf->pme = pme;
   lf->ewald = ewald;
   return PMNB_OK;

@@ -39,6 +39,12 @@
 // the reference's only when the mean error is within rounding of epsilon.
 // The positions and velocities of a given sweep count are the reference's
 // arithmetic (FP contraction is off for the whole library).
+//
+// Replicas.  A batch of independent systems (pmnb_batch) runs as one
+// integrator: each replica's groups occupy their own run of blocks (the
+// replica's first group block gb0, gnb blocks), its errors are summed over
+// those blocks only and its sweep count, hint and iteration record are its
+// own, as when the replica runs alone.  A single system is one replica.
 #pragma once
 
 namespace pmnb {
@@ -81,8 +87,14 @@ enum LfPre {
   kPreFullDrift = 4,  // doKickdoDrift without modifiers: v += F*h/m; x += v*h
 };
 
+// One replica's constraint groups: blocks [gb0, gb0 + gnb) hold its groups
+// goff .. goff + ng - 1, 256 to a block; ncons constraints.
+struct LfRep {
+  int gb0, gnb, goff, ng, ncons;
+};
+
 struct LfDev {
-  double *x, *v, *f, *out;  // positions, velocities, forces, 20-double block
+  double *x, *v, *f, *out;  // positions, velocities, forces, energy blocks
   double *last;             // SHAKE's previous positions
   const double *mass, *rmass;
   const int *g_atoms;       // NA per group, -1 padded
@@ -90,16 +102,20 @@ struct LfDev {
   const double *g_rest;     // NC per group: rest length
   const double *g_rinv;     // NC per group: 1 / (2 rest^2)
   const int *free_atoms;
-  int n_groups, n_free, ngb;  // ngb: blocks of group threads
+  int n_groups, n_free, ngb;  // ngb: blocks of group threads (all replicas)
   int n_cons;
-  int n_out;                // doubles of the energy block cleared with forces
+  int n_out;                // doubles of the energy blocks cleared with forces
+  const LfRep *reps;        // per replica
+  const int *blk_rep;       // replica of each group block
+  int n_reps;
   double h, h2, dtc;        // ts/TIMEFACTOR-scaled steps (see host)
   double eps[3];            // by stage
   int maxit[3];
   double *partial[3];       // by stage: (maxit + 1) levels x ngb blocks
-  int *hint;                // [stage * 2 + call parity]: last sweep count
+  int *hint;                // [8 rep + 2 stage + call parity]: sweep count
   double *xs, *vs;          // state after the first probe (per atom)
-  int *iters;               // sweep count per stage call (optional)
+  int *iters;               // sweep count per stage call and replica
+                            // ([call * n_reps + rep], optional)
   long long iters_cap;
   int *nonconv;             // [3] stage calls that hit max_iter
 };
@@ -302,25 +318,26 @@ __device__ __forceinline__ void pre_op(D3 &x, D3 &v, D3 f, double m,
 // sweep count plus a margin (the same value in every kernel of the call:
 // the apply kernel of call c writes slot c & 1, calls read slot (c + 1) & 1).
 __device__ __forceinline__ int lf_first_levels(const LfDev &d, int stage,
-                                               long long scall) {
-  const int hint = d.hint[stage * 2 + (int)((scall + 1) & 1)];
+                                               long long scall, int rep) {
+  const int hint = d.hint[8 * rep + stage * 2 + (int)((scall + 1) & 1)];
   const int L = hint < 0 ? kLfFirstLevels : hint + kLfMargin + 1;
   return min(max(L, 1), d.maxit[stage] + 1);
 }
 
-// First level in [j0, j1) whose mean error over all constraints is <=
-// epsilon, or -1.  Block-wide; every block sums the per-block partials in
-// the same order, so every block returns the same level.  A thread sums
-// blocks sub, sub + 8, ... of one level; its loads are issued in batches
-// before the (ordered) additions so that they overlap.
+// First level in [j0, j1) whose mean error over the replica's constraints
+// is <= epsilon, or -1.  Block-wide; every block of the replica sums its
+// per-block partials in the same order, so they all return the same level.
+// A thread sums blocks sub, sub + 8, ... (relative to the replica's first)
+// of one level; its loads are issued in batches before the (ordered)
+// additions so that they overlap.
 __device__ __forceinline__ int lf_scan(const LfDev &d, int stage, int j0,
-                                       int j1) {
+                                       int j1, const LfRep &R) {
   constexpr int kSub = 8, kChunk = kLfBlock / kSub, kBatch = 8;
   __shared__ double red[kChunk][kSub];
   __shared__ int hit;
   const int t = threadIdx.x;
-  const double *P = d.partial[stage];
-  const int nb = d.ngb;
+  const double *P = d.partial[stage] + R.gb0;
+  const int nb = R.gnb;
   for (int c0 = j0; c0 < j1; c0 += kChunk) {
     const int n = min(kChunk, j1 - c0);
     __syncthreads();
@@ -328,7 +345,7 @@ __device__ __forceinline__ int lf_scan(const LfDev &d, int stage, int j0,
       const int jj = t % kChunk, sub = t / kChunk;
       double s = 0;
       if (jj < n) {
-        const double *row = P + (size_t)(c0 + jj) * nb;
+        const double *row = P + (size_t)(c0 + jj) * d.ngb;
         for (int q0 = sub; q0 < nb; q0 += kSub * kBatch) {
           double v[kBatch];
 #pragma unroll
@@ -348,7 +365,7 @@ __device__ __forceinline__ int lf_scan(const LfDev &d, int stage, int j0,
       for (int jj = 0; jj < n; jj++) {
         double s = 0;
         for (int k = 0; k < kSub; k++) s += red[jj][k];
-        if (s / d.n_cons <= d.eps[stage]) {
+        if (s / R.ncons <= d.eps[stage]) {
           hit = c0 + jj;
           break;
         }
@@ -364,11 +381,12 @@ __device__ __forceinline__ int lf_scan(const LfDev &d, int stage, int j0,
 // Sweep count of a stage call: the first level j (0..maxit) whose mean
 // error is <= epsilon, else maxit + 1.
 __device__ __forceinline__ int lf_decide(const LfDev &d, int stage,
-                                         long long scall) {
-  const int L = lf_first_levels(d, stage, scall);
-  const int k = lf_scan(d, stage, 0, L);
+                                         long long scall, const LfRep &R,
+                                         int rep) {
+  const int L = lf_first_levels(d, stage, scall, rep);
+  const int k = lf_scan(d, stage, 0, L, R);
   if (k >= 0) return k;
-  const int k2 = lf_scan(d, stage, L, d.maxit[stage] + 1);
+  const int k2 = lf_scan(d, stage, L, d.maxit[stage] + 1, R);
   return k2 >= 0 ? k2 : d.maxit[stage] + 1;
 }
 
@@ -449,16 +467,24 @@ __global__ __launch_bounds__(kLfBlock) void k_lf_step(LfDev d, long long sc_appl
                                                       long long call) {
   using SH = LfShape<SHAPE>;
   int k = 0;
-  if (APPLY != kStNone) {
-    k = lf_decide(d, APPLY, sc_apply);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      d.hint[APPLY * 2 + (int)(sc_apply & 1)] = k;
-      if (d.iters && call < d.iters_cap) d.iters[call] = k;
+  const bool gblock = (int)blockIdx.x < d.ngb;
+  const int rep = gblock ? d.blk_rep[blockIdx.x] : 0;
+  const LfRep R = gblock ? d.reps[rep] : LfRep{0, 0, 0, 0, 0};
+  if (APPLY != kStNone && gblock) {
+    k = lf_decide(d, APPLY, sc_apply, R, rep);
+    if ((int)blockIdx.x == R.gb0 && threadIdx.x == 0) {
+      d.hint[8 * rep + APPLY * 2 + (int)(sc_apply & 1)] = k;
+      const long long slot = call * d.n_reps + rep;
+      if (d.iters && slot < d.iters_cap) d.iters[slot] = k;
       if (k > d.maxit[APPLY]) atomicAdd(&d.nonconv[APPLY], 1);
     }
   }
-  if (ZERO && blockIdx.x == 0 && (int)threadIdx.x < d.n_out) d.out[threadIdx.x] = 0;
-  if ((int)blockIdx.x >= d.ngb) {
+  if (ZERO) {
+    for (int t = blockIdx.x * kLfBlock + threadIdx.x; t < d.n_out;
+         t += gridDim.x * kLfBlock)
+      d.out[t] = 0;
+  }
+  if (!gblock) {
     const int t = ((int)blockIdx.x - d.ngb) * kLfBlock + threadIdx.x;
     if (t >= d.n_free) return;
     const int i = d.free_atoms[t];
@@ -471,11 +497,11 @@ __global__ __launch_bounds__(kLfBlock) void k_lf_step(LfDev d, long long sc_appl
     if (ZERO) st3(d.f, i, D3{0, 0, 0});
     return;
   }
-  const int gi = blockIdx.x * kLfBlock + threadIdx.x;
-  const bool act = gi < d.n_groups;
+  const int local = ((int)blockIdx.x - R.gb0) * kLfBlock + threadIdx.x;
+  const bool act = local < R.ng;
   LfGroup<SH> g;
-  // inactive threads hold a copy of the last group (read only)
-  load_group(d, act ? gi : d.n_groups - 1, g,
+  // inactive threads hold a copy of the replica's last group (read only)
+  load_group(d, R.goff + (act ? local : R.ng - 1), g,
              APPLY == kStShake || PROBE == kStShake);
   if (act) {
     for (int s = 0; s < k; s++) sweep<APPLY>(g, d.dtc, true);
@@ -496,7 +522,7 @@ __global__ __launch_bounds__(kLfBlock) void k_lf_step(LfDev d, long long sc_appl
     }
   }
   if (PROBE != kStNone) {
-    const int L = lf_first_levels(d, PROBE, sc_probe);
+    const int L = lf_first_levels(d, PROBE, sc_probe, rep);
     lf_probe<PROBE>(d, g, act, 0, L, L <= d.maxit[PROBE]);
   }
 }
@@ -506,13 +532,15 @@ __global__ __launch_bounds__(kLfBlock) void k_lf_step(LfDev d, long long sc_appl
 template <int STAGE, int SHAPE>
 __global__ __launch_bounds__(kLfBlock) void k_lf_probe(LfDev d, long long scall) {
   using SH = LfShape<SHAPE>;
-  const int L = lf_first_levels(d, STAGE, scall);
-  if (L > d.maxit[STAGE] || lf_scan(d, STAGE, 0, L) >= 0) return;
-  const int gi = blockIdx.x * kLfBlock + threadIdx.x;
-  const bool act = gi < d.n_groups;
+  const int rep = d.blk_rep[blockIdx.x];
+  const LfRep R = d.reps[rep];
+  const int L = lf_first_levels(d, STAGE, scall, rep);
+  if (L > d.maxit[STAGE] || lf_scan(d, STAGE, 0, L, R) >= 0) return;
+  const int local = ((int)blockIdx.x - R.gb0) * kLfBlock + threadIdx.x;
+  const bool act = local < R.ng;
   LfGroup<SH> g;
-  // inactive threads hold a copy of the last group (read only)
-  load_group(d, act ? gi : d.n_groups - 1, g, STAGE == kStShake);
+  // inactive threads hold a copy of the replica's last group (read only)
+  load_group(d, R.goff + (act ? local : R.ng - 1), g, STAGE == kStShake);
 #pragma unroll
   for (int a = 0; a < SH::NA; a++) {
     if (g.idx[a] < 0) continue;

@@ -8,10 +8,10 @@
 // One evaluation:
 //   k_pme_spread     one thread per atom: grid coordinates x = r n / L
 //                    wrapped into [0, n) as Grid::anterpolateCharge does
-//                    (Grid.h:124-131), the B-spline weights of
+//                    (Grid.h:107-131), the B-spline weights of
 //                    BSpline::set (BSpline.cpp:58-95), and q Mx My Mz onto
 //                    the order^3 stencil as 64-bit fixed-point integers
-//                    (2^-32) with atomics: integer sums do not depend on
+//                    (2^-40) with atomics: integer sums do not depend on
 //                    the order of the adds, so the grid is the same bits
 //                    on every run;
 //   k_pme_to_real    the integer grid to FP64, and cleared for the next run;
@@ -19,7 +19,7 @@
 //                    nz/2 + 1 half spectrum; the reference transforms the
 //                    full complex grid, whose other half is the conjugate);
 //   k_pme_scalar     one thread per spectrum point of the half spectrum,
-//                    the loop of Grid::scalarSum (Grid.h:214-304): theta(m)
+//                    the loop of Grid::scalarSum (Grid.h:237-341): theta(m)
 //                    = B(m) exp(-pi^2 m^2 / alpha^2) / (pi V m^2), energy
 //                    and virial terms (doubled where the reference doubles
 //                    them), the spectrum multiplied by theta; per-block sums
@@ -39,12 +39,19 @@ namespace pmnb {
 
 constexpr int kPmeBlock = 256;
 constexpr int kPmeSums = 7;  // energy, xx, xy, xz, yy, yz, zz
+// Charge grid resolution 2^-40: with order 8 on a 32^3 grid the rounding of
+// 2^-32 moves forces by 1e-9 of the largest, 2^-40 by 3e-12.  A grid value
+// must stay below 2^23 (8.4e6, about 4.6e5 elementary charges on one grid
+// point) in magnitude; each contribution is clamped to 2^22.
+constexpr double kPmeFixScale = 1099511627776.0;       // 2^40
+constexpr double kPmeFixInv = 1.0 / 1099511627776.0;
+constexpr double kPmeFixClamp = 4194304.0;             // 2^22
 
 struct PmeDev {
   int n;                   // atoms
   int nx, ny, nz, nzh;     // grid, nz / 2 + 1
   double hr[3];            // n / L per axis
-  int off[3];              // -(order - 1) / 2 + n per axis (Grid.h:510-512)
+  int off[3];              // -(order - 1) / 2 + n per axis (Grid.h:509-511)
   double Lr[3];            // 1 / L per axis
   double piVr;             // 1 / (pi V)
   double fac;              // pi^2 / alpha^2
@@ -95,7 +102,7 @@ __device__ __forceinline__ void pme_bspline(double w, double (&th)[P],
 
 // Grid coordinate of position p on an axis of n points: p n / L, brought
 // into [0, n) by adding or subtracting n as the reference's while loops do
-// (Grid.h:124-131); positions more than 64 boxes away and non-finite ones
+// (Grid.h:111-116); positions more than 64 boxes away and non-finite ones
 // are folded in one step, so every loop ends.
 __device__ __forceinline__ double pme_coord(double p, double hr, int n) {
   double x = p * hr;
@@ -139,9 +146,9 @@ __global__ __launch_bounds__(kPmeBlock) void k_pme_spread(PmeDev d,
       for (int w = 0; w < P; w++) {
         const int k1 = (w + b[2]) % d.nz;
         double val = ab * th[2][w];
-        val = fmin(fmax(val, -2147483647.0), 2147483647.0);
+        val = fmin(fmax(val, -kPmeFixClamp), kPmeFixClamp);
         atomicAdd(reinterpret_cast<unsigned long long *>(row + k1),
-                  (unsigned long long)(long long)__builtin_rint(val * kFixedScale));
+                  (unsigned long long)(long long)__builtin_rint(val * kPmeFixScale));
       }
     }
   }
@@ -151,7 +158,7 @@ __global__ __launch_bounds__(kPmeBlock) void k_pme_to_real(PmeDev d,
                                                            long long total) {
   const long long g = (long long)blockIdx.x * kPmeBlock + threadIdx.x;
   if (g >= total) return;
-  d.qreal[g] = (double)d.qfix[g] * kFixedInv;
+  d.qreal[g] = (double)d.qfix[g] * kPmeFixInv;
   d.qfix[g] = 0;
 }
 
@@ -268,7 +275,7 @@ __global__ __launch_bounds__(kPmeBlock) void k_pme_gather(PmeDev d, EwDev e,
 
 // One wave: lane c < kPmeSums adds sum c of every block in block order;
 // out[1] += energy / 2, and with the virial flag the virial slots XX..ZZ
-// += the virial sums / 2 (Grid.h:306-317).
+// += the virial sums / 2 (Grid.h:316-326).
 __global__ __launch_bounds__(64) void k_pme_finish(PmeDev d, double *out,
                                                    int virial) {
   __shared__ double tot[kPmeSums];

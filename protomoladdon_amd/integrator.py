@@ -222,6 +222,19 @@ class LeapfrogIntegrator:
             d_forces.data_ptr(), d_out.data_ptr(), int(n_steps), int(flags),
             ctypes.c_void_p(stream) if stream is not None else None))
 
+    def run_graph(self, topology, d_positions, d_velocities, d_forces, d_out,
+                  n_steps, n_replays=1, constraints=None, flags=0,
+                  stream=None):
+        """n_replays calls of run(n_steps) as replays of one captured HIP
+        graph (the same result, bit for bit, with one launch per replay)."""
+        self._ensure(topology, constraints)
+        lib = self.force._lib
+        _lib.check(lib, self.force._ctx, lib.pmnb_lf_run_graph(
+            self._lf, d_positions.data_ptr(), d_velocities.data_ptr(),
+            d_forces.data_ptr(), d_out.data_ptr(), int(n_steps),
+            int(n_replays), int(flags),
+            ctypes.c_void_p(stream) if stream is not None else None))
+
     def record_iterations(self, capacity):
         """Starts (capacity > 0) or stops (0) recording SHAKE/RATTLE sweep
         counts; returns (counts, n_calls, n_unconverged) recorded since the

@@ -129,3 +129,40 @@ def test_autotune_keeps_accuracy(replicas):
         a, b = bf.offsets[r], bf.offsets[r + 1]
         assert rms_rel(f[a:b], ref.forces) <= 1e-5
     bf.close()
+
+
+def test_config3_full_size_512_replicas():
+    """The 512-replica workload at its size (2.08 M atoms): graph replays
+    equal the eager evaluation bit for bit; every replica's forces within
+    RMS 1e-5 of the FP64 oracle and its energies within 2e-5; the pair
+    count equals the oracle's over all replicas."""
+    import torch
+    sizes = systems.config3_sizes()
+    ws = [systems.config3_replica(i, n) for i, n in enumerate(sizes)]
+    bf = BatchedCutoffForce.from_workloads(ws, precision="mixed")
+    print("512 replicas, atoms", bf.n_atoms, flush=True)
+    pos, frc0, out0 = batch_eval(bf, ws)
+    npairs = bf.pair_count()
+    frc = torch.full_like(pos, 7.0)
+    out = torch.full((len(ws) * OUT_SLOTS,), 3.0, dtype=torch.float64,
+                     device="cuda")
+    for k in range(3):
+        bf.run_graph(pos, frc, out, flags=1, n_steps=1)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(frc.cpu().numpy(), frc0)
+        np.testing.assert_array_equal(
+            out.cpu().numpy().reshape(len(ws), OUT_SLOTS), out0)
+        bf.evaluate_device(pos, frc, out, 1)
+        torch.cuda.synchronize()
+    bf.close()
+    total, worst = 0, 0.0
+    for r, w in enumerate(ws):
+        ref = O.evaluate(O.System.from_workload(w, flags=1))
+        a, b = bf.offsets[r], bf.offsets[r + 1]
+        total += ref.n_pairs
+        err = rms_rel(frc0[a:b], ref.forces)
+        worst = max(worst, err)
+        assert err <= 1e-5, (r, err)
+        np.testing.assert_allclose(out0[r, :2], ref.energies, rtol=2e-5)
+    print("worst replica rms rel force error", worst, flush=True)
+    assert npairs == total

@@ -148,3 +148,73 @@ def test_mixed_adjoint_close_to_fp64(chains):
     assert rms_rel(x32, x64) <= 1e-4
     assert rms_rel(q32, q64) <= 1e-4
     assert rms_rel(t32, t64) <= 1e-3
+
+
+def test_config4_full_size_force_matching():
+    """The force-matching workload at its size (250,000-atom chains): the
+    FP64 gradients of L = sum |F(theta) - F_target|^2 with respect to the 4
+    sigma, the 4 epsilon and 56 random charges match FP64 central finite
+    differences (h = 1e-6) within 1e-5; the position gradient matches a
+    directional central difference within 1e-6; the mixed gradients match
+    the FP64 ones within RMS 1e-4.  The pair list is recomputed in the
+    backward pass (pmnb_adjoint_device)."""
+    import torch
+    from protomoladdon_amd import grad as G
+    w = systems.config2()
+    t = w.topology
+    print("chains", w.n_atoms, flush=True)
+    rng = np.random.default_rng(1234)
+    q0 = t.scaled_charges / 18.2226123264
+    s0 = np.array(systems.CHAIN_SIGMA)
+    e0 = np.array(systems.CHAIN_EPS)
+    qt = q0 * (1 + 0.05 * rng.normal(size=q0.shape))
+    st = s0 * (1 + 0.05 * rng.normal(size=4))
+    et = e0 * (1 + 0.05 * rng.normal(size=4))
+    pos = dev(w.positions)
+    grads = {}
+    for prec in ("fp64", "mixed"):
+        f = w.make_force(prec)
+        with torch.no_grad():
+            target = G.cutoff_forces(f, t, pos, dev(qt), dev(st), dev(et), 0.5)
+        x = pos.clone().requires_grad_(True)
+        q = dev(q0).requires_grad_(True)
+        s = dev(s0).requires_grad_(True)
+        e = dev(e0).requires_grad_(True)
+        L = G.force_matching_loss(G.cutoff_forces(f, t, x, q, s, e, 0.5),
+                                  target)
+        L.backward()
+        grads[prec] = [a.grad.cpu().numpy() for a in (x, q, s, e)]
+        if prec == "fp64":
+            f64, target64 = f, target
+        else:
+            f.close()
+    for a, b in zip(grads["mixed"], grads["fp64"]):
+        assert rms_rel(a, b) <= 1e-4
+
+    def loss(th, xp=None):
+        with torch.no_grad():
+            F = G.cutoff_forces(f64, t, pos if xp is None else xp,
+                                dev(th[8:]), dev(th[:4]), dev(th[4:8]), 0.5)
+            return float(G.force_matching_loss(F, target64))
+    gx, gq, gs, ge = grads["fp64"]
+    theta = np.concatenate([s0, e0, q0])
+    an = np.concatenate([gs, ge, gq])
+    pick = np.concatenate([np.arange(8),
+                           8 + rng.choice(len(q0), 56, replace=False)])
+    h = 1e-6
+    worst = 0.0
+    gmax = np.abs(an[pick]).max()
+    for k in pick:
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd = (loss(tp) - loss(tm)) / (2 * h)
+        worst = max(worst, abs(fd - an[k]) / max(abs(fd), 1e-3 * gmax))
+    print("worst relative error over 64 parameters", worst, flush=True)
+    assert worst <= 1e-5
+    v = rng.normal(size=(w.n_atoms, 3))
+    fd = (loss(theta, pos + h * dev(v)) - loss(theta, pos - h * dev(v))) / (2 * h)
+    an_x = float(np.sum(gx * v))
+    print("dL/dx . v", an_x, "fd", fd, flush=True)
+    assert an_x == pytest.approx(fd, rel=1e-6)
+    f64.close()

@@ -5,7 +5,7 @@ k-space terms in its force group with the oracle leapfrog given the same
 terms.
 
 Bars: forces within 1e-9 of the largest oracle force component (the charge
-grid is summed in 64-bit fixed point at 2^-32, the FFTs are different
+grid is summed in 64-bit fixed point at 2^-40, the FFTs are different
 libraries), electrostatic energy within 1e-12 of its largest term (the
 point self energy), atomic virial within 1e-9 of its largest component;
 repeated evaluations bitwise equal.  Leapfrog: positions, velocities and
