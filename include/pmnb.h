@@ -138,6 +138,28 @@ int pmnb_evaluate_device(pmnb_ctx *ctx, const double *d_positions,
                          const double *d_mol_com, uint32_t flags,
                          void *stream);
 
+/* pmnb_evaluate_device without any allocation: same arguments, results
+   and kernels (bit for bit), with every per-evaluation array (cell counts
+   and starts, the sorted atom arrays, work items, partial sums, the
+   fixed-point force array, the grid and box) in one static device array
+   of 64 MiB per GPU instead of the context's buffers; nothing is
+   allocated or freed and no host copy is made.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid
+   allocation).  Covers one system on the tile path (algorithm cutoff >=
+   the switching cutoff, at most 8 LJ types): PMNB_ERR_UNSUPPORTED
+   otherwise; PMNB_ERR_CAPACITY when the system needs more than 64 MiB
+   (112 bytes per atom mixed, 136 FP64, plus 128 per 16 atoms and 8 per
+   cell: about 550,000 atoms mixed, 460,000 FP64).  Every call on one GPU
+   shares that GPU's array, which
+   keeps the last call's contents: calls that overlap on one GPU (other
+   streams or threads) race.  pmnb_last_pair_count does not see these
+   calls.  PMNB_ERR_STATE while kernel timing is on. */
+int pmnb_evaluate_device_noalloc(pmnb_ctx *ctx, const double *d_positions,
+                                 double *d_forces, double *d_out,
+                                 const double *d_mol_com, uint32_t flags,
+                                 void *stream);
+
 /* The pair set of one evaluation: every unordered pair that is not FULL
    excluded, is taken at its minimum image and has d^2 <= rc^2, where rc is
    the larger switching cutoff (force/OneAtomPairTwo.h:57-80).  Writes

@@ -42,6 +42,7 @@ EXPORTED = (
     "pmnb_lf_create", "pmnb_lf_run_device", "pmnb_lf_set_bonded",
     "pmnb_lf_forces_device", "pmnb_lf_iterations", "pmnb_lf_destroy",
     "pmnb_lf_run_graph", "pmnb_batch_lf_create",
+    "pmnb_evaluate_device_noalloc",
     "pmnb_ewald_create", "pmnb_ewald_info", "pmnb_ewald_evaluate_device",
     "pmnb_ewald_destroy", "pmnb_pme_create", "pmnb_pme_info",
     "pmnb_pme_evaluate_device", "pmnb_pme_destroy", "pmnb_lf_set_kspace",
@@ -112,6 +113,8 @@ def load():
     lib.pmnb_evaluate.argtypes = [_p, _p, _p, _p, _p, _p, _p, _u32]
     lib.pmnb_evaluate.restype = ctypes.c_int
     lib.pmnb_evaluate_device.argtypes = [_p, _p, _p, _p, _p, _u32, _p]
+    lib.pmnb_evaluate_device_noalloc.argtypes = [_p, _p, _p, _p, _p, _u32, _p]
+    lib.pmnb_evaluate_device_noalloc.restype = ctypes.c_int
     lib.pmnb_evaluate_device.restype = ctypes.c_int
     lib.pmnb_pairset.argtypes = [_p, _p, _p, _i64, ctypes.POINTER(_i64)]
     lib.pmnb_pairset.restype = ctypes.c_int

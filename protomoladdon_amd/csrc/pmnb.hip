@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -994,6 +995,10 @@ int run_device(pmnb_ctx *ctx, const double *d_pos, double *d_forces,
 
 }  // namespace
 
+namespace {
+void scratch_init(int device);  // the evaluation without allocation, below
+}  // namespace
+
 extern "C" {
 
 int pmnb_version(void) { return 100; }
@@ -1039,6 +1044,9 @@ int pmnb_create(const pmnb_params *params, pmnb_ctx **out) {
     delete ctx;
     return PMNB_ERR_HIP;
   }
+  // the device address of the static scratch on this device (once per
+  // device, with it current)
+  scratch_init(params->device);
   *out = ctx;
   return PMNB_OK;
 }
@@ -3259,6 +3267,165 @@ int pmnb_lf_set_kspace(pmnb_leapfrog *lf, pmnb_pme *pme, pmnb_ewald *ewald) {
   lf->pme = pme;
   lf->ewald = ewald;
   return PMNB_OK;
+}
+
+}  // extern "C"
+
+// --------------------------------------------- evaluation without allocation
+// pmnb_evaluate_device_noalloc keeps every per-evaluation array of the tile
+// path (cell counts and starts, the sorted structure of arrays, work items,
+// partial sums, the fixed-point force array, the system's grid and box) in
+// `scratch`, a file-scope device array of kScratchBytes that each GPU
+// holds for the life of the process.  Its device address on each GPU is
+// looked up once, by pmnb_create with that device current; host code uses
+// only that address (the array's own name is a host-side stand-in).
+namespace {
+
+constexpr size_t kScratchBytes = (size_t)64 << 20;  // 64 MiB per GPU
+constexpr int kScratchDevices = 64;
+
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kScratchDevices] = {};
+hipError_t g_scratch_err[kScratchDevices] = {};
+bool g_scratch_done[kScratchDevices] = {};
+
+// With `device` current: records the device address of `scratch` on it.
+void scratch_init(int device) {
+  if (device < 0 || device >= kScratchDevices) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_done[device]) return;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  g_scratch_err[device] = e;
+  g_scratch_ptr[device] = e == hipSuccess ? static_cast<unsigned char *>(ptr)
+                                          : nullptr;
+  g_scratch_done[device] = true;
+}
+
+// Carves 256-byte aligned arrays from a base address (or only counts the
+// bytes when base is null).
+struct Carve {
+  unsigned char *base;
+  size_t off = 0;
+  template <class T> T *take(size_t count) {
+    off = (off + 255) & ~(size_t)255;
+    T *p = base ? reinterpret_cast<T *>(base + off) : nullptr;
+    off += std::max<size_t>(count, 1) * sizeof(T);
+    return p;
+  }
+};
+
+// The tile path's arrays for one system of ctx, carved from `base`.
+EvalBufs carve_bufs(const pmnb_ctx *ctx, unsigned char *base, size_t &bytes,
+                    long long &nfix) {
+  Carve c{base};
+  const size_t N = (size_t)std::max(ctx->n, 1);
+  const size_t nc = (size_t)std::max(ctx->grid.ncell, 1);
+  const int W = ctx->prm.precision == PMNB_PREC_FP64 ? FixWords<true>::W
+                                                      : FixWords<false>::W;
+  const size_t items = (size_t)(ctx->n / kTileAtoms + ctx->ncol_total + 1);
+  EvalBufs B{};
+  B.ffix = c.take<long long>(N * W);
+  B.cell_of = c.take<int>(N);
+  B.count = c.take<int>(nc);
+  B.start = c.take<int>(nc);
+  B.rank = c.take<int>(N);
+  B.sorted = c.take<int>(N);
+  B.sx = c.take<double>(N);
+  B.sy = c.take<double>(N);
+  B.sz = c.take<double>(N);
+  B.sq = c.take<double>(N);
+  B.loc = c.take<float4>(N);
+  B.stype = c.take<int>(N);
+  B.smol = c.take<int>(N);
+  B.scell = c.take<int>(N);
+  B.srec = c.take<int4>(N);
+  B.partial = c.take<double>(items * ACC_N);
+  B.npairs = c.take<unsigned long long>(1);
+  B.items = c.take<int4>(items);
+  B.nitems = c.take<int>(1);
+  B.rep_first = c.take<int>(2);
+  B.reps = c.take<TileRep>(1);
+  B.col_rep = c.take<int>((size_t)std::max(ctx->ncol_total, 1));
+  B.box = c.take<Box>(1);
+  bytes = c.off;
+  nfix = (long long)(N * W);
+  return B;
+}
+
+// Writes the system's grid and box into the scratch, marks every column as
+// replica 0 and clears the fixed-point force array (the previous call on
+// this GPU may have been another system's, or may have stopped early).
+__global__ void k_scratch_setup(TileRep rep, Box box, TileRep *d_rep,
+                                Box *d_box, int *col_rep, int ncol,
+                                long long *ffix, long long nfix) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  if (t == 0) {
+    *d_rep = rep;
+    *d_box = box;
+  }
+  for (long long k = t; k < ncol; k += stride) col_rep[k] = 0;
+  for (long long k = t; k < nfix; k += stride) ffix[k] = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pmnb_evaluate_device_noalloc(pmnb_ctx *ctx, const double *d_positions,
+                                 double *d_forces, double *d_out,
+                                 const double *d_mol_com, uint32_t flags,
+                                 void *stream) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (ctx->batch || !ctx->use_tile || ctx->reps.size() != 1)
+    return fail(ctx, PMNB_ERR_UNSUPPORTED,
+                "the allocation-free evaluation covers one system on the tile "
+                "path (algorithm cutoff >= switching cutoff, at most 8 LJ "
+                "types)");
+  if (ctx->timing)
+    return fail(ctx, PMNB_ERR_STATE, "kernel timing is on");
+  if (ctx->n > 0 && (!d_positions || !d_forces || !d_out))
+    return fail(ctx, PMNB_ERR_ARG, "null device pointer");
+  if ((flags & PMNB_FLAG_MOLVIRIAL) && !d_mol_com)
+    return fail(ctx, PMNB_ERR_ARG, "molecular virial needs molecule centres");
+  const int dev = ctx->prm.device;
+  unsigned char *base = nullptr;
+  hipError_t serr = hipErrorInvalidDevice;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (dev >= 0 && dev < kScratchDevices && g_scratch_done[dev]) {
+      base = g_scratch_ptr[dev];
+      serr = g_scratch_err[dev];
+    }
+  }
+  if (!base)
+    return fail(ctx, PMNB_ERR_HIP, "no scratch address on device %d: %s", dev,
+                hipGetErrorString(serr));
+  size_t need = 0;
+  long long nfix = 0;
+  carve_bufs(ctx, nullptr, need, nfix);
+  if (need > kScratchBytes)
+    return fail(ctx, PMNB_ERR_CAPACITY,
+                "the evaluation needs %zu bytes of scratch; the static scratch "
+                "holds %zu (64 MiB)", need, kScratchBytes);
+  const EvalBufs B = carve_bufs(ctx, base, need, nfix);
+  HIPCHK(hipSetDevice(dev));
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  if (ctx->n == 0) return PMNB_OK;
+  const long long work = std::max<long long>(nfix, ctx->ncol_total);
+  const unsigned grid = (unsigned)std::min<long long>(
+      std::max<long long>(1, (work + kBlock - 1) / kBlock), 4096);
+  hipLaunchKernelGGL(k_scratch_setup, dim3(grid), dim3(kBlock), 0, s,
+                     ctx->reps[0], ctx->box, const_cast<TileRep *>(B.reps),
+                     const_cast<Box *>(B.box), const_cast<int *>(B.col_rep),
+                     ctx->ncol_total, B.ffix, nfix);
+  HIPCHK(hipGetLastError());
+  return run_device_bufs(ctx, B, d_positions, d_forces, d_out, d_mol_com,
+                         flags, s);
 }
 
 }  // extern "C"

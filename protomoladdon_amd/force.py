@@ -238,6 +238,20 @@ class NonbondedCutoffSystemForce:
             d_mol_com.data_ptr() if d_mol_com is not None else None,
             flags, sp))
 
+    def evaluate_device_noalloc(self, topology, d_positions, d_forces, d_out,
+                                flags=0, d_mol_com=None, stream=None):
+        """evaluate_device without allocating: the per-evaluation arrays
+        live in the library's static 64 MiB scratch on this device (one
+        system on the tile path; calls overlapping on one GPU race)."""
+        self._ensure(topology)
+        lib = self._lib
+        sp = ctypes.c_void_p(stream) if stream is not None else None
+        _lib.check(lib, self._ctx, lib.pmnb_evaluate_device_noalloc(
+            self._ctx, d_positions.data_ptr(), d_forces.data_ptr(),
+            d_out.data_ptr(),
+            d_mol_com.data_ptr() if d_mol_com is not None else None,
+            flags, sp))
+
     def set_parameters_device(self, topology, d_lj=None, d_scaled_charges=None,
                               stream=None):
         """Replaces the device copy of the LJ table (T x T x 4 float64
