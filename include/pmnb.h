@@ -295,10 +295,12 @@ int pmnb_batch_run_graph(pmnb_batch *b, const double *d_positions,
                          double *d_forces, double *d_out, uint32_t flags,
                          int32_t n_steps, void *stream);
 
-/* Autotuner: times every pair-kernel shape (buffered candidates per wave
-   with the waves per SIMD its LDS allows: 192 / 3, 128 / 4, 256 / 2 for
-   mixed) and persistent grid size on these positions
-   and keeps the fastest for later evaluations; returns the shape index,
+/* Autotuner: times every pair-kernel shape (the shell, and the buffered
+   candidates per wave with the waves per SIMD its LDS allows: mixed 0 full
+   shell 256 / 3, 1 full shell 128 / 4, 2 half shell 192 / 3; FP64 0 full
+   shell 192 / 3, 1 half shell 128 / 2) and persistent grid size on these
+   positions and keeps the fastest for later evaluations; returns the shape
+   index,
    the grid cap in blocks (0 = one block per work group of items) and its
    time per evaluation in ms (any pointer may be NULL).  The outputs are
    overwritten with accumulated sums. */
@@ -306,7 +308,8 @@ int pmnb_batch_autotune(pmnb_batch *b, const double *d_positions,
                         double *d_forces, double *d_out, uint32_t flags,
                         int32_t *shape, int32_t *grid_blocks, double *ms);
 
-/* Sets the pair-kernel shape (mixed 0 .. 2, FP64 0) and grid cap by hand. */
+/* Sets the pair-kernel shape (mixed 0 .. 2, FP64 0 .. 1, as the autotuner
+   numbers them) and grid cap by hand. */
 int pmnb_batch_set_shape(pmnb_batch *b, int32_t shape, int32_t grid_blocks);
 
 /* pmnb_force_pairs for the whole batch at device positions d_positions:

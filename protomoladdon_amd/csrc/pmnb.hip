@@ -562,11 +562,14 @@ int ensure_eval_buffers(pmnb_ctx *ctx) {
   return PMNB_OK;
 }
 
-// Pair-kernel shapes of the tile path: {buffered records per wave, waves
-// per SIMD the register allocation must allow}.  LDS per record: 56 bytes
-// (mixed), 112 (FP64), four waves per block.  A single system uses shape
-// 0; the batch autotuner chooses among them.
-constexpr int kMixedShapes = 3, kF64Shapes = 1;
+// Pair-kernel shapes of the tile path: {shell, buffered records per wave,
+// waves per SIMD the register allocation must allow}.  LDS per record: 32
+// bytes (mixed full shell), 56 (mixed half shell: + 3 fixed-point slots),
+// 64 / 112 (FP64), four waves per block.  Mixed: 0 full 256 / 3, 1 full
+// 128 / 4, 2 half 192 / 3; FP64: 0 full 192 / 3, 1 half 128 / 2.  A single
+// system uses its context's shape (0 unless set); the batch autotuner
+// chooses among them.
+constexpr int kMixedShapes = 3, kF64Shapes = 2;
 
 struct HalfArgs {
   int max_items;   // upper bound of the work items
@@ -586,13 +589,13 @@ struct HalfArgs {
   PairLog log;
 };
 
-template <bool F64, int EL, int SW, int BUF, int OCC>
+template <bool F64, int EL, int SW, int BUF, int OCC, bool HALF>
 void launch_half_shape(const HalfArgs &a) {
   if (a.single) {
     // a multiple of 8 blocks (FP64 maps blocks to XCD-contiguous items)
     const int nb = 8 * blocks_for(std::max(1, blocks_for(a.max_items,
                                                          kTmWaves)), 8);
-    hipLaunchKernelGGL((k_half<F64, EL, SW, BUF, OCC, false>), dim3(nb),
+    hipLaunchKernelGGL((k_half<F64, EL, SW, BUF, OCC, false, HALF>), dim3(nb),
                        dim3(kBlock), 0, a.s, *a.P, *a.S, a.reps, a.G, a.items,
                        a.nitems, a.ffix, a.partial, a.pstride, a.npairs,
                        a.log);
@@ -601,7 +604,7 @@ void launch_half_shape(const HalfArgs &a) {
   int nb = blocks_for(a.max_items, kTmWaves);
   if (a.grid_cap > 0) nb = std::min(nb, a.grid_cap);
   nb = std::max(nb, 1);
-  hipLaunchKernelGGL((k_half<F64, EL, SW, BUF, OCC, true>), dim3(nb),
+  hipLaunchKernelGGL((k_half<F64, EL, SW, BUF, OCC, true, HALF>), dim3(nb),
                      dim3(kBlock), 0, a.s, *a.P, *a.S, a.reps, a.G, a.items,
                      a.nitems, a.ffix, a.partial, a.pstride, a.npairs, a.log);
 }
@@ -609,11 +612,12 @@ void launch_half_shape(const HalfArgs &a) {
 template <bool F64, int EL, int SW>
 void launch_half(int shape, const HalfArgs &a) {
   if constexpr (F64) {
-    launch_half_shape<true, EL, SW, 128, 2>(a);
+    if (shape == 1) launch_half_shape<true, EL, SW, 128, 2, true>(a);
+    else launch_half_shape<true, EL, SW, 192, 3, false>(a);
   } else {
-    if (a.single || shape == 0) launch_half_shape<false, EL, SW, 192, 3>(a);
-    else if (shape == 1) launch_half_shape<false, EL, SW, 128, 4>(a);
-    else launch_half_shape<false, EL, SW, 256, 2>(a);
+    if (shape == 1) launch_half_shape<false, EL, SW, 128, 4, false>(a);
+    else if (shape == 2) launch_half_shape<false, EL, SW, 192, 3, true>(a);
+    else launch_half_shape<false, EL, SW, 256, 3, false>(a);
   }
 }
 

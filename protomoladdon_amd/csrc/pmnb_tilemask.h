@@ -1,38 +1,46 @@
-// Half-shell tile kernel with per-lane pair masks (k_half), for one system
-// or a batch of replicas (TileRep, pmnb_queue.h).
+// Tile kernel with per-lane pair masks (k_half), for one system or a batch
+// of replicas (TileRep, pmnb_queue.h), over the full or the half shell.
 //
 // One wave handles 16 consecutive sorted atoms of one column, 4 lanes per
-// atom (lane = atom slot + 16 * split).  Every unordered pair is evaluated
-// once (Newton's third law, as NonbondedCutoffForce::doEvaluate and
-// OneAtomPairTwo::doOneAtomPair do: force/nonbonded/NonbondedCutoffForce.h
+// atom (lane = atom slot + 16 * split).  Candidates stream into an LDS
+// buffer one z level at a time (stream_levels) after a test of their
+// distance to the wave's bounding box.  When the buffer is nearly full each
+// lane tests every 4th buffered candidate against its own atom into a pass
+// mask, then evaluates the set bits one by one, reading each candidate from
+// LDS one pair ahead of its arithmetic.  The LJ table and the exclusion
+// masks of the wave's own atoms are held in LDS.  Pairs within `band` of
+// the cutoff are re-decided with the reference's FP64 minimum image.  Each
+// item writes one partial row of energies and virials.
+//
+// Full shell (HALF = false, the path every single system takes): the
+// candidates are the 3 x 3 neighbour columns, every pair is visited from
+// both of its atoms and each visit takes half the pair's energy and virial.
+// An atom's force is summed by its own wave only (FP32 partials per buffer
+// added in 64-bit fixed point, or FP64 sums) and stored once into the
+// per-atom fixed-point array: no atomics.
+//
+// Half shell (HALF = true, a batch shape): every unordered pair is
+// evaluated once (Newton's third law, as NonbondedCutoffForce::doEvaluate
+// and OneAtomPairTwo::doOneAtomPair do: force/nonbonded/NonbondedCutoffForce.h
 // :48-52, force/OneAtomPairTwo.h:106-111): a wave takes its own column's
-// atoms of higher sorted index and four of the eight neighbour columns,
-// the image offsets (0, +1), (+1, -1), (+1, 0), (+1, +1).  Of a pair's two
+// atoms of higher sorted index and four of the eight neighbour columns, the
+// image offsets (0, +1), (+1, -1), (+1, 0), (+1, +1).  Of a pair's two
 // views, i -> j at image offset d and j -> i at -d, exactly one is taken:
 // for d != 0 one of d, -d is in that half, for d = 0 the lower sorted index
 // takes it.  With two columns on an axis the offsets -1 and +1 reach the
 // same column at images one box (>= 2 cutoffs) apart, so at most one image
-// of a pair is within the cutoff and it is taken once.
+// of a pair is within the cutoff and it is taken once.  The partner's force
+// is added into the candidate's LDS slot as 64-bit fixed-point integers,
+// and after each buffer the slots go to the per-atom fixed-point array with
+// global integer atomics.  Integer sums do not depend on the order of the
+// adds, so the result is bitwise reproducible.  On the 96,000-atom water
+// this is slower than the full shell: the LDS integer atomics and the
+// global atomics cost more than the pair visits they save.
 //
-// Candidates of those columns stream into an LDS buffer one z level at a
-// time (stream_levels) after a test of their distance to the wave's
-// bounding box.  When the buffer is nearly full each lane tests every 4th
-// buffered candidate against its own atom into a pass mask, then evaluates
-// the set bits one by one, reading each candidate from LDS one pair ahead
-// of its arithmetic.  The LJ table and the exclusion masks of the wave's own
-// atoms are held in LDS.
-//
-// Forces: the atom's own force is summed in registers (FP32 partials per
-// buffer added in 64-bit fixed point, or FP64 sums); the partner's force is
-// added into the candidate's LDS slot as 64-bit fixed-point integers, and
-// after each buffer the slots go to the per-atom fixed-point array in HBM
-// with integer atomics.  Integer sums do not depend on the order of the
-// adds, so the result is bitwise reproducible.  Mixed: x, y, z at 2^-32.
-// FP64: each component as (hi, lo) with value hi 2^-24 + lo 2^-56, exact
-// to 2^-57 per term.  k_half_finish converts the array into the caller's
-// forces (+=) and clears it for the next evaluation.  Pairs within `band`
-// of the cutoff are re-decided with the reference's FP64 minimum image.
-// Each item writes one partial row of energies and virials.
+// Fixed point: mixed x, y, z at 2^-32; FP64 each component as (hi, lo) with
+// value hi 2^-24 + lo 2^-56, exact to 2^-57 per term.  k_half_finish
+// converts the array into the caller's forces (+=) and clears it for the
+// next evaluation.
 #pragma once
 
 #include <type_traits>
@@ -176,14 +184,14 @@ __device__ __forceinline__ void tm_test(const float4 *__restrict__ rec, int n,
 //   offsets ox, oy and level base z0, all relative to the wave's home cell.
 // wx / wy: single column on that axis (taken once, per-pair image); wz: the
 // levels zA..zB cover every z cell once (per-pair image along z).
-template <class Emit>
+template <bool HALF, class Emit>
 __device__ __forceinline__ void stream_levels(
     const Sorted &S, const TileGrid &G, int coff, int cx, int cy, bool wx,
     bool wy, bool wz, int zlo, int zhi, int zA, int zB, int a0, float bx0,
     float bx1, float by0, float by1, float bz0, float bz1, float hi, int lane,
     Emit &&emit) {
   const int ox = lane / 3 - 1, oy = lane % 3 - 1;
-  const bool own = lane == 4;
+  const bool own = HALF && lane == 4;
   const float gx = wx ? 0.f
                       : fmaxf(0.f, fmaxf((float)ox * G.csx - bx1,
                                          bx0 - (float)(ox + 1) * G.csx));
@@ -191,8 +199,8 @@ __device__ __forceinline__ void stream_levels(
                       : fmaxf(0.f, fmaxf((float)oy * G.csy - by1,
                                          by0 - (float)(oy + 1) * G.csy));
   const float gxy2 = fmaf(gx, gx, gy * gy);
-  const bool cok = lane >= 4 && lane < 9 && !(wx && ox != 0) &&
-                   !(wy && oy != 0) && gxy2 <= hi;
+  const bool cok = lane >= (HALF ? 4 : 0) && lane < 9 &&
+                   !(wx && ox != 0) && !(wy && oy != 0) && gxy2 <= hi;
   int tx = cx + ox, ty = cy + oy;
   tx = tx < 0 ? tx + G.nx : (tx >= G.nx ? tx - G.nx : tx);
   ty = ty < 0 ? ty + G.ny : (ty >= G.ny ? ty - G.ny : ty);
@@ -234,7 +242,7 @@ __device__ __forceinline__ void stream_levels(
     int cnt_n = 0, start_n = 0;
     if (qn < nlev) level_cell(zn, cnt_n, start_n);
     // inclusive prefix of the counts over lanes 0 .. 8 (row shifts within
-    // the first 16 lanes; lanes 0 .. 3 count nothing)
+    // the first 16 lanes; in the half shell lanes 0 .. 3 count nothing)
     int inc = cnt;
     inc += __builtin_amdgcn_update_dpp(0, inc, 0x111, 0xf, 0xf, false);
     inc += __builtin_amdgcn_update_dpp(0, inc, 0x112, 0xf, 0xf, false);
@@ -244,14 +252,15 @@ __device__ __forceinline__ void stream_levels(
     if (total > 0) {
       const float z0 = (float)(zc - zlo) * G.csz;
       const int rel = start - (inc - cnt);  // record = run index + rel
-      int ends[4];
+      constexpr int K0 = HALF ? 4 : 0, NE = 8 - K0;
+      int ends[NE];
 #pragma unroll
-      for (int j = 0; j < 4; j++) ends[j] = __builtin_amdgcn_readlane(inc, 4 + j);
+      for (int j = 0; j < NE; j++) ends[j] = __builtin_amdgcn_readlane(inc, K0 + j);
       for (int base = 0; base < total; base += 64) {
         const int idx = base + lane;
-        int k = 4;
+        int k = K0;
 #pragma unroll
-        for (int j = 0; j < 4; j++) k += idx >= ends[j] ? 1 : 0;
+        for (int j = 0; j < NE; j++) k += idx >= ends[j] ? 1 : 0;
         const int t = idx + __shfl(rel, k, 64);
         emit(idx < total, t, (float)(k / 3 - 1) * G.csx,
              (float)(k % 3 - 1) * G.csy, z0);
@@ -270,7 +279,7 @@ __device__ __forceinline__ void stream_levels(
 // a batch of replicas (per-item TileRep, per-pair minimum image on axes
 // with a single column), a persistent item loop with a grid stride.  OCC:
 // waves per SIMD the register allocation must allow.
-template <bool F64, int ELEC, int ELSW, int BUF, int OCC, bool REP>
+template <bool F64, int ELEC, int ELSW, int BUF, int OCC, bool REP, bool HALF>
 __global__ void __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(OCC)))
 k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
@@ -284,7 +293,8 @@ k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
   __shared__ int4 reci[kTmWaves][BUF];   // sorted index, type, mol, atom
   __shared__ double4 recd[F64 ? kTmWaves : 1][F64 ? BUF : 1];
   // fixed-point force on each buffered record from this wave's atoms
-  __shared__ long long pfix[kTmWaves][W][BUF];
+  // (half shell only)
+  __shared__ long long pfix[HALF ? kTmWaves : 1][HALF ? W : 1][HALF ? BUF : 1];
   // exclusion masks of the wave's own atoms (excl_own layout)
   __shared__ unsigned long long xm[kTmWaves][kTileAtoms][4];
   __shared__ float4 ljs[F64 ? 1 : kQLj];  // FP32 LJ table (T * T <= kQLj)
@@ -292,7 +302,11 @@ k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
   // registers) where that measured faster
   auto U = [](auto v) { return (F64 || REP) ? uni(v) : v; };
   const int wave = uni((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
-  for (int k = lane; k < W * BUF; k += 64) (&pfix[wave][0][0])[k] = 0;
+  if (HALF)
+    for (int k = lane; k < W * BUF; k += 64) (&pfix[HALF ? wave : 0][0][0])[k] = 0;
+  // each pair is seen from both atoms in the full shell: half its energy
+  // and virial to each
+  constexpr double kShare = HALF ? 1.0 : 0.5;
   if (!F64) {
     for (int k = threadIdx.x; k < P.n_types * P.n_types; k += kBlock)
       ljs[k] = S.ljf[k];
@@ -382,9 +396,10 @@ k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
                     auto WM) __attribute__((always_inline)) {
       constexpr int M = decltype(WM)::value;
       const int t = q4.x;
-      // the atom itself and the wave's own atoms below it (records of the
-      // own column start above a0; other columns hold no atom of this wave)
-      if ((unsigned)(t - a0) <= (unsigned)slot) return;
+      // half shell: the atom itself and the wave's own atoms below it
+      // (records of the own column start above a0; other columns hold no
+      // atom of this wave); full shell: the atom itself
+      if (HALF ? (unsigned)(t - a0) <= (unsigned)slot : t == i) return;
       float dx = r.x - xi, dy = r.y - yi, dz = r.z - zi;
       if (M == 2 || (M == 3 && wx)) dx = fmaf(-L[0], rintf(dx * iL[0]), dx);
       if (M == 2 || (M == 3 && wy)) dy = fmaf(-L[1], rintf(dy * iL[1]), dy);
@@ -405,8 +420,10 @@ k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
       const bool same = q4.z == mi;
       const int excl = same ? excl_own(xm[wave][slot], q4.w - ai) : 0;
       if (excl == 2) return;
-      cnt++;
-      if (plog.out) log_pair(plog, ai, q4.w, excl);
+      if (HALF || i < t) {
+        cnt++;
+        if (plog.out) log_pair(plog, ai, q4.w, excl);
+      }
       const bool molv = (P.flags & 2u) && !same;
       if (F64) {
         const double *lp = ljrow64 + 4 * q4.y;
@@ -414,24 +431,26 @@ k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
         pair_f64<ELEC, ELSW>(P, e2d, excl != 1 ? lp[0] : lp[2],
                              excl != 1 ? lp[1] : lp[3], qi * recd[wave][k].w,
                              excl, ff, e1, e2);
-        acc[ACC_ELJ] += e1;
-        acc[ACC_EEL] += e2;
+        acc[ACC_ELJ] += kShare * e1;
+        acc[ACC_EEL] += kShare * e2;
         const double gx = ex * ff, gy = ey * ff, gz = ez * ff;
         fx64 -= gx;
         fy64 -= gy;
         fz64 -= gz;
-        long long h, l;
-        fix64_split(gx, h, l);
-        lds_add_i64(&pfix[wave][0][k], h);
-        lds_add_i64(&pfix[wave][1][k], l);
-        fix64_split(gy, h, l);
-        lds_add_i64(&pfix[wave][2][k], h);
-        lds_add_i64(&pfix[wave][3][k], l);
-        fix64_split(gz, h, l);
-        lds_add_i64(&pfix[wave][4][k], h);
-        lds_add_i64(&pfix[wave][5][k], l);
+        if constexpr (HALF) {
+          long long h, l;
+          fix64_split(gx, h, l);
+          lds_add_i64(&pfix[wave][0][k], h);
+          lds_add_i64(&pfix[wave][1][k], l);
+          fix64_split(gy, h, l);
+          lds_add_i64(&pfix[wave][2][k], h);
+          lds_add_i64(&pfix[wave][3][k], l);
+          fix64_split(gz, h, l);
+          lds_add_i64(&pfix[wave][4][k], h);
+          lds_add_i64(&pfix[wave][5][k], l);
+        }
         if (molv || (P.flags & 1u))
-          add_virial(acc, ACC_VXX, gx, gy, gz, ex, ey, ez, 1.0);
+          add_virial(acc, ACC_VXX, gx, gy, gz, ex, ey, ez, kShare);
         if (molv) {
           const Box &bm = rare_box(bptr);
           double cdx, cdy, cdz, cd2;
@@ -439,7 +458,7 @@ k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
                          S.com[3 * mi + 2], S.com[3 * q4.z],
                          S.com[3 * q4.z + 1], S.com[3 * q4.z + 2], cdx, cdy,
                          cdz, cd2);
-          add_virial(acc, ACC_MXX, gx, gy, gz, cdx, cdy, cdz, 1.0);
+          add_virial(acc, ACC_MXX, gx, gy, gz, cdx, cdy, cdz, kShare);
         }
       } else {
         const float4 lp = ljs[ljbase + q4.y];
@@ -452,9 +471,11 @@ k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
         px -= gx;
         py -= gy;
         pz -= gz;
-        lds_add_i64(&pfix[wave][0][k], to_fixed_f32(gx));
-        lds_add_i64(&pfix[wave][1][k], to_fixed_f32(gy));
-        lds_add_i64(&pfix[wave][2][k], to_fixed_f32(gz));
+        if constexpr (HALF) {
+          lds_add_i64(&pfix[wave][0][k], to_fixed_f32(gx));
+          lds_add_i64(&pfix[wave][1][k], to_fixed_f32(gy));
+          lds_add_i64(&pfix[wave][2][k], to_fixed_f32(gz));
+        }
         if (vir) {
           if (molv || (P.flags & 1u)) {
             v0 = fmaf(gx, dx, v0);
@@ -471,7 +492,7 @@ k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
                            S.com[3 * mi + 2], S.com[3 * q4.z],
                            S.com[3 * q4.z + 1], S.com[3 * q4.z + 2], cdx, cdy,
                            cdz, cd2);
-            add_virial(acc, ACC_MXX, gx, gy, gz, cdx, cdy, cdz, 1.0);
+            add_virial(acc, ACC_MXX, gx, gy, gz, cdx, cdy, cdz, kShare);
           }
         }
       }
@@ -563,16 +584,20 @@ k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
         Fx += to_fixed_f32(px);
         Fy += to_fixed_f32(py);
         Fz += to_fixed_f32(pz);
-        acc[ACC_ELJ] += (double)pelj;
-        acc[ACC_EEL] += (double)peel;
+        acc[ACC_ELJ] += kShare * (double)pelj;
+        acc[ACC_EEL] += kShare * (double)peel;
         if (vir) {
-          acc[ACC_VXX] += (double)v0;
-          acc[ACC_VXY] += (double)v1;
-          acc[ACC_VXZ] += (double)v2;
-          acc[ACC_VYY] += (double)v3;
-          acc[ACC_VYZ] += (double)v4;
-          acc[ACC_VZZ] += (double)v5;
+          acc[ACC_VXX] += kShare * (double)v0;
+          acc[ACC_VXY] += kShare * (double)v1;
+          acc[ACC_VXZ] += kShare * (double)v2;
+          acc[ACC_VYY] += kShare * (double)v3;
+          acc[ACC_VYZ] += kShare * (double)v4;
+          acc[ACC_VZZ] += kShare * (double)v5;
         }
+      }
+      if (!HALF) {
+        wave_sync_lds();  // the buffer is refilled next
+        return;
       }
       wave_sync_lds();  // every lane's slot adds are in
       for (int k = lane; k < n; k += 64) {
@@ -623,11 +648,12 @@ k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
         nbuf = 0;
       }
     };
-    stream_levels(S, G, coff, cx, cy, wx, wy, wz, zlo, zhi, zA, zB, a0, bx0,
-                  bx1, by0, by1, bz0, bz1, hi, lane, emit);
+    stream_levels<HALF>(S, G, coff, cx, cy, wx, wy, wz, zlo, zhi, zA, zB, a0,
+                        bx0, bx1, by0, by1, bz0, bz1, hi, lane, emit);
     if (nbuf > 0) process(nbuf);
-    // the 4 lanes of an atom add their partial forces, then the atom's
-    // force joins the partner forces other waves add to it
+    // the 4 lanes of an atom add their partial forces; in the half shell
+    // the atom's force then joins the partner forces other waves add to
+    // it, in the full shell it is the whole force (one writer per atom)
     if (F64) {
       fx64 += __shfl_down(fx64, 32, 64);
       fy64 += __shfl_down(fy64, 32, 64);
@@ -637,16 +663,20 @@ k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
       fz64 += __shfl_down(fz64, 16, 64);
       if (active && split == 0) {
         long long *dst = ffix + (size_t)W * ii;
-        long long h, l;
-        fix64_split(fx64, h, l);
-        global_add_i64(dst, h);
-        global_add_i64(dst + 1, l);
-        fix64_split(fy64, h, l);
-        global_add_i64(dst + 2, h);
-        global_add_i64(dst + 3, l);
-        fix64_split(fz64, h, l);
-        global_add_i64(dst + 4, h);
-        global_add_i64(dst + 5, l);
+        long long h[3], l[3];
+        fix64_split(fx64, h[0], l[0]);
+        fix64_split(fy64, h[1], l[1]);
+        fix64_split(fz64, h[2], l[2]);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          if (HALF) {
+            global_add_i64(dst + 2 * c, h[c]);
+            global_add_i64(dst + 2 * c + 1, l[c]);
+          } else {
+            dst[2 * c] = h[c];
+            dst[2 * c + 1] = l[c];
+          }
+        }
       }
     } else {
       Fx += __shfl_down(Fx, 32, 64);
@@ -657,9 +687,15 @@ k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
       Fz += __shfl_down(Fz, 16, 64);
       if (active && split == 0) {
         long long *dst = ffix + (size_t)W * ii;
-        global_add_i64(dst, Fx);
-        global_add_i64(dst + 1, Fy);
-        global_add_i64(dst + 2, Fz);
+        if (HALF) {
+          global_add_i64(dst, Fx);
+          global_add_i64(dst + 1, Fy);
+          global_add_i64(dst + 2, Fz);
+        } else {
+          dst[0] = Fx;
+          dst[1] = Fy;
+          dst[2] = Fz;
+        }
       }
     }
     // this item's energies and virials

@@ -166,3 +166,35 @@ def test_config3_full_size_512_replicas():
         np.testing.assert_allclose(out0[r, :2], ref.energies, rtol=2e-5)
     print("worst replica rms rel force error", worst, flush=True)
     assert npairs == total
+
+
+@pytest.mark.parametrize("precision,shape", [("mixed", 2), ("fp64", 1)])
+def test_half_shell_shape_matches_oracle(replicas, precision, shape):
+    """The half-shell pair kernel (each pair evaluated once, the partner's
+    force added through fixed-point LDS slots and global integer atomics),
+    selected as a batch shape: the pairs it evaluates are the oracle's, bit
+    for bit, forces and energies within the bars of the full shell, and
+    repeated evaluations are bitwise equal."""
+    bf = BatchedCutoffForce.from_workloads(replicas, precision=precision)
+    bf.set_shape(shape)
+    pos, frc, out = batch_eval(bf, replicas)
+    _, frc2, out2 = batch_eval(bf, replicas)
+    np.testing.assert_array_equal(frc2, frc)
+    np.testing.assert_array_equal(out2, out)
+    bpairs = bf.force_pairs(pos)
+    bf.close()
+    for r, w in enumerate(replicas):
+        a, b = bf.offsets[r], bf.offsets[r + 1]
+        ref = O.evaluate(O.System.from_workload(w, flags=1), record_pairs=True)
+        mine = bpairs[(bpairs[:, 0] >= a) & (bpairs[:, 0] < b)].copy()
+        mine[:, :2] -= a
+        np.testing.assert_array_equal(mine, O.sorted_pairs(ref.pairs))
+        if precision == "mixed":
+            assert rms_rel(frc[a:b], ref.forces) <= 1e-5
+            np.testing.assert_allclose(out[r, :2], ref.energies, rtol=2e-5)
+        else:
+            scale = max(1.0, np.abs(ref.forces).max())
+            assert np.abs(frc[a:b] - ref.forces).max() <= 1e-11 * scale
+            np.testing.assert_allclose(out[r, :2], ref.energies, rtol=1e-11)
+            np.testing.assert_allclose(out[r, 2:11], ref.virial, rtol=1e-10,
+                                       atol=1e-9 * np.abs(ref.virial).max())
