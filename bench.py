@@ -153,6 +153,48 @@ def time_ewald_kspace(w, alpha, accuracy, reps, torch):
             "atom_kvector_terms_per_s": w.n_atoms * nk / (ms * 1e-3)}
 
 
+def time_pme_kspace(w, grid, order, alpha, reps, torch, ewald_alpha=None):
+    """The k-space terms of NonbondedPMEwaldSystemForce (B-spline
+    spreading, FFTs, scalar sum, gather, correction, self and
+    charged-system terms) on the device: ms per evaluation, and the RMS
+    relative difference of its k-space forces from the direct Ewald sum's
+    at the same alpha (device, accuracy 1e-8)."""
+    from protomoladdon_amd.ewald import NonbondedFullEwaldKSpace
+    from protomoladdon_amd.pme import NonbondedPMEwaldKSpace
+    force = w.make_force("fp64")
+    ks = NonbondedPMEwaldKSpace(force, grid, 9.0, order=order, alpha=alpha)
+    dev = torch.device("cuda:0")
+    x = torch.from_numpy(np.ascontiguousarray(w.positions)).to(dev)
+    f = torch.zeros_like(x)
+    out = torch.zeros(20, dtype=torch.float64, device=dev)
+    sp = torch.cuda.current_stream(dev).cuda_stream
+    ks.evaluate_device(w.topology, x, f, out, 1, stream=sp)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ks.evaluate_device(w.topology, x, f, out, 1, stream=sp)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / reps
+    f.zero_()
+    ks.evaluate_device(w.topology, x, f, out, 1, stream=sp)
+    torch.cuda.synchronize()
+    fp = f.cpu().numpy()
+    a, kc = ks.info(w.topology)
+    ks.close()
+    ew = NonbondedFullEwaldKSpace(force, alpha, 1e-8)
+    fe = torch.zeros_like(x)
+    oe = torch.zeros(20, dtype=torch.float64, device=dev)
+    ew.evaluate_device(w.topology, x, fe, oe, 1, stream=sp)
+    torch.cuda.synchronize()
+    fr = fe.cpu().numpy()
+    ew.close()
+    force.close()
+    return {"ms": ms, "grid": list(grid), "order": order, "alpha": a,
+            "n_atoms": w.n_atoms,
+            "rms_rel_force_diff_vs_direct_ewald": rms_rel(fp, fr),
+            "energy_diff_vs_direct_ewald": float(out[1].item() - oe[1].item())}
+
+
 def host_cpu():
     """(model name, logical CPUs visible to this process)."""
     import platform
@@ -515,6 +557,8 @@ def main():
     ewald = {"water3k": time_ewald_kspace(systems.config0(), -1.0, 1e-5, 20,
                                           torch),
              "water96k": time_ewald_kspace(w, 0.32, 1e-5, 3, torch)}
+    print("pme k-space ...", file=sys.stderr, flush=True)
+    pme = {"water96k": time_pme_kspace(w, (96, 96, 96), 4, 0.32, 20, torch)}
 
     # the host-array entry point (pmnb_evaluate: positions in, forces out
     # through PCIe, synchronous), for the record; never the headline value
@@ -626,6 +670,7 @@ def main():
                              "whole step on the device, virial on",
                "mixed": md_mx, "fp64": md_64},
         "ewald_kspace": ewald,
+        "pme_kspace": pme,
         "host_api_ms_per_step": host_ms,
         "accuracy_rms_rel_force_err_vs_fp64_oracle": accuracy,
         "roofline": {"bound": "valu", "achieved": achieved,

@@ -587,26 +587,48 @@ struct HalfArgs {
   bool single;     // one replica without wrapped x/y
   TileGrid G;      // its column grid
   PairLog log;
+  double *forces;  // the caller's forces (k_tile adds into them)
 };
 
-template <bool F64, int EL, int SW, int BUF, int OCC, bool HALF>
-void launch_half_shape(const HalfArgs &a) {
+// The single-system full-shell kernel adds its forces straight into the
+// caller's array; every other shape goes through the fixed-point array.
+bool direct_forces(const HalfArgs &a, int shape, bool mixed) {
+  return a.single && (mixed ? shape != 2 : shape != 1);
+}
+
+template <bool F64, int EL, int SW, int BUF, int OCC, bool HALF, bool LOG>
+void launch_half_log(const HalfArgs &a) {
   if (a.single) {
     // a multiple of 8 blocks (FP64 maps blocks to XCD-contiguous items)
     const int nb = 8 * blocks_for(std::max(1, blocks_for(a.max_items,
                                                          kTmWaves)), 8);
-    hipLaunchKernelGGL((k_half<F64, EL, SW, BUF, OCC, false, HALF>), dim3(nb),
-                       dim3(kBlock), 0, a.s, *a.P, *a.S, a.reps, a.G, a.items,
-                       a.nitems, a.ffix, a.partial, a.pstride, a.npairs,
-                       a.log);
+    if constexpr (!HALF)
+      hipLaunchKernelGGL((k_tile<F64, EL, SW, F64 ? 192 : 256, LOG>),
+                         dim3(nb), dim3(kBlock), 0, a.s, *a.P, *a.S, a.G,
+                         a.items, a.nitems, a.forces, a.partial, a.pstride,
+                         a.npairs, a.log);
+    else
+      hipLaunchKernelGGL((k_half<F64, EL, SW, BUF, OCC, false, HALF, LOG>),
+                         dim3(nb), dim3(kBlock), 0, a.s, *a.P, *a.S, a.reps,
+                         a.G, a.items, a.nitems, a.ffix, a.partial,
+                         a.pstride, a.npairs, a.log);
     return;
   }
   int nb = blocks_for(a.max_items, kTmWaves);
   if (a.grid_cap > 0) nb = std::min(nb, a.grid_cap);
   nb = std::max(nb, 1);
-  hipLaunchKernelGGL((k_half<F64, EL, SW, BUF, OCC, true, HALF>), dim3(nb),
-                     dim3(kBlock), 0, a.s, *a.P, *a.S, a.reps, a.G, a.items,
-                     a.nitems, a.ffix, a.partial, a.pstride, a.npairs, a.log);
+  hipLaunchKernelGGL((k_half<F64, EL, SW, BUF, OCC, true, HALF, LOG>),
+                     dim3(nb), dim3(kBlock), 0, a.s, *a.P, *a.S, a.reps, a.G,
+                     a.items, a.nitems, a.ffix, a.partial, a.pstride, a.npairs,
+                     a.log);
+}
+
+// The pair record (a.log.out set) runs the same kernel code with the
+// record compiled in.
+template <bool F64, int EL, int SW, int BUF, int OCC, bool HALF>
+void launch_half_shape(const HalfArgs &a) {
+  if (a.log.out) launch_half_log<F64, EL, SW, BUF, OCC, HALF, true>(a);
+  else launch_half_log<F64, EL, SW, BUF, OCC, HALF, false>(a);
 }
 
 template <bool F64, int EL, int SW>
@@ -889,17 +911,21 @@ int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
     int *nitems = B.nitems;
     int *rep_first = B.rep_first;
     long long *ffix = B.ffix;
+    bool direct = false;
     if (ev_built) HIPCHK(hipEventRecord(ev_built, s));
     if (ctx->n > 0) {
       const bool single = !ctx->batch && nrep == 1 &&
                           !ctx->reps[0].wrap[0] && !ctx->reps[0].wrap[1];
       HalfArgs a{max_items, ctx->grid_cap, s, &P, &S, reps, items, nitems,
-                 ffix, part, np, max_items, single, ctx->tile, ctx->plog};
+                 ffix, part, np, max_items, single, ctx->tile, ctx->plog,
+                 d_forces};
+      direct = direct_forces(a, ctx->shape, mixed);
       launch_pair_half(mixed, el, sw, ctx->shape, a);
     }
     if (ev_stop) HIPCHK(hipEventRecord(ev_stop, s));
-    // fixed-point forces into the caller's array, per-replica sums
-    const int ab = blocks_for(ctx->n, 1024);
+    // fixed-point forces into the caller's array (unless the pair kernel
+    // added them itself), per-replica sums
+    const int ab = direct ? 0 : blocks_for(ctx->n, 1024);
     const dim3 fg((unsigned)(ab + nrep * ACC_N));
     if (mixed)
       hipLaunchKernelGGL(k_half_finish<false>, fg, dim3(1024), 0, s, ffix,

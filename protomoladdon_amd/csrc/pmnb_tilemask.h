@@ -278,8 +278,11 @@ __device__ __forceinline__ void stream_levels(
 // grid and box as arguments, x and y with at least two columns; REP = true:
 // a batch of replicas (per-item TileRep, per-pair minimum image on axes
 // with a single column), a persistent item loop with a grid stride.  OCC:
-// waves per SIMD the register allocation must allow.
-template <bool F64, int ELEC, int ELSW, int BUF, int OCC, bool REP, bool HALF>
+// waves per SIMD the register allocation must allow.  LOG: the pair record
+// of pmnb_force_pairs is compiled in (the same code, with each evaluated
+// pair appended to plog).
+template <bool F64, int ELEC, int ELSW, int BUF, int OCC, bool REP, bool HALF,
+          bool LOG>
 __global__ void __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(OCC)))
 k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
@@ -420,10 +423,8 @@ k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
       const bool same = q4.z == mi;
       const int excl = same ? excl_own(xm[wave][slot], q4.w - ai) : 0;
       if (excl == 2) return;
-      if (HALF || i < t) {
-        cnt++;
-        if (plog.out) log_pair(plog, ai, q4.w, excl);
-      }
+      cnt += (HALF || i < t) ? 1u : 0u;
+      if (LOG && (HALF || i < t)) log_pair(plog, ai, q4.w, excl);
       const bool molv = (P.flags & 2u) && !same;
       if (F64) {
         const double *lp = ljrow64 + 4 * q4.y;
@@ -708,6 +709,366 @@ k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
     if (!REP) break;  // one item per wave
   }
   unsigned long long tot = block_count<kTmWaves>(cnt);
+  if (threadIdx.x == 0 && tot) atomicAdd(npairs, tot);
+}
+
+// Tests the buffered records against one atom (mask_test layout).  WRAPZ:
+// minimum image along z.
+template <bool WRAPZ, int NW>
+__device__ __forceinline__ void tile_test(const float4 *__restrict__ rec,
+                                          int n, int split, float xi,
+                                          float yi, float zi, float hi,
+                                          float Lz, float iLz,
+                                          unsigned (&m)[NW]) {
+  auto test = [&](int j) -> unsigned {
+    const float4 r = rec[split + kTmSplit * j];
+    const float dx = r.x - xi, dy = r.y - yi;
+    float dz = r.z - zi;
+    if (WRAPZ) dz = fmaf(-Lz, rintf(dz * iLz), dz);
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    return (unsigned)(r2 <= hi);
+  };
+  mask_test<0, NW>((n + kTmSplit - 1) / kTmSplit, m, test);
+}
+
+// The pair kernel of one system (the path every single system takes): the
+// full shell of k_half with REP = false, written for one replica only: the
+// column grid and box are kernel arguments, x and y have at least two
+// columns, z takes the minimum image only when the cutoff reaches every z
+// cell, and each atom's force is added straight into the caller's array
+// (+=, one writer per atom).  Fewer scalar registers than the
+// replica-general form, measured faster on one large system.  LOG: the
+// pair record of pmnb_force_pairs is compiled in.
+template <bool F64, int ELEC, int ELSW, int BUF, bool LOG>
+__global__ void __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(3)))
+k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
+       const int *__restrict__ nitems, double *__restrict__ forces,
+       double *__restrict__ partial, int pstride,
+       unsigned long long *__restrict__ npairs, PairLog plog) {
+  constexpr int kNW = (BUF / kTmSplit + 31) / 32;  // mask words
+  __shared__ float4 recf[kTmWaves][BUF];
+  __shared__ int4 reci[kTmWaves][BUF];   // sorted index, type, mol, atom
+  __shared__ double4 recd[F64 ? kTmWaves : 1][F64 ? BUF : 1];
+  // exclusion masks of the wave's own atoms (excl_own layout)
+  __shared__ unsigned long long xm[kTmWaves][kTileAtoms][4];
+  __shared__ float4 ljs[F64 ? 1 : kQLj];  // FP32 LJ table (T * T <= kQLj)
+  if (!F64) {
+    for (int k = threadIdx.x; k < P.n_types * P.n_types; k += kBlock)
+      ljs[k] = S.ljf[k];
+    __syncthreads();
+  }
+  // FP64: values equal on all lanes of the wave are marked uniform (uni), so
+  // the work item, its z range and box live in scalar registers (in the
+  // FP32 kernel the extra scalar registers spill and cost more than they
+  // save)
+  auto U = [](auto v) { return F64 ? uni(v) : v; };
+  const int wave = U((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+  // FP64: blocks go to the 8 XCDs in turn; block b takes the items of block
+  // position (b % 8) * per + b / 8 (the grid is a multiple of 8), so each
+  // XCD works through one contiguous range of items and the candidates that
+  // neighbouring items share stay in that XCD's L2 (measured faster for the
+  // FP64 kernel, slightly slower for the FP32 one).
+  const int per = gridDim.x / 8;
+  const int blk =
+      F64 ? (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8) : blockIdx.x;
+  const int item_idx = blk * kTmWaves + wave;
+  double acc[ACC_N];
+#pragma unroll
+  for (int k = 0; k < ACC_N; k++) acc[k] = 0.0;
+  unsigned cnt = 0;
+  if (item_idx < *nitems) {
+    const int4 item = items[item_idx];
+    const int a0 = U(item.x), a1 = U(item.y), col = U(item.z);
+    const int cx = col / G.ny, cy = col % G.ny;
+    const int slot = lane & (kTileAtoms - 1), split = lane >> 4;
+    const int i = a0 + slot;
+    const bool active = i < a1;
+    const int ii = active ? i : a0;
+    const F32Params &F = P.f;
+    const float hi = F.cut2 + F.band, lo = F.cut2 - F.band;
+    const int zlo = U(S.cell[a0] % G.nz), zhi = U(S.cell[a1 - 1] % G.nz);
+    const float4 li = S.loc[ii];
+    const int zci = S.cell[ii] % G.nz;
+    const float xi = li.x, yi = li.y;
+    const float zi = (float)(zci - zlo) * G.csz + li.z;
+    const int ti = S.type[ii], mi = S.mol[ii], ai = S.atom[ii];
+    if (split == 0) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) xm[wave][slot][k] = S.xown[4 * ai + k];
+    }
+    wave_sync_lds();
+    const double qi = S.q[ii];
+    const double xd = S.x[ii], yd = S.y[ii], zd = S.z[ii];
+    const int ljbase = ti * P.n_types;  // row of this atom's type in ljs
+    const double *ljrow64 = S.lj + 4 * (ti * P.n_types);
+    // bounding box of the wave's atoms in home-relative coordinates
+    const float bx0 = U(wave_min(active ? xi : 1e30f)),
+                bx1 = U(wave_max(active ? xi : -1e30f));
+    const float by0 = U(wave_min(active ? yi : 1e30f)),
+                by1 = U(wave_max(active ? yi : -1e30f));
+    const float bz0 = U(wave_min(active ? zi : 1e30f)),
+                bz1 = U(wave_max(active ? zi : -1e30f));
+    int zA = zlo - G.rz, zB = zhi + G.rz;
+    const bool clampz = zB - zA + 1 > G.nz;
+    if (clampz) zB = zA + G.nz - 1;
+    const float Lz = F.Lz, iLz = F.iLz;
+    long long Fx = 0, Fy = 0, Fz = 0;
+    double fx64 = 0.0, fy64 = 0.0, fz64 = 0.0;
+    const bool vir = (P.flags & 3u) != 0;
+    int nbuf = 0;
+
+    // FP32 partials of one buffer (flushed by process)
+    float px = 0.f, py = 0.f, pz = 0.f, pelj = 0.f, peel = 0.f;
+    float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
+
+    // One pair: candidate record k of this wave's buffer, seen from atom i
+    // (each pair is seen from both atoms: half its energy and virial here).
+    // WZ: minimum image along z, 0 no, 1 yes, 2 as the runtime flag (a
+    // compile-time tag, so the common case carries no select).
+    auto eval = [&](const float4 &r, const int4 &q4, int k,
+                    auto WZ) __attribute__((always_inline)) {
+      constexpr int M = decltype(WZ)::value;
+      const int t = q4.x;
+      if (t == i) return;
+      float dx = r.x - xi, dy = r.y - yi, dz = r.z - zi;
+      if (M == 1 || (M == 2 && clampz)) dz = fmaf(-Lz, rintf(dz * iLz), dz);
+      const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+      double ex = 0.0, ey = 0.0, ez = 0.0, e2d = 0.0;
+      if (F64) {
+        const double4 rd = recd[wave][k];
+        min_diff_exact(*S.box, xd, yd, zd, rd.x, rd.y, rd.z, ex, ey, ez, e2d);
+        if (e2d > P.cut2) return;
+      } else if (r2 >= lo) {
+        // near the cutoff: decide with the reference's FP64 minimum image
+        min_diff_exact(*S.box, xd, yd, zd, S.x[t], S.y[t], S.z[t], ex, ey, ez,
+                       e2d);
+        if (e2d > P.cut2) return;
+      }
+      const bool same = q4.z == mi;
+      const int excl = same ? excl_own(xm[wave][slot], q4.w - ai) : 0;
+      if (excl == 2) return;
+      if (i < t) {
+        cnt++;
+        if (LOG) log_pair(plog, ai, q4.w, excl);
+      }
+      const bool molv = (P.flags & 2u) && !same;
+      if (F64) {
+        const double *lp = ljrow64 + 4 * q4.y;
+        double ff, e1, e2;
+        pair_f64<ELEC, ELSW>(P, e2d, excl != 1 ? lp[0] : lp[2],
+                             excl != 1 ? lp[1] : lp[3], qi * recd[wave][k].w,
+                             excl, ff, e1, e2);
+        acc[ACC_ELJ] += 0.5 * e1;
+        acc[ACC_EEL] += 0.5 * e2;
+        const double gx = ex * ff, gy = ey * ff, gz = ez * ff;
+        fx64 -= gx;
+        fy64 -= gy;
+        fz64 -= gz;
+        if (molv || (P.flags & 1u))
+          add_virial(acc, ACC_VXX, gx, gy, gz, ex, ey, ez, 0.5);
+        if (molv) {
+          double cdx, cdy, cdz, cd2;
+          min_diff_exact(*S.box, S.com[3 * mi], S.com[3 * mi + 1],
+                         S.com[3 * mi + 2], S.com[3 * q4.z],
+                         S.com[3 * q4.z + 1], S.com[3 * q4.z + 2], cdx, cdy,
+                         cdz, cd2);
+          add_virial(acc, ACC_MXX, gx, gy, gz, cdx, cdy, cdz, 0.5);
+        }
+      } else {
+        const float4 lp = ljs[ljbase + q4.y];
+        float ff, e1, e2;
+        pair_f32<ELEC>(F, r2, excl != 1 ? lp.x : lp.z, excl != 1 ? lp.y : lp.w,
+                       li.w * r.w, excl, ff, e1, e2);
+        pelj += e1;
+        peel += e2;
+        const float gx = dx * ff, gy = dy * ff, gz = dz * ff;
+        px -= gx;
+        py -= gy;
+        pz -= gz;
+        if (vir) {
+          if (molv || (P.flags & 1u)) {
+            v0 = fmaf(gx, dx, v0);
+            v1 = fmaf(gx, dy, v1);
+            v2 = fmaf(gx, dz, v2);
+            v3 = fmaf(gy, dy, v3);
+            v4 = fmaf(gy, dz, v4);
+            v5 = fmaf(gz, dz, v5);
+          }
+          if (molv) {
+            double cdx, cdy, cdz, cd2;
+            min_diff_exact(*S.box, S.com[3 * mi], S.com[3 * mi + 1],
+                           S.com[3 * mi + 2], S.com[3 * q4.z],
+                           S.com[3 * q4.z + 1], S.com[3 * q4.z + 2], cdx, cdy,
+                           cdz, cd2);
+            add_virial(acc, ACC_MXX, gx, gy, gz, cdx, cdy, cdz, 0.5);
+          }
+        }
+      }
+    };
+
+    // Tests the buffered candidates against this lane's atom, then
+    // evaluates the pairs that pass; the next pair's records are read from
+    // LDS before the current pair's arithmetic.
+    auto process = [&](int n) __attribute__((always_inline)) {
+      // records n .. n + 3 (the last group of 4) get NaN coordinates, which
+      // fail the distance test
+      if (lane < kTmSplit && n + lane < BUF) {
+        const float nan = __builtin_nanf("");
+        recf[wave][n + lane] = make_float4(nan, nan, nan, 0.f);
+      }
+      wave_sync_lds();
+      unsigned m[kNW];
+      if (clampz)
+        tile_test<true>(recf[wave], n, split, xi, yi, zi, hi, Lz, iLz, m);
+      else
+        tile_test<false>(recf[wave], n, split, xi, yi, zi, hi, Lz, iLz, m);
+      if (!active) {
+#pragma unroll
+        for (int w = 0; w < kNW; w++) m[w] = 0u;
+      }
+      const int jmax = (n + kTmSplit - 1) / kTmSplit;
+      px = py = pz = pelj = peel = 0.f;
+      v0 = v1 = v2 = v3 = v4 = v5 = 0.f;
+      auto run = [&](auto WZ) __attribute__((always_inline)) {
+        if (!mask_any(m)) return;
+        if constexpr (F64) {
+          int k = mask_pop<0, kNW, false>(m, split, jmax);
+          float4 r = recf[wave][k];
+          int4 q4 = reci[wave][k];
+          while (true) {
+            const bool more = mask_any(m);
+            int kn = k;
+            if (more) kn = mask_pop<0, kNW, false>(m, split, jmax);
+            const float4 rn = recf[wave][kn];
+            const int4 q4n = reci[wave][kn];
+            eval(r, q4, k, WZ);
+            if (!more) break;
+            r = rn;
+            q4 = q4n;
+            k = kn;
+          }
+        } else {
+          // two record sets in turn (no register moves); the next pop is
+          // taken unconditionally and its index replaced by 0 when the
+          // mask was empty, so every LDS read stays in the buffer
+          int k0 = mask_pop<0, kNW, true>(m, split, jmax);
+          float4 r0 = recf[wave][k0];
+          int4 q0 = reci[wave][k0];
+          while (true) {
+            const bool more1 = mask_any(m);
+            int k1 = mask_pop<0, kNW, true>(m, split, jmax);
+            k1 = more1 ? k1 : 0;
+            const float4 r1 = recf[wave][k1];
+            const int4 q1 = reci[wave][k1];
+            eval(r0, q0, k0, WZ);
+            if (!more1) break;
+            const bool more0 = mask_any(m);
+            k0 = mask_pop<0, kNW, true>(m, split, jmax);
+            k0 = more0 ? k0 : 0;
+            r0 = recf[wave][k0];
+            q0 = reci[wave][k0];
+            eval(r1, q1, k1, WZ);
+            if (!more0) break;
+          }
+        }
+      };
+      // FP64 keeps one loop with a runtime flag (fewer spilled registers)
+      if (F64)
+        run(std::integral_constant<int, 2>{});
+      else if (clampz)
+        run(std::integral_constant<int, 1>{});
+      else
+        run(std::integral_constant<int, 0>{});
+      if (!F64) {
+        Fx += to_fixed((double)px);
+        Fy += to_fixed((double)py);
+        Fz += to_fixed((double)pz);
+        acc[ACC_ELJ] += 0.5 * (double)pelj;
+        acc[ACC_EEL] += 0.5 * (double)peel;
+        if (vir) {
+          acc[ACC_VXX] += 0.5 * (double)v0;
+          acc[ACC_VXY] += 0.5 * (double)v1;
+          acc[ACC_VXZ] += 0.5 * (double)v2;
+          acc[ACC_VYY] += 0.5 * (double)v3;
+          acc[ACC_VYZ] += 0.5 * (double)v4;
+          acc[ACC_VZZ] += 0.5 * (double)v5;
+        }
+      }
+      wave_sync_lds();  // buffer is refilled next
+    };
+
+    auto emit = [&](bool ok, int t, float oxc, float oyc,
+                    float z0) __attribute__((always_inline)) {
+      float4 rf = make_float4(0.f, 0.f, 0.f, 0.f);
+      int4 ri = make_int4(0, 0, 0, 0);
+      if (ok) {
+        const float4 lt = S.loc[t];
+        ri = S.rec[t];
+        rf.x = oxc + lt.x;
+        rf.y = oyc + lt.y;
+        rf.z = z0 + lt.z;
+        rf.w = lt.w;
+        float gzv = fmaxf(0.f, fmaxf(bz0 - rf.z, rf.z - bz1));
+        if (clampz) gzv = 0.f;
+        const float gxv = fmaxf(0.f, fmaxf(bx0 - rf.x, rf.x - bx1));
+        const float gyv = fmaxf(0.f, fmaxf(by0 - rf.y, rf.y - by1));
+        ok = fmaf(gxv, gxv, fmaf(gyv, gyv, gzv * gzv)) <= hi;
+      }
+      const unsigned long long ball = __ballot(ok);
+      if (ok) {
+        const int pos = nbuf + lane_prefix(ball);
+        recf[wave][pos] = rf;
+        reci[wave][pos] = ri;
+        if (F64) recd[wave][pos] = make_double4(S.x[t], S.y[t], S.z[t],
+                                                S.q[t]);
+      }
+      nbuf += __popcll(ball);
+      if (nbuf > BUF - 64) {
+        process(nbuf);
+        nbuf = 0;
+      }
+    };
+    stream_levels<false>(S, G, 0, cx, cy, false, false, clampz, zlo, zhi, zA,
+                         zB, a0, bx0, bx1, by0, by1, bz0, bz1, hi, lane, emit);
+    if (nbuf > 0) process(nbuf);
+    // the 4 lanes of an atom add their partial forces
+    if (F64) {
+      fx64 += __shfl_down(fx64, 32, 64);
+      fy64 += __shfl_down(fy64, 32, 64);
+      fz64 += __shfl_down(fz64, 32, 64);
+      fx64 += __shfl_down(fx64, 16, 64);
+      fy64 += __shfl_down(fy64, 16, 64);
+      fz64 += __shfl_down(fz64, 16, 64);
+      if (active && split == 0) {
+        forces[3 * ai] += fx64;
+        forces[3 * ai + 1] += fy64;
+        forces[3 * ai + 2] += fz64;
+      }
+    } else {
+      Fx += __shfl_down(Fx, 32, 64);
+      Fy += __shfl_down(Fy, 32, 64);
+      Fz += __shfl_down(Fz, 32, 64);
+      Fx += __shfl_down(Fx, 16, 64);
+      Fy += __shfl_down(Fy, 16, 64);
+      Fz += __shfl_down(Fz, 16, 64);
+      if (active && split == 0) {
+        forces[3 * ai] += (double)Fx * kFixedInv;
+        forces[3 * ai + 1] += (double)Fy * kFixedInv;
+        forces[3 * ai + 2] += (double)Fz * kFixedInv;
+      }
+    }
+  }
+  // this item's energies and virials (one row per item)
+  if (item_idx < *nitems) {
+#pragma unroll
+    for (int k = 0; k < ACC_N; k++) {
+      double v = acc[k];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+      if (lane == 0) partial[(size_t)k * pstride + item_idx] = v;
+    }
+  }
+  unsigned long long tot = block_count(cnt);
   if (threadIdx.x == 0 && tot) atomicAdd(npairs, tot);
 }
 
