@@ -23,8 +23,12 @@
 // its force uses the exact erfc derivative), so the Jacobian is that of the
 // forces actually computed.
 //
-// Work decomposition, candidate streaming, test and pair queue are those of
-// k_tq (pmnb_queue.h).  Each atom's dL/dx and dL/dq~ are summed by its own
+// Work decomposition and candidate streaming are those of the tile-path
+// pair kernels (one wave per work item of 16 sorted atoms, candidates of the
+// 3 x 3 neighbour columns buffered in LDS after a bounding-box test); the
+// test appends passing (atom, record) entries to an LDS pair queue at
+// ballot + mbcnt positions, which the wave consumes one pair per lane.  Each
+// atom's dL/dx and dL/dq~ are summed by its own
 // wave in per-lane LDS slots (fixed order); the LJ-parameter gradients are
 // summed per lane in registers (one per unordered type pair, selected by
 // comparison), then per item in a fixed order.
