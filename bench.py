@@ -116,11 +116,16 @@ def time_md(w, precision, steps, warmup, flags, torch):
     force.close()
     ms_eager = 1e3 * dt / steps
     ms_graph = 1e3 * dtg / (chunk * reps)
-    return {"value": ns_per_day(1e3 / ms_graph), "unit": "ns/day",
-            "ms_per_step": ms_graph, "steps": chunk * reps,
-            "graph": "%d replays of one captured run(%d)" % (reps, chunk),
+    # the headline is the faster of the two ways of running the same steps
+    graph_faster = ms_graph < ms_eager
+    ms = ms_graph if graph_faster else ms_eager
+    return {"value": ns_per_day(1e3 / ms), "unit": "ns/day",
+            "ms_per_step": ms, "steps": chunk * reps if graph_faster else steps,
+            "mode": ("graph: %d replays of one captured run(%d)" % (reps, chunk)
+                     if graph_faster else "eager: one run(%d) call" % steps),
             "eager_ms_per_step": ms_eager,
-            "eager_ns_per_day": ns_per_day(1e3 / ms_eager),
+            "graph_ms_per_step": ms_graph,
+            "graph": "%d replays of one captured run(%d)" % (reps, chunk),
             "mean_shake_sweeps": float(shake_its.mean()),
             "mean_rattle_sweeps": float(rattle_its.mean()),
             "unconverged_calls": int(unconverged + unconverged_g)}

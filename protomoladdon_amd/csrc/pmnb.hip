@@ -103,6 +103,10 @@ struct pmnb_ctx {
   // 6 FP64), all zero between evaluations (the finish kernel clears it)
   DevBuf d_ffix;
   const void *ffix_zeroed = nullptr;  // allocation last cleared
+  // d_count holds zeros at the start of the next evaluation on the tile path
+  // (the finish kernel clears it after the pair kernel): no clearing launch
+  bool counts_zero = false;
+  const void *count_seen = nullptr;   // d_count allocation the flag is about
   // pair record of the force kernels (parity tests): on for one evaluation
   PairLog plog{nullptr, 0, nullptr};
   DevBuf d_plog, d_plog_count;
@@ -488,6 +492,10 @@ int ensure_eval_buffers(pmnb_ctx *ctx) {
   const size_t N = (size_t)std::max(n, 1);
   HIPCHK(ctx->d_cell_of.ensure(N * sizeof(int)));
   HIPCHK(ctx->d_count.ensure((size_t)nc * sizeof(int)));
+  if (ctx->count_seen != ctx->d_count.p) {
+    ctx->counts_zero = false;
+    ctx->count_seen = ctx->d_count.p;
+  }
   HIPCHK(ctx->d_start.ensure((size_t)nc * sizeof(int)));
   HIPCHK(ctx->d_fill.ensure(N * sizeof(int)));  // per-atom rank in its cell
   HIPCHK(ctx->d_sorted.ensure(N * sizeof(int)));
@@ -770,16 +778,20 @@ EvalBufs ctx_bufs(const pmnb_ctx *ctx) {
 }
 
 // On the tile path every replica's cells are numbered from its offset
-// (ctx->grid.ncell is then the total).
+// (ctx->grid.ncell is then the total).  counts_zero: B.count is known to
+// hold zeros (tile path only), so no clearing launch is needed; the
+// work-item kernel clears the pair counter.  Leaves ctx->counts_zero false:
+// the counts are in use.
 int build_cells(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
-                hipStream_t s) {
+                hipStream_t s, bool counts_zero = false) {
   const int n = ctx->n, nc = ctx->grid.ncell;
   const TileRep *reps = B.reps;
   const int *rep_of = ctx->batch ? ctx->d_rep_of.as<int>() : nullptr;
   int *rank = B.rank;
-  hipLaunchKernelGGL(k_clear, dim3(blocks_for(nc)), dim3(kBlock), 0, s,
-                     B.count, nc,
-                     B.npairs);
+  ctx->counts_zero = false;
+  if (!(counts_zero && ctx->use_tile))
+    hipLaunchKernelGGL(k_clear, dim3(blocks_for(nc)), dim3(kBlock), 0, s,
+                       B.count, nc, B.npairs);
   if (n > 0 && ctx->use_tile) {
     hipLaunchKernelGGL(k_assign_r<TileRep>, dim3(blocks_for(n)), dim3(kBlock),
                        0, s, reps, rep_of, d_pos, n, B.cell_of,
@@ -795,8 +807,8 @@ int build_cells(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
                        B.count, B.start, nc,
                        reps, B.col_rep, ctx->ncol_total,
                        (int)ctx->reps.size(), B.items,
-                       B.nitems, B.rep_first);
-  } else {
+                       B.nitems, B.rep_first, B.npairs);
+  } else if (!ctx->use_tile) {
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s,
                        B.count, B.start, nc);
   }
@@ -869,9 +881,11 @@ int check_ready(pmnb_ctx *ctx) {
   return PMNB_OK;
 }
 
+// own: B are the context's buffers (not the static scratch), whose cell
+// counts the finish kernel leaves cleared for the next evaluation.
 int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
                     double *d_forces, double *d_out, const double *d_com,
-                    uint32_t flags, hipStream_t s) {
+                    uint32_t flags, hipStream_t s, bool own) {
   int rc = PMNB_OK;
   if ((flags & PMNB_FLAG_MOLVIRIAL) && !d_com)
     return fail(ctx, PMNB_ERR_ARG, "molecular virial needs molecule centres");
@@ -889,7 +903,7 @@ int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
     ev_stop = ctx->ev_pool[ctx->ev_used + 2];
     ctx->ev_used += 3;
   }
-  rc = build_cells(ctx, B, d_pos, s);
+  rc = build_cells(ctx, B, d_pos, s, own && ctx->counts_zero);
   if (rc) return rc;
   const ForceParams P = make_params(ctx, flags);
   const Sorted S = make_sorted(ctx, B, d_com);
@@ -925,17 +939,20 @@ int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
     if (ev_stop) HIPCHK(hipEventRecord(ev_stop, s));
     // fixed-point forces into the caller's array (unless the pair kernel
     // added them itself), per-replica sums
+    // and the cell counts cleared for the next evaluation
     const int ab = direct ? 0 : blocks_for(ctx->n, 1024);
-    const dim3 fg((unsigned)(ab + nrep * ACC_N));
+    const int rb = nrep * ACC_N, nc = ctx->grid.ncell;
+    const dim3 fg((unsigned)(ab + rb + blocks_for(nc, 1024)));
     if (mixed)
       hipLaunchKernelGGL(k_half_finish<false>, fg, dim3(1024), 0, s, ffix,
                          S.atom, ctx->n, d_forces, ab, part, max_items,
-                         rep_first, d_out);
+                         rep_first, d_out, rb, B.count, nc);
     else
       hipLaunchKernelGGL(k_half_finish<true>, fg, dim3(1024), 0, s, ffix,
                          S.atom, ctx->n, d_forces, ab, part, max_items,
-                         rep_first, d_out);
+                         rep_first, d_out, rb, B.count, nc);
     HIPCHK(hipGetLastError());
+    if (own) ctx->counts_zero = true;
     return PMNB_OK;
   }
   // only / cap of the direct-scan kernels: everything (no list) or the
@@ -1020,7 +1037,7 @@ int run_device(pmnb_ctx *ctx, const double *d_pos, double *d_forces,
   const int rc = ensure_eval_buffers(ctx);
   if (rc) return rc;
   return run_device_bufs(ctx, ctx_bufs(ctx), d_pos, d_forces, d_out, d_com,
-                         flags, s);
+                         flags, s, true);
 }
 
 }  // namespace
@@ -1781,6 +1798,9 @@ int pmnb_batch_run_graph(pmnb_batch *b, const double *d_positions,
     if (!b->cap_stream)
       HIPCHK(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
     hipStream_t cs = b->cap_stream;
+    // a replay must not rely on what ran before it: the captured
+    // evaluation clears the cell counts itself
+    ctx->counts_zero = false;
     HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((n3 + kBlock - 1) / kBlock)),
                        dim3(kBlock), 0, cs, d_forces, n3);
@@ -1789,6 +1809,7 @@ int pmnb_batch_run_graph(pmnb_batch *b, const double *d_positions,
     rc = run_device(ctx, d_positions, d_forces, d_out, nullptr, flags, cs);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(cs, &g);
+    ctx->counts_zero = false;  // the captured work has not run
     if (rc) {
       if (g) (void)hipGraphDestroy(g);
       return rc;
@@ -1919,7 +1940,7 @@ struct pmnb_leapfrog {
   bool shake = false, rattle = false;  // attached and effective
   int n = 0, ng = 0, nf = 0, ngb = 0, nfb = 0, n_cons = 0, shape = 0;
   DevBuf mass, rmass, g_atoms, g_cons, g_rest, g_rinv, free_atoms, last, xs,
-      vs, partial_s, partial_r, iters, nonconv, hint;
+      vs, xk, vk, vr, partial_s, partial_r, iters, nonconv, hint;
   bool last_ready = false;
   long long calls = 0, iters_cap = 0;
   long long scalls[3] = {0, 0, 0};  // calls per stage
@@ -1987,6 +2008,9 @@ LfDev lf_dev(pmnb_leapfrog *lf, double *x, double *v, double *f,
   d.hint = lf->hint.as<int>();
   d.xs = lf->xs.as<double>();
   d.vs = lf->vs.as<double>();
+  d.xk = lf->xk.as<double>();
+  d.vk = lf->vk.as<double>();
+  d.vr = lf->vr.as<double>();
   d.iters = lf->iters_cap > 0 ? lf->iters.as<int>() : nullptr;
   d.iters_cap = lf->iters_cap;
   d.nonconv = lf->nonconv.as<int>();
@@ -2300,6 +2324,9 @@ int lf_create_impl(pmnb_ctx *ctx, const pmnb_lf_params *params,
   if (e == hipSuccess) e = lf->last.ensure(N * 3 * sizeof(double));
   if (e == hipSuccess) e = lf->xs.ensure(N * 3 * sizeof(double));
   if (e == hipSuccess) e = lf->vs.ensure(N * 3 * sizeof(double));
+  if (e == hipSuccess) e = lf->xk.ensure(N * 3 * sizeof(double));
+  if (e == hipSuccess) e = lf->vk.ensure(N * 3 * sizeof(double));
+  if (e == hipSuccess) e = lf->vr.ensure(N * 3 * sizeof(double));
   if (e == hipSuccess)
     e = lf->partial_s.ensure((size_t)(std::max(p.shake_max_iter, 0) + 1) * ngb *
                              sizeof(double));
@@ -2452,7 +2479,8 @@ void pmnb_lf_destroy(pmnb_leapfrog *lf) {
   if (lf->cap_stream) (void)hipStreamDestroy(lf->cap_stream);
   DevBuf *bufs[] = {&lf->mass, &lf->rmass, &lf->g_atoms, &lf->g_cons,
                     &lf->g_rest, &lf->g_rinv, &lf->free_atoms, &lf->last,
-                    &lf->xs, &lf->vs, &lf->partial_s, &lf->partial_r,
+                    &lf->xs, &lf->vs, &lf->xk, &lf->vk, &lf->vr,
+                    &lf->partial_s, &lf->partial_r,
                     &lf->iters, &lf->nonconv, &lf->hint, &lf->bptr,
                     &lf->bent, &lf->aptr, &lf->aent, &lf->bonds, &lf->bk,
                     &lf->br0, &lf->angles, &lf->ak, &lf->at0, &lf->ubk,
@@ -2599,11 +2627,15 @@ int pmnb_lf_run_graph(pmnb_leapfrog *lf, double *d_positions,
     if (!lf->cap_stream)
       HIPCHK(hipStreamCreateWithFlags(&lf->cap_stream, hipStreamNonBlocking));
     hipStream_t cs = lf->cap_stream;
+    // a replay must not rely on what ran before it: its first evaluation
+    // clears the cell counts itself
+    ctx->counts_zero = false;
     HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
     rc = pmnb_lf_run_device(lf, d_positions, d_velocities, d_forces, d_out,
                             n_steps, flags, cs);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(cs, &g);
+    ctx->counts_zero = false;  // the captured work has not run
     if (rc) {
       if (g) (void)hipGraphDestroy(g);
       return rc;
@@ -3455,7 +3487,7 @@ int pmnb_evaluate_device_noalloc(pmnb_ctx *ctx, const double *d_positions,
                      ctx->ncol_total, B.ffix, nfix);
   HIPCHK(hipGetLastError());
   return run_device_bufs(ctx, B, d_positions, d_forces, d_out, d_mol_com,
-                         flags, s);
+                         flags, s, false);
 }
 
 }  // extern "C"

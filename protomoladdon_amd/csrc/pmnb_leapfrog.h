@@ -52,7 +52,7 @@ namespace pmnb {
 constexpr int kLfBlock = 256;
 constexpr int kLfWaves = kLfBlock / 64;
 constexpr int kLfFirstLevels = 16;  // first probe's levels with no history
-constexpr int kLfMargin = 2;        // levels past the previous sweep count
+constexpr int kLfMargin = 1;        // levels past the previous sweep count
 constexpr int kLfMaxAtoms = 4;      // largest group
 constexpr int kLfMaxCons = 6;       // most constraints in a group
 
@@ -114,6 +114,8 @@ struct LfDev {
   double *partial[3];       // by stage: (maxit + 1) levels x ngb blocks
   int *hint;                // [8 rep + 2 stage + call parity]: sweep count
   double *xs, *vs;          // state after the first probe (per atom)
+  double *xk, *vk, *vr;     // state after the predicted sweep count, stored
+                            // by the probe: SHAKE x, v; RATTLE v (per atom)
   int *iters;               // sweep count per stage call and replica
                             // ([call * n_reps + rep], optional)
   long long iters_cap;
@@ -314,14 +316,30 @@ __device__ __forceinline__ void pre_op(D3 &x, D3 &v, D3 f, double m,
   }
 }
 
+// The previous call's sweep count of a stage (-1 before the first call):
+// the same value in every kernel of call `scall` (the apply kernel of call c
+// writes slot c & 1, calls read slot (c + 1) & 1).
+__device__ __forceinline__ int lf_hint(const LfDev &d, int stage,
+                                       long long scall, int rep) {
+  return d.hint[8 * rep + stage * 2 + (int)((scall + 1) & 1)];
+}
+
 // Levels the first probe of stage call `scall` records: the previous call's
-// sweep count plus a margin (the same value in every kernel of the call:
-// the apply kernel of call c writes slot c & 1, calls read slot (c + 1) & 1).
+// sweep count plus a margin.
 __device__ __forceinline__ int lf_first_levels(const LfDev &d, int stage,
                                                long long scall, int rep) {
-  const int hint = d.hint[8 * rep + stage * 2 + (int)((scall + 1) & 1)];
+  const int hint = lf_hint(d, stage, scall, rep);
   const int L = hint < 0 ? kLfFirstLevels : hint + kLfMargin + 1;
   return min(max(L, 1), d.maxit[stage] + 1);
+}
+
+// Predicted sweep count: the first probe of the call stores the state after
+// this many sweeps (xk, vk or vr), and the apply starts from it when the
+// decided count is at least as large; 0: nothing stored.  It never exceeds
+// the first probe's levels, after the last of which the probe still sweeps.
+__device__ __forceinline__ int lf_kept_level(const LfDev &d, int stage,
+                                             long long scall, int rep) {
+  return max(lf_hint(d, stage, scall, rep), 0);
 }
 
 // First level in [j0, j1) whose mean error over the replica's constraints
@@ -329,11 +347,13 @@ __device__ __forceinline__ int lf_first_levels(const LfDev &d, int stage,
 // per-block partials in the same order, so they all return the same level.
 // A thread sums blocks sub, sub + 8, ... (relative to the replica's first)
 // of one level; its loads are issued in batches before the (ordered)
-// additions so that they overlap.
+// additions so that they overlap.  The first wave then tests the levels of
+// the chunk in parallel (one lane per level) and takes the lowest passing.
 __device__ __forceinline__ int lf_scan(const LfDev &d, int stage, int j0,
                                        int j1, const LfRep &R) {
   constexpr int kSub = 8, kChunk = kLfBlock / kSub, kBatch = 8;
-  __shared__ double red[kChunk][kSub];
+  static_assert(kChunk <= 64, "one lane per level of a chunk");
+  __shared__ double red[kSub][kChunk];
   __shared__ int hit;
   const int t = threadIdx.x;
   const double *P = d.partial[stage] + R.gb0;
@@ -357,19 +377,19 @@ __device__ __forceinline__ int lf_scan(const LfDev &d, int stage, int j0,
           for (int u = 0; u < kBatch; u++) s += v[u];
         }
       }
-      red[jj][sub] = s;
+      red[sub][jj] = s;
     }
-    if (t == 0) hit = -1;
     __syncthreads();
-    if (t == 0) {
-      for (int jj = 0; jj < n; jj++) {
+    if (t < 64) {
+      bool pass = false;
+      if (t < n) {
         double s = 0;
-        for (int k = 0; k < kSub; k++) s += red[jj][k];
-        if (s / R.ncons <= d.eps[stage]) {
-          hit = c0 + jj;
-          break;
-        }
+#pragma unroll
+        for (int k = 0; k < kSub; k++) s += red[k][t];
+        pass = s / R.ncons <= d.eps[stage];
       }
+      const unsigned long long m = __ballot(pass);
+      if (t == 0) hit = m ? c0 + __builtin_ctzll(m) : -1;
     }
     __syncthreads();
     const int hv = hit;
@@ -390,39 +410,64 @@ __device__ __forceinline__ int lf_decide(const LfDev &d, int stage,
   return k2 >= 0 ? k2 : d.maxit[stage] + 1;
 }
 
-__device__ __forceinline__ double wave_sum(double s) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-  return s;
-}
-
 // Records the error levels [j0, j1) of a stage for this block's groups from
-// the state in g (active threads only): one partial per block and level,
-// the four wave sums added in wave order.  With `store`, the state after
-// j1 - j0 sweeps is left in xs / vs.  Block-wide (all threads call it).
+// the state in g (active threads only): one partial per block and level.
+// The levels run back to back, each thread leaving its group's error in
+// LDS; each chunk of levels is then summed in a fixed order (16 runs of 16
+// threads per level, then the 16 run sums).  With `store`, the state after
+// j1 - j0 sweeps is left in xs / vs.  With kept > 0 (and j0 = 0), the full
+// state after `kept` sweeps (velocities updated as the apply does) is left
+// in xk, vk (SHAKE) or vr (RATTLE).  Block-wide (all threads call it).
 template <int STAGE, class SH>
 __device__ __forceinline__ void lf_probe(const LfDev &d, LfGroup<SH> &g,
-                                         bool act, int j0, int j1, bool store) {
-  constexpr int kLevels = 32;
-  __shared__ double wred[kLfWaves][kLevels];
-  const int wave = threadIdx.x >> 6;
+                                         bool act, int j0, int j1, bool store,
+                                         int kept) {
+  constexpr int kLevels = 16, kRun = kLfBlock / kLevels;
+  static_assert(kRun == 16 && kLevels * kRun == kLfBlock, "reduction shape");
+  __shared__ double elev[kLevels][kLfBlock];
+  __shared__ double runs[kRun][kLevels];
+  const int t = threadIdx.x;
   double *P = d.partial[STAGE] + blockIdx.x;
   for (int c0 = j0; c0 < j1; c0 += kLevels) {
     const int n = min(kLevels, j1 - c0);
-    // branch-free level body (inactive threads hold a copy of a real
-    // group and only contribute 0): the wave sum of one level overlaps the
-    // next sweep.  The sweep after the last level is only used by `store`.
+    // inactive threads hold a copy of a real group and only contribute 0.
+    // The sweep after the last level is only used by `store` and `kept`.
+    // SHAKE's velocity updates (which never feed back into positions or
+    // errors) are made up to the kept level only.
     for (int jj = 0; jj < n; jj++) {
+      const int lev = c0 + jj;
       const double e = group_error<STAGE>(g);
-      sweep<STAGE>(g, d.dtc, false);
-      wred[wave][jj] = wave_sum(act ? e : 0.0);
+      elev[jj][t] = act ? e : 0.0;
+      sweep<STAGE>(g, d.dtc, lev < kept);
+      if (lev + 1 == kept && act) {
+#pragma unroll
+        for (int k = 0; k < SH::NA; k++) {
+          if (g.idx[k] < 0) continue;
+          if (STAGE == kStShake) {
+            st3(d.xk, g.idx[k], g.x[k]);
+            st3(d.vk, g.idx[k], g.v[k]);
+          } else {
+            st3(d.vr, g.idx[k], g.v[k]);
+          }
+        }
+      }
     }
     __syncthreads();
-    if ((int)threadIdx.x < n) {
+    {
+      const int jj = t / kRun, r = t % kRun;
+      double s = 0;
+      if (jj < n) {
+#pragma unroll
+        for (int k = 0; k < kRun; k++) s += elev[jj][r + kRun * k];
+      }
+      runs[r][jj] = s;
+    }
+    __syncthreads();
+    if (t < n) {
       double s = 0;
 #pragma unroll
-      for (int w = 0; w < kLfWaves; w++) s += wred[w][threadIdx.x];
-      P[(size_t)(c0 + threadIdx.x) * d.ngb] = s;
+      for (int r = 0; r < kRun; r++) s += runs[r][t];
+      P[(size_t)(c0 + t) * d.ngb] = s;
     }
     __syncthreads();
   }
@@ -504,7 +549,26 @@ __global__ __launch_bounds__(kLfBlock) void k_lf_step(LfDev d, long long sc_appl
   load_group(d, R.goff + (act ? local : R.ng - 1), g,
              APPLY == kStShake || PROBE == kStShake);
   if (act) {
-    for (int s = 0; s < k; s++) sweep<APPLY>(g, d.dtc, true);
+    int s0 = 0;
+    if (APPLY != kStNone) {
+      // start from the state this call's probe kept, when it is on the way
+      const int kept = lf_kept_level(d, APPLY, sc_apply, rep);
+      if (kept > 0 && k >= kept) {
+#pragma unroll
+        for (int a = 0; a < SH::NA; a++) {
+          const int i = g.idx[a];
+          if (i < 0) continue;
+          if (APPLY == kStShake) {
+            g.x[a] = ld3(d.xk, i);
+            g.v[a] = ld3(d.vk, i);
+          } else {
+            g.v[a] = ld3(d.vr, i);
+          }
+        }
+        s0 = kept;
+      }
+    }
+    for (int s = s0; s < k; s++) sweep<APPLY>(g, d.dtc, true);
 #pragma unroll
     for (int a = 0; a < SH::NA; a++) {
       const int i = g.idx[a];
@@ -523,7 +587,8 @@ __global__ __launch_bounds__(kLfBlock) void k_lf_step(LfDev d, long long sc_appl
   }
   if (PROBE != kStNone) {
     const int L = lf_first_levels(d, PROBE, sc_probe, rep);
-    lf_probe<PROBE>(d, g, act, 0, L, L <= d.maxit[PROBE]);
+    lf_probe<PROBE>(d, g, act, 0, L, L <= d.maxit[PROBE],
+                    lf_kept_level(d, PROBE, sc_probe, rep));
   }
 }
 
@@ -547,7 +612,7 @@ __global__ __launch_bounds__(kLfBlock) void k_lf_probe(LfDev d, long long scall)
     if (STAGE == kStShake) g.x[a] = ld3(d.xs, g.idx[a]);
     else g.v[a] = ld3(d.vs, g.idx[a]);
   }
-  lf_probe<STAGE>(d, g, act, L, d.maxit[STAGE] + 1, false);
+  lf_probe<STAGE>(d, g, act, L, d.maxit[STAGE] + 1, false, 0);
 }
 
 // ------------------------------------------------------------ bonded terms

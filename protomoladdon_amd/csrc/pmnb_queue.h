@@ -121,13 +121,16 @@ __device__ __forceinline__ void build_items(
 }
 
 // Cell starts (exclusive scan of the counts) and then the work items, in
-// one block: the items read the starts this block has just written.
+// one block: the items read the starts this block has just written.  It
+// also clears the pair counter of the evaluation.
 __global__ void __launch_bounds__(1024)
 k_scan_items(const int *__restrict__ count, int *__restrict__ start,
              int ncell, const TileRep *__restrict__ reps,
              const int *__restrict__ col_rep, int ncol, int nrep,
              int4 *__restrict__ items, int *__restrict__ nitems,
-             int *__restrict__ rep_first) {
+             int *__restrict__ rep_first,
+             unsigned long long *__restrict__ npairs) {
+  if (threadIdx.x == 0) *npairs = 0ull;
   scan_counts(count, start, ncell);
   __syncthreads();
   build_items(reps, col_rep, start, count, ncol, nrep, items, nitems,

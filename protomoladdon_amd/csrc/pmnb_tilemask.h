@@ -1074,14 +1074,16 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
 
 // After the pair kernel: blocks below atom_blocks convert the fixed-point
 // force of each sorted slot into the caller's forces (+= at the original
-// index) and clear it; the others sum the item partials of one (replica,
-// accumulator) each (finish_rep_block).
+// index) and clear it; the next rep_blocks sum the item partials of one
+// (replica, accumulator) each (finish_rep_block); the rest clear the cell
+// counts for the next evaluation (nothing reads them after the pair kernel).
 template <bool F64>
 __global__ void __launch_bounds__(1024)
 k_half_finish(long long *__restrict__ ffix, const int *__restrict__ atom,
               int n, double *__restrict__ forces, int atom_blocks,
               const double *__restrict__ partial, int pstride,
-              const int *__restrict__ rep_first, double *__restrict__ out) {
+              const int *__restrict__ rep_first, double *__restrict__ out,
+              int rep_blocks, int *__restrict__ count, int ncell) {
   constexpr int W = FixWords<F64>::W;
   if ((int)blockIdx.x < atom_blocks) {
     const int s = blockIdx.x * 1024 + threadIdx.x;
@@ -1103,6 +1105,11 @@ k_half_finish(long long *__restrict__ ffix, const int *__restrict__ atom,
     return;
   }
   const int b = blockIdx.x - atom_blocks;
+  if (b >= rep_blocks) {
+    const int c = (b - rep_blocks) * 1024 + threadIdx.x;
+    if (c < ncell) count[c] = 0;
+    return;
+  }
   finish_rep_block(partial, pstride, rep_first, out, b / ACC_N, b % ACC_N);
 }
 
