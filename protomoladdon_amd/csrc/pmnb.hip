@@ -15,6 +15,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/pmnb.h"
@@ -64,6 +65,11 @@ struct DevBuf {
 
 }  // namespace
 
+// pmnb_evaluate moves positions in in this many chunks (systems of at
+// least kHostChunkMin coordinates)
+constexpr int kHostChunks = 4;
+constexpr int kHostChunkMin = 1 << 16;
+
 struct pmnb_ctx {
   pmnb_params prm{};
   hipStream_t stream = nullptr;
@@ -78,6 +84,8 @@ struct pmnb_ctx {
   std::vector<int4> shell;  // (0,0,0) then the reference's delta list
   int per_pair_wrap = 0;
   long long last_pairs = 0;
+  // device counter of the last evaluation's pairs (pmnb_last_pair_count)
+  const unsigned long long *last_np = nullptr;
   // device: topology (original order)
   DevBuf d_type, d_q, d_mol, d_xfull, d_xmod, d_xown, d_lj, d_ljf;
   // device: per evaluation
@@ -113,6 +121,7 @@ struct pmnb_ctx {
   // pinned host staging of pmnb_evaluate: positions, forces, out + count
   double *h_stage = nullptr;
   size_t h_stage_bytes = 0;
+  hipEvent_t h_ev[1] = {};  // pmnb_evaluate: results are on the host
   int shape = 0;                  // tile-path kernel shape (mixed)
   int grid_cap = 0;               // persistent pair grid: max blocks (0: none)
   DevBuf d_items, d_nitems, d_reps, d_col_rep, d_rep_first;
@@ -882,11 +891,18 @@ int check_ready(pmnb_ctx *ctx) {
 }
 
 // own: B are the context's buffers (not the static scratch), whose cell
-// counts the finish kernel leaves cleared for the next evaluation.
+// counts the finish kernel leaves cleared for the next evaluation.  fresh:
+// on the tile path the forces and the 20 output doubles are written, not
+// added to (the caller's arrays need no clearing).  np_out (tile path):
+// the finish kernel also stores the pair count there.
 int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
                     double *d_forces, double *d_out, const double *d_com,
-                    uint32_t flags, hipStream_t s, bool own) {
+                    uint32_t flags, hipStream_t s, bool own,
+                    bool fresh = false,
+                    unsigned long long *np_out = nullptr) {
   int rc = PMNB_OK;
+  flags &= ~kFlagFresh;  // internal: never taken from a caller
+  if (fresh && ctx->use_tile) flags |= kFlagFresh;
   if ((flags & PMNB_FLAG_MOLVIRIAL) && !d_com)
     return fail(ctx, PMNB_ERR_ARG, "molecular virial needs molecule centres");
   // timing: [start, built) = cell build (+ work items or neighbour lists),
@@ -905,6 +921,7 @@ int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
   }
   rc = build_cells(ctx, B, d_pos, s, own && ctx->counts_zero);
   if (rc) return rc;
+  ctx->last_np = B.npairs;
   const ForceParams P = make_params(ctx, flags);
   const Sorted S = make_sorted(ctx, B, d_com);
   const int nb = std::max(1, blocks_for(ctx->n));
@@ -942,15 +959,18 @@ int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
     // and the cell counts cleared for the next evaluation
     const int ab = direct ? 0 : blocks_for(ctx->n, 1024);
     const int rb = nrep * ACC_N, nc = ctx->grid.ncell;
+    const int fresh = (flags & kFlagFresh) ? 1 : 0;
     const dim3 fg((unsigned)(ab + rb + blocks_for(nc, 1024)));
     if (mixed)
       hipLaunchKernelGGL(k_half_finish<false>, fg, dim3(1024), 0, s, ffix,
                          S.atom, ctx->n, d_forces, ab, part, max_items,
-                         rep_first, d_out, rb, B.count, nc);
+                         rep_first, d_out, rb, B.count, nc, fresh, np,
+                         np_out);
     else
       hipLaunchKernelGGL(k_half_finish<true>, fg, dim3(1024), 0, s, ffix,
                          S.atom, ctx->n, d_forces, ab, part, max_items,
-                         rep_first, d_out, rb, B.count, nc);
+                         rep_first, d_out, rb, B.count, nc, fresh, np,
+                         np_out);
     HIPCHK(hipGetLastError());
     if (own) ctx->counts_zero = true;
     return PMNB_OK;
@@ -1033,11 +1053,11 @@ int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
 
 int run_device(pmnb_ctx *ctx, const double *d_pos, double *d_forces,
                double *d_out, const double *d_com, uint32_t flags,
-               hipStream_t s) {
+               hipStream_t s, bool fresh = false) {
   const int rc = ensure_eval_buffers(ctx);
   if (rc) return rc;
   return run_device_bufs(ctx, ctx_bufs(ctx), d_pos, d_forces, d_out, d_com,
-                         flags, s, true);
+                         flags, s, true, fresh);
 }
 
 }  // namespace
@@ -1265,28 +1285,33 @@ int pmnb_evaluate(pmnb_ctx *ctx, const double *positions, double *forces,
   const size_t N = (size_t)std::max(ctx->n, 1);
   const size_t n3 = (size_t)ctx->n * 3;
   hipStream_t s = ctx->stream;
+  // pinned staging, mapped into the device and coherent (allocated once per
+  // size): positions go in by DMA in chunks; on the tile path the kernels
+  // write the forces, the 20 output doubles and the pair count straight
+  // into it, with no copy back (else: device results copied back at once)
+  const size_t nres = n3 + 21;
   HIPCHK(ctx->d_pos.ensure(N * 3 * sizeof(double)));
-  HIPCHK(ctx->d_force.ensure(N * 3 * sizeof(double)));
-  HIPCHK(ctx->d_out.ensure(20 * sizeof(double)));
-  // pinned staging (allocated once per size): DMA both ways at full PCIe
-  // rate instead of the runtime's pageable bounce buffers
   const size_t want = (2 * N * 3 + 24) * sizeof(double);
   if (ctx->h_stage_bytes < want) {
     if (ctx->h_stage) HIPCHK(hipHostFree(ctx->h_stage));
     ctx->h_stage = nullptr;
     ctx->h_stage_bytes = 0;
-    HIPCHK(hipHostMalloc((void **)&ctx->h_stage, want, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&ctx->h_stage, want,
+                         hipHostMallocMapped | hipHostMallocCoherent));
     ctx->h_stage_bytes = want;
   }
   double *h_pos = ctx->h_stage, *h_frc = ctx->h_stage + N * 3;
-  double *h_out = ctx->h_stage + 2 * N * 3;
-  if (n3) {
-    std::memcpy(h_pos, positions, n3 * sizeof(double));
-    HIPCHK(hipMemcpyAsync(ctx->d_pos.p, h_pos, n3 * sizeof(double),
-                          hipMemcpyHostToDevice, s));
+  double *h_out = h_frc + n3;
+  for (hipEvent_t &e : ctx->h_ev)
+    if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  // the host's copy into the staging buffer overlaps the transfers
+  const size_t nch = n3 >= (size_t)kHostChunkMin ? kHostChunks : 1;
+  for (size_t c = 0; c < nch && n3; c++) {
+    const size_t a = n3 * c / nch, b = n3 * (c + 1) / nch;
+    std::memcpy(h_pos + a, positions + a, (b - a) * sizeof(double));
+    HIPCHK(hipMemcpyAsync(ctx->d_pos.as<double>() + a, h_pos + a,
+                          (b - a) * sizeof(double), hipMemcpyHostToDevice, s));
   }
-  launch_fill_i32(ctx->d_force.as<int>(), (int)(N * 6), 0, s);
-  launch_fill_i32(ctx->d_out.as<int>(), 40, 0, s);
   const double *d_com = nullptr;
   if (flags & PMNB_FLAG_MOLVIRIAL) {
     // centres are indexed by molecule id
@@ -1296,17 +1321,32 @@ int pmnb_evaluate(pmnb_ctx *ctx, const double *positions, double *forces,
                           hipMemcpyHostToDevice, s));
     d_com = ctx->d_com.as<double>();
   }
-  rc = run_device(ctx, ctx->d_pos.as<double>(), ctx->d_force.as<double>(),
-                  ctx->d_out.as<double>(), d_com, flags, s);
+  rc = ensure_eval_buffers(ctx);
   if (rc) return rc;
-  if (n3)
-    HIPCHK(hipMemcpyAsync(h_frc, ctx->d_force.p, n3 * sizeof(double),
+  if (ctx->use_tile) {
+    double *g_frc = nullptr;  // device address of h_frc
+    HIPCHK(hipHostGetDevicePointer((void **)&g_frc, h_frc, 0));
+    rc = run_device_bufs(ctx, ctx_bufs(ctx), ctx->d_pos.as<double>(), g_frc,
+                         g_frc + n3, d_com, flags, s, true, true,
+                         reinterpret_cast<unsigned long long *>(g_frc + n3 + 20));
+    if (rc) return rc;
+  } else {
+    HIPCHK(ctx->d_force.ensure((N * 3 + 24) * sizeof(double)));
+    double *d_res = ctx->d_force.as<double>();
+    launch_fill_i32(ctx->d_force.as<int>(), (int)(2 * nres), 0, s);
+    EvalBufs B = ctx_bufs(ctx);
+    B.npairs = reinterpret_cast<unsigned long long *>(d_res + n3 + 20);
+    rc = run_device_bufs(ctx, B, ctx->d_pos.as<double>(), d_res, d_res + n3,
+                         d_com, flags, s, true, false);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(h_frc, d_res, nres * sizeof(double),
                           hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(h_out, ctx->d_out.p, 20 * sizeof(double),
-                        hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(h_out + 20, ctx->d_npairs.p, sizeof(unsigned long long),
-                        hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
+  }
+  HIPCHK(hipEventRecord(ctx->h_ev[0], s));
+  hipError_t q;
+  while ((q = hipEventQuery(ctx->h_ev[0])) == hipErrorNotReady) {
+  }
+  HIPCHK(q);
   for (size_t i = 0; i < n3; i++) forces[i] += h_frc[i];
   energies[0] += h_out[0];
   energies[1] += h_out[1];
@@ -1482,12 +1522,12 @@ int pmnb_pair_path(const pmnb_ctx *ctx, int32_t *path) {
 
 int pmnb_last_pair_count(pmnb_ctx *ctx, int64_t *n_pairs) {
   if (!ctx || !n_pairs) return PMNB_ERR_ARG;
-  if (ctx->d_npairs.p) {
+  if (ctx->last_np) {
     // the counter of the last launch, on whichever stream it ran
     HIPCHK(hipSetDevice(ctx->prm.device));
     HIPCHK(hipDeviceSynchronize());
     unsigned long long np = 0;
-    HIPCHK(hipMemcpy(&np, ctx->d_npairs.p, sizeof np, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&np, ctx->last_np, sizeof np, hipMemcpyDeviceToHost));
     ctx->last_pairs = (long long)np;
   }
   *n_pairs = ctx->last_pairs;
@@ -1517,6 +1557,8 @@ void pmnb_destroy(pmnb_ctx *ctx) {
                     &ctx->d_ffix, &ctx->d_plog, &ctx->d_plog_count};
   for (DevBuf *b : bufs) b->release();
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : ctx->h_ev)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;

@@ -61,6 +61,11 @@ struct F32Params {
   int same_switch;  // LJ and electrostatic switches are the same C2
 };
 
+// Internal evaluation flag (never a caller's flag): the tile path writes
+// the forces and the 20 output doubles instead of adding to them, so the
+// caller need not clear them first.
+constexpr uint32_t kFlagFresh = 1u << 31;
+
 struct ForceParams {
   Box box;
   Grid grid;

@@ -144,7 +144,7 @@ k_scan_items(const int *__restrict__ count, int *__restrict__ start,
 __device__ __forceinline__ void finish_rep_block(
     const double *__restrict__ partial, int pstride,
     const int *__restrict__ rep_first, double *__restrict__ out, int r,
-    int k) {
+    int k, int fresh = 0) {
   __shared__ double red[16];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i0 = rep_first[r], i1 = rep_first[r + 1];
@@ -167,19 +167,21 @@ __device__ __forceinline__ void finish_rep_block(
   v = 0.0;
   for (int w = 0; w < 16; w++) v += red[w];
   double *o = out + 20 * r;
+  // fresh: every one of the 20 slots is written by exactly one accumulator
+  auto put = [&](int slot) { o[slot] = fresh ? v : o[slot] + v; };
   if (k < 2) {
-    o[k] += v;
+    put(k);
   } else {
     const int base = k < ACC_MXX ? 2 : 11;
     const int u = k < ACC_MXX ? k - ACC_VXX : k - ACC_MXX;
     // u: 0 xx, 1 xy, 2 xz, 3 yy, 4 yz, 5 zz
     switch (u) {
-      case 0: o[base + 0] += v; break;
-      case 1: o[base + 1] += v; o[base + 3] += v; break;
-      case 2: o[base + 2] += v; o[base + 6] += v; break;
-      case 3: o[base + 4] += v; break;
-      case 4: o[base + 5] += v; o[base + 7] += v; break;
-      default: o[base + 8] += v; break;
+      case 0: put(base + 0); break;
+      case 1: put(base + 1); put(base + 3); break;
+      case 2: put(base + 2); put(base + 6); break;
+      case 3: put(base + 4); break;
+      case 4: put(base + 5); put(base + 7); break;
+      default: put(base + 8); break;
     }
   }
 }

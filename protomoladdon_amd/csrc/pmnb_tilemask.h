@@ -1041,9 +1041,15 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
       fy64 += __shfl_down(fy64, 16, 64);
       fz64 += __shfl_down(fz64, 16, 64);
       if (active && split == 0) {
-        forces[3 * ai] += fx64;
-        forces[3 * ai + 1] += fy64;
-        forces[3 * ai + 2] += fz64;
+        if (P.flags & kFlagFresh) {
+          forces[3 * ai] = fx64;
+          forces[3 * ai + 1] = fy64;
+          forces[3 * ai + 2] = fz64;
+        } else {
+          forces[3 * ai] += fx64;
+          forces[3 * ai + 1] += fy64;
+          forces[3 * ai + 2] += fz64;
+        }
       }
     } else {
       Fx += __shfl_down(Fx, 32, 64);
@@ -1053,9 +1059,15 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
       Fy += __shfl_down(Fy, 16, 64);
       Fz += __shfl_down(Fz, 16, 64);
       if (active && split == 0) {
-        forces[3 * ai] += (double)Fx * kFixedInv;
-        forces[3 * ai + 1] += (double)Fy * kFixedInv;
-        forces[3 * ai + 2] += (double)Fz * kFixedInv;
+        if (P.flags & kFlagFresh) {
+          forces[3 * ai] = (double)Fx * kFixedInv;
+          forces[3 * ai + 1] = (double)Fy * kFixedInv;
+          forces[3 * ai + 2] = (double)Fz * kFixedInv;
+        } else {
+          forces[3 * ai] += (double)Fx * kFixedInv;
+          forces[3 * ai + 1] += (double)Fy * kFixedInv;
+          forces[3 * ai + 2] += (double)Fz * kFixedInv;
+        }
       }
     }
   }
@@ -1074,31 +1086,34 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
 
 // After the pair kernel: blocks below atom_blocks convert the fixed-point
 // force of each sorted slot into the caller's forces (+= at the original
-// index) and clear it; the next rep_blocks sum the item partials of one
-// (replica, accumulator) each (finish_rep_block); the rest clear the cell
-// counts for the next evaluation (nothing reads them after the pair kernel).
+// index, = with fresh) and clear it; the next rep_blocks sum the item
+// partials of one (replica, accumulator) each (finish_rep_block), the
+// first of them also storing the pair count in np_out when given; the rest
+// clear the cell counts for the next evaluation (nothing reads them after
+// the pair kernel).
 template <bool F64>
 __global__ void __launch_bounds__(1024)
 k_half_finish(long long *__restrict__ ffix, const int *__restrict__ atom,
               int n, double *__restrict__ forces, int atom_blocks,
               const double *__restrict__ partial, int pstride,
               const int *__restrict__ rep_first, double *__restrict__ out,
-              int rep_blocks, int *__restrict__ count, int ncell) {
+              int rep_blocks, int *__restrict__ count, int ncell, int fresh,
+              const unsigned long long *__restrict__ npairs,
+              unsigned long long *__restrict__ np_out) {
   constexpr int W = FixWords<F64>::W;
   if ((int)blockIdx.x < atom_blocks) {
     const int s = blockIdx.x * 1024 + threadIdx.x;
     if (s >= n) return;
     long long *src = ffix + (size_t)W * s;
     const int a = atom[s];
-    if (F64) {
-      for (int c = 0; c < 3; c++) {
-        forces[3 * a + c] += fix64_value(src[2 * c], src[2 * c + 1]);
+    for (int c = 0; c < 3; c++) {
+      const double f = F64 ? fix64_value(src[2 * c], src[2 * c + 1])
+                           : (double)src[c] * kFixedInv;
+      forces[3 * a + c] = fresh ? f : forces[3 * a + c] + f;
+      if (F64) {
         src[2 * c] = 0;
         src[2 * c + 1] = 0;
-      }
-    } else {
-      for (int c = 0; c < 3; c++) {
-        forces[3 * a + c] += (double)src[c] * kFixedInv;
+      } else {
         src[c] = 0;
       }
     }
@@ -1110,7 +1125,9 @@ k_half_finish(long long *__restrict__ ffix, const int *__restrict__ atom,
     if (c < ncell) count[c] = 0;
     return;
   }
-  finish_rep_block(partial, pstride, rep_first, out, b / ACC_N, b % ACC_N);
+  if (np_out && b == 0 && threadIdx.x == 0) *np_out = *npairs;
+  finish_rep_block(partial, pstride, rep_first, out, b / ACC_N, b % ACC_N,
+                   fresh);
 }
 
 }  // namespace pmnb

@@ -206,6 +206,41 @@ def test_fp64_is_deterministic_and_accumulates():
     np.testing.assert_array_equal(e3.table, e1.table)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_host_entry_equals_device_entry_on_changing_positions(precision):
+    """pmnb_evaluate (host arrays; the tile path writes its results into
+    mapped host memory) against pmnb_evaluate_device on the same context,
+    bit for bit, over calls whose positions change every time, so that a
+    result left over from an earlier call cannot pass; forces and energies
+    are added to what the caller's arrays already hold."""
+    import torch
+    from protomoladdon_amd.topology import ScalarStructure
+    w = systems.small_water(n_mol=1500)
+    force = w.make_force(precision)
+    rng = np.random.default_rng(7)
+    dev = torch.device("cuda:0")
+    for k in range(4):
+        pos = w.positions + rng.normal(0.0, 0.02, w.positions.shape)
+        print("call", k, precision, flush=True)
+        fh = np.full((w.n_atoms, 3), 0.25)
+        e = ScalarStructure(virial=True)
+        e[SS.COULOMB] = 3.0
+        force.evaluate(w.topology, pos, fh, e)
+        n_host = force.last_pair_count()
+        x = torch.tensor(pos, dtype=torch.float64, device=dev)
+        fd = torch.zeros_like(x)
+        out = torch.zeros(20, dtype=torch.float64, device=dev)
+        force.evaluate_device(w.topology, x, fd, out, 1)
+        torch.cuda.synchronize()
+        n_dev = force.last_pair_count()
+        o = out.cpu().numpy()
+        np.testing.assert_array_equal(fh, 0.25 + fd.cpu().numpy())
+        assert e[SS.LENNARDJONES] == o[0] and e[SS.COULOMB] == 3.0 + o[1]
+        np.testing.assert_array_equal(e.table[7:16], o[2:11])
+        assert n_host == n_dev > 0
+    force.close()
+
+
 # ----------------------------------------------------------------- mixed
 @pytest.mark.parametrize("make", [
     systems.config0,
