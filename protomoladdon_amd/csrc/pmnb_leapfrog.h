@@ -511,19 +511,7 @@ __global__ __launch_bounds__(kLfBlock) void k_lf_step(LfDev d, long long sc_appl
                                                       long long sc_probe,
                                                       long long call) {
   using SH = LfShape<SHAPE>;
-  int k = 0;
   const bool gblock = (int)blockIdx.x < d.ngb;
-  const int rep = gblock ? d.blk_rep[blockIdx.x] : 0;
-  const LfRep R = gblock ? d.reps[rep] : LfRep{0, 0, 0, 0, 0};
-  if (APPLY != kStNone && gblock) {
-    k = lf_decide(d, APPLY, sc_apply, R, rep);
-    if ((int)blockIdx.x == R.gb0 && threadIdx.x == 0) {
-      d.hint[8 * rep + APPLY * 2 + (int)(sc_apply & 1)] = k;
-      const long long slot = call * d.n_reps + rep;
-      if (d.iters && slot < d.iters_cap) d.iters[slot] = k;
-      if (k > d.maxit[APPLY]) atomicAdd(&d.nonconv[APPLY], 1);
-    }
-  }
   if (ZERO) {
     for (int t = blockIdx.x * kLfBlock + threadIdx.x; t < d.n_out;
          t += gridDim.x * kLfBlock)
@@ -542,31 +530,55 @@ __global__ __launch_bounds__(kLfBlock) void k_lf_step(LfDev d, long long sc_appl
     if (ZERO) st3(d.f, i, D3{0, 0, 0});
     return;
   }
+  const int rep = d.blk_rep[blockIdx.x];
+  const LfRep R = d.reps[rep];
   const int local = ((int)blockIdx.x - R.gb0) * kLfBlock + threadIdx.x;
   const bool act = local < R.ng;
   LfGroup<SH> g;
   // inactive threads hold a copy of the replica's last group (read only)
   load_group(d, R.goff + (act ? local : R.ng - 1), g,
              APPLY == kStShake || PROBE == kStShake);
+  // what the apply and the per-atom update read, loaded before the sweep
+  // count is decided so that the loads overlap the decision: the state this
+  // call's probe kept, the forces and the masses
+  const int kept =
+      APPLY != kStNone ? lf_kept_level(d, APPLY, sc_apply, rep) : 0;
+  D3 kx[SH::NA], kv[SH::NA], fa[SH::NA];
+  double ma[SH::NA];
+#pragma unroll
+  for (int a = 0; a < SH::NA; a++) {
+    const int i = g.idx[a] < 0 ? 0 : g.idx[a];  // padding: read, not used
+    if (APPLY == kStShake && kept > 0) {
+      kx[a] = ld3(d.xk, i);
+      kv[a] = ld3(d.vk, i);
+    }
+    if (APPLY == kStRattle && kept > 0) kv[a] = ld3(d.vr, i);
+    if (PRE != kPreNone) {
+      fa[a] = ld3(d.f, i);
+      ma[a] = d.mass[i];
+    }
+  }
+  int k = 0;
+  if (APPLY != kStNone) {
+    k = lf_decide(d, APPLY, sc_apply, R, rep);
+    if ((int)blockIdx.x == R.gb0 && threadIdx.x == 0) {
+      d.hint[8 * rep + APPLY * 2 + (int)(sc_apply & 1)] = k;
+      const long long slot = call * d.n_reps + rep;
+      if (d.iters && slot < d.iters_cap) d.iters[slot] = k;
+      if (k > d.maxit[APPLY]) atomicAdd(&d.nonconv[APPLY], 1);
+    }
+  }
   if (act) {
     int s0 = 0;
-    if (APPLY != kStNone) {
-      // start from the state this call's probe kept, when it is on the way
-      const int kept = lf_kept_level(d, APPLY, sc_apply, rep);
-      if (kept > 0 && k >= kept) {
+    // start from the state this call's probe kept, when it is on the way
+    if (APPLY != kStNone && kept > 0 && k >= kept) {
 #pragma unroll
-        for (int a = 0; a < SH::NA; a++) {
-          const int i = g.idx[a];
-          if (i < 0) continue;
-          if (APPLY == kStShake) {
-            g.x[a] = ld3(d.xk, i);
-            g.v[a] = ld3(d.vk, i);
-          } else {
-            g.v[a] = ld3(d.vr, i);
-          }
-        }
-        s0 = kept;
+      for (int a = 0; a < SH::NA; a++) {
+        if (g.idx[a] < 0) continue;
+        if (APPLY == kStShake) g.x[a] = kx[a];
+        g.v[a] = kv[a];
       }
+      s0 = kept;
     }
     for (int s = s0; s < k; s++) sweep<APPLY>(g, d.dtc, true);
 #pragma unroll
@@ -577,7 +589,7 @@ __global__ __launch_bounds__(kLfBlock) void k_lf_step(LfDev d, long long sc_appl
         g.l[a] = g.x[a];
         st3(d.last, i, g.x[a]);
       }
-      if (PRE != kPreNone) pre_op<PRE>(g.x[a], g.v[a], ld3(d.f, i), d.mass[i], d);
+      if (PRE != kPreNone) pre_op<PRE>(g.x[a], g.v[a], fa[a], ma[a], d);
       if (APPLY != kStNone || PRE != kPreNone) {
         st3(d.x, i, g.x[a]);
         st3(d.v, i, g.v[a]);
