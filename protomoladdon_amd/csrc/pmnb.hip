@@ -789,19 +789,25 @@ EvalBufs ctx_bufs(const pmnb_ctx *ctx) {
 // On the tile path every replica's cells are numbered from its offset
 // (ctx->grid.ncell is then the total).  counts_zero: B.count is known to
 // hold zeros (tile path only), so no clearing launch is needed; the
-// work-item kernel clears the pair counter.  Leaves ctx->counts_zero false:
-// the counts are in use.
+// work-item kernel clears the pair counter.  assigned (tile path): the
+// counts, cells and ranks of these positions are already in B (written by
+// the integrator's kernel that produced them).  Leaves ctx->counts_zero
+// false: the counts are in use.
 int build_cells(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
-                hipStream_t s, bool counts_zero = false) {
+                hipStream_t s, bool counts_zero = false,
+                bool assigned = false) {
   const int n = ctx->n, nc = ctx->grid.ncell;
   const TileRep *reps = B.reps;
   const int *rep_of = ctx->batch ? ctx->d_rep_of.as<int>() : nullptr;
   int *rank = B.rank;
   ctx->counts_zero = false;
-  if (!(counts_zero && ctx->use_tile))
+  assigned = assigned && ctx->use_tile;
+  if (!((counts_zero || assigned) && ctx->use_tile))
     hipLaunchKernelGGL(k_clear, dim3(blocks_for(nc)), dim3(kBlock), 0, s,
                        B.count, nc, B.npairs);
-  if (n > 0 && ctx->use_tile) {
+  if (assigned) {
+    // cells and ranks came with the positions (the integrator's kernel)
+  } else if (n > 0 && ctx->use_tile) {
     hipLaunchKernelGGL(k_assign_r<TileRep>, dim3(blocks_for(n)), dim3(kBlock),
                        0, s, reps, rep_of, d_pos, n, B.cell_of,
                        B.count, rank);
@@ -899,7 +905,8 @@ int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
                     double *d_forces, double *d_out, const double *d_com,
                     uint32_t flags, hipStream_t s, bool own,
                     bool fresh = false,
-                    unsigned long long *np_out = nullptr) {
+                    unsigned long long *np_out = nullptr,
+                    bool assigned = false) {
   int rc = PMNB_OK;
   flags &= ~kFlagFresh;  // internal: never taken from a caller
   if (fresh && ctx->use_tile) flags |= kFlagFresh;
@@ -919,7 +926,7 @@ int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
     ev_stop = ctx->ev_pool[ctx->ev_used + 2];
     ctx->ev_used += 3;
   }
-  rc = build_cells(ctx, B, d_pos, s, own && ctx->counts_zero);
+  rc = build_cells(ctx, B, d_pos, s, own && ctx->counts_zero, assigned);
   if (rc) return rc;
   ctx->last_np = B.npairs;
   const ForceParams P = make_params(ctx, flags);
@@ -1053,11 +1060,11 @@ int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
 
 int run_device(pmnb_ctx *ctx, const double *d_pos, double *d_forces,
                double *d_out, const double *d_com, uint32_t flags,
-               hipStream_t s, bool fresh = false) {
+               hipStream_t s, bool fresh = false, bool assigned = false) {
   const int rc = ensure_eval_buffers(ctx);
   if (rc) return rc;
   return run_device_bufs(ctx, ctx_bufs(ctx), d_pos, d_forces, d_out, d_com,
-                         flags, s, true, fresh);
+                         flags, s, true, fresh, nullptr, assigned);
 }
 
 }  // namespace
@@ -2163,9 +2170,10 @@ LfBonded lf_bonded(pmnb_leapfrog *lf) {
 // energies into out[20], out[21] when `energies`).
 int lf_calc_forces(pmnb_leapfrog *lf, const double *d_pos, double *d_force,
                    double *d_out, uint32_t flags, bool energies,
-                   hipStream_t s) {
+                   hipStream_t s, bool assigned = false) {
   pmnb_ctx *ctx = lf->ctx;
-  int rc = run_device(ctx, d_pos, d_force, d_out, nullptr, flags, s);
+  int rc = run_device(ctx, d_pos, d_force, d_out, nullptr, flags, s, false,
+                      assigned);
   if (rc) return rc;
   if (lf->pme) {
     rc = pmnb_pme_evaluate_device(lf->pme, d_pos, d_force, d_out,
@@ -2458,11 +2466,30 @@ int pmnb_lf_run_device(pmnb_leapfrog *lf, double *d_positions,
     k.apply = st;
     return r;
   };
+  // the kernel that clears the forces also assigns the cells of the
+  // evaluation that follows (tile path; on zeroed counts)
+  const bool assign = ctx->use_tile;
+  LfDev da = d;
+  if (assign) {
+    rc = ensure_eval_buffers(ctx);
+    if (rc) return rc;
+    const EvalBufs B = ctx_bufs(ctx);
+    da.c_reps = B.reps;
+    da.c_rep_of = ctx->batch ? ctx->d_rep_of.as<int>() : nullptr;
+    da.c_cell_of = B.cell_of;
+    da.c_count = B.count;
+    da.c_rank = B.rank;
+  }
   auto force = [&](bool last) -> int {
     k.zero = true;
-    int r = lf_flush(lf, k, d, s);
+    if (assign && !ctx->counts_zero)
+      hipLaunchKernelGGL(k_clear, dim3(blocks_for(ctx->grid.ncell)),
+                         dim3(kBlock), 0, s, da.c_count, ctx->grid.ncell,
+                         ctx->d_npairs.as<unsigned long long>());
+    int r = lf_flush(lf, k, assign ? da : d, s);
     if (r) return r;
-    return lf_calc_forces(lf, d_positions, d_forces, d_out, flags, last, s);
+    return lf_calc_forces(lf, d_positions, d_forces, d_out, flags, last, s,
+                          assign);
   };
   // doHalfKickdoDrift (LeapfrogIntegrator.cpp:33-58), SHAKE post-drift
   if ((rc = pre(kPreFirst))) return rc;

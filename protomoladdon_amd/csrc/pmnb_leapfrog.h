@@ -45,7 +45,15 @@
 // replica's first group block gb0, gnb blocks), its errors are summed over
 // those blocks only and its sweep count, hint and iteration record are its
 // own, as when the replica runs alone.  A single system is one replica.
+//
+// Cells.  The kernel that clears the forces before an evaluation (ZERO) also
+// does the evaluation's cell assignment on the tile path when given the
+// cell arrays (as k_assign_r: cell of every atom, one global count atomic
+// per distinct cell of a block, arrival rank), from the positions it has
+// just written, which saves the evaluation that launch.
 #pragma once
+
+#include "pmnb_queue.h"
 
 namespace pmnb {
 
@@ -120,6 +128,10 @@ struct LfDev {
                             // ([call * n_reps + rep], optional)
   long long iters_cap;
   int *nonconv;             // [3] stage calls that hit max_iter
+  // cell assignment of the next evaluation (tile path; c_count null: none)
+  const TileRep *c_reps;
+  const int *c_rep_of;      // replica of each atom (null: one replica)
+  int *c_cell_of, *c_count, *c_rank;
 };
 
 struct D3 {
@@ -502,6 +514,56 @@ __device__ __forceinline__ void load_group(const LfDev &d, int gi,
   }
 }
 
+// Cell assignment of the atoms idx[0..NA) (negative: none) at positions x
+// (k_assign_r's scheme for a block of up to kLfBlock * NA atoms): the block
+// counts its atoms per cell in an LDS table, takes one global atomic add
+// per distinct cell and gives each atom the cell's base plus its rank.
+// Block-wide (all threads call it).
+template <int NA>
+__device__ __forceinline__ void lf_assign_cells(const LfDev &d,
+                                                const int (&idx)[NA],
+                                                const D3 (&x)[NA]) {
+  constexpr int NS = 1024;
+  static_assert(kLfBlock * NA <= NS, "one slot per atom of the block");
+  __shared__ int key[NS], lcount[NS], base[NS];
+  const int t = threadIdx.x;
+  for (int q = t; q < NS; q += kLfBlock) {
+    key[q] = -1;
+    lcount[q] = 0;
+  }
+  __syncthreads();
+  int slot[NA], lrank[NA];
+#pragma unroll
+  for (int a = 0; a < NA; a++) {
+    slot[a] = -1;
+    lrank[a] = 0;
+    const int i = idx[a];
+    if (i < 0) continue;
+    const TileRep &R = d.c_reps[d.c_rep_of ? d.c_rep_of[i] : 0];
+    int cx, cy, cz;
+    double vx, vy, vz;
+    cell_coords(R.box, R.grid, x[a].x, x[a].y, x[a].z, cx, cy, cz, vx, vy, vz);
+    const int c = R.cell_off + (cx * R.grid.n[1] + cy) * R.grid.n[2] + cz;
+    d.c_cell_of[i] = c;
+    // at most NS distinct cells in NS slots: the probe ends
+    int h = c & (NS - 1);
+    while (true) {
+      const int old = atomicCAS(&key[h], -1, c);
+      if (old == -1 || old == c) break;
+      h = (h + 1) & (NS - 1);
+    }
+    slot[a] = h;
+    lrank[a] = atomicAdd(&lcount[h], 1);
+  }
+  __syncthreads();
+  for (int q = t; q < NS; q += kLfBlock)
+    if (key[q] >= 0) base[q] = atomicAdd(&d.c_count[key[q]], lcount[q]);
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < NA; a++)
+    if (slot[a] >= 0) d.c_rank[idx[a]] = base[slot[a]] + lrank[a];
+}
+
 // One fused launch: apply stage APPLY (sweep count from its probe), then the
 // per-atom update PRE, store, then the first probe of stage PROBE; with
 // ZERO, clear the forces and the energy block for the next evaluation.
@@ -519,15 +581,24 @@ __global__ __launch_bounds__(kLfBlock) void k_lf_step(LfDev d, long long sc_appl
   }
   if (!gblock) {
     const int t = ((int)blockIdx.x - d.ngb) * kLfBlock + threadIdx.x;
-    if (t >= d.n_free) return;
-    const int i = d.free_atoms[t];
-    if (PRE != kPreNone) {
-      D3 x = ld3(d.x, i), v = ld3(d.v, i);
-      pre_op<PRE>(x, v, ld3(d.f, i), d.mass[i], d);
-      st3(d.x, i, x);
-      st3(d.v, i, v);
+    int ia[1] = {-1};
+    D3 xa[1];
+    if (t < d.n_free) {
+      const int i = d.free_atoms[t];
+      D3 x = ld3(d.x, i);
+      if (PRE != kPreNone) {
+        D3 v = ld3(d.v, i);
+        pre_op<PRE>(x, v, ld3(d.f, i), d.mass[i], d);
+        st3(d.x, i, x);
+        st3(d.v, i, v);
+      }
+      if (ZERO) st3(d.f, i, D3{0, 0, 0});
+      ia[0] = i;
+      xa[0] = x;
     }
-    if (ZERO) st3(d.f, i, D3{0, 0, 0});
+    if constexpr (ZERO) {
+      if (d.c_count) lf_assign_cells(d, ia, xa);
+    }
     return;
   }
   const int rep = d.blk_rep[blockIdx.x];
@@ -595,6 +666,14 @@ __global__ __launch_bounds__(kLfBlock) void k_lf_step(LfDev d, long long sc_appl
         st3(d.v, i, g.v[a]);
       }
       if (ZERO) st3(d.f, i, D3{0, 0, 0});
+    }
+  }
+  if constexpr (ZERO) {
+    if (d.c_count) {
+      int ia[SH::NA];
+#pragma unroll
+      for (int a = 0; a < SH::NA; a++) ia[a] = act ? g.idx[a] : -1;
+      lf_assign_cells(d, ia, g.x);
     }
   }
   if (PROBE != kStNone) {
