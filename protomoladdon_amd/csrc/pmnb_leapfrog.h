@@ -50,7 +50,8 @@
 // does the evaluation's cell assignment on the tile path when given the
 // cell arrays (as k_assign_r: cell of every atom, one global count atomic
 // per distinct cell of a block, arrival rank), from the positions it has
-// just written, which saves the evaluation that launch.
+// just written, which saves the evaluation that launch (single systems; a
+// batch measured faster with the evaluation's own launch).
 #pragma once
 
 #include "pmnb_queue.h"
@@ -349,8 +350,12 @@ __device__ __forceinline__ int lf_first_levels(const LfDev &d, int stage,
 // this many sweeps (xk, vk or vr), and the apply starts from it when the
 // decided count is at least as large; 0: nothing stored.  It never exceeds
 // the first probe's levels, after the last of which the probe still sweeps.
+// A batch of replicas keeps nothing: with many waves per SIMD the sweeps
+// cost less than moving the kept state through memory (measured on 512
+// water replicas); the results are the same bits either way.
 __device__ __forceinline__ int lf_kept_level(const LfDev &d, int stage,
                                              long long scall, int rep) {
+  if (d.n_reps > 1) return 0;
   return max(lf_hint(d, stage, scall, rep), 0);
 }
 

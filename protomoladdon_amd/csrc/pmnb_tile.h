@@ -2,7 +2,8 @@
 // pmnb_queue.h and pmnb_adjoint.h).
 //
 // Cell grid: x/y columns at least one cutoff wide (a neighbour's column
-// offset is -1, 0 or +1) and z cells of about a quarter cutoff; atoms are
+// offset is -1, 0 or +1) and z cells of about a third (mixed) or a quarter
+// (FP64) of the cutoff; atoms are
 // sorted column by column, z cell by z cell, ascending atom index inside a
 // cell, so a column's z-range of cells is one contiguous run of the sorted
 // arrays (two when it wraps).  One wave handles 16 consecutive sorted atoms
