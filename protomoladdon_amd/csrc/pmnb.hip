@@ -111,6 +111,8 @@ struct pmnb_ctx {
   // 6 FP64), all zero between evaluations (the finish kernel clears it)
   DevBuf d_ffix;
   const void *ffix_zeroed = nullptr;  // allocation last cleared
+  DevBuf d_bucket;                    // tile path: cap atoms per cell
+  int bucket_cap = 64;
   // d_count holds zeros at the start of the next evaluation on the tile path
   // (the finish kernel clears it after the pair kernel): no clearing launch
   bool counts_zero = false;
@@ -503,6 +505,16 @@ void launch_fill_i32(int *p, int n, int v, hipStream_t s) {
                        n, v);
 }
 
+// Atom slots per cell of the tile path's buckets: 8 times the mean
+// occupancy, a power of two in [64, kSortLds] (a fuller cell is read back
+// from the cell index of every atom instead).
+int tile_bucket_cap(int n, int nc) {
+  const long long mean = ((long long)n + std::max(nc, 1) - 1) / std::max(nc, 1);
+  int cap = 64;
+  while (cap < 8 * mean && cap < kSortLds) cap *= 2;
+  return cap;
+}
+
 int ensure_eval_buffers(pmnb_ctx *ctx) {
   const int n = ctx->n, nc = ctx->grid.ncell;
   const size_t N = (size_t)std::max(n, 1);
@@ -525,6 +537,9 @@ int ensure_eval_buffers(pmnb_ctx *ctx) {
   HIPCHK(ctx->d_scell.ensure(N * sizeof(int)));
   if (ctx->use_tile) {
     HIPCHK(ctx->d_srec.ensure(N * sizeof(int4)));
+    ctx->bucket_cap = tile_bucket_cap(n, nc);
+    HIPCHK(ctx->d_bucket.ensure((size_t)std::max(nc, 1) * ctx->bucket_cap *
+                                sizeof(int)));
     const int W = ctx->prm.precision == PMNB_PREC_FP64 ? FixWords<true>::W
                                                         : FixWords<false>::W;
     HIPCHK(ctx->d_ffix.ensure(N * W * sizeof(long long)));
@@ -751,6 +766,8 @@ void launch_adj_t(int T, int max_items, hipStream_t s, const ForceParams &P,
 // pmnb_evaluate_device_noalloc, carved from the static scratch array.
 struct EvalBufs {
   int *cell_of, *count, *start, *rank, *sorted;
+  int *bucket;  // tile path: cap atom slots per cell, in arrival order
+  int cap;
   double *sx, *sy, *sz, *sq;
   float4 *loc;
   int *stype, *smol, *scell;
@@ -772,6 +789,8 @@ EvalBufs ctx_bufs(const pmnb_ctx *ctx) {
   B.start = ctx->d_start.as<int>();
   B.rank = ctx->d_fill.as<int>();
   B.sorted = ctx->d_sorted.as<int>();
+  B.bucket = ctx->d_bucket.as<int>();
+  B.cap = ctx->bucket_cap;
   B.sx = ctx->d_sx.as<double>();
   B.sy = ctx->d_sy.as<double>();
   B.sz = ctx->d_sz.as<double>();
@@ -813,11 +832,11 @@ int build_cells(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
     hipLaunchKernelGGL(k_clear, dim3(blocks_for(nc)), dim3(kBlock), 0, s,
                        B.count, nc, B.npairs);
   if (assigned) {
-    // cells and ranks came with the positions (the integrator's kernel)
+    // cells and buckets came with the positions (the integrator's kernel)
   } else if (n > 0 && ctx->use_tile) {
     hipLaunchKernelGGL(k_assign_r<TileRep>, dim3(blocks_for(n)), dim3(kBlock),
                        0, s, reps, rep_of, d_pos, n, B.cell_of,
-                       B.count, rank);
+                       B.count, B.bucket, B.cap);
   } else if (n > 0) {
     hipLaunchKernelGGL(k_assign, dim3(blocks_for(n)), dim3(kBlock), 0, s,
                        ctx->box, ctx->grid, d_pos, n, B.cell_of,
@@ -834,7 +853,7 @@ int build_cells(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s,
                        B.count, B.start, nc);
   }
-  if (n > 0) {
+  if (n > 0 && !ctx->use_tile) {
     hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, s,
                        B.cell_of, n, B.start,
                        rank, B.sorted);
@@ -843,7 +862,8 @@ int build_cells(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
     hipLaunchKernelGGL(k_sort_gather_r<TileRep>,
                        dim3(blocks_for(nc, kBlock / 64)), dim3(kBlock), 0, s,
                        reps, rep_of, d_pos, B.start,
-                       B.count, nc, ctx->d_type.as<int>(),
+                       B.count, nc, B.bucket, B.cap, B.cell_of, n,
+                       ctx->d_type.as<int>(),
                        ctx->d_q.as<double>(), ctx->d_mol.as<int>(),
                        B.sorted, B.sx,
                        B.sy, B.sz,
@@ -1568,7 +1588,8 @@ void pmnb_destroy(pmnb_ctx *ctx) {
                     &ctx->d_items,
                     &ctx->d_nitems, &ctx->d_reps, &ctx->d_col_rep,
                     &ctx->d_rep_first, &ctx->d_rep_of, &ctx->d_partial_lj,
-                    &ctx->d_ffix, &ctx->d_plog, &ctx->d_plog_count};
+                    &ctx->d_ffix, &ctx->d_plog, &ctx->d_plog_count,
+                    &ctx->d_bucket};
   for (DevBuf *b : bufs) b->release();
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   for (hipEvent_t e : ctx->h_ev)
@@ -2486,7 +2507,8 @@ int pmnb_lf_run_device(pmnb_leapfrog *lf, double *d_positions,
     da.c_rep_of = ctx->batch ? ctx->d_rep_of.as<int>() : nullptr;
     da.c_cell_of = B.cell_of;
     da.c_count = B.count;
-    da.c_rank = B.rank;
+    da.c_bucket = B.bucket;
+    da.c_cap = B.cap;
   }
   auto force = [&](bool last) -> int {
     k.zero = true;
@@ -3472,6 +3494,8 @@ EvalBufs carve_bufs(const pmnb_ctx *ctx, unsigned char *base, size_t &bytes,
   B.start = c.take<int>(nc);
   B.rank = c.take<int>(N);
   B.sorted = c.take<int>(N);
+  B.cap = tile_bucket_cap(ctx->n, (int)nc);
+  B.bucket = c.take<int>(nc * (size_t)B.cap);
   B.sx = c.take<double>(N);
   B.sy = c.take<double>(N);
   B.sz = c.take<double>(N);

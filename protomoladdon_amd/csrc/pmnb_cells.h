@@ -241,27 +241,29 @@ __global__ void k_gather(Box b, Grid g, const double *__restrict__ pos,
 // Cells of a block's atoms are few (consecutive atoms are neighbours): the
 // block counts its atoms per cell in an LDS table (open addressing on the
 // cell index), takes one global atomic add per distinct cell, all issued
-// together, and gives each atom the cell's base plus its rank in the block.
-// Requires blockDim.x == kBlock.
+// together, and gives each atom the cell's base plus its rank in the block:
+// its arrival rank in the cell, at which it is written into the cell's row
+// of `bucket` (cap entries per cell; an atom ranked beyond cap is found
+// again from cell_of by k_sort_gather_r).  Requires blockDim.x == kBlock.
 template <class RepT>
 __global__ void __launch_bounds__(kBlock)
 k_assign_r(const RepT *__restrict__ reps, const int *__restrict__ rep_of,
            const double *__restrict__ pos, int n, int *__restrict__ cell_of,
-           int *__restrict__ count, int *__restrict__ rank) {
+           int *__restrict__ count, int *__restrict__ bucket, int cap) {
   __shared__ int key[kBlock], lcount[kBlock], base[kBlock];
   const int t = threadIdx.x;
   const int i = blockIdx.x * kBlock + t;
   key[t] = -1;
   lcount[t] = 0;
   __syncthreads();
-  int slot = -1, lrank = 0;
+  int slot = -1, lrank = 0, c = 0;
   if (i < n) {
     const RepT &R = reps[rep_of ? rep_of[i] : 0];
     int cx, cy, cz;
     double vx, vy, vz;
     cell_coords(R.box, R.grid, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2],
                 cx, cy, cz, vx, vy, vz);
-    const int c = R.cell_off + (cx * R.grid.n[1] + cy) * R.grid.n[2] + cz;
+    c = R.cell_off + (cx * R.grid.n[1] + cy) * R.grid.n[2] + cz;
     cell_of[i] = c;
     // at most kBlock distinct cells in kBlock slots: the probe ends
     int h = c & (kBlock - 1);
@@ -276,19 +278,25 @@ k_assign_r(const RepT *__restrict__ reps, const int *__restrict__ rep_of,
   __syncthreads();
   if (key[t] >= 0) base[t] = atomicAdd(&count[key[t]], lcount[t]);
   __syncthreads();
-  if (i < n) rank[i] = base[slot] + lrank;
+  if (i < n) {
+    const int r = base[slot] + lrank;
+    if (r < cap) bucket[(size_t)c * cap + r] = i;
+  }
 }
 
 // Per-cell ordering and gather of the tile path in one pass: one wave per
-// cell puts the cell's entries in ascending atom order (as k_sort_cells)
-// and writes, for each sorted slot, the atom's FP64 position, the FP32
-// offset from its cell's lower corner (computed in FP64) with its charge,
-// type, charge, molecule, cell and packed record.
+// cell takes the cell's atoms from its bucket row (or, for a cell of more
+// than cap atoms, from cell_of), puts them in ascending atom order (as
+// k_sort_cells) and writes, for each sorted slot, the atom index, its FP64
+// position, the FP32 offset from its cell's lower corner (computed in FP64)
+// with its charge, type, charge, molecule, cell and packed record.
 template <class RepT>
 __global__ void __launch_bounds__(kBlock)
 k_sort_gather_r(const RepT *__restrict__ reps, const int *__restrict__ rep_of,
                 const double *__restrict__ pos, const int *__restrict__ start,
                 const int *__restrict__ count, int ncell,
+                const int *__restrict__ bucket, int cap,
+                const int *__restrict__ cell_of, int n,
                 const int *__restrict__ type, const double *__restrict__ q,
                 const int *__restrict__ mol, int *__restrict__ sorted,
                 double *__restrict__ sx, double *__restrict__ sy,
@@ -297,6 +305,7 @@ k_sort_gather_r(const RepT *__restrict__ reps, const int *__restrict__ rep_of,
                 int *__restrict__ smol, int *__restrict__ scell,
                 int4 *__restrict__ srec) {
   __shared__ int keys[kBlock / 64][kSortLds];
+  __shared__ int found[kBlock / 64];
   const int w = threadIdx.x >> 6;
   const int c = blockIdx.x * (kBlock / 64) + w;
   const int lane = threadIdx.x & 63;
@@ -305,6 +314,17 @@ k_sort_gather_r(const RepT *__restrict__ reps, const int *__restrict__ rep_of,
   if (m == 0) return;
   const int s0 = start[c];
   int *seg = sorted + s0;
+  const int *row = bucket + (size_t)c * cap;  // the first min(m, cap) atoms
+  // a cell of more than cap atoms: the wave finds them all in cell_of and
+  // appends them to dst (any order; sorted below)
+  auto collect = [&](int *dst) {
+    if (lane == 0) found[w] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int i = lane; i < n; i += 64)
+      if (cell_of[i] == c) dst[atomicAdd(&found[w], 1)] = i;
+  };
   // slot s0 + k receives atom a
   auto put = [&](int k, int a) {
     const int s = s0 + k;
@@ -329,8 +349,8 @@ k_sort_gather_r(const RepT *__restrict__ reps, const int *__restrict__ rep_of,
     scell[s] = c;
     srec[s] = make_int4(s, type[a], mol[a], a);
   };
-  if (m <= 64) {
-    int v = lane < m ? seg[lane] : 0x7fffffff;
+  if (m <= 64) {  // cap >= 64: every atom is in the row
+    int v = lane < m ? row[lane] : 0x7fffffff;
     for (int k = 2; k <= 64; k <<= 1) {
       for (int j = k >> 1; j > 0; j >>= 1) {
         const int other = __shfl_xor(v, j, 64);
@@ -341,7 +361,11 @@ k_sort_gather_r(const RepT *__restrict__ reps, const int *__restrict__ rep_of,
     }
     if (lane < m) put(lane, v);
   } else if (m <= kSortLds) {
-    for (int k = lane; k < m; k += 64) keys[w][k] = seg[k];
+    if (m <= cap) {
+      for (int k = lane; k < m; k += 64) keys[w][k] = row[k];
+    } else {
+      collect(keys[w]);
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -352,6 +376,11 @@ k_sort_gather_r(const RepT *__restrict__ reps, const int *__restrict__ rep_of,
       put(rank, v);
     }
   } else {
+    // m > kSortLds >= cap
+    collect(seg);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (lane == 0) {
       for (int i = 1; i < m; i++) {
         int v = seg[i];

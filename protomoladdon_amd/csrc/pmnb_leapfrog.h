@@ -49,7 +49,7 @@
 // Cells.  The kernel that clears the forces before an evaluation (ZERO) also
 // does the evaluation's cell assignment on the tile path when given the
 // cell arrays (as k_assign_r: cell of every atom, one global count atomic
-// per distinct cell of a block, arrival rank), from the positions it has
+// per distinct cell of a block, bucket slot), from the positions it has
 // just written, which saves the evaluation that launch (single systems; a
 // batch measured faster with the evaluation's own launch).
 #pragma once
@@ -132,7 +132,8 @@ struct LfDev {
   // cell assignment of the next evaluation (tile path; c_count null: none)
   const TileRep *c_reps;
   const int *c_rep_of;      // replica of each atom (null: one replica)
-  int *c_cell_of, *c_count, *c_rank;
+  int *c_cell_of, *c_count, *c_bucket;
+  int c_cap;                // atom slots per cell of c_bucket
 };
 
 struct D3 {
@@ -522,8 +523,9 @@ __device__ __forceinline__ void load_group(const LfDev &d, int gi,
 // Cell assignment of the atoms idx[0..NA) (negative: none) at positions x
 // (k_assign_r's scheme for a block of up to kLfBlock * NA atoms): the block
 // counts its atoms per cell in an LDS table, takes one global atomic add
-// per distinct cell and gives each atom the cell's base plus its rank.
-// Block-wide (all threads call it).
+// per distinct cell and writes each atom into its cell's bucket row at the
+// cell's base plus its rank (below the row's capacity).  Block-wide (all
+// threads call it).
 template <int NA>
 __device__ __forceinline__ void lf_assign_cells(const LfDev &d,
                                                 const int (&idx)[NA],
@@ -565,8 +567,11 @@ __device__ __forceinline__ void lf_assign_cells(const LfDev &d,
     if (key[q] >= 0) base[q] = atomicAdd(&d.c_count[key[q]], lcount[q]);
   __syncthreads();
 #pragma unroll
-  for (int a = 0; a < NA; a++)
-    if (slot[a] >= 0) d.c_rank[idx[a]] = base[slot[a]] + lrank[a];
+  for (int a = 0; a < NA; a++) {
+    if (slot[a] < 0) continue;
+    const int r = base[slot[a]] + lrank[a];
+    if (r < d.c_cap) d.c_bucket[(size_t)key[slot[a]] * d.c_cap + r] = idx[a];
+  }
 }
 
 // One fused launch: apply stage APPLY (sweep count from its probe), then the

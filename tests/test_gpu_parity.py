@@ -206,6 +206,40 @@ def test_fp64_is_deterministic_and_accumulates():
     np.testing.assert_array_equal(e3.table, e1.table)
 
 
+def dense_cell_water(n_mol, seed=11):
+    """Water in a 60 A box with every molecule moved rigidly so that its
+    oxygen lies in 7 x 7 x 1.6 A at the origin corner: one cell of the tile
+    grid holds far more atoms than its bucket row (the mean occupancy is
+    below one atom, so a row has 64 slots), and the gather reads that cell
+    back from the cell index of every atom (in LDS up to 512 atoms, beyond
+    in the cell's segment)."""
+    w = systems.water_workload("water_dense_cell", n_mol, 60.0, seed, 9.0,
+                               7.5, "ewald", 0.32)
+    rng = np.random.default_rng(seed)
+    X = w.positions.reshape(n_mol, 3, 3).copy()
+    o_new = rng.uniform([0.5, 0.5, 0.35], [7.5, 7.5, 1.95], size=(n_mol, 3))
+    X += (o_new - X[:, 0, :])[:, None, :]
+    return w, np.ascontiguousarray(X.reshape(-1, 3))
+
+
+@pytest.mark.parametrize("n_mol", [100, 300])
+def test_fp64_cell_beyond_its_bucket(n_mol):
+    w, pos = dense_cell_water(n_mol)
+    check_fp64(w, flags=1, positions=pos)
+
+
+def test_mixed_pair_set_of_a_cell_beyond_its_bucket():
+    w, pos = dense_cell_water(100)
+    s = O.System.from_workload(w, positions=pos, flags=1)
+    ref = O.evaluate(s, record_pairs=True)
+    _, _, npairs, (pairs, fpairs) = run_gpu(w, "mixed", 1, pos,
+                                            force_pairs=True)
+    ref_pairs = O.sorted_pairs(ref.pairs)
+    np.testing.assert_array_equal(pairs, ref_pairs)
+    np.testing.assert_array_equal(fpairs, ref_pairs)
+    assert npairs == ref.n_pairs
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_host_entry_equals_device_entry_on_changing_positions(precision):
     """pmnb_evaluate (host arrays; the tile path writes its results into
