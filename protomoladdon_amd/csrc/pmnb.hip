@@ -927,13 +927,17 @@ int check_ready(pmnb_ctx *ctx) {
 // counts the finish kernel leaves cleared for the next evaluation.  fresh:
 // on the tile path the forces and the 20 output doubles are written, not
 // added to (the caller's arrays need no clearing).  np_out (tile path):
-// the finish kernel also stores the pair count there.
+// the finish kernel also stores the pair count there.  finish_skipped
+// (given): a single-system tile evaluation whose pair kernel wrote the
+// forces itself launches no finish kernel (no energies or virial reach
+// d_out, and the caller clears the cell counts before the next
+// evaluation); *finish_skipped tells whether it did so.
 int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
                     double *d_forces, double *d_out, const double *d_com,
                     uint32_t flags, hipStream_t s, bool own,
                     bool fresh = false,
                     unsigned long long *np_out = nullptr,
-                    bool assigned = false) {
+                    bool assigned = false, bool *finish_skipped = nullptr) {
   int rc = PMNB_OK;
   flags &= ~kFlagFresh;  // internal: never taken from a caller
   if (fresh && ctx->use_tile) flags |= kFlagFresh;
@@ -988,6 +992,10 @@ int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
       launch_pair_half(mixed, el, sw, ctx->shape, a);
     }
     if (ev_stop) HIPCHK(hipEventRecord(ev_stop, s));
+    if (finish_skipped) {
+      *finish_skipped = direct;
+      if (direct) return PMNB_OK;
+    }
     // fixed-point forces into the caller's array (unless the pair kernel
     // added them itself), per-replica sums
     // and the cell counts cleared for the next evaluation
@@ -1087,11 +1095,13 @@ int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
 
 int run_device(pmnb_ctx *ctx, const double *d_pos, double *d_forces,
                double *d_out, const double *d_com, uint32_t flags,
-               hipStream_t s, bool fresh = false, bool assigned = false) {
+               hipStream_t s, bool fresh = false, bool assigned = false,
+               bool *finish_skipped = nullptr) {
   const int rc = ensure_eval_buffers(ctx);
   if (rc) return rc;
   return run_device_bufs(ctx, ctx_bufs(ctx), d_pos, d_forces, d_out, d_com,
-                         flags, s, true, fresh, nullptr, assigned);
+                         flags, s, true, fresh, nullptr, assigned,
+                         finish_skipped);
 }
 
 }  // namespace
@@ -2019,6 +2029,7 @@ struct pmnb_leapfrog {
   DevBuf mass, rmass, g_atoms, g_cons, g_rest, g_rinv, free_atoms, last, xs,
       vs, xk, vk, vr, partial_s, partial_r, iters, nonconv, hint;
   bool last_ready = false;
+  bool zero_counts = false;  // the next step kernel clears the cell counts
   long long calls = 0, iters_cap = 0;
   long long scalls[3] = {0, 0, 0};  // calls per stage
   // bonded terms evaluated with the force (pmnb_lf_set_bonded)
@@ -2142,9 +2153,16 @@ hipError_t lf_launch_probe(int stage, const LfDev &d, long long scall,
 
 // Launches one fused kernel, then the continuation probe of the stage it
 // probes.
-int lf_flush(pmnb_leapfrog *lf, LfKernel &k, const LfDev &d, hipStream_t s) {
+int lf_flush(pmnb_leapfrog *lf, LfKernel &k, const LfDev &d0, hipStream_t s) {
   pmnb_ctx *ctx = lf->ctx;
   if (k.empty()) return PMNB_OK;
+  LfDev d = d0;
+  const bool zc = lf->zero_counts;
+  if (zc) {
+    // the evaluation before launched no finish kernel: clear its counts
+    d.z_count = ctx->d_count.as<int>();
+    d.z_nc = ctx->grid.ncell;
+  }
   const long long call = k.apply != kStNone ? lf->calls++ : -1;
   const long long sc_apply = lf->scalls[k.apply];
   const long long sc_probe = lf->scalls[k.probe];
@@ -2169,6 +2187,10 @@ int lf_flush(pmnb_leapfrog *lf, LfKernel &k, const LfDev &d, hipStream_t s) {
     }
   }
   if (k.apply != kStNone) lf->scalls[k.apply]++;
+  if (zc) {
+    lf->zero_counts = false;
+    ctx->counts_zero = true;
+  }
   k = LfKernel{};
   return PMNB_OK;
 }
@@ -2195,13 +2217,21 @@ LfBonded lf_bonded(pmnb_leapfrog *lf) {
 
 // calculateForces of the integrator's force group after the forces and the
 // energy block were cleared: the cutoff force, then the bonded terms (their
-// energies into out[20], out[21] when `energies`).
+// energies into out[20], out[21] when `energies`).  Only the last
+// evaluation of a run leaves its energies in d_out (the next step clears
+// them), so the others (energies false) skip the cutoff force's virial and,
+// on a single system, its energy reduction; the cell counts are then
+// cleared by the integrator's next kernel (lf->zero_counts).
 int lf_calc_forces(pmnb_leapfrog *lf, const double *d_pos, double *d_force,
                    double *d_out, uint32_t flags, bool energies,
                    hipStream_t s, bool assigned = false) {
   pmnb_ctx *ctx = lf->ctx;
-  int rc = run_device(ctx, d_pos, d_force, d_out, nullptr, flags, s, false,
-                      assigned);
+  bool skipped = false;
+  const uint32_t fl =
+      energies ? flags : flags & ~(PMNB_FLAG_VIRIAL | PMNB_FLAG_MOLVIRIAL);
+  int rc = run_device(ctx, d_pos, d_force, d_out, nullptr, fl, s, false,
+                      assigned, energies ? nullptr : &skipped);
+  lf->zero_counts = skipped;
   if (rc) return rc;
   if (lf->pme) {
     rc = pmnb_pme_evaluate_device(lf->pme, d_pos, d_force, d_out,
@@ -2512,10 +2542,14 @@ int pmnb_lf_run_device(pmnb_leapfrog *lf, double *d_positions,
   }
   auto force = [&](bool last) -> int {
     k.zero = true;
-    if (assign && !ctx->counts_zero)
+    if (assign && !ctx->counts_zero) {
+      // (also what a clearing left to this kernel would have done: the
+      // assignment's atomics must not race a clearing in the same kernel)
       hipLaunchKernelGGL(k_clear, dim3(blocks_for(ctx->grid.ncell)),
                          dim3(kBlock), 0, s, da.c_count, ctx->grid.ncell,
                          ctx->d_npairs.as<unsigned long long>());
+      lf->zero_counts = false;
+    }
     int r = lf_flush(lf, k, assign ? da : d, s);
     if (r) return r;
     return lf_calc_forces(lf, d_positions, d_forces, d_out, flags, last, s,

@@ -134,6 +134,8 @@ struct LfDev {
   const int *c_rep_of;      // replica of each atom (null: one replica)
   int *c_cell_of, *c_count, *c_bucket;
   int c_cap;                // atom slots per cell of c_bucket
+  int *z_count;             // cell counts to clear (null: none)
+  int z_nc;
 };
 
 struct D3 {
@@ -584,6 +586,11 @@ __global__ __launch_bounds__(kLfBlock) void k_lf_step(LfDev d, long long sc_appl
                                                       long long call) {
   using SH = LfShape<SHAPE>;
   const bool gblock = (int)blockIdx.x < d.ngb;
+  if (d.z_count) {
+    for (int t = blockIdx.x * kLfBlock + threadIdx.x; t < d.z_nc;
+         t += gridDim.x * kLfBlock)
+      d.z_count[t] = 0;
+  }
   if (ZERO) {
     for (int t = blockIdx.x * kLfBlock + threadIdx.x; t < d.n_out;
          t += gridDim.x * kLfBlock)
