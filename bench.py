@@ -662,8 +662,9 @@ def main():
             "workload": "water96k_ewald MD: 32,000 waters (96,000 atoms), "
                         "98.6 A periodic box, LJ O-O + Ewald real space "
                         "alpha 0.32, C2 7.5->9 A; one step = leapfrog 1 fs "
-                        "+ SHAKE/RATTLE + cell-list rebuild + force, virial "
-                        "on",
+                        "+ SHAKE/RATTLE + cell-list rebuild + force; energies "
+                        "and virial (flag on) summed for the last step of "
+                        "each run() call, the values the caller reads",
             "n_atoms": w.n_atoms,
             "pairs_per_step": P,
         },
@@ -685,7 +686,8 @@ def main():
                             "force_only": sps / sps64,
                             "pair_kernel": f64["kernel_ms"] / mx["kernel_ms"]},
         "md": {"integrator": "Leapfrog 1 fs + SHAKE/RATTLE (1e-5, 30), "
-                             "whole step on the device, virial on",
+                             "whole step on the device; energies and virial "
+                             "of the last step of each run() call",
                "mixed": md_mx, "fp64": md_64},
         "ewald_kspace": ewald,
         "pme_kspace": pme,

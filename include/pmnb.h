@@ -373,7 +373,9 @@ int pmnb_lf_create(pmnb_ctx *ctx, const pmnb_lf_params *params,
    doubles when bonded terms are set: [20] bond, [21] angle energy),
    holding the last evaluation's energies on exit (calculateForces clears
    them before every evaluation).  flags: as pmnb_evaluate_device, without
-   PMNB_FLAG_MOLVIRIAL. */
+   PMNB_FLAG_MOLVIRIAL; the virial, like the energies, is the last
+   evaluation's (the cutoff force's energy sums and virial are formed for
+   that evaluation only, the only one whose values reach d_out). */
 int pmnb_lf_run_device(pmnb_leapfrog *lf, double *d_positions,
                        double *d_velocities, double *d_forces, double *d_out,
                        int32_t n_steps, uint32_t flags, void *stream);
