@@ -505,13 +505,13 @@ void launch_fill_i32(int *p, int n, int v, hipStream_t s) {
                        n, v);
 }
 
-// Atom slots per cell of the tile path's buckets: 8 times the mean
+// Atom slots per cell of the tile path's buckets: at least 4 times the mean
 // occupancy, a power of two in [64, kSortLds] (a fuller cell is read back
 // from the cell index of every atom instead).
 int tile_bucket_cap(int n, int nc) {
   const long long mean = ((long long)n + std::max(nc, 1) - 1) / std::max(nc, 1);
   int cap = 64;
-  while (cap < 8 * mean && cap < kSortLds) cap *= 2;
+  while (cap < 4 * mean && cap < kSortLds) cap *= 2;
   return cap;
 }
 
@@ -959,7 +959,7 @@ int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
   }
   rc = build_cells(ctx, B, d_pos, s, own && ctx->counts_zero, assigned);
   if (rc) return rc;
-  ctx->last_np = B.npairs;
+  if (own) ctx->last_np = B.npairs;  // not the shared scratch's counter
   const ForceParams P = make_params(ctx, flags);
   const Sorted S = make_sorted(ctx, B, d_com);
   const int nb = std::max(1, blocks_for(ctx->n));
