@@ -2139,20 +2139,7 @@ hipError_t lf_launch_step(const LfKernel &k, const LfDev &d, int grid,
 }
 #undef PMNB_LF_CASE
 
-template <int SHAPE>
-hipError_t lf_launch_probe(int stage, const LfDev &d, long long scall,
-                           hipStream_t s) {
-  if (stage == kStShake)
-    hipLaunchKernelGGL((k_lf_probe<kStShake, SHAPE>), dim3(d.ngb),
-                       dim3(kLfBlock), 0, s, d, scall);
-  else
-    hipLaunchKernelGGL((k_lf_probe<kStRattle, SHAPE>), dim3(d.ngb),
-                       dim3(kLfBlock), 0, s, d, scall);
-  return hipGetLastError();
-}
-
-// Launches one fused kernel, then the continuation probe of the stage it
-// probes.
+// Launches one fused kernel.
 int lf_flush(pmnb_leapfrog *lf, LfKernel &k, const LfDev &d0, hipStream_t s) {
   pmnb_ctx *ctx = lf->ctx;
   if (k.empty()) return PMNB_OK;
@@ -2177,14 +2164,6 @@ int lf_flush(pmnb_leapfrog *lf, LfKernel &k, const LfDev &d0, hipStream_t s) {
     if (e != hipSuccess)
       return fail(ctx, PMNB_ERR_HIP, "leapfrog step launch failed: %s",
                   hipGetErrorString(e));
-    if (k.probe != kStNone && lf->ngb > 0) {
-      e = lf->shape == kShapeWater ? lf_launch_probe<kShapeWater>(k.probe, d, sc_probe, s)
-          : lf->shape == kShape33  ? lf_launch_probe<kShape33>(k.probe, d, sc_probe, s)
-                                   : lf_launch_probe<kShape46>(k.probe, d, sc_probe, s);
-      if (e != hipSuccess)
-        return fail(ctx, PMNB_ERR_HIP, "leapfrog probe launch failed: %s",
-                    hipGetErrorString(e));
-    }
   }
   if (k.apply != kStNone) lf->scalls[k.apply]++;
   if (zc) {

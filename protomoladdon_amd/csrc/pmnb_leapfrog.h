@@ -23,14 +23,15 @@
 // Sweep count.  The reference sweeps every constraint until the mean error
 // over all constraints is <= epsilon (at most max_iter + 1 sweeps), a
 // decision over the whole system.  A "probe" records the summed group
-// errors after 0, 1, ..., L-1 sweeps (one partial per block) without keeping
-// the swept state; the kernel that applies the stage reduces these partials
-// in a fixed order (every block computes the same value), finds the
-// reference's sweep count k, and sweeps k times from the stage input.  L is
-// the previous call's sweep count of the same stage plus a margin; when the
-// error is still above epsilon after L levels, one continuation launch
-// probes the remaining levels from the state stored after L sweeps (it
-// returns at once otherwise).  No host synchronisation, no atomics on the
+// errors after 0, 1, ..., L-1 sweeps (one partial per block); the kernel
+// that applies the stage reduces these partials in a fixed order (every
+// block computes the same value), finds the reference's sweep count k, and
+// sweeps k times from the stage input (or from the state the probe kept
+// after the previous count, below).  L is the previous call's sweep count of
+// the same stage plus a margin (every level on the first call); when the
+// error is still above epsilon after L levels, the apply kernel's blocks
+// compute the remaining levels themselves from the state the probe stored
+// after L sweeps (lf_fallback).  No host synchronisation, no atomics on the
 // decision.
 //
 // The global error is summed per group then per block rather than in the
@@ -60,7 +61,7 @@ namespace pmnb {
 
 constexpr int kLfBlock = 256;
 constexpr int kLfWaves = kLfBlock / 64;
-constexpr int kLfFirstLevels = 16;  // first probe's levels with no history
+constexpr int kLfFirstLevels = 32;  // first probe's levels with no history
 constexpr int kLfMargin = 1;        // levels past the previous sweep count
 constexpr int kLfMaxAtoms = 4;      // largest group
 constexpr int kLfMaxCons = 6;       // most constraints in a group
@@ -418,18 +419,6 @@ __device__ __forceinline__ int lf_scan(const LfDev &d, int stage, int j0,
   return -1;
 }
 
-// Sweep count of a stage call: the first level j (0..maxit) whose mean
-// error is <= epsilon, else maxit + 1.
-__device__ __forceinline__ int lf_decide(const LfDev &d, int stage,
-                                         long long scall, const LfRep &R,
-                                         int rep) {
-  const int L = lf_first_levels(d, stage, scall, rep);
-  const int k = lf_scan(d, stage, 0, L, R);
-  if (k >= 0) return k;
-  const int k2 = lf_scan(d, stage, L, d.maxit[stage] + 1, R);
-  return k2 >= 0 ? k2 : d.maxit[stage] + 1;
-}
-
 // Records the error levels [j0, j1) of a stage for this block's groups from
 // the state in g (active threads only): one partial per block and level.
 // The levels run back to back, each thread leaving its group's error in
@@ -438,14 +427,55 @@ __device__ __forceinline__ int lf_decide(const LfDev &d, int stage,
 // j1 - j0 sweeps is left in xs / vs.  With kept > 0 (and j0 = 0), the full
 // state after `kept` sweeps (velocities updated as the apply does) is left
 // in xk, vk (SHAKE) or vr (RATTLE).  Block-wide (all threads call it).
+constexpr int kLfLevels = 16;                  // levels per chunk
+constexpr int kLfRun = kLfBlock / kLfLevels;    // threads per run
+static_assert(kLfRun == 16, "reduction shape");
+
+// LDS of the level reductions, one object per kernel shared by the probe
+// and the apply kernel's fallback: each thread's group error per level of a
+// chunk, and the run sums.
+__device__ __forceinline__ double (&lf_elev())[kLfLevels][kLfBlock] {
+  __shared__ double e[kLfLevels][kLfBlock];
+  return e;
+}
+__device__ __forceinline__ double (&lf_runs())[kLfRun][kLfLevels] {
+  __shared__ double r[kLfRun][kLfLevels];
+  return r;
+}
+
+// The block sums of the first n levels in lf_elev() in a fixed order (16
+// runs of 16 threads per level, then the run sums): thread t < n returns
+// level t's.  Block-wide.
+__device__ __forceinline__ double lf_level_sums(int n) {
+  auto &elev = lf_elev();
+  auto &runs = lf_runs();
+  const int t = threadIdx.x;
+  __syncthreads();
+  {
+    const int jj = t / kLfRun, r = t % kLfRun;
+    double s = 0;
+    if (jj < n) {
+#pragma unroll
+      for (int k = 0; k < kLfRun; k++) s += elev[jj][r + kLfRun * k];
+    }
+    runs[r][jj] = s;
+  }
+  __syncthreads();
+  double s = 0;
+  if (t < n) {
+#pragma unroll
+    for (int r = 0; r < kLfRun; r++) s += runs[r][t];
+  }
+  __syncthreads();  // the LDS is refilled next
+  return s;
+}
+
 template <int STAGE, class SH>
 __device__ __forceinline__ void lf_probe(const LfDev &d, LfGroup<SH> &g,
                                          bool act, int j0, int j1, bool store,
                                          int kept) {
-  constexpr int kLevels = 16, kRun = kLfBlock / kLevels;
-  static_assert(kRun == 16 && kLevels * kRun == kLfBlock, "reduction shape");
-  __shared__ double elev[kLevels][kLfBlock];
-  __shared__ double runs[kRun][kLevels];
+  constexpr int kLevels = kLfLevels;
+  auto &elev = lf_elev();
   const int t = threadIdx.x;
   double *P = d.partial[STAGE] + blockIdx.x;
   for (int c0 = j0; c0 < j1; c0 += kLevels) {
@@ -472,24 +502,8 @@ __device__ __forceinline__ void lf_probe(const LfDev &d, LfGroup<SH> &g,
         }
       }
     }
-    __syncthreads();
-    {
-      const int jj = t / kRun, r = t % kRun;
-      double s = 0;
-      if (jj < n) {
-#pragma unroll
-        for (int k = 0; k < kRun; k++) s += elev[jj][r + kRun * k];
-      }
-      runs[r][jj] = s;
-    }
-    __syncthreads();
-    if (t < n) {
-      double s = 0;
-#pragma unroll
-      for (int r = 0; r < kRun; r++) s += runs[r][t];
-      P[(size_t)(c0 + t) * d.ngb] = s;
-    }
-    __syncthreads();
+    const double s = lf_level_sums(n);
+    if (t < n) P[(size_t)(c0 + t) * d.ngb] = s;
   }
   if (store && act) {
 #pragma unroll
@@ -522,19 +536,99 @@ __device__ __forceinline__ void load_group(const LfDev &d, int gi,
   }
 }
 
+// The levels [L, maxit] of a stage when the first probe's L levels did not
+// decide (the count rose by more than the margin): every block of the
+// replica computes each of the replica's block partials itself, from the
+// state the first probe stored after L sweeps (xs / vs), exactly as that
+// block's probe would (same sweeps, same LDS order), and sums them per level
+// in lf_scan's order (block q into the sum of q mod 8, in increasing q, then
+// the eight sums in order).  All blocks reach the same count, the one the
+// partials would give; slow (each block redoes the replica) but rare: the
+// first call probes every level, and a steady count needs L = count + 2.
+template <int STAGE, class SH>
+__device__ int lf_fallback(const LfDev &d, int L, const LfRep &R) {
+  constexpr int kSub = 8;
+  __shared__ double acc[kLfLevels][kSub];
+  __shared__ int hit;
+  auto &elev = lf_elev();
+  const int t = threadIdx.x;
+  const int maxl = d.maxit[STAGE] + 1;
+  for (int c0 = L; c0 < maxl; c0 += kLfLevels) {
+    const int n = min(kLfLevels, maxl - c0);
+    if (t < kLfLevels * kSub) acc[t / kSub][t % kSub] = 0.0;
+    for (int vb = 0; vb < R.gnb; vb++) {
+      const int local = vb * kLfBlock + t;
+      const bool act = local < R.ng;
+      LfGroup<SH> g;
+      load_group(d, R.goff + (act ? local : R.ng - 1), g, STAGE == kStShake);
+#pragma unroll
+      for (int a = 0; a < SH::NA; a++) {
+        if (g.idx[a] < 0) continue;
+        if (STAGE == kStShake) g.x[a] = ld3(d.xs, g.idx[a]);
+        else g.v[a] = ld3(d.vs, g.idx[a]);
+      }
+      for (int lev = L; lev < c0; lev++) sweep<STAGE>(g, d.dtc, false);
+      for (int jj = 0; jj < n; jj++) {
+        const double e = group_error<STAGE>(g);
+        elev[jj][t] = act ? e : 0.0;
+        sweep<STAGE>(g, d.dtc, false);
+      }
+      const double s = lf_level_sums(n);
+      if (t < n) acc[t][vb % kSub] += s;
+    }
+    __syncthreads();
+    if (t < 64) {
+      bool pass = false;
+      if (t < n) {
+        double s = 0;
+#pragma unroll
+        for (int k = 0; k < kSub; k++) s += acc[t][k];
+        pass = s / R.ncons <= d.eps[STAGE];
+      }
+      const unsigned long long m = __ballot(pass);
+      if (t == 0) hit = m ? c0 + __builtin_ctzll(m) : -1;
+    }
+    __syncthreads();
+    const int hv = hit;
+    __syncthreads();  // acc and hit are rewritten by the next chunk
+    if (hv >= 0) return hv;
+  }
+  return maxl;
+}
+
+// Sweep count of a stage call: the first level j (0..maxit) whose mean
+// error is <= epsilon, else maxit + 1.  Block-wide.
+template <int STAGE, class SH>
+__device__ __forceinline__ int lf_decide(const LfDev &d, long long scall,
+                                         const LfRep &R, int rep) {
+  const int L = lf_first_levels(d, STAGE, scall, rep);
+  const int k = lf_scan(d, STAGE, 0, L, R);
+  if (k >= 0) return k;
+  if (L > d.maxit[STAGE]) return d.maxit[STAGE] + 1;
+  return lf_fallback<STAGE, SH>(d, L, R);
+}
+
 // Cell assignment of the atoms idx[0..NA) (negative: none) at positions x
 // (k_assign_r's scheme for a block of up to kLfBlock * NA atoms): the block
 // counts its atoms per cell in an LDS table, takes one global atomic add
 // per distinct cell and writes each atom into its cell's bucket row at the
 // cell's base plus its rank (below the row's capacity).  Block-wide (all
 // threads call it).
+constexpr int kLfCellSlots = 1024;
+// LDS of lf_assign_cells, one object per kernel whatever NA
+__device__ __forceinline__ int (&lf_cell_lds())[3][kLfCellSlots] {
+  __shared__ int t[3][kLfCellSlots];
+  return t;
+}
+
 template <int NA>
 __device__ __forceinline__ void lf_assign_cells(const LfDev &d,
                                                 const int (&idx)[NA],
                                                 const D3 (&x)[NA]) {
-  constexpr int NS = 1024;
+  constexpr int NS = kLfCellSlots;
   static_assert(kLfBlock * NA <= NS, "one slot per atom of the block");
-  __shared__ int key[NS], lcount[NS], base[NS];
+  auto &lds = lf_cell_lds();
+  int *key = lds[0], *lcount = lds[1], *base = lds[2];
   const int t = threadIdx.x;
   for (int q = t; q < NS; q += kLfBlock) {
     key[q] = -1;
@@ -647,8 +741,8 @@ __global__ __launch_bounds__(kLfBlock) void k_lf_step(LfDev d, long long sc_appl
     }
   }
   int k = 0;
-  if (APPLY != kStNone) {
-    k = lf_decide(d, APPLY, sc_apply, R, rep);
+  if constexpr (APPLY != kStNone) {
+    k = lf_decide<APPLY, SH>(d, sc_apply, R, rep);
     if ((int)blockIdx.x == R.gb0 && threadIdx.x == 0) {
       d.hint[8 * rep + APPLY * 2 + (int)(sc_apply & 1)] = k;
       const long long slot = call * d.n_reps + rep;
@@ -698,29 +792,6 @@ __global__ __launch_bounds__(kLfBlock) void k_lf_step(LfDev d, long long sc_appl
     lf_probe<PROBE>(d, g, act, 0, L, L <= d.maxit[PROBE],
                     lf_kept_level(d, PROBE, sc_probe, rep));
   }
-}
-
-// The levels after the first probe's, from the state it stored; returns at
-// once when the first probe already decided.
-template <int STAGE, int SHAPE>
-__global__ __launch_bounds__(kLfBlock) void k_lf_probe(LfDev d, long long scall) {
-  using SH = LfShape<SHAPE>;
-  const int rep = d.blk_rep[blockIdx.x];
-  const LfRep R = d.reps[rep];
-  const int L = lf_first_levels(d, STAGE, scall, rep);
-  if (L > d.maxit[STAGE] || lf_scan(d, STAGE, 0, L, R) >= 0) return;
-  const int local = ((int)blockIdx.x - R.gb0) * kLfBlock + threadIdx.x;
-  const bool act = local < R.ng;
-  LfGroup<SH> g;
-  // inactive threads hold a copy of the replica's last group (read only)
-  load_group(d, R.goff + (act ? local : R.ng - 1), g, STAGE == kStShake);
-#pragma unroll
-  for (int a = 0; a < SH::NA; a++) {
-    if (g.idx[a] < 0) continue;
-    if (STAGE == kStShake) g.x[a] = ld3(d.xs, g.idx[a]);
-    else g.v[a] = ld3(d.vs, g.idx[a]);
-  }
-  lf_probe<STAGE>(d, g, act, L, d.maxit[STAGE] + 1, false, 0);
 }
 
 // ------------------------------------------------------------ bonded terms

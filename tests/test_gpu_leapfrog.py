@@ -133,6 +133,36 @@ def test_two_runs_keep_shake_state():
     dev.close()
 
 
+def test_sweep_count_jump_beyond_the_first_probe():
+    """Velocities scaled by 4 between two runs: the next calls need more
+    sweeps than the previous count plus the margin, so the apply kernels
+    compute the levels past the first probe themselves; the trajectory and
+    every sweep count equal the oracle's."""
+    w, cons = water_case("ewald")
+    t = w.topology
+    v0 = maxwell(t.masses, 13)
+    print("water300 sweep count jump", flush=True)
+    dev = Device(w, "fp64", cons, DEFAULT, DEFAULT, v0)
+    s, st = oracle_state(w, v0)
+    dev.run(4)
+    its_ref = list(O.leapfrog_run(s, st, 4, t.masses, cons[0], cons[1], 1.0,
+                                  shake=DEFAULT, rattle=DEFAULT))
+    dev.v *= 4.0
+    st.v *= 4.0
+    dev.run(4)
+    its_ref += list(O.leapfrog_run(s, st, 4, t.masses, cons[0], cons[1], 1.0,
+                                   shake=DEFAULT, rattle=DEFAULT))
+    its_dev, _ = dev.iterations()
+    compare(dev, st, its_dev, its_ref)
+    # call order per run: S, (R, S) x 3, R; the first calls of the second
+    # run follow the last calls of the first
+    s1, r1 = its_ref[6], its_ref[7]
+    s2, r2 = its_ref[8], its_ref[9]
+    print("SHAKE", s1, "->", s2, "RATTLE", r1, "->", r2, flush=True)
+    assert s2 >= s1 + 2 or r2 >= r1 + 2
+    dev.close()
+
+
 def test_four_atom_groups_and_free_atoms():
     """Groups of 4 atoms and 4 constraints (a water plus the O of the next
     water, at its current distance) with the other atoms unconstrained:
