@@ -3505,7 +3505,7 @@ EvalBufs carve_bufs(const pmnb_ctx *ctx, unsigned char *base, size_t &bytes,
   B.cell_of = c.take<int>(N);
   B.count = c.take<int>(nc);
   B.start = c.take<int>(nc);
-  B.rank = c.take<int>(N);
+  B.rank = nullptr;  // the tile path fills bucket rows instead
   B.sorted = c.take<int>(N);
   B.cap = tile_bucket_cap(ctx->n, (int)nc);
   B.bucket = c.take<int>(nc * (size_t)B.cap);
