@@ -149,9 +149,9 @@ int pmnb_evaluate_device(pmnb_ctx *ctx, const double *d_positions,
    allocation).  Covers one system on the tile path (algorithm cutoff >=
    the switching cutoff, at most 8 LJ types): PMNB_ERR_UNSUPPORTED
    otherwise; PMNB_ERR_CAPACITY when the system needs more than 64 MiB
-   (108 bytes per atom mixed, 132 FP64, plus 128 per 16 atoms and, per
+   (100 bytes per atom mixed, 124 FP64, plus 128 per 16 atoms and, per
    cell, 8 and 4 per bucket slot, at least 4 times the mean occupancy:
-   about 490,000 atoms of water mixed, 400,000 FP64).  Every call on one GPU
+   about 520,000 atoms of water mixed, 425,000 FP64).  Every call on one GPU
    shares that GPU's array, which
    keeps the last call's contents: calls that overlap on one GPU (other
    streams or threads) race.  pmnb_last_pair_count does not see these

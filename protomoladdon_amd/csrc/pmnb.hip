@@ -536,7 +536,7 @@ int ensure_eval_buffers(pmnb_ctx *ctx) {
   HIPCHK(ctx->d_smol.ensure(N * sizeof(int)));
   HIPCHK(ctx->d_scell.ensure(N * sizeof(int)));
   if (ctx->use_tile) {
-    HIPCHK(ctx->d_srec.ensure(N * sizeof(int4)));
+    HIPCHK(ctx->d_srec.ensure(N * sizeof(int2)));
     ctx->bucket_cap = tile_bucket_cap(n, nc);
     HIPCHK(ctx->d_bucket.ensure((size_t)std::max(nc, 1) * ctx->bucket_cap *
                                 sizeof(int)));
@@ -771,7 +771,7 @@ struct EvalBufs {
   double *sx, *sy, *sz, *sq;
   float4 *loc;
   int *stype, *smol, *scell;
-  int4 *srec;
+  int2 *srec;
   long long *ffix;
   double *partial;
   unsigned long long *npairs;
@@ -799,7 +799,7 @@ EvalBufs ctx_bufs(const pmnb_ctx *ctx) {
   B.stype = ctx->d_stype.as<int>();
   B.smol = ctx->d_smol.as<int>();
   B.scell = ctx->d_scell.as<int>();
-  B.srec = ctx->d_srec.as<int4>();
+  B.srec = ctx->d_srec.as<int2>();
   B.ffix = ctx->d_ffix.as<long long>();
   B.partial = ctx->d_partial.as<double>();
   B.npairs = ctx->d_npairs.as<unsigned long long>();
@@ -3517,7 +3517,7 @@ EvalBufs carve_bufs(const pmnb_ctx *ctx, unsigned char *base, size_t &bytes,
   B.stype = c.take<int>(N);
   B.smol = c.take<int>(N);
   B.scell = c.take<int>(N);
-  B.srec = c.take<int4>(N);
+  B.srec = c.take<int2>(N);
   B.partial = c.take<double>(items * ACC_N);
   B.npairs = c.take<unsigned long long>(1);
   B.items = c.take<int4>(items);

@@ -303,7 +303,7 @@ k_sort_gather_r(const RepT *__restrict__ reps, const int *__restrict__ rep_of,
                 double *__restrict__ sz, float4 *__restrict__ sloc,
                 int *__restrict__ stype, double *__restrict__ sq,
                 int *__restrict__ smol, int *__restrict__ scell,
-                int4 *__restrict__ srec) {
+                int2 *__restrict__ srec) {
   __shared__ int keys[kBlock / 64][kSortLds];
   __shared__ int found[kBlock / 64];
   const int w = threadIdx.x >> 6;
@@ -347,7 +347,7 @@ k_sort_gather_r(const RepT *__restrict__ reps, const int *__restrict__ rep_of,
     sq[s] = q[a];
     smol[s] = mol[a];
     scell[s] = c;
-    srec[s] = make_int4(s, type[a], mol[a], a);
+    srec[s] = make_int2(type[a] | (mol[a] << 8), a);
   };
   if (m <= 64) {  // cap >= 64: every atom is in the row
     int v = lane < m ? row[lane] : 0x7fffffff;

@@ -25,8 +25,9 @@ struct Sorted {
   const double *q;
   const int *mol;
   const int *cell;               // linear cell of each sorted slot
-  const int4 *rec;               // {sorted slot, type, molecule, original
-                                 // index} (tile path only)
+  const int2 *rec;               // {type | molecule << 8, original index}
+                                 // (tile path only; < 256 types and
+                                 // < 2^24 atoms there)
   const int *cstart, *ccount;    // per-cell segment of the sorted order
   const int4 *delta;             // shell offsets, (0,0,0) first
   const unsigned long long *xfull, *xmod;  // exclusion masks (original idx)

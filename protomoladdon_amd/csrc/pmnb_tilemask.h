@@ -625,7 +625,10 @@ k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
       int4 ri = make_int4(0, 0, 0, 0);
       if (ok) {
         const float4 lt = S.loc[t];
-        ri = S.rec[t];
+        {
+          const int2 rr = S.rec[t];
+          ri = make_int4(t, rr.x & 255, rr.x >> 8, rr.y);
+        }
         rf.x = oxc + lt.x;
         rf.y = oyc + lt.y;
         rf.z = z0 + lt.z;
@@ -1004,7 +1007,10 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
       int4 ri = make_int4(0, 0, 0, 0);
       if (ok) {
         const float4 lt = S.loc[t];
-        ri = S.rec[t];
+        {
+          const int2 rr = S.rec[t];
+          ri = make_int4(t, rr.x & 255, rr.x >> 8, rr.y);
+        }
         rf.x = oxc + lt.x;
         rf.y = oyc + lt.y;
         rf.z = z0 + lt.z;
