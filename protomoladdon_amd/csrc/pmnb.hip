@@ -999,17 +999,17 @@ int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
     // fixed-point forces into the caller's array (unless the pair kernel
     // added them itself), per-replica sums
     // and the cell counts cleared for the next evaluation
-    const int ab = direct ? 0 : blocks_for(ctx->n, kFinBlock);
+    const int ab = direct ? 0 : blocks_for(ctx->n, 1024);
     const int rb = nrep * ACC_N, nc = ctx->grid.ncell;
     const int fresh = (flags & kFlagFresh) ? 1 : 0;
-    const dim3 fg((unsigned)(ab + rb + blocks_for(nc, kFinBlock)));
+    const dim3 fg((unsigned)(ab + rb + blocks_for(nc, 1024)));
     if (mixed)
-      hipLaunchKernelGGL(k_half_finish<false>, fg, dim3(kFinBlock), 0, s, ffix,
+      hipLaunchKernelGGL(k_half_finish<false>, fg, dim3(1024), 0, s, ffix,
                          S.atom, ctx->n, d_forces, ab, part, max_items,
                          rep_first, d_out, rb, B.count, nc, fresh, np,
                          np_out);
     else
-      hipLaunchKernelGGL(k_half_finish<true>, fg, dim3(kFinBlock), 0, s, ffix,
+      hipLaunchKernelGGL(k_half_finish<true>, fg, dim3(1024), 0, s, ffix,
                          S.atom, ctx->n, d_forces, ab, part, max_items,
                          rep_first, d_out, rb, B.count, nc, fresh, np,
                          np_out);

@@ -138,47 +138,34 @@ k_scan_items(const int *__restrict__ count, int *__restrict__ start,
 }
 
 // Per-replica sum of one accumulator of the item partials (accumulator k
-// of item b at partial[k * pstride + b]) in a fixed reduction tree over
-// 1024 virtual threads, run by a block of kFinBlock threads (each standing
-// for 1024 / kFinBlock of them, its virtual waves taken in turn), added
-// into out[20 r + ..]:
+// of item b at partial[k * pstride + b]) by one block of 1024 threads with
+// a fixed reduction tree, added into out[20 r + ..]:
 // [0] LJ, [1] electrostatic, [2..10] virial 3x3, [11..19] molecular virial.
-// The tree: virtual thread v sums items i0 + v + 1024 m into four running
-// sums by m mod 4 (while four remain), adds them pairwise, its 64-lane
-// virtual wave sums by shuffles, and thread 0 adds the 16 wave sums in
-// order.
-constexpr int kFinBlock = 256;
 __device__ __forceinline__ void finish_rep_block(
     const double *__restrict__ partial, int pstride,
     const int *__restrict__ rep_first, double *__restrict__ out, int r,
     int k, int fresh = 0) {
-  constexpr int kVirt = 1024, kPer = kVirt / kFinBlock;
-  static_assert(kFinBlock % 64 == 0 && kVirt % kFinBlock == 0, "tree shape");
-  __shared__ double red[kVirt / 64];
+  __shared__ double red[16];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i0 = rep_first[r], i1 = rep_first[r + 1];
   const double *pk = partial + (size_t)k * pstride;
-#pragma unroll
-  for (int j = 0; j < kPer; j++) {
-    const int vt = (int)threadIdx.x + kFinBlock * j;
-    // four independent sums per virtual thread, so the loads overlap
-    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
-    int b = i0 + vt;
-    for (; b + 3072 < i1; b += 4096) {
-      v0 += pk[b];
-      v1 += pk[b + 1024];
-      v2 += pk[b + 2048];
-      v3 += pk[b + 3072];
-    }
-    for (; b < i1; b += 1024) v0 += pk[b];
-    double v = (v0 + v1) + (v2 + v3);
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if (lane == 0) red[wave + (kFinBlock / 64) * j] = v;
+  // four independent sums per thread, so the loads overlap
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+  int b = i0 + (int)threadIdx.x;
+  for (; b + 3072 < i1; b += 4096) {
+    v0 += pk[b];
+    v1 += pk[b + 1024];
+    v2 += pk[b + 2048];
+    v3 += pk[b + 3072];
   }
+  for (; b < i1; b += 1024) v0 += pk[b];
+  double v = (v0 + v1) + (v2 + v3);
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if (lane == 0) red[wave] = v;
   __syncthreads();
   if (threadIdx.x != 0) return;
-  double v = 0.0;
-  for (int w = 0; w < kVirt / 64; w++) v += red[w];
+  v = 0.0;
+  for (int w = 0; w < 16; w++) v += red[w];
   double *o = out + 20 * r;
   // fresh: every one of the 20 slots is written by exactly one accumulator
   auto put = [&](int slot) { o[slot] = fresh ? v : o[slot] + v; };

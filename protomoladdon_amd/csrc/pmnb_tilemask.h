@@ -1098,7 +1098,7 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
 // clear the cell counts for the next evaluation (nothing reads them after
 // the pair kernel).
 template <bool F64>
-__global__ void __launch_bounds__(kFinBlock)
+__global__ void __launch_bounds__(1024)
 k_half_finish(long long *__restrict__ ffix, const int *__restrict__ atom,
               int n, double *__restrict__ forces, int atom_blocks,
               const double *__restrict__ partial, int pstride,
@@ -1108,7 +1108,7 @@ k_half_finish(long long *__restrict__ ffix, const int *__restrict__ atom,
               unsigned long long *__restrict__ np_out) {
   constexpr int W = FixWords<F64>::W;
   if ((int)blockIdx.x < atom_blocks) {
-    const int s = blockIdx.x * kFinBlock + threadIdx.x;
+    const int s = blockIdx.x * 1024 + threadIdx.x;
     if (s >= n) return;
     long long *src = ffix + (size_t)W * s;
     const int a = atom[s];
@@ -1127,7 +1127,7 @@ k_half_finish(long long *__restrict__ ffix, const int *__restrict__ atom,
   }
   const int b = blockIdx.x - atom_blocks;
   if (b >= rep_blocks) {
-    const int c = (b - rep_blocks) * kFinBlock + threadIdx.x;
+    const int c = (b - rep_blocks) * 1024 + threadIdx.x;
     if (c < ncell) count[c] = 0;
     return;
   }
