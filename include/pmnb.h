@@ -122,7 +122,12 @@ int pmnb_set_box(pmnb_ctx *ctx, const double e1[3], const double e2[3],
    PMNB_FLAG_VIRIAL or PMNB_FLAG_MOLVIRIAL is set, molvirial[9]
    (MOLVIRIALXX..) when PMNB_FLAG_MOLVIRIAL is set, which also needs
    mol_com (3 doubles per molecule id: Molecule::position).  Pointers that
-   a flag does not use may be NULL. */
+   a flag does not use may be NULL.  For 3 * n_atoms >= 65536 the copy of
+   the positions into the context's staging buffer and the force add are
+   shared by the calling thread and host threads that the context starts on
+   its first such call (4 in all, or PMNB_HOST_THREADS from the environment
+   at that call; 1 = the calling thread alone); they poll for 1 ms after
+   each call, then sleep until the next, and end with pmnb_destroy. */
 int pmnb_evaluate(pmnb_ctx *ctx, const double *positions, double *forces,
                   double *energies, double *virial, double *molvirial,
                   const double *mol_com, uint32_t flags);

@@ -9,12 +9,18 @@
 #include <hipfft/hipfft.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -63,12 +69,104 @@ struct DevBuf {
   template <class T> T *as() const { return static_cast<T *>(p); }
 };
 
+// Host threads that share pmnb_evaluate's staging copy and force add.  The
+// caller's thread is member 0; the others wait for a task posted through a
+// sequence number.  After a task a worker polls for the next one for
+// kPollNs (the force add follows the copy by the kernels' time, and the
+// next call's copy follows the add), then sleeps on a condition variable
+// until a task or the end.
+class HostCrew {
+ public:
+  ~HostCrew() {
+    if (th_.empty()) return;
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (std::thread &t : th_) t.join();
+  }
+  int size() const { return (int)th_.size() + 1; }
+  // n members in all (n - 1 threads); once
+  void start(int n) {
+    const unsigned s0 = seq_.load(std::memory_order_relaxed);
+    for (int k = 1; k < n; k++)
+      th_.emplace_back([this, k, s0] { loop(k, s0); });
+  }
+  // f(k) for k = 0 .. size() - 1, f(0) on the calling thread; returns when
+  // every member has returned
+  template <class F>
+  void run(F &&f) {
+    if (th_.empty()) {
+      f(0);
+      return;
+    }
+    std::function<void(int)> task(f);
+    task_ = &task;
+    busy_.store((int)th_.size(), std::memory_order_relaxed);
+    seq_.fetch_add(1, std::memory_order_seq_cst);
+    if (sleepers_.load(std::memory_order_seq_cst) > 0) {
+      std::lock_guard<std::mutex> g(mu_);
+      cv_.notify_all();
+    }
+    f(0);
+    while (busy_.load(std::memory_order_acquire) > 0) relax();
+    task_ = nullptr;
+  }
+
+ private:
+  static constexpr long long kPollNs = 1000000;
+  static void relax() { __builtin_ia32_pause(); }
+  // seen: the sequence number before the first task
+  void loop(int k, unsigned seen) {
+    while (true) {
+      const auto t0 = std::chrono::steady_clock::now();
+      unsigned now = seq_.load(std::memory_order_acquire);
+      while (now == seen && !stop_.load(std::memory_order_acquire)) {
+        if (std::chrono::duration_cast<std::chrono::nanoseconds>(
+                std::chrono::steady_clock::now() - t0).count() > kPollNs) {
+          std::unique_lock<std::mutex> g(mu_);
+          sleepers_.fetch_add(1, std::memory_order_seq_cst);
+          cv_.wait(g, [&] {
+            return stop_ || seq_.load(std::memory_order_seq_cst) != seen;
+          });
+          sleepers_.fetch_sub(1, std::memory_order_relaxed);
+        } else {
+          for (int i = 0; i < 64; i++) relax();
+        }
+        now = seq_.load(std::memory_order_acquire);
+      }
+      if (now == seen) return;  // stopped
+      seen = now;
+      (*task_)(k);
+      busy_.fetch_sub(1, std::memory_order_release);
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::atomic<unsigned> seq_{0};
+  std::atomic<int> busy_{0}, sleepers_{0};
+  const std::function<void(int)> *task_ = nullptr;
+  std::atomic<bool> stop_{false};
+};
+
+// Members of pmnb_evaluate's host crew: PMNB_HOST_THREADS when set (1 = the
+// caller's thread alone), else 4 (at most the hardware threads).
+int host_crew_size() {
+  if (const char *e = std::getenv("PMNB_HOST_THREADS")) {
+    const int v = std::atoi(e);
+    if (v >= 1) return std::min(v, 64);
+  }
+  const int hw = (int)std::thread::hardware_concurrency();
+  return std::max(1, std::min(4, hw));
+}
+
 }  // namespace
 
-// pmnb_evaluate moves positions in in this many chunks (systems of at
-// least kHostChunkMin coordinates)
-constexpr int kHostChunks = 4;
-constexpr int kHostChunkMin = 1 << 16;
+// pmnb_evaluate shares its host copy and add over the crew for systems of
+// at least kHostCrewMin coordinates
+constexpr int kHostCrewMin = 1 << 16;
 
 struct pmnb_ctx {
   pmnb_params prm{};
@@ -124,6 +222,8 @@ struct pmnb_ctx {
   double *h_stage = nullptr;
   size_t h_stage_bytes = 0;
   hipEvent_t h_ev[1] = {};  // pmnb_evaluate: results are on the host
+  HostCrew crew;             // pmnb_evaluate: host threads (started once)
+  bool crew_started = false;
   int shape = 0;                  // tile-path kernel shape (mixed)
   int grid_cap = 0;               // persistent pair grid: max blocks (0: none)
   DevBuf d_items, d_nitems, d_reps, d_col_rep, d_rep_first;
@@ -817,11 +917,14 @@ EvalBufs ctx_bufs(const pmnb_ctx *ctx) {
 // hold zeros (tile path only), so no clearing launch is needed; the
 // work-item kernel clears the pair counter.  assigned (tile path): the
 // counts, cells and ranks of these positions are already in B (written by
-// the integrator's kernel that produced them).  Leaves ctx->counts_zero
-// false: the counts are in use.
+// the integrator's kernel that produced them).  pos_copy (not with
+// assigned): d_pos is read once, by the cell assignment, which stores the
+// positions there for the kernels after it (d_pos may then be host memory
+// mapped into the device).  Leaves ctx->counts_zero false: the counts are
+// in use.
 int build_cells(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
                 hipStream_t s, bool counts_zero = false,
-                bool assigned = false) {
+                bool assigned = false, double *pos_copy = nullptr) {
   const int n = ctx->n, nc = ctx->grid.ncell;
   const TileRep *reps = B.reps;
   const int *rep_of = ctx->batch ? ctx->d_rep_of.as<int>() : nullptr;
@@ -836,12 +939,13 @@ int build_cells(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
   } else if (n > 0 && ctx->use_tile) {
     hipLaunchKernelGGL(k_assign_r<TileRep>, dim3(blocks_for(n)), dim3(kBlock),
                        0, s, reps, rep_of, d_pos, n, B.cell_of,
-                       B.count, B.bucket, B.cap);
+                       B.count, B.bucket, B.cap, pos_copy);
   } else if (n > 0) {
     hipLaunchKernelGGL(k_assign, dim3(blocks_for(n)), dim3(kBlock), 0, s,
                        ctx->box, ctx->grid, d_pos, n, B.cell_of,
-                       B.count, rank);
+                       B.count, rank, pos_copy);
   }
+  if (pos_copy && !assigned) d_pos = pos_copy;
   if (ctx->use_tile) {
     // starts and the work items of the tile path in one block
     hipLaunchKernelGGL(k_scan_items, dim3(1), dim3(1024), 0, s,
@@ -937,7 +1041,8 @@ int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
                     uint32_t flags, hipStream_t s, bool own,
                     bool fresh = false,
                     unsigned long long *np_out = nullptr,
-                    bool assigned = false, bool *finish_skipped = nullptr) {
+                    bool assigned = false, bool *finish_skipped = nullptr,
+                    double *pos_copy = nullptr) {
   int rc = PMNB_OK;
   flags &= ~kFlagFresh;  // internal: never taken from a caller
   if (fresh && ctx->use_tile) flags |= kFlagFresh;
@@ -957,7 +1062,8 @@ int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
     ev_stop = ctx->ev_pool[ctx->ev_used + 2];
     ctx->ev_used += 3;
   }
-  rc = build_cells(ctx, B, d_pos, s, own && ctx->counts_zero, assigned);
+  rc = build_cells(ctx, B, d_pos, s, own && ctx->counts_zero, assigned,
+                   pos_copy);
   if (rc) return rc;
   if (own) ctx->last_np = B.npairs;  // not the shared scratch's counter
   const ForceParams P = make_params(ctx, flags);
@@ -1330,9 +1436,12 @@ int pmnb_evaluate(pmnb_ctx *ctx, const double *positions, double *forces,
   const size_t n3 = (size_t)ctx->n * 3;
   hipStream_t s = ctx->stream;
   // pinned staging, mapped into the device and coherent (allocated once per
-  // size): positions go in by DMA in chunks; on the tile path the kernels
-  // write the forces, the 20 output doubles and the pair count straight
-  // into it, with no copy back (else: device results copied back at once)
+  // size).  The host threads copy the positions in, and the cell assignment
+  // reads them from there over the bus and keeps a device copy for the
+  // kernels after it: no DMA transfer, whose hand-over between the copy
+  // engine and the kernels took longer than the data.  On the tile path the
+  // kernels write the forces, the 20 output doubles and the pair count
+  // straight into the buffer (else: device results copied back at once).
   const size_t nres = n3 + 21;
   HIPCHK(ctx->d_pos.ensure(N * 3 * sizeof(double)));
   const size_t want = (2 * N * 3 + 24) * sizeof(double);
@@ -1346,16 +1455,25 @@ int pmnb_evaluate(pmnb_ctx *ctx, const double *positions, double *forces,
   }
   double *h_pos = ctx->h_stage, *h_frc = ctx->h_stage + N * 3;
   double *h_out = h_frc + n3;
+  double *g_stage = nullptr;  // device address of h_stage
+  HIPCHK(hipHostGetDevicePointer((void **)&g_stage, ctx->h_stage, 0));
+  const double *g_pos = g_stage;
+  double *g_frc = g_stage + N * 3;
   for (hipEvent_t &e : ctx->h_ev)
     if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  // the host's copy into the staging buffer overlaps the transfers
-  const size_t nch = n3 >= (size_t)kHostChunkMin ? kHostChunks : 1;
-  for (size_t c = 0; c < nch && n3; c++) {
-    const size_t a = n3 * c / nch, b = n3 * (c + 1) / nch;
-    std::memcpy(h_pos + a, positions + a, (b - a) * sizeof(double));
-    HIPCHK(hipMemcpyAsync(ctx->d_pos.as<double>() + a, h_pos + a,
-                          (b - a) * sizeof(double), hipMemcpyHostToDevice, s));
+  // large systems: the copy in and the force add are split over the crew
+  const bool big = n3 >= (size_t)kHostCrewMin;
+  if (big && !ctx->crew_started) {
+    ctx->crew.start(host_crew_size());
+    ctx->crew_started = true;
   }
+  const int T = big ? ctx->crew.size() : 1;
+  if (n3)
+    ctx->crew.run([&](int k) {
+      if (k >= T) return;
+      const size_t a = n3 * k / T, b = n3 * (k + 1) / T;
+      std::memcpy(h_pos + a, positions + a, (b - a) * sizeof(double));
+    });
   const double *d_com = nullptr;
   if (flags & PMNB_FLAG_MOLVIRIAL) {
     // centres are indexed by molecule id
@@ -1368,11 +1486,10 @@ int pmnb_evaluate(pmnb_ctx *ctx, const double *positions, double *forces,
   rc = ensure_eval_buffers(ctx);
   if (rc) return rc;
   if (ctx->use_tile) {
-    double *g_frc = nullptr;  // device address of h_frc
-    HIPCHK(hipHostGetDevicePointer((void **)&g_frc, h_frc, 0));
-    rc = run_device_bufs(ctx, ctx_bufs(ctx), ctx->d_pos.as<double>(), g_frc,
-                         g_frc + n3, d_com, flags, s, true, true,
-                         reinterpret_cast<unsigned long long *>(g_frc + n3 + 20));
+    rc = run_device_bufs(ctx, ctx_bufs(ctx), g_pos, g_frc, g_frc + n3, d_com,
+                         flags, s, true, true,
+                         reinterpret_cast<unsigned long long *>(g_frc + n3 + 20),
+                         false, nullptr, ctx->d_pos.as<double>());
     if (rc) return rc;
   } else {
     HIPCHK(ctx->d_force.ensure((N * 3 + 24) * sizeof(double)));
@@ -1380,8 +1497,9 @@ int pmnb_evaluate(pmnb_ctx *ctx, const double *positions, double *forces,
     launch_fill_i32(ctx->d_force.as<int>(), (int)(2 * nres), 0, s);
     EvalBufs B = ctx_bufs(ctx);
     B.npairs = reinterpret_cast<unsigned long long *>(d_res + n3 + 20);
-    rc = run_device_bufs(ctx, B, ctx->d_pos.as<double>(), d_res, d_res + n3,
-                         d_com, flags, s, true, false);
+    rc = run_device_bufs(ctx, B, g_pos, d_res, d_res + n3, d_com, flags, s,
+                         true, false, nullptr, false, nullptr,
+                         ctx->d_pos.as<double>());
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(h_frc, d_res, nres * sizeof(double),
                           hipMemcpyDeviceToHost, s));
@@ -1391,7 +1509,11 @@ int pmnb_evaluate(pmnb_ctx *ctx, const double *positions, double *forces,
   while ((q = hipEventQuery(ctx->h_ev[0])) == hipErrorNotReady) {
   }
   HIPCHK(q);
-  for (size_t i = 0; i < n3; i++) forces[i] += h_frc[i];
+  ctx->crew.run([&](int k) {
+    if (k >= T) return;
+    const size_t a = n3 * k / T, b = n3 * (k + 1) / T;
+    for (size_t i = a; i < b; i++) forces[i] += h_frc[i];
+  });
   energies[0] += h_out[0];
   energies[1] += h_out[1];
   if (flags & (PMNB_FLAG_VIRIAL | PMNB_FLAG_MOLVIRIAL))

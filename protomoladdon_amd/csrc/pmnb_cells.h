@@ -58,16 +58,24 @@ __device__ __forceinline__ void cell_coords(const Box &b, const Grid &g,
 }
 
 // Cell of every atom; rank[i] is the atom's arrival rank in its cell (any
-// order: k_sort_cells restores ascending atom order afterwards).
+// order: k_sort_cells restores ascending atom order afterwards).  With
+// pos_copy the positions read (from host memory mapped into the device,
+// say) are also stored there for the kernels after this one.
 __global__ void k_assign(Box b, Grid g, const double *__restrict__ pos,
                          int n, int *__restrict__ cell_of,
-                         int *__restrict__ count, int *__restrict__ rank) {
+                         int *__restrict__ count, int *__restrict__ rank,
+                         double *__restrict__ pos_copy) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int cx, cy, cz;
   double vx, vy, vz;
-  cell_coords(b, g, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], cx, cy, cz,
-              vx, vy, vz);
+  const double px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+  if (pos_copy) {
+    pos_copy[3 * i] = px;
+    pos_copy[3 * i + 1] = py;
+    pos_copy[3 * i + 2] = pz;
+  }
+  cell_coords(b, g, px, py, pz, cx, cy, cz, vx, vy, vz);
   int c = (cx * g.n[1] + cy) * g.n[2] + cz;
   cell_of[i] = c;
   rank[i] = atomicAdd(&count[c], 1);
@@ -244,12 +252,14 @@ __global__ void k_gather(Box b, Grid g, const double *__restrict__ pos,
 // together, and gives each atom the cell's base plus its rank in the block:
 // its arrival rank in the cell, at which it is written into the cell's row
 // of `bucket` (cap entries per cell; an atom ranked beyond cap is found
-// again from cell_of by k_sort_gather_r).  Requires blockDim.x == kBlock.
+// again from cell_of by k_sort_gather_r).  pos_copy as in k_assign.
+// Requires blockDim.x == kBlock.
 template <class RepT>
 __global__ void __launch_bounds__(kBlock)
 k_assign_r(const RepT *__restrict__ reps, const int *__restrict__ rep_of,
            const double *__restrict__ pos, int n, int *__restrict__ cell_of,
-           int *__restrict__ count, int *__restrict__ bucket, int cap) {
+           int *__restrict__ count, int *__restrict__ bucket, int cap,
+           double *__restrict__ pos_copy) {
   __shared__ int key[kBlock], lcount[kBlock], base[kBlock];
   const int t = threadIdx.x;
   const int i = blockIdx.x * kBlock + t;
@@ -261,8 +271,13 @@ k_assign_r(const RepT *__restrict__ reps, const int *__restrict__ rep_of,
     const RepT &R = reps[rep_of ? rep_of[i] : 0];
     int cx, cy, cz;
     double vx, vy, vz;
-    cell_coords(R.box, R.grid, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2],
-                cx, cy, cz, vx, vy, vz);
+    const double px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+    if (pos_copy) {
+      pos_copy[3 * i] = px;
+      pos_copy[3 * i + 1] = py;
+      pos_copy[3 * i + 2] = pz;
+    }
+    cell_coords(R.box, R.grid, px, py, pz, cx, cy, cz, vx, vy, vz);
     c = R.cell_off + (cx * R.grid.n[1] + cy) * R.grid.n[2] + cz;
     cell_of[i] = c;
     // at most kBlock distinct cells in kBlock slots: the probe ends

@@ -240,22 +240,35 @@ def test_mixed_pair_set_of_a_cell_beyond_its_bucket():
     assert npairs == ref.n_pairs
 
 
+@pytest.mark.parametrize("n_mol,threads", [
+    (1500, None),   # below the chunked staging: the caller's thread alone
+    (8000, None),   # 24,000 atoms: chunked copy and add on the host threads
+    (8000, "1"),
+    (8000, "3"),
+])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_host_entry_equals_device_entry_on_changing_positions(precision):
+def test_host_entry_equals_device_entry_on_changing_positions(
+        precision, n_mol, threads, monkeypatch):
     """pmnb_evaluate (host arrays; the tile path writes its results into
     mapped host memory) against pmnb_evaluate_device on the same context,
     bit for bit, over calls whose positions change every time, so that a
     result left over from an earlier call cannot pass; forces and energies
-    are added to what the caller's arrays already hold."""
+    are added to what the caller's arrays already hold.  threads: the host
+    threads that share the staging copy and the force add
+    (PMNB_HOST_THREADS, read when a context first stages a large system)."""
     import torch
     from protomoladdon_amd.topology import ScalarStructure
-    w = systems.small_water(n_mol=1500)
+    if threads is None:
+        monkeypatch.delenv("PMNB_HOST_THREADS", raising=False)
+    else:
+        monkeypatch.setenv("PMNB_HOST_THREADS", threads)
+    w = systems.small_water(n_mol=n_mol)
     force = w.make_force(precision)
     rng = np.random.default_rng(7)
     dev = torch.device("cuda:0")
     for k in range(4):
         pos = w.positions + rng.normal(0.0, 0.02, w.positions.shape)
-        print("call", k, precision, flush=True)
+        print("call", k, precision, n_mol, threads, flush=True)
         fh = np.full((w.n_atoms, 3), 0.25)
         e = ScalarStructure(virial=True)
         e[SS.COULOMB] = 3.0
