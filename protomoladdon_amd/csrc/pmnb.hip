@@ -727,6 +727,9 @@ struct HalfArgs {
   TileGrid G;      // its column grid
   PairLog log;
   double *forces;  // the caller's forces (k_tile adds into them)
+  // no energy or virial rows are read (the finish kernel is skipped): the
+  // mixed single-system kernel leaves them out
+  bool forces_only = false;
 };
 
 // The single-system full-shell kernel adds its forces straight into the
@@ -741,7 +744,18 @@ void launch_half_log(const HalfArgs &a) {
     // a multiple of 8 blocks (FP64 maps blocks to XCD-contiguous items)
     const int nb = 8 * blocks_for(std::max(1, blocks_for(a.max_items,
                                                          kTmWaves)), 8);
-    if constexpr (!HALF)
+    if constexpr (!HALF && !F64 && !LOG) {
+      if (a.forces_only)
+        hipLaunchKernelGGL((k_tile<false, EL, SW, 256, false, false>),
+                           dim3(nb), dim3(kBlock), 0, a.s, *a.P, *a.S, a.G,
+                           a.items, a.nitems, a.forces, a.partial, a.pstride,
+                           a.npairs, a.log);
+      else
+        hipLaunchKernelGGL((k_tile<false, EL, SW, 256, false>), dim3(nb),
+                           dim3(kBlock), 0, a.s, *a.P, *a.S, a.G, a.items,
+                           a.nitems, a.forces, a.partial, a.pstride,
+                           a.npairs, a.log);
+    } else if constexpr (!HALF)
       hipLaunchKernelGGL((k_tile<F64, EL, SW, F64 ? 192 : 256, LOG>),
                          dim3(nb), dim3(kBlock), 0, a.s, *a.P, *a.S, a.G,
                          a.items, a.nitems, a.forces, a.partial, a.pstride,
@@ -1035,7 +1049,8 @@ int check_ready(pmnb_ctx *ctx) {
 // (given): a single-system tile evaluation whose pair kernel wrote the
 // forces itself launches no finish kernel (no energies or virial reach
 // d_out, and the caller clears the cell counts before the next
-// evaluation); *finish_skipped tells whether it did so.
+// evaluation), and its pair kernel forms no energy or virial rows;
+// *finish_skipped tells whether it did so.
 int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
                     double *d_forces, double *d_out, const double *d_com,
                     uint32_t flags, hipStream_t s, bool own,
@@ -1095,6 +1110,7 @@ int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
                  ffix, part, np, max_items, single, ctx->tile, ctx->plog,
                  d_forces};
       direct = direct_forces(a, ctx->shape, mixed);
+      a.forces_only = finish_skipped != nullptr && direct;
       launch_pair_half(mixed, el, sw, ctx->shape, a);
     }
     if (ev_stop) HIPCHK(hipEventRecord(ev_stop, s));

@@ -741,8 +741,11 @@ __device__ __forceinline__ void tile_test(const float4 *__restrict__ rec,
 // cell, and each atom's force is added straight into the caller's array
 // (+=, one writer per atom).  Fewer scalar registers than the
 // replica-general form, measured faster on one large system.  LOG: the
-// pair record of pmnb_force_pairs is compiled in.
-template <bool F64, int ELEC, int ELSW, int BUF, bool LOG>
+// pair record of pmnb_force_pairs is compiled in.  ENER = false: forces
+// and the pair count only, no energy or virial rows (for evaluations whose
+// energies nobody reads, such as the steps of a dynamics run before its
+// last).
+template <bool F64, int ELEC, int ELSW, int BUF, bool LOG, bool ENER = true>
 __global__ void __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(3)))
 k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
@@ -818,7 +821,7 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
     const float Lz = F.Lz, iLz = F.iLz;
     long long Fx = 0, Fy = 0, Fz = 0;
     double fx64 = 0.0, fy64 = 0.0, fz64 = 0.0;
-    const bool vir = (P.flags & 3u) != 0;
+    const bool vir = ENER && (P.flags & 3u) != 0;
     int nbuf = 0;
 
     // FP32 partials of one buffer (flushed by process)
@@ -862,8 +865,10 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
         pair_f64<ELEC, ELSW>(P, e2d, excl != 1 ? lp[0] : lp[2],
                              excl != 1 ? lp[1] : lp[3], qi * recd[wave][k].w,
                              excl, ff, e1, e2);
-        acc[ACC_ELJ] += 0.5 * e1;
-        acc[ACC_EEL] += 0.5 * e2;
+        if (ENER) {
+          acc[ACC_ELJ] += 0.5 * e1;
+          acc[ACC_EEL] += 0.5 * e2;
+        }
         const double gx = ex * ff, gy = ey * ff, gz = ez * ff;
         fx64 -= gx;
         fy64 -= gy;
@@ -883,8 +888,10 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
         float ff, e1, e2;
         pair_f32<ELEC>(F, r2, excl != 1 ? lp.x : lp.z, excl != 1 ? lp.y : lp.w,
                        li.w * r.w, excl, ff, e1, e2);
-        pelj += e1;
-        peel += e2;
+        if (ENER) {
+          pelj += e1;
+          peel += e2;
+        }
         const float gx = dx * ff, gy = dy * ff, gz = dz * ff;
         px -= gx;
         py -= gy;
@@ -987,8 +994,10 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
         Fx += to_fixed((double)px);
         Fy += to_fixed((double)py);
         Fz += to_fixed((double)pz);
-        acc[ACC_ELJ] += 0.5 * (double)pelj;
-        acc[ACC_EEL] += 0.5 * (double)peel;
+        if (ENER) {
+          acc[ACC_ELJ] += 0.5 * (double)pelj;
+          acc[ACC_EEL] += 0.5 * (double)peel;
+        }
         if (vir) {
           acc[ACC_VXX] += 0.5 * (double)v0;
           acc[ACC_VXY] += 0.5 * (double)v1;
@@ -1078,7 +1087,7 @@ k_tile(ForceParams P, Sorted S, TileGrid G, const int4 *__restrict__ items,
     }
   }
   // this item's energies and virials (one row per item)
-  if (item_idx < *nitems) {
+  if (ENER && item_idx < *nitems) {
 #pragma unroll
     for (int k = 0; k < ACC_N; k++) {
       double v = acc[k];
