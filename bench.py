@@ -30,10 +30,12 @@ sys.path.insert(0, ROOT)
 METRIC = ("ns/day + pair evals/s, 96k-atom LJ+Ewald-real; "
           "mixed-prec vs FP64 on GPU")
 FLOP_PER_PAIR = 90        # LJ + Ewald real (A&S) + shared C2 + atomic virial
+FLOP_PER_PAIR_FORCES = 73  # the same without the energy sums (2) and virial (15)
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
 PEAK_FP64_TFLOPS = 78.6   # MI355X published FP64 vector peak
 DT_FS = 1.0
 GRAPH_CHUNK = 20          # MD steps per captured run (replayed)
+MD_KERNEL_STEPS = 20      # steps of the run whose pair kernel is timed
 
 
 def ns_per_day(steps_per_s):
@@ -102,6 +104,17 @@ def time_md(w, precision, steps, warmup, flags, torch):
     its, n_calls, unconverged = lf.record_iterations(0)
     # call order: SHAKE, then (RATTLE, SHAKE) per later step, then RATTLE
     shake_its, rattle_its = its[0::2], its[1::2]
+    # the pair kernel of the dynamics' evaluations (HIP events on the launch
+    # stream): run(1), whose one evaluation forms energies, then run(K),
+    # K - 1 evaluations without energies and the last with them
+    force.kernel_timing(True)
+    lf.run(t, x, v, f, out, 1, cons, flags=flags, stream=sp)
+    _, k_one, n_one = force.kernel_timing(False)
+    force.kernel_timing(True)
+    lf.run(t, x, v, f, out, MD_KERNEL_STEPS, cons, flags=flags, stream=sp)
+    _, k_run, n_run = force.kernel_timing(False)
+    k_energy = k_one / max(n_one, 1)
+    k_forces = (k_run - k_energy) / max(n_run - 1, 1)
     # the same steps as replays of one captured run(chunk) (pmnb_lf_run_graph)
     chunk = GRAPH_CHUNK if steps >= GRAPH_CHUNK else steps
     reps = max(1, steps // chunk)
@@ -126,6 +139,8 @@ def time_md(w, precision, steps, warmup, flags, torch):
             "eager_ms_per_step": ms_eager,
             "graph_ms_per_step": ms_graph,
             "graph": "%d replays of one captured run(%d)" % (reps, chunk),
+            "pair_kernel_ms_forces_only": k_forces,
+            "pair_kernel_ms_with_energies": k_energy,
             "mean_shake_sweeps": float(shake_its.mean()),
             "mean_rattle_sweeps": float(rattle_its.mean()),
             "unconverged_calls": int(unconverged + unconverged_g)}
@@ -629,6 +644,8 @@ def main():
     P = mx["pairs"]
     achieved = P * FLOP_PER_PAIR / (mx["kernel_ms"] * 1e-3) / 1e12
     achieved64 = P * FLOP_PER_PAIR / (f64["kernel_ms"] * 1e-3) / 1e12
+    md_achieved = P * FLOP_PER_PAIR_FORCES / (
+        md_mx["pair_kernel_ms_forces_only"] * 1e-3) / 1e12
 
     # HBM bytes per launch of the pair kernel, from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE passes (scaled by a copy kernel of known size)
@@ -705,7 +722,17 @@ def main():
                      "cell_build_ms": mx["build_ms"],
                      "achieved_incl_cell_build": P * FLOP_PER_PAIR / (
                          (mx["kernel_ms"] + mx["build_ms"]) * 1e-3) / 1e12,
-                     "flop_per_pair": FLOP_PER_PAIR},
+                     "flop_per_pair": FLOP_PER_PAIR,
+                     # the dynamics' pair kernel (every evaluation of a
+                     # run but its last): forces only, 73 of the 90 FLOP
+                     "md_kernel": {
+                         "kernel": "k_tile<mixed, Ewald-real, C2, forces only>",
+                         "kernel_ms": md_mx["pair_kernel_ms_forces_only"],
+                         "flop_per_pair": FLOP_PER_PAIR_FORCES,
+                         "achieved": md_achieved,
+                         "frac": md_achieved / PEAK_FP32_TFLOPS,
+                         "frac_without_packed_fp32":
+                             md_achieved / PEAK_FP64_TFLOPS}},
         "cpu_baseline": cpu,
     }
     line.update(extras)
