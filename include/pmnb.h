@@ -315,7 +315,10 @@ int pmnb_batch_autotune(pmnb_batch *b, const double *d_positions,
                         int32_t *shape, int32_t *grid_blocks, double *ms);
 
 /* Sets the pair-kernel shape (mixed 0 .. 2, FP64 0 .. 1, as the autotuner
-   numbers them) and grid cap by hand. */
+   numbers them) and grid cap by hand.  The shape (autotuned or set) is the
+   one of evaluations that form energies; a mixed evaluation whose energies
+   nobody reads (a dynamics step before the last of a run) takes shape 0
+   without energy sums, the fastest there. */
 int pmnb_batch_set_shape(pmnb_batch *b, int32_t shape, int32_t grid_blocks);
 
 /* pmnb_force_pairs for the whole batch at device positions d_positions:

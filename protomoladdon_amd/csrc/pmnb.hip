@@ -770,6 +770,16 @@ void launch_half_log(const HalfArgs &a) {
   int nb = blocks_for(a.max_items, kTmWaves);
   if (a.grid_cap > 0) nb = std::min(nb, a.grid_cap);
   nb = std::max(nb, 1);
+  if constexpr (!F64 && !LOG) {
+    if (a.forces_only) {
+      hipLaunchKernelGGL((k_half<false, EL, SW, BUF, OCC, true, HALF, false,
+                                 false>),
+                         dim3(nb), dim3(kBlock), 0, a.s, *a.P, *a.S, a.reps,
+                         a.G, a.items, a.nitems, a.ffix, a.partial, a.pstride,
+                         a.npairs, a.log);
+      return;
+    }
+  }
   hipLaunchKernelGGL((k_half<F64, EL, SW, BUF, OCC, true, HALF, LOG>),
                      dim3(nb), dim3(kBlock), 0, a.s, *a.P, *a.S, a.reps, a.G,
                      a.items, a.nitems, a.ffix, a.partial, a.pstride, a.npairs,
@@ -1109,9 +1119,17 @@ int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
       HalfArgs a{max_items, ctx->grid_cap, s, &P, &S, reps, items, nitems,
                  ffix, part, np, max_items, single, ctx->tile, ctx->plog,
                  d_forces};
-      direct = direct_forces(a, ctx->shape, mixed);
-      a.forces_only = finish_skipped != nullptr && direct;
-      launch_pair_half(mixed, el, sw, ctx->shape, a);
+      // no energies wanted: the mixed kernels leave them out, in the full
+      // shell of 256 records whatever shape the evaluations with energies
+      // use (114 VGPRs, four waves per SIMD; the half shell's fixed-point
+      // slots in LDS hold it to three: 3.8 against 4.8 ms per dynamics step
+      // of the 512 water replicas).  A single system's kernel then needs
+      // no finish; a batch's finish still converts the forces but sums no
+      // energies.
+      a.forces_only = finish_skipped != nullptr;
+      const int shape = a.forces_only && mixed ? 0 : ctx->shape;
+      direct = direct_forces(a, shape, mixed);
+      launch_pair_half(mixed, el, sw, shape, a);
     }
     if (ev_stop) HIPCHK(hipEventRecord(ev_stop, s));
     if (finish_skipped) {
@@ -1122,7 +1140,7 @@ int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
     // added them itself), per-replica sums
     // and the cell counts cleared for the next evaluation
     const int ab = direct ? 0 : blocks_for(ctx->n, 1024);
-    const int rb = nrep * ACC_N, nc = ctx->grid.ncell;
+    const int rb = finish_skipped ? 0 : nrep * ACC_N, nc = ctx->grid.ncell;
     const int fresh = (flags & kFlagFresh) ? 1 : 0;
     const dim3 fg((unsigned)(ab + rb + blocks_for(nc, 1024)));
     if (mixed)

@@ -280,9 +280,10 @@ __device__ __forceinline__ void stream_levels(
 // with a single column), a persistent item loop with a grid stride.  OCC:
 // waves per SIMD the register allocation must allow.  LOG: the pair record
 // of pmnb_force_pairs is compiled in (the same code, with each evaluated
-// pair appended to plog).
+// pair appended to plog).  ENER = false: forces and the pair count only,
+// no energy or virial rows.
 template <bool F64, int ELEC, int ELSW, int BUF, int OCC, bool REP, bool HALF,
-          bool LOG>
+          bool LOG, bool ENER = true>
 __global__ void __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(OCC)))
 k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
@@ -317,7 +318,7 @@ k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
   __syncthreads();
   const F32Params &F = P.f;
   const float hi = F.cut2 + F.band, lo = F.cut2 - F.band;
-  const bool vir = (P.flags & 3u) != 0;
+  const bool vir = ENER && (P.flags & 3u) != 0;
   const int n_items = *nitems;
   // FP64 single system: blocks go to the 8 XCDs in turn; block b takes the
   // items of block position (b % 8) * per + b / 8 (the grid is a multiple
@@ -432,8 +433,10 @@ k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
         pair_f64<ELEC, ELSW>(P, e2d, excl != 1 ? lp[0] : lp[2],
                              excl != 1 ? lp[1] : lp[3], qi * recd[wave][k].w,
                              excl, ff, e1, e2);
-        acc[ACC_ELJ] += kShare * e1;
-        acc[ACC_EEL] += kShare * e2;
+        if (ENER) {
+          acc[ACC_ELJ] += kShare * e1;
+          acc[ACC_EEL] += kShare * e2;
+        }
         const double gx = ex * ff, gy = ey * ff, gz = ez * ff;
         fx64 -= gx;
         fy64 -= gy;
@@ -450,9 +453,9 @@ k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
           lds_add_i64(&pfix[wave][4][k], h);
           lds_add_i64(&pfix[wave][5][k], l);
         }
-        if (molv || (P.flags & 1u))
+        if (ENER && (molv || (P.flags & 1u)))
           add_virial(acc, ACC_VXX, gx, gy, gz, ex, ey, ez, kShare);
-        if (molv) {
+        if (ENER && molv) {
           const Box &bm = rare_box(bptr);
           double cdx, cdy, cdz, cd2;
           min_diff_exact(bm, S.com[3 * mi], S.com[3 * mi + 1],
@@ -466,8 +469,10 @@ k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
         float ff, e1, e2;
         pair_f32<ELEC>(F, r2, excl != 1 ? lp.x : lp.z, excl != 1 ? lp.y : lp.w,
                        li.w * r.w, excl, ff, e1, e2);
-        pelj += e1;
-        peel += e2;
+        if (ENER) {
+          pelj += e1;
+          peel += e2;
+        }
         const float gx = dx * ff, gy = dy * ff, gz = dz * ff;
         px -= gx;
         py -= gy;
@@ -585,8 +590,10 @@ k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
         Fx += to_fixed_f32(px);
         Fy += to_fixed_f32(py);
         Fz += to_fixed_f32(pz);
-        acc[ACC_ELJ] += kShare * (double)pelj;
-        acc[ACC_EEL] += kShare * (double)peel;
+        if (ENER) {
+          acc[ACC_ELJ] += kShare * (double)pelj;
+          acc[ACC_EEL] += kShare * (double)peel;
+        }
         if (vir) {
           acc[ACC_VXX] += kShare * (double)v0;
           acc[ACC_VXY] += kShare * (double)v1;
@@ -703,11 +710,13 @@ k_half(ForceParams P, Sorted S, const TileRep *__restrict__ reps,
       }
     }
     // this item's energies and virials
+    if (ENER) {
 #pragma unroll
-    for (int k = 0; k < ACC_N; k++) {
-      double v = acc[k];
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-      if (lane == 0) partial[(size_t)k * pstride + item_idx] = v;
+      for (int k = 0; k < ACC_N; k++) {
+        double v = acc[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) partial[(size_t)k * pstride + item_idx] = v;
+      }
     }
     if (!REP) break;  // one item per wave
   }
