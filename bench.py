@@ -649,11 +649,13 @@ def main():
 
     # HBM bytes per launch of the pair kernel, from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE passes (scaled by a copy kernel of known size)
-    traffic = None
+    traffic = traffic_fo = None
     tpath = os.path.join(ROOT, "profiles", "k_tile_water96k_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+            tj = json.load(fh)
+        traffic = tj.get("hbm_bytes_per_launch")
+        traffic_fo = tj.get("forces_only", {}).get("hbm_bytes_per_launch")
 
     cpu, accuracy = None, None
     if args.cpu_budget > 0:
@@ -731,6 +733,7 @@ def main():
                          "flop_per_pair": FLOP_PER_PAIR_FORCES,
                          "achieved": md_achieved,
                          "frac": md_achieved / PEAK_FP32_TFLOPS,
+                         "traffic": traffic_fo,
                          "frac_without_packed_fp32":
                              md_achieved / PEAK_FP64_TFLOPS}},
         "cpu_baseline": cpu,
