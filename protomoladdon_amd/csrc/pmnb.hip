@@ -727,8 +727,7 @@ struct HalfArgs {
   TileGrid G;      // its column grid
   PairLog log;
   double *forces;  // the caller's forces (k_tile adds into them)
-  // no energy or virial rows are read (the finish kernel is skipped): the
-  // mixed single-system kernel leaves them out
+  // no energy or virial rows are read: the mixed kernels leave them out
   bool forces_only = false;
 };
 
@@ -1056,11 +1055,12 @@ int check_ready(pmnb_ctx *ctx) {
 // on the tile path the forces and the 20 output doubles are written, not
 // added to (the caller's arrays need no clearing).  np_out (tile path):
 // the finish kernel also stores the pair count there.  finish_skipped
-// (given): a single-system tile evaluation whose pair kernel wrote the
-// forces itself launches no finish kernel (no energies or virial reach
-// d_out, and the caller clears the cell counts before the next
-// evaluation), and its pair kernel forms no energy or virial rows;
-// *finish_skipped tells whether it did so.
+// (given): no energies or virial are wanted in d_out; the mixed pair
+// kernels form none, a single-system tile evaluation whose pair kernel
+// wrote the forces itself launches no finish kernel (the caller clears the
+// cell counts before the next evaluation), and a batch's finish kernel
+// sums no energies; *finish_skipped tells whether the finish was left
+// out.
 int run_device_bufs(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
                     double *d_forces, double *d_out, const double *d_com,
                     uint32_t flags, hipStream_t s, bool own,
