@@ -167,6 +167,10 @@ int host_crew_size() {
 // pmnb_evaluate shares its host copy and add over the crew for systems of
 // at least kHostCrewMin coordinates
 constexpr int kHostCrewMin = 1 << 16;
+// Above this many cells the tile path scans the cell counts over the
+// columns in three launches instead of one block (512 water replicas,
+// about 63,000 cells in 5,500 columns: 146 us in one block)
+constexpr int kScanCellsOneBlock = 16384;
 
 struct pmnb_ctx {
   pmnb_params prm{};
@@ -226,7 +230,7 @@ struct pmnb_ctx {
   bool crew_started = false;
   int shape = 0;                  // tile-path kernel shape (mixed)
   int grid_cap = 0;               // persistent pair grid: max blocks (0: none)
-  DevBuf d_items, d_nitems, d_reps, d_col_rep, d_rep_first;
+  DevBuf d_items, d_nitems, d_reps, d_col_rep, d_rep_first, d_colscan;
   double box_key[12] = {0};       // e1, e2, e3, origin of the current box
   unsigned long long topo_gen = 0;  // pmnb_set_topology calls so far
   int cap = 0;                    // list slots per atom
@@ -677,6 +681,7 @@ int ensure_eval_buffers(pmnb_ctx *ctx) {
                                sizeof(int4)));
     HIPCHK(ctx->d_nitems.ensure(sizeof(int)));
     HIPCHK(ctx->d_rep_first.ensure((size_t)(nrep + 1) * sizeof(int)));
+    HIPCHK(ctx->d_colscan.ensure((size_t)3 * std::max(ncol, 1) * sizeof(int)));
     if (!ctx->rep_uploaded) {
       HIPCHK(ctx->d_reps.ensure((size_t)nrep * sizeof(TileRep)));
       HIPCHK(hipMemcpy(ctx->d_reps.p, ctx->reps.data(),
@@ -900,6 +905,9 @@ struct EvalBufs {
   unsigned long long *npairs;
   int4 *items;
   int *nitems, *rep_first;
+  // 3 ints per global column (atoms, first atom, first item) for the
+  // column scan of many cells; null: the one-block scan
+  int *colscan;
   const TileRep *reps;
   const int *col_rep;
   const Box *box;
@@ -929,6 +937,7 @@ EvalBufs ctx_bufs(const pmnb_ctx *ctx) {
   B.items = ctx->d_items.as<int4>();
   B.nitems = ctx->d_nitems.as<int>();
   B.rep_first = ctx->d_rep_first.as<int>();
+  B.colscan = ctx->d_colscan.as<int>();
   B.reps = ctx->d_reps.as<TileRep>();
   B.col_rep = ctx->d_col_rep.as<int>();
   B.box = ctx->d_box.as<Box>();
@@ -969,7 +978,19 @@ int build_cells(pmnb_ctx *ctx, const EvalBufs &B, const double *d_pos,
                        B.count, rank, pos_copy);
   }
   if (pos_copy && !assigned) d_pos = pos_copy;
-  if (ctx->use_tile) {
+  if (ctx->use_tile && B.colscan && nc > kScanCellsOneBlock) {
+    // starts and work items over the columns (many cells: a batch)
+    const int ncol = ctx->ncol_total;
+    int *col_n = B.colscan, *col_a0 = col_n + ncol, *col_i0 = col_a0 + ncol;
+    hipLaunchKernelGGL(k_col_atoms, dim3(blocks_for(ncol)), dim3(kBlock), 0,
+                       s, B.count, reps, B.col_rep, ncol, col_n);
+    hipLaunchKernelGGL(k_col_scan, dim3(1), dim3(1024), 0, s, col_n, ncol,
+                       (int)ctx->reps.size(), col_a0, col_i0, B.nitems,
+                       B.rep_first, B.npairs);
+    hipLaunchKernelGGL(k_col_fill, dim3(blocks_for(ncol)), dim3(kBlock), 0,
+                       s, B.count, B.start, reps, B.col_rep, ncol, col_a0,
+                       col_i0, B.items, B.rep_first);
+  } else if (ctx->use_tile) {
     // starts and the work items of the tile path in one block
     hipLaunchKernelGGL(k_scan_items, dim3(1), dim3(1024), 0, s,
                        B.count, B.start, nc,
@@ -1755,7 +1776,7 @@ void pmnb_destroy(pmnb_ctx *ctx) {
                     &ctx->d_nitems, &ctx->d_reps, &ctx->d_col_rep,
                     &ctx->d_rep_first, &ctx->d_rep_of, &ctx->d_partial_lj,
                     &ctx->d_ffix, &ctx->d_plog, &ctx->d_plog_count,
-                    &ctx->d_bucket};
+                    &ctx->d_bucket, &ctx->d_colscan};
   for (DevBuf *b : bufs) b->release();
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   for (hipEvent_t e : ctx->h_ev)

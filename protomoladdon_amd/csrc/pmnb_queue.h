@@ -137,6 +137,93 @@ k_scan_items(const int *__restrict__ count, int *__restrict__ start,
               rep_first);
 }
 
+// The same starts and items for many cells, in three launches over the
+// global columns (a column's cells are consecutive, and columns are
+// numbered in cell order), so the loads spread over the chip instead of
+// one block.  Columns first: the atoms of every column.
+__device__ __forceinline__ void col_cells(const TileRep *__restrict__ reps,
+                                          const int *__restrict__ col_rep,
+                                          int gc, int &r, int &lc,
+                                          int &first, int &nz) {
+  r = col_rep[gc];
+  const TileRep &R = reps[r];
+  nz = R.tg.nz;
+  lc = gc - R.col_off;
+  first = R.cell_off + lc * nz;
+}
+
+__global__ void k_col_atoms(const int *__restrict__ count,
+                            const TileRep *__restrict__ reps,
+                            const int *__restrict__ col_rep, int ncol,
+                            int *__restrict__ col_n) {
+  const int gc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gc >= ncol) return;
+  int r, lc, first, nz;
+  col_cells(reps, col_rep, gc, r, lc, first, nz);
+  int a = 0;
+  for (int z = 0; z < nz; z++) a += count[first + z];
+  col_n[gc] = a;
+}
+
+// One block of 1024: each column's first atom (col_a0) and first work item
+// (col_i0), exclusive scans in column order; the item count, the end of
+// rep_first, and the pair counter cleared.
+__global__ void __launch_bounds__(1024)
+k_col_scan(const int *__restrict__ col_n, int ncol, int nrep,
+           int *__restrict__ col_a0, int *__restrict__ col_i0,
+           int *__restrict__ nitems, int *__restrict__ rep_first,
+           unsigned long long *__restrict__ npairs) {
+  const int t = threadIdx.x;
+  if (t == 0) *npairs = 0ull;
+  const int chunk = (ncol + 1023) / 1024;
+  const int b0 = min(ncol, t * chunk), b1 = min(ncol, b0 + chunk);
+  int sa = 0, si = 0;
+  for (int gc = b0; gc < b1; gc++) {
+    const int a = col_n[gc];
+    sa += a;
+    si += (a + kTileAtoms - 1) / kTileAtoms;
+  }
+  int ta, ti;
+  int oa = block_scan_1024(sa, &ta);
+  int oi = block_scan_1024(si, &ti);
+  for (int gc = b0; gc < b1; gc++) {
+    const int a = col_n[gc];
+    col_a0[gc] = oa;
+    col_i0[gc] = oi;
+    oa += a;
+    oi += (a + kTileAtoms - 1) / kTileAtoms;
+  }
+  if (t == 1023) {
+    *nitems = ti;
+    rep_first[nrep] = ti;
+  }
+}
+
+// Per column: the starts of its cells and its work items (as build_items).
+__global__ void k_col_fill(const int *__restrict__ count,
+                           int *__restrict__ start,
+                           const TileRep *__restrict__ reps,
+                           const int *__restrict__ col_rep, int ncol,
+                           const int *__restrict__ col_a0,
+                           const int *__restrict__ col_i0,
+                           int4 *__restrict__ items,
+                           int *__restrict__ rep_first) {
+  const int gc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gc >= ncol) return;
+  int r, lc, first, nz;
+  col_cells(reps, col_rep, gc, r, lc, first, nz);
+  const int a0 = col_a0[gc];
+  int o = col_i0[gc];
+  int run = a0;
+  for (int z = 0; z < nz; z++) {
+    start[first + z] = run;
+    run += count[first + z];
+  }
+  if (lc == 0) rep_first[r] = o;
+  for (int a = a0; a < run; a += kTileAtoms)
+    items[o++] = make_int4(a, min(a + kTileAtoms, run), lc, r);
+}
+
 // Per-replica sum of one accumulator of the item partials (accumulator k
 // of item b at partial[k * pstride + b]) by one block of 1024 threads with
 // a fixed reduction tree, added into out[20 r + ..]:
