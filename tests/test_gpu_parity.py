@@ -344,6 +344,34 @@ def test_large_config_properties(which):
     assert np.abs(fm.sum(axis=0)).max() < 1e-6 * np.abs(fm).sum() + 1e-3
 
 
+def test_single_system_of_many_cells():
+    """A single system of more than 16,384 tile cells (27,000 waters with a
+    5 A cutoff: 18 x 18 columns of 55 z cells mixed, 74 FP64), whose cell
+    starts and work items are built over the columns in three launches
+    instead of one block: the pair record of both precisions equals the
+    oracle's pair set, FP64 forces within 1e-11, mixed RMS within 1e-5."""
+    w = systems.small_water(n_mol=27000, rc=5.0, rs=4.0)
+    L = (27000 / systems.WATER_DENSITY) ** (1.0 / 3.0)
+    assert int(L / 5.0) ** 2 * int(L / (5.0 / 3.0)) > 16384
+    fc = w.make_force("mixed")
+    assert fc.pair_path(w.topology) == "tile"
+    fc.close()
+    ref = O.evaluate(O.System.from_workload(w), record_pairs=True)
+    ref_pairs = O.sorted_pairs(ref.pairs)
+    print(w.name, w.n_atoms, "atoms, pairs", ref.n_pairs, flush=True)
+    for precision in ("fp64", "mixed"):
+        f, e, npairs, (_, fp) = run_gpu(w, precision, 0, with_pairs=False,
+                                        force_pairs=True)
+        print(precision, "pairs", npairs, flush=True)
+        assert npairs == ref.n_pairs
+        np.testing.assert_array_equal(fp, ref_pairs)
+        if precision == "fp64":
+            scale = max(1.0, np.abs(ref.forces).max())
+            assert np.abs(f - ref.forces).max() <= 1e-11 * scale
+        else:
+            assert rms_rel(f, ref.forces) <= 1e-5
+
+
 # ------------------------------------------------------------ general path
 # Algorithm cutoff below the switching cutoff (the pair set is the one the
 # reference's cell grid and delta list reach:
