@@ -17,10 +17,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <functional>
 #include <mutex>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -87,11 +87,17 @@ class HostCrew {
     for (std::thread &t : th_) t.join();
   }
   int size() const { return (int)th_.size() + 1; }
-  // n members in all (n - 1 threads); once
+  // n members in all (n - 1 threads); once.  A thread that cannot be
+  // started leaves the crew smaller (no exception leaves the C ABI).
   void start(int n) {
     const unsigned s0 = seq_.load(std::memory_order_relaxed);
-    for (int k = 1; k < n; k++)
-      th_.emplace_back([this, k, s0] { loop(k, s0); });
+    for (int k = 1; k < n; k++) {
+      try {
+        th_.emplace_back([this, k, s0] { loop(k, s0); });
+      } catch (...) {
+        break;
+      }
+    }
   }
   // f(k) for k = 0 .. size() - 1, f(0) on the calling thread; returns when
   // every member has returned
@@ -101,8 +107,10 @@ class HostCrew {
       f(0);
       return;
     }
-    std::function<void(int)> task(f);
-    task_ = &task;
+    // the task as a plain function and argument: nothing allocated
+    using Fn = typename std::remove_reference<F>::type;
+    task_fn_ = [](void *p, int k) { (*static_cast<Fn *>(p))(k); };
+    task_arg_ = (void *)&f;
     busy_.store((int)th_.size(), std::memory_order_relaxed);
     seq_.fetch_add(1, std::memory_order_seq_cst);
     if (sleepers_.load(std::memory_order_seq_cst) > 0) {
@@ -111,7 +119,8 @@ class HostCrew {
     }
     f(0);
     while (busy_.load(std::memory_order_acquire) > 0) relax();
-    task_ = nullptr;
+    task_fn_ = nullptr;
+    task_arg_ = nullptr;
   }
 
  private:
@@ -138,7 +147,7 @@ class HostCrew {
       }
       if (now == seen) return;  // stopped
       seen = now;
-      (*task_)(k);
+      task_fn_(task_arg_, k);
       busy_.fetch_sub(1, std::memory_order_release);
     }
   }
@@ -147,7 +156,8 @@ class HostCrew {
   std::condition_variable cv_;
   std::atomic<unsigned> seq_{0};
   std::atomic<int> busy_{0}, sleepers_{0};
-  const std::function<void(int)> *task_ = nullptr;
+  void (*task_fn_)(void *, int) = nullptr;  // the posted task
+  void *task_arg_ = nullptr;
   std::atomic<bool> stop_{false};
 };
 
